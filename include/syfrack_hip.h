@@ -1,0 +1,251 @@
+/*
+ * syfrack_hip.h -- C ABI of the MI355X (gfx950) sphere-traced distance-estimator path.
+ *
+ * The library owns every device allocation behind opaque handles.  Callers own the host
+ * buffers they pass; no pointer is retained after a call returns.  Every entry point returns
+ * an int status (SYF_OK or a negative SYF_E_* code) and leaves a human-readable message,
+ * naming the offending field where there is one, in a thread-local string read with
+ * syf_last_error().
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * SyFrack source tree).  The Rust host (syfrack-render) binds these through an
+ * `extern "C"` block; nothing here uses C++ or torch types.
+ */
+#ifndef SYFRACK_HIP_H
+#define SYFRACK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SYF_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- status codes
+ * Replaces anyhow::Result at the seams: config errors (renderer.rs:81-138),
+ * device/scope errors (renderer.rs:651-666), TDR timeout (renderer.rs:749-768),
+ * wavefront status bits (wavefront.rs:414-417). */
+#define SYF_OK 0
+#define SYF_E_INVALID (-1)   /* invalid argument or configuration; message names the field */
+#define SYF_E_DEVICE (-2)    /* HIP runtime error, device limit or out of device memory */
+#define SYF_E_QUEUE (-3)     /* queue invariant broken (capacity overflow, path left alive) */
+#define SYF_E_TIMEOUT (-4)   /* watchdog expired or device lost */
+#define SYF_E_ADMISSION (-5) /* request refused: its footprint exceeds the HBM budget */
+#define SYF_E_NOT_FOUND (-6) /* unknown request id */
+#define SYF_E_NO_DEVICE (-7) /* no HIP device with that index */
+
+/* ---------------------------------------------------------------- scene description
+ * The flat job description is what Uniforms::from_scene (uniforms.rs:748-853) and
+ * ParamLayout::pack (graph_wgsl.rs:72-183) produce for the classic graphs
+ * graphs/{mandelbulb,mandelbox,menger,sphere}.ron: one fixed topology per family, its
+ * node parameters as plain fields. */
+#define SYF_FAMILY_MANDELBULB 0u /* Iterate{Trap, Power, AddOrigin} -> BulbFence */
+#define SYF_FAMILY_MANDELBOX 1u  /* Iterate{BoxFold, Trap, SphereFold, Scale, AddOrigin} -> NormFence */
+#define SYF_FAMILY_MENGER 2u     /* Iterate{MengerFold, Trap} -> Box */
+#define SYF_FAMILY_SPHERE 3u     /* Trap -> Sphere  U  Plane */
+
+typedef struct syf_fractal {
+    uint32_t family;          /* SYF_FAMILY_* */
+    uint32_t iterations;      /* Iterate.iterations (1..64, graph/mod.rs:44-73) */
+    float bailout;            /* Iterate.bailout */
+    float power;              /* Power.power (Mandelbulb) */
+    float scale;              /* Scale.factor (Mandelbox) or MengerFold.scale (Menger) */
+    float min_radius;         /* SphereFold.min_radius */
+    float fixed_radius;       /* SphereFold.fixed_radius */
+    float fold_limit;         /* BoxFold.limit */
+    float offset[3];          /* MengerFold.offset */
+    float half[3];            /* Box.half (Menger fence) */
+    float radius;             /* Sphere.radius (sphere graph) */
+    float plane_normal[3];    /* Plane.normal (sphere graph ground) */
+    float plane_offset;       /* Plane.offset */
+    float trap_scale;         /* Trap.scale of slot 0 */
+    float safety;             /* Graph.safety; step scale = safety / Lipschitz (eval.rs:879) */
+    float bounding_radius_hint;
+} syf_fractal;
+
+#define SYF_CAMERA_FREE 0u
+#define SYF_CAMERA_ORBIT 1u
+/* scene/camera.rs:5-35; the basis is derived as Camera::basis (camera.rs:58-104). */
+typedef struct syf_camera {
+    uint32_t mode;            /* SYF_CAMERA_FREE or SYF_CAMERA_ORBIT */
+    float position[3];        /* Free */
+    float target[3];          /* Free */
+    float roll;               /* degrees, both modes */
+    float point[3];           /* Orbit */
+    float distance;           /* Orbit */
+    float azimuth;            /* Orbit, degrees */
+    float elevation;          /* Orbit, degrees */
+    float focal_mm;           /* 36 mm sensor: tan(half horizontal fov) = 18 / focal_mm */
+    float f_number;           /* 0 = pinhole */
+    float focus_distance;
+} syf_camera;
+
+#define SYF_LIGHT_AREA 1u /* GpuLight KIND_AREA (uniforms.rs:676-716) */
+#define SYF_LIGHT_ENV 2u  /* GpuLight KIND_ENV (uniforms.rs:732-744) */
+#define SYF_MAX_LIGHTS 4
+typedef struct syf_light {
+    uint32_t kind;            /* SYF_LIGHT_AREA or SYF_LIGHT_ENV */
+    float position[3];        /* area: centre */
+    float normal[3];          /* area: emitting side */
+    float size[2];            /* area: full width and height */
+    float color[3];           /* area: linear colour */
+    float intensity;          /* both: multiplies the colour(s) */
+    float zenith[3];          /* env */
+    float horizon[3];         /* env */
+    float ground[3];          /* env */
+} syf_light;
+
+/* scene/material.rs:138-165 and SurfaceVariation (material.rs:70-91). */
+typedef struct syf_material {
+    float albedo[3];
+    float roughness;
+    float metallic;
+    float specular;
+    uint32_t surface_enabled;      /* SurfaceVariation present */
+    uint32_t surface_slot;         /* trap slot 0..3 that drives the variation */
+    float surface_roughness[2];    /* roughness at t = 0 and t = 1 */
+    float surface_metallic[2];     /* metallic at t = 0 and t = 1 */
+} syf_material;
+
+/* One render job: the scene plus RenderConfig (renderer.rs:17-43) fields. */
+typedef struct syf_job_desc {
+    syf_fractal fractal;
+    syf_camera camera;
+    syf_material material;
+    syf_light lights[SYF_MAX_LIGHTS];
+    uint32_t light_count;
+    uint32_t width;
+    uint32_t height;
+    uint32_t max_bounces;          /* 0..32 */
+    uint32_t max_steps;            /* >= 1 */
+    float pixel_tolerance;         /* 0.125..1.0 */
+    float cavity_darkness;         /* Atmosphere.cavity_darkness, 0..1 */
+    uint32_t reserved;
+    uint64_t seed;                 /* Scene.seed; folded lo ^ hi as uniforms.rs:824 */
+} syf_job_desc;
+
+/* ---------------------------------------------------------------- planes
+ * Device planes of a job, read back with syf_readback.  Order follows the reference
+ * session planes (session.rs:1217-1243) where they exist. */
+#define SYF_PLANE_ACCUM 0u   /* float4 per pixel: sum of radiance, sample count */
+#define SYF_PLANE_DEPTH 1u   /* float per pixel: first-hit march distance, -1 on a miss */
+#define SYF_PLANE_AUX 2u     /* float4 per pixel: first-hit normal, steps; (0,0,0,-1) on a miss */
+#define SYF_PLANE_DEPTH64 3u /* double per pixel: FP64-refined hit distance, -1 on a miss */
+#define SYF_PLANE_STEPS 4u   /* uint32 per pixel: march steps of the primary ray, hit or miss */
+
+/* Work counters of the last launch on a job, and its device time. */
+typedef struct syf_stats {
+    uint64_t rays;             /* traced segments: primary + bounce + shadow */
+    uint64_t primary_rays;
+    uint64_t hits;             /* primary rays that hit the form */
+    uint64_t de_evals;         /* FP32 distance evaluations (march steps + normal taps) */
+    uint64_t de_iterations;    /* FP32 Iterate-body executions over all evaluations */
+    uint64_t de64_evals;       /* FP64 evaluations (final-hit refine) */
+    uint64_t de64_iterations;
+    uint64_t steps;            /* march steps over all segments */
+    double kernel_ms;          /* device time of the last call's launches, HIP events */
+    double march_ms;           /* device time of its dominant (FP32 march) kernel alone */
+} syf_stats;
+
+/* ---------------------------------------------------------------- opaque handles */
+typedef struct syf_ctx syf_ctx;
+typedef struct syf_job syf_job;
+typedef struct syf_planar syf_planar;
+typedef struct syf_server syf_server;
+
+/* Thread-local text of the last error on this thread ("" when none). */
+const char* syf_last_error(void);
+int syf_abi_version(void);
+
+/* Device context: one HIP device and its stream pool.  Replaces GpuContext::new
+ * (gpu.rs:15-34) for this path. */
+int syf_ctx_create(int device, syf_ctx** out);
+int syf_ctx_destroy(syf_ctx* ctx);
+
+/* Render job: validates the description like RenderConfig::validate (renderer.rs:81-138),
+ * derives the camera basis (camera.rs:58-104), the step scale (eval.rs:879-881), the ray
+ * extent (uniforms.rs:781-782) and allocates its device planes.  Replaces
+ * RenderSession::build (session.rs:187-219). */
+int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out);
+int syf_job_destroy(syf_job* job);
+/* Zero the accumulation and first-hit planes. */
+int syf_job_clear(syf_job* job);
+/* Wait for every launch queued on the job's stream. */
+int syf_job_sync(syf_job* job);
+/* Counters and device time of the last launch; waits for it first. */
+int syf_job_stats(syf_job* job, syf_stats* out);
+
+/* Distance probe: evaluates the field at n points (x, y, z, ignored) and writes
+ * (distance, trap0, iterations, material) and the four traps per point, the layout of
+ * de_probe.wgsl:13-17.  Replaces parity::probe_scene (parity.rs:219-332) and
+ * Renderer::probe_distances (renderer.rs:645-648).  Host buffers; synchronous. */
+int syf_de_probe(syf_job* job, const float* xyz4, size_t n, float* out4, float* traps4);
+
+/* Primary-ray sphere trace of sample `sample` over the whole image: FP32 march with the
+ * pixel-footprint hit rule (march_segment, pathtrace.wgsl:78-145), tetrahedral normal
+ * (common.wgsl:376-387), FP64 refine of each hit.  Fills DEPTH, AUX, DEPTH64 and STEPS.
+ * Asynchronous on the job's stream.  This is the first-hit half of Renderer::dispatch
+ * (renderer.rs:729-771) with max_bounces = 0. */
+int syf_trace_primary(syf_job* job, uint32_t sample);
+
+/* Megakernel path trace of a tile: for each pixel in [x, x+w) x [y, y+h), samples
+ * sample_start .. sample_start+sample_count-1 are traced and added to ACCUM in ascending
+ * sample order; DEPTH/AUX take the first hit when sample_start == 0.  The one-launch form
+ * of Renderer::dispatch (renderer.rs:729-771) running pathtrace.wgsl main (:1203-1276).
+ * Asynchronous. */
+int syf_render_tile(syf_job* job, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                    uint32_t sample_start, uint32_t sample_count);
+
+/* Wavefront path trace of a tile, same result contract as syf_render_tile: per sample,
+ * generate -> (march -> shade/extend -> compact) x rounds -> accumulate, with ray state in
+ * SoA device queues.  Replaces Wavefront::dispatch (wavefront.rs:310-419).  Synchronous:
+ * returns SYF_E_QUEUE when a queue invariant is broken. */
+int syf_render_wavefront(syf_job* job, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                         uint32_t sample_start, uint32_t sample_count);
+
+/* Copy a device plane (SYF_PLANE_*) to a host buffer of exactly `bytes` bytes.
+ * Replaces the readbacks of RenderSession::readback (session.rs:1217-1243). */
+int syf_readback(syf_job* job, uint32_t plane, void* host, size_t bytes);
+
+/* ---------------------------------------------------------------- 2D escape time
+ * PlanarFractal::sample (planar.rs:77-194) / planar_sample (planar.wgsl:16-91) for the
+ * parameter-plane families, plus FP64 and deep-zoom perturbation modes that have no
+ * reference counterpart. */
+#define SYF_PLANAR_MANDELBROT 2u  /* family numbering of planar.wgsl:29 */
+#define SYF_PLANAR_BURNING_SHIP 3u
+#define SYF_PLANAR_JULIA 1u
+#define SYF_PLANAR_FP32 0u        /* direct FP32 iteration (the reference's precision) */
+#define SYF_PLANAR_FP64 1u        /* direct FP64 iteration: counts bit-exact */
+#define SYF_PLANAR_PERTURB 2u     /* FP64 reference orbit + FP32 deltas, rebasing on glitch */
+typedef struct syf_planar_desc {
+    uint32_t family;          /* SYF_PLANAR_* family */
+    uint32_t mode;            /* SYF_PLANAR_FP32 / FP64 / PERTURB */
+    uint32_t width;
+    uint32_t height;
+    uint32_t iterations;      /* maximum iterations (budget) */
+    uint32_t reserved;
+    double center[2];         /* centre of the view in the complex plane */
+    double span;              /* width of the view in the complex plane */
+    double julia[2];          /* Julia constant */
+    float bailout;            /* escape radius */
+    float rotation;           /* view rotation, degrees */
+} syf_planar_desc;
+#define SYF_PLANAR_SMOOTH 0u  /* float per pixel: smooth iteration count */
+#define SYF_PLANAR_ITER 1u    /* uint32 per pixel: iteration count */
+
+int syf_planar_create(syf_ctx* ctx, const syf_planar_desc* desc, syf_planar** out);
+int syf_planar_destroy(syf_planar* p);
+/* Evaluate every pixel (asynchronous; the reference orbit is built on the host first in
+ * PERTURB mode). */
+int syf_planar_run(syf_planar* p);
+int syf_planar_sync(syf_planar* p);
+int syf_planar_readback(syf_planar* p, uint32_t plane, void* host, size_t bytes);
+/* Iterations evaluated, glitch re-references, and device time of the last run. */
+int syf_planar_stats(syf_planar* p, uint64_t* iterations, uint64_t* rebases, double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SYFRACK_HIP_H */

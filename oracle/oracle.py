@@ -1,0 +1,172 @@
+"""ctypes binding of the oracle library (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py, always as the checker.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+_LIB = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    lib.orc_pcg.restype = C.c_uint32
+    lib.orc_pcg.argtypes = [C.c_uint32]
+    lib.orc_rng_init.restype = C.c_uint32
+    lib.orc_rng_init.argtypes = [C.c_uint32] * 4
+    lib.orc_fold_seed.restype = C.c_uint32
+    lib.orc_fold_seed.argtypes = [C.c_uint64]
+    lib.orc_camera_basis.restype = None
+    lib.orc_camera_basis.argtypes = [P, P]
+    lib.orc_camera_rays.restype = None
+    lib.orc_camera_rays.argtypes = [P, C.c_uint32, P, P]
+    lib.orc_de_probe.restype = C.c_int
+    lib.orc_de_probe.argtypes = [P, C.c_int, P, C.c_size_t, P, P, P, P]
+    lib.orc_spherical_power.restype = None
+    lib.orc_spherical_power.argtypes = [P, C.c_size_t, C.c_uint32, P]
+    lib.orc_trace_primary.restype = C.c_int
+    lib.orc_trace_primary.argtypes = [P, C.c_uint32, C.c_int] + [C.c_uint32] * 4 + [P] * 5
+    lib.orc_first_root_bruteforce.restype = C.c_double
+    lib.orc_first_root_bruteforce.argtypes = [P, P, P, C.c_double, C.c_double]
+    lib.orc_planar_sample_f32.restype = C.c_uint32
+    lib.orc_planar_sample_f32.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_float, C.c_float, C.c_float,
+                                          C.c_float, C.c_float, P]
+    lib.orc_planar_offsets.restype = None
+    lib.orc_planar_offsets.argtypes = [P, P, P]
+    lib.orc_escape_f64.restype = None
+    lib.orc_escape_f64.argtypes = [P] + [C.c_uint32] * 4 + [P, P]
+    lib.orc_escape_f128.restype = None
+    lib.orc_escape_f128.argtypes = [P, P, P, C.c_size_t, P, P]
+    if hasattr(lib, "orc_render"):
+        lib.orc_render.restype = C.c_int
+        lib.orc_render.argtypes = [P] + [C.c_uint32] * 6 + [P, P, P]
+    _LIB = lib
+    return lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def de_probe(fractal, points: np.ndarray, precision: int = 32):
+    """Distance, traps (n, 4), iterations and body executions at points (n, 3 or 4)."""
+    lib = load()
+    pts = np.zeros((len(points), 4), np.float32)
+    pts[:, :3] = np.asarray(points, np.float32)[:, :3]
+    n = len(pts)
+    dist = np.zeros(n, np.float64)
+    traps = np.zeros((n, 4), np.float64)
+    iters = np.zeros(n, np.uint32)
+    runs = np.zeros(n, np.uint32)
+    rc = lib.orc_de_probe(C.byref(fractal), precision, _ptr(pts), n, _ptr(dist), _ptr(traps), _ptr(iters), _ptr(runs))
+    assert rc == 0
+    return dist, traps, iters, runs
+
+
+def spherical_power(points: np.ndarray, power: int):
+    lib = load()
+    pts = np.zeros((len(points), 4), np.float32)
+    pts[:, :3] = np.asarray(points, np.float32)[:, :3]
+    out = np.zeros((len(pts), 4), np.float32)
+    lib.orc_spherical_power(_ptr(pts), len(pts), power, _ptr(out))
+    return out[:, :3], out[:, 3]
+
+
+def camera_basis(desc) -> np.ndarray:
+    out = np.zeros(16, np.float32)
+    load().orc_camera_basis(C.byref(desc), _ptr(out))
+    return out
+
+
+def camera_rays(desc, sample: int):
+    n = desc.width * desc.height
+    ro = np.zeros((n, 3), np.float32)
+    rd = np.zeros((n, 3), np.float32)
+    load().orc_camera_rays(C.byref(desc), sample, _ptr(ro), _ptr(rd))
+    return ro, rd
+
+
+def trace_primary(desc, sample: int = 0, precision: int = 32, rect=None):
+    """Primary trace of a pixel rectangle (x0, y0, w, h); default the whole image."""
+    lib = load()
+    x0, y0, w, h = rect if rect is not None else (0, 0, desc.width, desc.height)
+    n = w * h
+    depth = np.zeros(n, np.float64)
+    depth64 = np.zeros(n, np.float64)
+    aux = np.zeros((n, 4), np.float32)
+    steps = np.zeros(n, np.uint32)
+    counts = np.zeros(5, np.uint64)
+    rc = lib.orc_trace_primary(C.byref(desc), sample, precision, x0, y0, w, h, _ptr(depth), _ptr(depth64), _ptr(aux),
+                               _ptr(steps), _ptr(counts))
+    assert rc == 0
+    return {"depth": depth.reshape(h, w), "depth64": depth64.reshape(h, w), "aux": aux.reshape(h, w, 4),
+            "steps": steps.reshape(h, w), "counts": dict(zip(
+                ["de_evals", "de_iterations", "steps", "de64_evals", "de64_iterations"], counts.tolist()))}
+
+
+def first_root(fractal, ro, rd, t_max: float, dt: float) -> float:
+    a = np.asarray(ro, np.float64)
+    b = np.asarray(rd, np.float64)
+    return load().orc_first_root_bruteforce(C.byref(fractal), _ptr(a), _ptr(b), t_max, dt)
+
+
+def planar_sample_f32(family, power, iterations, julia, bailout, trap_radius, stripes, x, y):
+    out = np.zeros(5, np.float32)
+    j = np.asarray(julia, np.float32)
+    it = load().orc_planar_sample_f32(family, power, iterations, _ptr(j), bailout, trap_radius, stripes, x, y,
+                                      _ptr(out))
+    return int(it), out
+
+
+def planar_offsets(desc):
+    fx = np.zeros(desc.width, np.float64)
+    fy = np.zeros(desc.height, np.float64)
+    load().orc_planar_offsets(C.byref(desc), _ptr(fx), _ptr(fy))
+    return fx, fy
+
+
+def escape_f64(desc, rect=None):
+    x0, y0, w, h = rect if rect is not None else (0, 0, desc.width, desc.height)
+    iters = np.zeros(w * h, np.uint32)
+    smooth = np.zeros(w * h, np.float32)
+    load().orc_escape_f64(C.byref(desc), x0, y0, w, h, _ptr(iters), _ptr(smooth))
+    return iters.reshape(h, w), smooth.reshape(h, w)
+
+
+def escape_f128(desc, px: np.ndarray, py: np.ndarray):
+    px = np.ascontiguousarray(px, np.uint32)
+    py = np.ascontiguousarray(py, np.uint32)
+    iters = np.zeros(len(px), np.uint32)
+    smooth = np.zeros(len(px), np.float32)
+    load().orc_escape_f128(C.byref(desc), _ptr(px), _ptr(py), len(px), _ptr(iters), _ptr(smooth))
+    return iters, smooth
+
+
+def render(desc, rect, sample_start: int, sample_count: int):
+    """Path-traced accumulation (float4 per pixel), first-hit depth and aux of a rectangle."""
+    lib = load()
+    x0, y0, w, h = rect
+    accum = np.zeros((h, w, 4), np.float32)
+    depth = np.zeros((h, w), np.float32)
+    aux = np.zeros((h, w, 4), np.float32)
+    rc = lib.orc_render(C.byref(desc), x0, y0, w, h, sample_start, sample_count, _ptr(accum), _ptr(depth), _ptr(aux))
+    assert rc == 0
+    return accum, depth, aux

@@ -1,0 +1,234 @@
+"""ctypes mirror of include/syfrack_hip.h and the loader of the HIP library.
+
+The library is built in-tree (``syfrack_amd/libsyfrack_hip.so``).  There is no fallback:
+when the library is missing, ``load()`` raises, so nothing on the product path can run on
+anything but the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsyfrack_hip.so")
+
+SYF_OK = 0
+SYF_E_INVALID = -1
+SYF_E_DEVICE = -2
+SYF_E_QUEUE = -3
+SYF_E_TIMEOUT = -4
+SYF_E_ADMISSION = -5
+SYF_E_NOT_FOUND = -6
+SYF_E_NO_DEVICE = -7
+
+FAMILY_MANDELBULB = 0
+FAMILY_MANDELBOX = 1
+FAMILY_MENGER = 2
+FAMILY_SPHERE = 3
+
+CAMERA_FREE = 0
+CAMERA_ORBIT = 1
+
+LIGHT_AREA = 1
+LIGHT_ENV = 2
+MAX_LIGHTS = 4
+
+PLANE_ACCUM = 0
+PLANE_DEPTH = 1
+PLANE_AUX = 2
+PLANE_DEPTH64 = 3
+PLANE_STEPS = 4
+
+PLANAR_JULIA = 1
+PLANAR_MANDELBROT = 2
+PLANAR_BURNING_SHIP = 3
+PLANAR_FP32 = 0
+PLANAR_FP64 = 1
+PLANAR_PERTURB = 2
+PLANAR_SMOOTH = 0
+PLANAR_ITER = 1
+
+F3 = C.c_float * 3
+F2 = C.c_float * 2
+
+
+class SyfFractal(C.Structure):
+    _fields_ = [
+        ("family", C.c_uint32),
+        ("iterations", C.c_uint32),
+        ("bailout", C.c_float),
+        ("power", C.c_float),
+        ("scale", C.c_float),
+        ("min_radius", C.c_float),
+        ("fixed_radius", C.c_float),
+        ("fold_limit", C.c_float),
+        ("offset", F3),
+        ("half", F3),
+        ("radius", C.c_float),
+        ("plane_normal", F3),
+        ("plane_offset", C.c_float),
+        ("trap_scale", C.c_float),
+        ("safety", C.c_float),
+        ("bounding_radius_hint", C.c_float),
+    ]
+
+
+class SyfCamera(C.Structure):
+    _fields_ = [
+        ("mode", C.c_uint32),
+        ("position", F3),
+        ("target", F3),
+        ("roll", C.c_float),
+        ("point", F3),
+        ("distance", C.c_float),
+        ("azimuth", C.c_float),
+        ("elevation", C.c_float),
+        ("focal_mm", C.c_float),
+        ("f_number", C.c_float),
+        ("focus_distance", C.c_float),
+    ]
+
+
+class SyfLight(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("position", F3),
+        ("normal", F3),
+        ("size", F2),
+        ("color", F3),
+        ("intensity", C.c_float),
+        ("zenith", F3),
+        ("horizon", F3),
+        ("ground", F3),
+    ]
+
+
+class SyfMaterial(C.Structure):
+    _fields_ = [
+        ("albedo", F3),
+        ("roughness", C.c_float),
+        ("metallic", C.c_float),
+        ("specular", C.c_float),
+        ("surface_enabled", C.c_uint32),
+        ("surface_slot", C.c_uint32),
+        ("surface_roughness", F2),
+        ("surface_metallic", F2),
+    ]
+
+
+class SyfJobDesc(C.Structure):
+    _fields_ = [
+        ("fractal", SyfFractal),
+        ("camera", SyfCamera),
+        ("material", SyfMaterial),
+        ("lights", SyfLight * MAX_LIGHTS),
+        ("light_count", C.c_uint32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("max_bounces", C.c_uint32),
+        ("max_steps", C.c_uint32),
+        ("pixel_tolerance", C.c_float),
+        ("cavity_darkness", C.c_float),
+        ("reserved", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class SyfStats(C.Structure):
+    _fields_ = [
+        ("rays", C.c_uint64),
+        ("primary_rays", C.c_uint64),
+        ("hits", C.c_uint64),
+        ("de_evals", C.c_uint64),
+        ("de_iterations", C.c_uint64),
+        ("de64_evals", C.c_uint64),
+        ("de64_iterations", C.c_uint64),
+        ("steps", C.c_uint64),
+        ("kernel_ms", C.c_double),
+        ("march_ms", C.c_double),
+    ]
+
+
+class SyfPlanarDesc(C.Structure):
+    _fields_ = [
+        ("family", C.c_uint32),
+        ("mode", C.c_uint32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("iterations", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("center", C.c_double * 2),
+        ("span", C.c_double),
+        ("julia", C.c_double * 2),
+        ("bailout", C.c_float),
+        ("rotation", C.c_float),
+    ]
+
+
+# Every symbol the header declares, with its ctypes signature.
+_P = C.c_void_p
+SIGNATURES = {
+    "syf_last_error": (C.c_char_p, []),
+    "syf_abi_version": (C.c_int, []),
+    "syf_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "syf_ctx_destroy": (C.c_int, [_P]),
+    "syf_job_create": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.POINTER(_P)]),
+    "syf_job_destroy": (C.c_int, [_P]),
+    "syf_job_clear": (C.c_int, [_P]),
+    "syf_job_sync": (C.c_int, [_P]),
+    "syf_job_stats": (C.c_int, [_P, C.POINTER(SyfStats)]),
+    "syf_de_probe": (C.c_int, [_P, _P, C.c_size_t, _P, _P]),
+    "syf_trace_primary": (C.c_int, [_P, C.c_uint32]),
+    "syf_render_tile": (C.c_int, [_P] + [C.c_uint32] * 6),
+    "syf_render_wavefront": (C.c_int, [_P] + [C.c_uint32] * 6),
+    "syf_readback": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    "syf_planar_create": (C.c_int, [_P, C.POINTER(SyfPlanarDesc), C.POINTER(_P)]),
+    "syf_planar_destroy": (C.c_int, [_P]),
+    "syf_planar_run": (C.c_int, [_P]),
+    "syf_planar_sync": (C.c_int, [_P]),
+    "syf_planar_readback": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    "syf_planar_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+}
+
+
+class SyfError(RuntimeError):
+    """A negative status from the library, with its syf_last_error() text."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"syfrack_hip error {code}: {message}")
+        self.code = code
+        self.message = message
+
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library (once).  Raises when it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the HIP path has no fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != SYF_OK:
+        msg = load().syf_last_error().decode("utf-8", "replace")
+        raise SyfError(code, msg)
+
+
+def header_symbols(header_path: str) -> list:
+    """Names of the functions declared in a C header (``int syf_x(`` style)."""
+    import re
+    text = open(header_path, encoding="utf-8").read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(syf_\w+)\s*\(", text, re.M)))

@@ -1,0 +1,43 @@
+// Internal definitions of the C ABI's opaque handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "params.hpp"
+
+struct syf_ctx {
+    int device = 0;
+    int cu_count = 0;
+    char arch[64] = {0};
+};
+
+namespace syf {
+struct WavefrontArena;
+}
+
+struct syf_job {
+    syf_ctx* ctx = nullptr;
+    syf_job_desc desc{};
+    syf::JobP P{};
+    size_t pixels = 0;
+    hipStream_t stream = nullptr;
+    void* plane[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    unsigned long long* ctr = nullptr;
+    float* t_prev = nullptr;  // per pixel: march position before the hit (FP64 refine bracket)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // start, after dominant kernel, end
+    bool timed = false;
+    double extra_ms = 0.0;
+    syf::WavefrontArena* wf = nullptr;
+};
+
+namespace syf {
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+void clear_error();
+int lower_fractal(const syf_fractal& f, FractalP& F);
+int lower_job(const syf_job_desc& d, JobP& P);
+size_t plane_bytes(uint32_t plane, size_t pixels);
+void release_wavefront(syf_job* j);
+}  // namespace syf

@@ -1,0 +1,427 @@
+// Device-side building blocks of the gfx950 distance-estimator path: vector algebra, the
+// classic distance fields, the PCG stream, camera rays, the sphere-tracing march, the
+// tetrahedral normal and the FP64 final-hit refine.
+//
+// Everything is written for the wave64 FP32 VALU (no MFMA: per-ray branchy arithmetic).
+// Functions are templated on the arithmetic type: float for the march, double for the
+// final-hit refine.  The library is compiled with -ffp-contract=off, so each expression
+// rounds as written, in the order of the reference formulas cited below (paths relative to
+// the SyFrack source tree).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "params.hpp"
+
+namespace syf {
+
+template <class T> struct V3 {
+    T x, y, z;
+};
+template <class T> __device__ __forceinline__ V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> __device__ __forceinline__ V3<T> add(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> __device__ __forceinline__ V3<T> sub(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> __device__ __forceinline__ V3<T> scl(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> __device__ __forceinline__ T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> __device__ __forceinline__ T len3(V3<T> a) { return sqrt(dot3(a, a)); }
+template <class T> __device__ __forceinline__ V3<T> nrm(V3<T> a) { return scl(a, T(1) / len3(a)); }
+template <class T> __device__ __forceinline__ T clampT(T v, T lo, T hi) { return fmin(fmax(v, lo), hi); }
+template <class T> __device__ __forceinline__ V3<T> cvt(V3<float> a) { return {T(a.x), T(a.y), T(a.z)}; }
+template <class T> __device__ __forceinline__ V3<T> cross3(V3<T> a, V3<T> b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ V3<float> ld3(const float* p) { return {p[0], p[1], p[2]}; }
+
+// Result of one field evaluation: distance, normalised trap slot 0 (the classics write
+// only slot 0; slots 1-3 are 1), iteration count clamped to the iteration product, and the
+// number of Iterate-body executions (work accounting).
+template <class T> struct De {
+    T d;
+    T trap0;
+    uint32_t iter;
+    uint32_t runs;
+};
+
+// integral_spherical_power (eval.rs:129-178).
+template <class T>
+__device__ __forceinline__ void spherical_power(V3<T>& p, T r, uint32_t n, T& deriv) {
+    if (n == 1u) {
+        deriv = T(1);
+        return;
+    }
+    T rho = sqrt(p.x * p.x + p.y * p.y);
+    T tx = clampT(p.z / r, T(-1), T(1)), ty = rho / r;
+    T fx = T(1), fy = T(0);
+    if (rho > T(0)) {
+        fx = p.x / rho;
+        fy = p.y / rho;
+    }
+    if (n == 8u || n == 6u) {
+        T t2x = tx * tx - ty * ty, t2y = T(2) * tx * ty;
+        T f2x = fx * fx - fy * fy, f2y = T(2) * fx * fy;
+        T t4x = t2x * t2x - t2y * t2y, t4y = T(2) * t2x * t2y;
+        T f4x = f2x * f2x - f2y * f2y, f4y = T(2) * f2x * f2y;
+        T r2 = r * r;
+        T r4 = r2 * r2;
+        T ax, ay, bx, by, prev;
+        if (n == 8u) {
+            ax = t4x * t4x - t4y * t4y;
+            ay = T(2) * t4x * t4y;
+            bx = f4x * f4x - f4y * f4y;
+            by = T(2) * f4x * f4y;
+            prev = r4 * r2 * r;
+        } else {
+            ax = t2x * t4x - t2y * t4y;
+            ay = t2x * t4y + t2y * t4x;
+            bx = f2x * f4x - f2y * f4y;
+            by = f2x * f4y + f2y * f4x;
+            prev = r4 * r;
+        }
+        T s = prev * r;
+        p = mk((ay * bx) * s, (ay * by) * s, ax * s);
+        deriv = T(n) * prev;
+        return;
+    }
+    T ax = T(1), ay = T(0), bx = T(1), by = T(0);
+    uint32_t e = n;
+    while (e != 0u) {
+        if (e & 1u) {
+            T nx = ax * tx - ay * ty, ny = ax * ty + ay * tx;
+            ax = nx;
+            ay = ny;
+            nx = bx * fx - by * fy;
+            ny = bx * fy + by * fx;
+            bx = nx;
+            by = ny;
+        }
+        e >>= 1;
+        if (e != 0u) {
+            T nx = tx * tx - ty * ty, ny = tx * ty + ty * tx;
+            tx = nx;
+            ty = ny;
+            nx = fx * fx - fy * fy;
+            ny = fx * fy + fy * fx;
+            fx = nx;
+            fy = ny;
+        }
+    }
+    T prev = T(1);
+    for (uint32_t i = 1; i < n; ++i) prev = prev * r;
+    T s = prev * r;
+    p = mk((ay * bx) * s, (ay * by) * s, ax * s);
+    deriv = T(n) * prev;
+}
+
+// Trigonometric Power (eval.rs:462-475) for non-integer powers or tiny radii.
+template <class T> __device__ __forceinline__ void trig_power(V3<T>& p, T radius, T power, T& dr) {
+    T r = fmax(radius, T(1e-12));
+    T theta = radius >= T(1e-12) ? atan2(sqrt(p.x * p.x + p.y * p.y), p.z) : acos(clampT(p.z / r, T(-1), T(1)));
+    T phi = atan2(p.y, p.x);
+    dr = dr * (pow(r, power - T(1)) * power);
+    T zr = pow(r, power);
+    T st = sin(theta * power), ct = cos(theta * power);
+    T sp = sin(phi * power), cp = cos(phi * power);
+    p = mk((st * cp) * zr, (st * sp) * zr, ct * zr);
+}
+
+// Graph::evaluate for one classic family (eval.rs:98-114, :346-556, :558-599, :601-621).
+template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval(const FractalP& F, V3<T> c) {
+    V3<T> z = c;
+    T dr = T(1);
+    T trap = T(INFINITY);
+    uint32_t iter = 0;
+    T d;
+    const T bail = T(F.bailout);
+    if constexpr (FAM == SYF_FAMILY_MANDELBULB || FAM == FAM_BULB_TRIG) {
+        for (uint32_t k = 0; k < F.iterations; ++k) {
+            T r = len3(z);
+            if (r > bail) break;
+            trap = fmin(trap, r);
+            if (FAM == SYF_FAMILY_MANDELBULB && r >= T(1e-12)) {
+                T deriv;
+                spherical_power(z, r, F.ipower, deriv);
+                dr = dr * deriv;
+            } else {
+                trig_power(z, r, T(F.power), dr);
+            }
+            z = add(z, c);
+            dr = dr + T(1);
+            ++iter;
+        }
+        T r = fmax(len3(z), T(1e-12));
+        d = T(0.5) * log(r) * r / dr;
+    } else if constexpr (FAM == SYF_FAMILY_MANDELBOX) {
+        const T L = T(F.fold_limit), mr2 = T(F.mr2), fr2 = T(F.fr2), k_in = T(F.fr2_over_mr2);
+        const T s = T(F.scale), as = T(F.abs_scale);
+        for (uint32_t k = 0; k < F.iterations; ++k) {
+            if (len3(z) > bail) break;
+            z = mk(clampT(z.x, -L, L) * T(2) - z.x, clampT(z.y, -L, L) * T(2) - z.y, clampT(z.z, -L, L) * T(2) - z.z);
+            T r2 = dot3(z, z);
+            trap = fmin(trap, sqrt(r2));
+            T kf = r2 < mr2 ? k_in : (r2 < fr2 ? fr2 / r2 : T(1));
+            z = scl(z, kf);
+            dr = dr * kf;
+            z = scl(z, s);
+            dr = dr * as;
+            z = add(z, c);
+            dr = dr + T(1);
+            ++iter;
+        }
+        d = len3(z) / fabs(dr);
+    } else if constexpr (FAM == SYF_FAMILY_MENGER) {
+        const T s = T(F.scale), ox = T(F.off_s1[0]), oy = T(F.off_s1[1]), oz = T(F.off_s1[2]);
+        const T thr = T(F.menger_shift_threshold);
+        for (uint32_t k = 0; k < F.iterations; ++k) {
+            if (len3(z) > bail) break;
+            T wx = fabs(z.x), wy = fabs(z.y), wz = fabs(z.z), tmp;
+            if (wx < wy) { tmp = wx; wx = wy; wy = tmp; }
+            if (wx < wz) { tmp = wx; wx = wz; wz = tmp; }
+            if (wy < wz) { tmp = wy; wy = wz; wz = tmp; }
+            wx = wx * s - ox;
+            wy = wy * s - oy;
+            wz = wz * s - oz;
+            if (wz < thr) wz = wz + oz;
+            z = mk(wx, wy, wz);
+            dr = dr * s;
+            trap = fmin(trap, len3(z));
+            ++iter;
+        }
+        T qx = fabs(z.x) - T(F.half[0]), qy = fabs(z.y) - T(F.half[1]), qz = fabs(z.z) - T(F.half[2]);
+        V3<T> qm = mk(fmax(qx, T(0)), fmax(qy, T(0)), fmax(qz, T(0)));
+        d = (len3(qm) + fmin(fmax(qx, fmax(qy, qz)), T(0))) / dr;
+    } else {  // sphere graph: Trap -> Sphere U Plane
+        T r = len3(z);
+        trap = r;
+        T a = (r - T(F.radius)) / dr;
+        T b = (dot3(z, mk(T(F.plane_n[0]), T(F.plane_n[1]), T(F.plane_n[2]))) - T(F.plane_offset)) / dr;
+        d = fmin(a, b);
+    }
+    De<T> o;
+    o.d = d;
+    o.trap0 = isfinite(trap) ? fmin(trap / T(F.trap_scale), T(1)) : T(1);
+    o.iter = iter < F.iteration_product ? iter : F.iteration_product;
+    o.runs = iter;
+    return o;
+}
+
+// Family chosen at run time (uniform per job or per tile): used where several families
+// share one launch.
+template <class T> __device__ __forceinline__ De<T> de_eval_any(const FractalP& F, V3<T> p) {
+    switch (F.family) {
+    case SYF_FAMILY_MANDELBULB:
+        return F.ipower != 0u ? de_eval<T, SYF_FAMILY_MANDELBULB>(F, p) : de_eval<T, FAM_BULB_TRIG>(F, p);
+    case SYF_FAMILY_MANDELBOX: return de_eval<T, SYF_FAMILY_MANDELBOX>(F, p);
+    case SYF_FAMILY_MENGER: return de_eval<T, SYF_FAMILY_MENGER>(F, p);
+    default: return de_eval<T, SYF_FAMILY_SPHERE>(F, p);
+    }
+}
+
+// ---------------------------------------------------------------- RNG (pathtrace.wgsl:20-37)
+
+__device__ __host__ __forceinline__ uint32_t pcg(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+__device__ __host__ __forceinline__ uint32_t rng_init(uint32_t px, uint32_t py, uint32_t sample, uint32_t seed) {
+    uint32_t h = pcg(seed);
+    h = pcg(h ^ px);
+    h = pcg(h ^ py);
+    return pcg(h ^ sample);
+}
+__device__ __forceinline__ float rand01(uint32_t& s) {
+    s = pcg(s);
+    return float(s >> 8u) * (1.0f / 16777216.0f);
+}
+
+// camera_ray (pathtrace.wgsl:781-800).
+__device__ __forceinline__ void camera_ray(const CameraP& C, uint32_t w, uint32_t h, uint32_t px, uint32_t py,
+                                           uint32_t& rng, V3<float>& ro, V3<float>& rd) {
+    float jx = rand01(rng);
+    float jy = rand01(rng);
+    float ndcx = ((float(px) + jx) / float(w)) * 2.0f - 1.0f;
+    float ndcy = ((float(py) + jy) / float(h)) * 2.0f - 1.0f;
+    V3<float> f = ld3(C.forward), r = ld3(C.right), u = ld3(C.up);
+    V3<float> dir = nrm(add(add(f, scl(r, ndcx * C.tan_half)), scl(u, -ndcy * C.tan_half / C.aspect)));
+    V3<float> origin = ld3(C.origin);
+    if (C.lens_radius > 0.0f) {
+        V3<float> focus = add(origin, scl(dir, C.focus / fmaxf(dot3(dir, f), 1e-4f)));
+        float rr = C.lens_radius * sqrtf(rand01(rng));
+        float phi = 2.0f * 3.14159265358979f * rand01(rng);
+        origin = add(add(origin, scl(r, rr * cosf(phi))), scl(u, rr * sinf(phi)));
+        dir = nrm(sub(focus, origin));
+    }
+    ro = origin;
+    rd = dir;
+}
+
+// ---------------------------------------------------------------- march (pathtrace.wgsl:78-145)
+
+struct Work {
+    uint32_t de_evals = 0, de_iters = 0, steps = 0, de64_evals = 0, de64_iters = 0;
+};
+
+template <class T> struct Hit {
+    bool hit, exhausted;
+    T t;
+    T t_prev;  // march position evaluated before the hit: lower end of the refine bracket
+    uint32_t steps;
+    De<T> s;
+};
+
+// Sphere tracing with over-relaxation and fallback (Keinert et al. 2014); hit when the
+// distance, relative to t, falls below the pixel cone times pixel_tolerance.
+template <class T, uint32_t FAM>
+__device__ __forceinline__ Hit<T> march(const FractalP& F, V3<T> ro, V3<T> rd, T pixel_cone, T pixel_tol,
+                                        uint32_t max_steps, T t_end, Work& w) {
+    const T safety = T(F.safety);
+    const T hit_cone = pixel_cone * pixel_tol;
+    T t = T(0), omega = T(1.2), prev_radius = T(0), step_len = T(0), cand_t = T(0), cand_err = T(1e30);
+    T t_last = T(0);
+    De<T> cand{};
+    Hit<T> out;
+    out.hit = false;
+    out.exhausted = false;
+    out.t = t_end;
+    out.t_prev = T(0);
+    out.steps = 0;
+    out.s = De<T>{T(200), T(1), 0u, 0u};
+    for (uint32_t i = 0; i < max_steps; ++i) {
+        if (t >= t_end) return out;
+        out.steps = i + 1u;
+        De<T> s;
+        if constexpr (FAM == 0xffffffffu) s = de_eval_any<T>(F, add(ro, scl(rd, t)));
+        else s = de_eval<T, FAM>(F, add(ro, scl(rd, t)));
+        w.de_evals++;
+        w.de_iters += s.runs;
+        w.steps++;
+        T radius = fabs(s.d) * safety;
+        bool sor_fail = omega > T(1) && (radius + prev_radius) < step_len * T(1.0001);
+        if (sor_fail) {
+            step_len = step_len - omega * step_len;
+            omega = T(1);
+        } else {
+            step_len = radius * omega;
+        }
+        prev_radius = radius;
+        T err = radius / fmax(t, T(1e-6));
+        if (!sor_fail && err < cand_err) {
+            cand_t = t;
+            cand = s;
+            cand_err = err;
+        }
+        if (!sor_fail && err < hit_cone) {
+            out.hit = true;
+            out.t = t;
+            out.t_prev = t_last;
+            out.s = s;
+            return out;
+        }
+        if (!sor_fail && t + step_len >= t_end) {
+            if (radius >= t_end - t) return out;
+            omega = T(1);
+            step_len = radius;
+        }
+        t_last = t;
+        t = t + step_len;
+    }
+    if (cand_err < hit_cone * T(4)) {
+        out.hit = true;
+        out.t = cand_t;
+        out.t_prev = cand_t;  // best-candidate hit: no bracket, the refine keeps t
+        out.s = cand;
+    } else {
+        out.exhausted = true;
+    }
+    return out;
+}
+
+// normal_at (common.wgsl:376-387).
+template <class T, uint32_t FAM>
+__device__ __forceinline__ V3<T> normal_at(const FractalP& F, V3<T> p, T h, Work& w) {
+    V3<T> g = mk(T(0), T(0), T(0));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const T kx = (i == 0 || i == 3) ? T(1) : T(-1);
+        const T ky = (i >= 2) ? T(1) : T(-1);
+        const T kz = (i == 1 || i == 3) ? T(1) : T(-1);
+        V3<T> k = mk(kx, ky, kz);
+        De<T> s;
+        if constexpr (FAM == 0xffffffffu) s = de_eval_any<T>(F, add(p, scl(k, h)));
+        else s = de_eval<T, FAM>(F, add(p, scl(k, h)));
+        w.de_evals++;
+        w.de_iters += s.runs;
+        g = add(g, scl(k, s.d));
+    }
+    return nrm(g);
+}
+
+// Final-hit refine in FP64: the crossing of g(t) = |d(t)| * safety - hit_cone * t (the
+// march's own pixel-footprint hit rule) between the last position before the hit (g > 0)
+// and the hit (g <= 0), by Illinois regula falsi to a bracket of 1e-12 * t (at most 40
+// evaluations after the two end points); the bracket midpoint is returned.  Without an
+// FP64 sign change between the end points the march t is kept.
+constexpr int REFINE_ITERS = 40;
+template <uint32_t FAM>
+__device__ __forceinline__ double refine64(const FractalP& F, V3<double> ro, V3<double> rd, double t_prev,
+                                           double t_hit, double hit_cone, Work& w) {
+    const double safety = double(F.safety);
+    auto g = [&](double t) {
+        De<double> s;
+        if constexpr (FAM == 0xffffffffu) s = de_eval_any<double>(F, add(ro, scl(rd, t)));
+        else s = de_eval<double, FAM>(F, add(ro, scl(rd, t)));
+        w.de64_evals++;
+        w.de64_iters += s.runs;
+        return fabs(s.d) * safety - hit_cone * t;
+    };
+    if (!(t_prev < t_hit)) return t_hit;
+    double a = t_prev, b = t_hit;
+    double ga = g(a), gb = g(b);
+    if (!(ga > 0.0 && gb <= 0.0)) return t_hit;
+    int side = 0;
+    for (int i = 0; i < REFINE_ITERS; ++i) {
+        if (b - a <= 1e-12 * b) break;
+        double c = b - gb * (b - a) / (gb - ga);
+        if (!(c > a && c < b)) c = 0.5 * (a + b);
+        double gc = g(c);
+        if (gc > 0.0) {
+            a = c;
+            ga = gc;
+            if (side == 1) gb *= 0.5;
+            side = 1;
+        } else {
+            b = c;
+            gb = gc;
+            if (side == -1) ga *= 0.5;
+            side = -1;
+        }
+    }
+    return 0.5 * (a + b);
+}
+
+// ---------------------------------------------------------------- wave-level helpers
+
+// Sum over the 64 lanes of a wave (butterfly on the cross-lane permute network).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Adds a wave's work counters to the launch totals: one atomic per counter per wave.
+__device__ __forceinline__ void flush_work(const Work& w, uint32_t hits, uint32_t rays, unsigned long long* ctr) {
+    uint32_t a = wave_sum(w.de_evals), b = wave_sum(w.de_iters), c = wave_sum(w.steps);
+    uint32_t d = wave_sum(w.de64_evals), e = wave_sum(w.de64_iters), f = wave_sum(hits), g = wave_sum(rays);
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(&ctr[CTR_DE_EVALS], (unsigned long long)a);
+        atomicAdd(&ctr[CTR_DE_ITERS], (unsigned long long)b);
+        atomicAdd(&ctr[CTR_STEPS], (unsigned long long)c);
+        atomicAdd(&ctr[CTR_DE64_EVALS], (unsigned long long)d);
+        atomicAdd(&ctr[CTR_DE64_ITERS], (unsigned long long)e);
+        atomicAdd(&ctr[CTR_HITS], (unsigned long long)f);
+        atomicAdd(&ctr[CTR_RAYS], (unsigned long long)g);
+    }
+}
+
+}  // namespace syf

@@ -1,0 +1,109 @@
+// Per-job constants shared by the host side of the library and its kernels.  A job's
+// parameters travel as one kernel argument (scalar constant memory on gfx950), lowered on
+// the host from syf_job_desc the way Uniforms::from_scene (uniforms.rs:748-853) lowers a
+// scene.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/syfrack_hip.h"
+
+namespace syf {
+
+// Classic-graph parameters.  Products the reference recomputes per iteration from node
+// parameters alone (fr2/mr2, offset*(scale-1), -0.5*offset.z*(scale-1)) are folded once
+// on the host with the same f32 operations, so the values are identical.
+struct FractalP {
+    uint32_t family;
+    uint32_t iterations;
+    uint32_t iteration_product;
+    uint32_t ipower;          // integer power 1..16, or 0 for the trigonometric form
+    float bailout;
+    float power;
+    float scale;
+    float abs_scale;
+    float mr2;                // min_radius^2
+    float fr2;                // fixed_radius^2
+    float fr2_over_mr2;
+    float fold_limit;
+    float off_s1[3];          // MengerFold offset * (scale - 1)
+    float menger_shift_threshold;  // -0.5 * offset.z * (scale - 1)
+    float half[3];
+    float radius;
+    float plane_n[3];
+    float plane_offset;
+    float trap_scale;
+    float safety;             // step scale
+};
+
+struct CameraP {
+    float origin[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float tan_half;
+    float aspect;
+    float lens_radius;
+    float focus;
+};
+
+// Area light as GpuLight (uniforms.rs:676-716): a = centre, b = u * width, c = v * height.
+struct LightP {
+    uint32_t kind;
+    float a[3];
+    float b[3];
+    float c[3];
+    float n[3];               // area_normal = safe_normalize(cross(b, c))
+    float color[3];           // colour * intensity (env: zenith)
+    float horizon[3];
+    float ground[3];
+    float area;               // |cross(b, c)|
+};
+
+struct MaterialP {
+    float albedo[3];
+    float roughness;
+    float metallic;
+    float specular;
+    uint32_t surface_enabled;
+    uint32_t surface_slot;
+    float surface[4];         // roughness t=0, t=1, metallic t=0, t=1
+};
+
+struct JobP {
+    FractalP f;
+    CameraP cam;
+    MaterialP mat;
+    LightP lights[SYF_MAX_LIGHTS];
+    uint32_t light_count;
+    uint32_t width;
+    uint32_t height;
+    uint32_t seed;
+    uint32_t max_bounces;
+    uint32_t max_steps;
+    float pixel_tolerance;
+    float pixel_cone;         // tan_half * 2 / width (pathtrace.wgsl:809)
+    float t_end;              // ray_extent (common.wgsl:206-208)
+    float cavity_darkness;
+};
+
+// Kernel specialisation of a family: the Mandelbulb with a non-integer power takes the
+// trigonometric form, instantiated separately so the integer path stays lean.
+constexpr uint32_t FAM_BULB_TRIG = 4u;
+inline uint32_t kernel_family(const FractalP& F) {
+    return (F.family == SYF_FAMILY_MANDELBULB && F.ipower == 0u) ? FAM_BULB_TRIG : F.family;
+}
+
+enum : uint32_t {
+    CTR_DE_EVALS = 0,
+    CTR_DE_ITERS = 1,
+    CTR_STEPS = 2,
+    CTR_DE64_EVALS = 3,
+    CTR_DE64_ITERS = 4,
+    CTR_HITS = 5,
+    CTR_RAYS = 6,
+    CTR_PRIMARY = 7,
+    CTR_COUNT = 8,
+};
+
+}  // namespace syf

@@ -1,0 +1,178 @@
+"""Host-side mirror of the reference's Renderer API for this path, over the C ABI.
+
+``Renderer.probe_distances`` mirrors Renderer::probe_distances (renderer.rs:645-648),
+``Renderer.render`` mirrors Renderer::render (renderer.rs:668-693) returning a
+``RenderOutput`` with accumulation, depth and normals planes (renderer.rs:151-165), and
+``Renderer.trace_primary`` is its first-hit half (max_bounces = 0) with the FP64-refined hit
+distance.  Every call runs the HIP kernels of libsyfrack_hip.so; there is no other path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .scene import RenderConfig, Scene
+
+
+@dataclass
+class DeSample:
+    distance: np.ndarray    # (n,) f32
+    traps: np.ndarray       # (n, 4) f32, normalised to [0, 1]
+    iterations: np.ndarray  # (n,) u32
+    material: np.ndarray    # (n,) u32
+
+
+@dataclass
+class RenderOutput:
+    accum: Optional[np.ndarray]   # (h, w, 4): radiance sums and sample counts
+    depth: np.ndarray             # (h, w): first-hit distance, -1 on a miss
+    normals: np.ndarray           # (h, w, 4): normal xyz and steps; (0, 0, 0, -1) on a miss
+    depth64: Optional[np.ndarray] = None  # (h, w): FP64-refined hit distance
+    steps: Optional[np.ndarray] = None    # (h, w): primary march steps
+    stats: Optional[dict] = None
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One HIP device (syf_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = abi.load()
+        h = C.c_void_p()
+        abi.check(self.lib.syf_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.syf_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Job:
+    """A render job (syf_job): the lowered scene plus its device planes."""
+
+    def __init__(self, ctx: Context, desc: abi.SyfJobDesc):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.desc = desc
+        h = C.c_void_p()
+        abi.check(self.lib.syf_job_create(ctx.handle, C.byref(desc), C.byref(h)))
+        self.handle = h
+        self.width = desc.width
+        self.height = desc.height
+
+    def close(self):
+        if self.handle:
+            self.lib.syf_job_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self):
+        abi.check(self.lib.syf_job_clear(self.handle))
+
+    def sync(self):
+        abi.check(self.lib.syf_job_sync(self.handle))
+
+    def stats(self) -> dict:
+        s = abi.SyfStats()
+        abi.check(self.lib.syf_job_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in abi.SyfStats._fields_}
+
+    def trace_primary(self, sample: int = 0):
+        abi.check(self.lib.syf_trace_primary(self.handle, sample))
+
+    def render_tile(self, x, y, w, h, sample_start, sample_count):
+        abi.check(self.lib.syf_render_tile(self.handle, x, y, w, h, sample_start, sample_count))
+
+    def render_wavefront(self, x, y, w, h, sample_start, sample_count):
+        abi.check(self.lib.syf_render_wavefront(self.handle, x, y, w, h, sample_start, sample_count))
+
+    def readback(self, plane: int) -> np.ndarray:
+        h, w = self.height, self.width
+        shapes = {
+            abi.PLANE_ACCUM: ((h, w, 4), np.float32),
+            abi.PLANE_DEPTH: ((h, w), np.float32),
+            abi.PLANE_AUX: ((h, w, 4), np.float32),
+            abi.PLANE_DEPTH64: ((h, w), np.float64),
+            abi.PLANE_STEPS: ((h, w), np.uint32),
+        }
+        shape, dt = shapes[plane]
+        out = np.empty(shape, dt)
+        abi.check(self.lib.syf_readback(self.handle, plane, _p(out), out.nbytes))
+        return out
+
+    def probe(self, points: np.ndarray) -> DeSample:
+        pts = np.zeros((len(points), 4), np.float32)
+        pts[:, :3] = np.asarray(points, np.float32)[:, :3]
+        out = np.zeros((len(pts), 4), np.float32)
+        traps = np.zeros((len(pts), 4), np.float32)
+        abi.check(self.lib.syf_de_probe(self.handle, _p(pts), len(pts), _p(out), _p(traps)))
+        return DeSample(out[:, 0].copy(), traps, out[:, 2].astype(np.uint32), out[:, 3].astype(np.uint32))
+
+
+class Renderer:
+    """Renderer (renderer.rs:254-266) for the distance-estimator path on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self.ctx = Context(device)
+
+    def job(self, scene: Scene, cfg: RenderConfig) -> Job:
+        return Job(self.ctx, scene.desc(cfg))
+
+    def probe_distances(self, scene: Scene, points: np.ndarray) -> DeSample:
+        cfg = RenderConfig(1, 1, 1, 0, 1)
+        j = self.job(scene, cfg)
+        try:
+            return j.probe(points)
+        finally:
+            j.close()
+
+    def trace_primary(self, scene: Scene, cfg: RenderConfig, sample: int = 0) -> RenderOutput:
+        j = self.job(scene, cfg)
+        try:
+            j.trace_primary(sample)
+            j.sync()
+            return RenderOutput(None, j.readback(abi.PLANE_DEPTH), j.readback(abi.PLANE_AUX),
+                                j.readback(abi.PLANE_DEPTH64), j.readback(abi.PLANE_STEPS), j.stats())
+        finally:
+            j.close()
+
+    def render(self, scene: Scene, cfg: RenderConfig, wavefront: bool = False) -> RenderOutput:
+        """Progressive tile loop like RenderSession::step (session.rs:694-877): every tile of
+        ``cfg.tile_size`` gets all samples, dispatched in chunks of 8 samples."""
+        j = self.job(scene, cfg)
+        try:
+            ts = max(1, cfg.tile_size)
+            spd = 8
+            for s0 in range(0, cfg.samples_per_pixel, spd):
+                n = min(spd, cfg.samples_per_pixel - s0)
+                for y in range(0, cfg.height, ts):
+                    for x in range(0, cfg.width, ts):
+                        w, h = min(ts, cfg.width - x), min(ts, cfg.height - y)
+                        if wavefront:
+                            j.render_wavefront(x, y, w, h, s0, n)
+                        else:
+                            j.render_tile(x, y, w, h, s0, n)
+            j.sync()
+            return RenderOutput(j.readback(abi.PLANE_ACCUM), j.readback(abi.PLANE_DEPTH), j.readback(abi.PLANE_AUX),
+                                stats=j.stats())
+        finally:
+            j.close()

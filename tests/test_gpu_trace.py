@@ -1,0 +1,190 @@
+"""GPU parity of the distance probe and the primary-ray trace against the oracle.
+
+Runs on an MI355X through the C ABI (libsyfrack_hip.so).  Tolerances:
+  * DE probe vs the reference's 64-point fixtures: distance 1e-5 relative (absolute below
+    1), traps 1e-5, iterations exact (graph_parity.rs:34-65);
+  * DE probe vs the FP32 oracle on 4096 LCG points (gpu_de.rs:40-91 corpus, seed 11):
+    iterations exact, distance within 1e-5 relative;
+  * primary trace vs the FP64 oracle: >= 99.9 % hit/miss agreement and refined hit
+    distance within 1e-4 relative (per config, the observed error is printed).
+"""
+import numpy as np
+import pytest
+
+from helpers import lcg_points, load_de_fixture
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["sphere", "mandelbulb", "mandelbox", "menger"]
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    from syfrack_amd.render import Renderer
+    return Renderer(0)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_probe_matches_reference_fixtures(renderer, name):
+    from syfrack_amd.scene import Camera, Scene, Fractal
+    doc, p, d, traps, iters = load_de_fixture(name)
+    s = renderer.probe_distances(Scene(Fractal.bundled(name), Camera()), p)
+    err = np.abs(s.distance - d) / np.maximum(np.abs(d), 1.0)
+    print(f"{name}: worst relative distance error {err.max():.3e}")
+    assert err.max() <= 1e-5
+    assert np.abs(s.traps - traps).max() <= 1e-5
+    assert np.array_equal(s.iterations, iters)
+
+
+@pytest.mark.parametrize("name,extent", [("mandelbulb", 1.6), ("mandelbox", 8.0), ("menger", 1.6), ("sphere", 3.0)])
+def test_probe_matches_oracle_on_corpus(renderer, name, extent):
+    from oracle import oracle
+    from syfrack_amd.scene import Camera, Scene, Fractal
+    f = Fractal.bundled(name)
+    p = lcg_points(4096, extent, 11)
+    s = renderer.probe_distances(Scene(f, Camera()), p)
+    dist, traps, iters, _ = oracle.de_probe(f.c_struct(), p, 32)
+    assert np.array_equal(s.iterations, iters)
+    err = np.abs(s.distance - dist) / np.maximum(np.abs(dist), 1.0)
+    assert err.max() <= 1e-5, err.max()
+    assert np.abs(s.traps[:, 0] - traps[:, 0]).max() <= 1e-5
+
+
+@pytest.mark.parametrize("power", [2, 3, 6, 8, 16, 6.25])
+def test_probe_mandelbulb_powers(renderer, power):
+    from oracle import oracle
+    from syfrack_amd.scene import Camera, Scene, Fractal
+    f = Fractal.bundled("mandelbulb")
+    f.power = float(power)
+    p = lcg_points(2048, 1.6, 11)
+    p = np.concatenate([p, np.array([[0, 0, 1], [0, 0, -1], [1e-6, 0, 1], [0, 0, 0.5]], np.float32)])
+    s = renderer.probe_distances(Scene(f, Camera()), p)
+    dist, traps, iters, _ = oracle.de_probe(f.c_struct(), p, 32)
+    if float(power).is_integer():
+        # exact spherical product: IEEE arithmetic on both sides
+        assert np.array_equal(s.iterations, iters)
+        err = np.abs(s.distance - dist) / np.maximum(np.abs(dist), 1.0)
+        assert err.max() <= 1e-5, err.max()
+    else:
+        # trigonometric form: device libm vs glibc differ by ulps that the orbit amplifies;
+        # the reference's own GPU/CPU gate (gpu_de.rs:40-91): points with |d| < 0.02 skipped,
+        # |delta d| <= 1e-3 + 2 % |d|, iterations equal, at most 0.5 % of points off
+        keep = np.abs(dist) >= 0.02
+        bad = (np.abs(s.distance - dist) > 1e-3 + 0.02 * np.abs(dist)) | (s.iterations != iters)
+        assert np.mean(bad[keep]) <= 0.005, np.mean(bad[keep])
+
+
+def _trace_vs_oracle(renderer, scene, cfg, rect_rows=None):
+    from oracle import oracle
+    out = renderer.trace_primary(scene, cfg)
+    desc = scene.desc(cfg)
+    if rect_rows is None:
+        ref64 = oracle.trace_primary(desc, 0, 64)
+        ref32 = oracle.trace_primary(desc, 0, 32)
+        sl = slice(None)
+    else:
+        y0, h = rect_rows
+        ref64 = oracle.trace_primary(desc, 0, 64, rect=(0, y0, cfg.width, h))
+        ref32 = oracle.trace_primary(desc, 0, 32, rect=(0, y0, cfg.width, h))
+        sl = slice(y0, y0 + h)
+    return out, ref32, ref64, sl
+
+
+def test_mandelbulb_primary_320x180_matches_fp64_oracle(renderer):
+    from syfrack_amd import scene as sc
+    scene, cfg = sc.mandelbulb_primary(320, 180)
+    out, ref32, ref64, sl = _trace_vs_oracle(renderer, scene, cfg)
+    hit_g, hit_o = out.depth >= 0, ref64["depth"] >= 0
+    agree = np.mean(hit_g == hit_o)
+    both = hit_g & hit_o
+    rel = np.abs(out.depth64[both] - ref64["depth64"][both]) / ref64["depth64"][both]
+    relm = np.abs(out.depth[both] - ref64["depth"][both]) / ref64["depth"][both]
+    print(f"320x180: hit/miss agreement {agree:.5f}; refined distance rel err max {rel.max():.3e} "
+          f"p99 {np.percentile(rel, 99):.3e}; march distance rel err max {relm.max():.3e}")
+    assert agree >= 0.999
+    assert np.mean(rel <= 1e-4) >= 0.999
+    # against the FP32 oracle the march itself agrees step for step
+    assert np.mean(out.steps == ref32["steps"]) >= 0.99
+    hits = out.depth >= 0
+    n = out.normals[hits][:, :3]
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-4)
+    assert np.all(out.normals[~hits] == np.array([0, 0, 0, -1], np.float32))
+    assert out.stats["primary_rays"] == 0 or out.stats["primary_rays"] == 320 * 180
+
+
+@pytest.mark.parametrize("name", ["mandelbox", "menger"])
+def test_other_families_primary_match_fp64_oracle(renderer, name):
+    from syfrack_amd import scene as sc
+    f = sc.Fractal.bundled(name)
+    if name == "mandelbox":
+        cam = sc.Camera.orbit((0, 0, 0), 11.0, 28.0, 22.0, 35.0)
+    else:
+        cam = sc.Camera.orbit((0, 0, 0), 3.6, 40.0, 30.0, 40.0)
+    scene = sc.Scene(f, cam, seed=5)
+    cfg = sc.RenderConfig(160, 90, 1, 0, 512)
+    out, ref32, ref64, sl = _trace_vs_oracle(renderer, scene, cfg)
+    hit_g, hit_o = out.depth >= 0, ref64["depth"] >= 0
+    agree = np.mean(hit_g == hit_o)
+    both = hit_g & hit_o
+    rel = np.abs(out.depth64[both] - ref64["depth64"][both]) / ref64["depth64"][both]
+    print(f"{name}: hits {both.sum()}, agreement {agree:.5f}, refined rel err max {rel.max():.3e}")
+    assert both.sum() > 100
+    assert agree >= 0.999
+    assert np.mean(rel <= 1e-4) >= 0.999
+
+
+def test_mandelbulb_1080p_properties_and_centre_rows(renderer):
+    """Full 1920x1080: every ray accounted for, unit normals, steps >= 1, misses flagged,
+    and the centre rows against the FP64 oracle."""
+    from syfrack_amd import scene as sc
+    scene, cfg = sc.mandelbulb_primary(1920, 1080)
+    out, ref32, ref64, sl = _trace_vs_oracle(renderer, scene, cfg, rect_rows=(536, 8))
+    assert np.all(out.steps >= 1) and np.all(out.steps <= 512)
+    hits = out.depth >= 0
+    assert 0.2 < hits.mean() < 0.6
+    assert np.allclose(np.linalg.norm(out.normals[hits][:, :3], axis=1), 1.0, atol=1e-4)
+    assert np.all(out.depth64[~hits] == -1.0)
+    g = out.depth[sl] >= 0
+    o = ref64["depth"] >= 0
+    assert np.mean(g == o) >= 0.999
+    both = g & o
+    rel = np.abs(out.depth64[sl][both] - ref64["depth64"][both]) / ref64["depth64"][both]
+    assert np.mean(rel <= 1e-4) >= 0.999
+    st = out.stats
+    assert st["hits"] == int(hits.sum())
+    assert st["steps"] == int(out.steps.astype(np.int64).sum())
+    assert st["de_evals"] == st["steps"] + 4 * st["hits"]
+
+
+def test_sphere_scene_centre_depth_and_tolerance(renderer):
+    """gpu_render.rs:479-490 and :113-145 on the device: a unit sphere seen from distance 3
+    has centre depth 2 +- 0.05, and a finer pixel tolerance lowers the hit-radius error."""
+    from syfrack_amd import scene as sc
+    f = sc.Fractal.bundled("sphere")
+    cam = sc.Camera.free((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), 50.0)
+    scene = sc.Scene(f, cam, seed=3)
+    errs = []
+    for tol in (1.0, 0.25):
+        cfg = sc.RenderConfig(64, 64, 1, 0, 512, pixel_tolerance=tol)
+        out = renderer.trace_primary(scene, cfg)
+        assert abs(out.depth[32, 32] - 2.0) < 0.05
+        t = out.depth[20:44, 20:44].astype(np.float64)
+        nz = out.normals[20:44, 20:44, 2].astype(np.float64)
+        assert np.all(t > 0) and np.all(nz > 0)
+        radius = 3.0 * nz - np.sqrt(np.maximum(t * t - 9.0 * (1.0 - nz * nz), 0.0))
+        errs.append(np.mean(np.abs(radius - 1.0)))
+        # the FP64 refine lands on the hit rule's crossing |p| - 1 = cone * t; head-on that
+        # is t = 2 / (1 + cone)
+        cone = float(np.float32(np.float32(np.float32(18.0) / np.float32(50.0)) * np.float32(2.0)
+                                / np.float32(64)) * np.float32(tol))
+        assert abs(out.depth64[32, 32] - 2.0 / (1.0 + cone)) < 2e-4
+    assert errs[0] > 1e-5 and errs[1] < errs[0] * 0.7
+
+
+def test_invalid_config_names_the_field(renderer):
+    from syfrack_amd import abi, scene as sc
+    scene, cfg = sc.mandelbulb_primary(32, 32)
+    cfg.pixel_tolerance = 2.0
+    with pytest.raises(abi.SyfError) as e:
+        renderer.trace_primary(scene, cfg)
+    assert e.value.code == abi.SYF_E_INVALID and "pixel_tolerance" in e.value.message
