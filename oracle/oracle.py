@@ -170,3 +170,14 @@ def render(desc, rect, sample_start: int, sample_count: int):
     rc = lib.orc_render(C.byref(desc), x0, y0, w, h, sample_start, sample_count, _ptr(accum), _ptr(depth), _ptr(aux))
     assert rc == 0
     return accum, depth, aux
+
+
+def planar_f32(desc, rect=None):
+    x0, y0, w, h = rect if rect is not None else (0, 0, desc.width, desc.height)
+    iters = np.zeros(w * h, np.uint32)
+    smooth = np.zeros(w * h, np.float32)
+    lib = load()
+    lib.orc_planar_f32.restype = None
+    lib.orc_planar_f32.argtypes = [C.c_void_p] + [C.c_uint32] * 4 + [C.c_void_p, C.c_void_p]
+    lib.orc_planar_f32(C.byref(desc), x0, y0, w, h, _ptr(iters), _ptr(smooth))
+    return iters.reshape(h, w), smooth.reshape(h, w)

@@ -483,6 +483,458 @@ double refine64(const Fractal<double>& F, V3<double> ro, V3<double> rd, double t
     return 0.5 * (a + b);
 }
 
+// ---------------------------------------------------------------- path tracing (FP32)
+//
+// The radiance estimator of pathtrace.wgsl:802-1201 restricted to the scene subset of this
+// path: area and environment lights, one base material with an optional trap-driven surface
+// variation (Lambert + GGX metal), no fog, sun, HDRI, portal, ocean, optics, texture,
+// guiding or phenomena (each of those branches is inert when its feature is absent).
+
+constexpr float PI_F = 3.14159265358979f;
+constexpr uint32_t NO_LIGHT = 0xffffffffu;
+typedef V3<float> F3;
+
+struct LightL {
+    uint32_t kind;
+    F3 a, b, c, color, horizon, ground, n;
+    float area;
+};
+
+struct SceneL {
+    Fractal<float> F;
+    Basis cam;
+    uint32_t width, height, seed, max_bounces, max_steps;
+    float pixel_tol, pixel_cone, t_end, cavity;
+    LightL lights[SYF_MAX_LIGHTS];
+    uint32_t light_count;
+    syf_material mat;
+};
+
+inline F3 f3(const float* p) { return v3(p[0], p[1], p[2]); }
+inline float luminance(F3 c) { return dot(c, v3(0.2126f, 0.7152f, 0.0722f)); }
+// WGSL mix(a, b, t) = a * (1 - t) + b * t
+inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+inline F3 mix3(F3 a, F3 b, float t) { return v3(mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)); }
+inline F3 mul3(F3 a, F3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline F3 reflect3(F3 e1, F3 e2) { return e1 - e2 * (2.0f * dot(e2, e1)); }
+// common.wgsl:242-247
+inline F3 safe_normalize(F3 v) {
+    float l = length(v);
+    float rcp = 1.0f / l;
+    if (!(l > 0.0f) || !(rcp > 0.0f) || !(rcp <= 3.4028235e38f)) return v3(0.0f, 1.0f, 0.0f);
+    return v * rcp;
+}
+inline float smoothstep01(float x) {
+    float t = clampv((x - 0.0f) / (1.0f - 0.0f), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+SceneL lower_scene(const syf_job_desc& d) {
+    SceneL S;
+    S.F = lower_fractal<float>(d.fractal);
+    S.cam = camera_basis(d.camera, d.width, d.height);
+    S.width = d.width;
+    S.height = d.height;
+    S.seed = uint32_t(d.seed) ^ uint32_t(d.seed >> 32);
+    S.max_bounces = d.max_bounces;
+    S.max_steps = d.max_steps;
+    S.pixel_tol = d.pixel_tolerance;
+    S.pixel_cone = S.cam.tan_half * 2.0f / float(d.width);
+    S.cavity = d.cavity_darkness;
+    float p0z = std::fmax(2.0f * d.fractal.bounding_radius_hint + length(S.cam.origin), 200.0f);
+    S.t_end = std::fmax(200.0f, p0z);
+    S.light_count = d.light_count;
+    for (uint32_t i = 0; i < d.light_count; ++i) {
+        const syf_light& l = d.lights[i];
+        LightL& L = S.lights[i];
+        L = LightL{};
+        L.kind = l.kind;
+        if (l.kind == SYF_LIGHT_AREA) {  // GpuLight::from_light (uniforms.rs:676-716)
+            F3 n0 = f3(l.normal);
+            float rcp = 1.0f / length(n0);
+            F3 n = (std::isfinite(rcp) && rcp > 0.0f) ? n0 * rcp : v3(0.0f, 1.0f, 0.0f);
+            F3 helper = std::fabs(n.y) > 0.999f ? v3(0.0f, 0.0f, 1.0f) : v3(0.0f, 1.0f, 0.0f);
+            F3 u = normalize(cross(helper, n));
+            F3 vv = cross(n, u);
+            L.a = f3(l.position);
+            L.b = u * l.size[0];
+            L.c = vv * l.size[1];
+            L.n = safe_normalize(cross(L.b, L.c));  // area_normal (lights.wgsl:164-166)
+            L.area = length(cross(L.b, L.c));       // emitter_area (lights.wgsl:69-74)
+            L.color = f3(l.color) * l.intensity;
+        } else {
+            L.color = f3(l.zenith) * l.intensity;  // env: a = zenith, b = horizon, c = ground
+            L.horizon = f3(l.horizon) * l.intensity;
+            L.ground = f3(l.ground) * l.intensity;
+        }
+    }
+    S.mat = d.material;
+    return S;
+}
+
+struct Surf {
+    F3 n, diffuse, f0;
+    float alpha;
+};
+
+// Branchless orthonormal basis, columns t, bt, n (pathtrace.wgsl:194-201).
+inline void tangent_frame(F3 n, F3* t, F3* bt) {
+    float s = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n.z);
+    float b = n.x * n.y * a;
+    *t = v3(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
+    *bt = v3(b, s + n.y * n.y * a, -n.y);
+}
+inline F3 frame_mul(F3 t, F3 bt, F3 n, F3 x) { return t * x.x + bt * x.y + n * x.z; }
+// pathtrace.wgsl:203-208
+F3 cosine_sample(F3 n, float r1, float r2) {
+    float phi = 2.0f * PI_F * r1;
+    float r = std::sqrt(r2);
+    F3 t, bt;
+    tangent_frame(n, &t, &bt);
+    return frame_mul(t, bt, n, v3(r * std::cos(phi), r * std::sin(phi), std::sqrt(std::fmax(0.0f, 1.0f - r2))));
+}
+inline float ggx_d(float nh, float a2) {
+    float d = nh * nh * (a2 - 1.0f) + 1.0f;
+    return a2 / (PI_F * d * d);
+}
+inline float smith_g1(float nv, float a2) { return 2.0f * nv / (nv + std::sqrt(a2 + (1.0f - a2) * nv * nv)); }
+inline F3 fresnel(F3 f0, float vh) {
+    float k = std::pow(1.0f - vh, 5.0f);
+    return v3(f0.x + (1.0f - f0.x) * k, f0.y + (1.0f - f0.y) * k, f0.z + (1.0f - f0.z) * k);
+}
+// Heitz 2018 VNDF sampling (pathtrace.wgsl:234-249).
+F3 sample_ggx_h(F3 n, F3 v, float alpha, float r1, float r2) {
+    F3 t, bt;
+    tangent_frame(n, &t, &bt);
+    F3 lv = v3(dot(t, v), dot(bt, v), dot(n, v));
+    F3 vh = safe_normalize(v3(alpha * lv.x, alpha * lv.y, std::fmax(lv.z, 1e-5f)));
+    float lensq = vh.x * vh.x + vh.y * vh.y;
+    F3 t1 = v3(1.0f, 0.0f, 0.0f);
+    if (lensq > 0.0f) {
+        float sl = std::sqrt(lensq);
+        t1 = v3(-vh.y / sl, vh.x / sl, 0.0f);
+    }
+    F3 t2 = cross(vh, t1);
+    float r = std::sqrt(r1);
+    float phi = 2.0f * PI_F * r2;
+    float x = r * std::cos(phi);
+    float blend = 0.5f * (1.0f + vh.z);
+    float y = mixf(std::sqrt(std::fmax(0.0f, 1.0f - x * x)), r * std::sin(phi), blend);
+    F3 nh = t1 * x + t2 * y + vh * std::sqrt(std::fmax(0.0f, 1.0f - x * x - y * y));
+    return frame_mul(t, bt, n, safe_normalize(v3(alpha * nh.x, alpha * nh.y, std::fmax(nh.z, 0.0f))));
+}
+// make_surface subset (pathtrace.wgsl:271-386): base material, SurfaceVariation.
+Surf make_surface(const SceneL& S, F3 n, float trap0, uint32_t iterations) {
+    const syf_material& m = S.mat;
+    F3 albedo = f3(m.albedo);
+    float rough = m.roughness, metallic = m.metallic, spec = m.specular;
+    if (m.surface_enabled) {
+        float value = m.surface_slot == 0 ? trap0 : 1.0f;  // classic graphs write slot 0 only
+        float t = clampv(value, 0.0f, 1.0f);
+        rough = m.surface_roughness[0] + (m.surface_roughness[1] - m.surface_roughness[0]) * t;
+        metallic = m.surface_metallic[0] + (m.surface_metallic[1] - m.surface_metallic[0]) * t;
+    }
+    rough = clampv(rough, 0.02f, 1.0f);
+    Surf s;
+    s.n = n;
+    s.diffuse = albedo * (1.0f - metallic);
+    s.f0 = mix3(v3(spec, spec, spec), albedo, metallic);
+    s.alpha = rough * rough;
+    (void)iterations;
+    return s;
+}
+// brdf_eval (pathtrace.wgsl:394-420), Lambert + GGX, no Oren-Nayar term.
+F3 brdf_eval(const Surf& s, F3 v, F3 l) {
+    float nl = dot(s.n, l), nv = dot(s.n, v);
+    if (nl <= 0.0f || nv <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    F3 h = normalize(v + l);
+    float nh = std::fmax(dot(s.n, h), 0.0f);
+    float vh = std::fmax(dot(v, h), 0.0f);
+    float a2 = s.alpha * s.alpha;
+    F3 f = fresnel(s.f0, vh);
+    float k = ggx_d(nh, a2) * smith_g1(nv, a2) * smith_g1(nl, a2);
+    float den = std::fmax(4.0f * nv * nl, 1e-4f);
+    F3 spec = v3(k * f.x / den, k * f.y / den, k * f.z / den);
+    F3 one_f = v3(1.0f - f.x, 1.0f - f.y, 1.0f - f.z);
+    F3 dif = mul3(s.diffuse, one_f) * 1.0f;
+    return v3(dif.x / PI_F + spec.x, dif.y / PI_F + spec.y, dif.z / PI_F + spec.z);
+}
+inline float mis_power(float a, float b) {
+    float scale = std::fmax(a, b);
+    if (scale <= 0.0f) return 0.0f;
+    float pa = a / scale, pb = b / scale;
+    return pa * pa / (pa * pa + pb * pb);
+}
+inline float spec_probability(const Surf& s, float nv) {
+    F3 f = fresnel(s.f0, nv);
+    return clampv(luminance(f) / std::fmax(luminance(f) + luminance(s.diffuse), 1e-4f), 0.05f, 0.95f);
+}
+// bsdf_pdf (pathtrace.wgsl:443-452); continuation_pdf without guiding or transmission.
+float bsdf_pdf(const Surf& s, F3 v, F3 l, float p_spec) {
+    float nl = dot(s.n, l);
+    if (nl <= 0.0f) return 0.0f;
+    F3 hh = normalize(v + l);
+    float nh = std::fmax(dot(s.n, hh), 0.0f);
+    float a2 = s.alpha * s.alpha;
+    float nv = std::fmax(dot(s.n, v), 1e-4f);
+    float pdf_spec = ggx_d(nh, a2) * smith_g1(nv, a2) / (4.0f * nv);
+    return p_spec * pdf_spec + (1.0f - p_spec) * nl / PI_F;
+}
+// environment_value / environment_probability (pathtrace.wgsl:164-175)
+F3 environment_value(const LightL& l, F3 dir) {
+    if (dir.y >= 0.0f) return mix3(l.horizon, l.color, std::sqrt(dir.y));
+    return mix3(l.horizon, l.ground, std::sqrt(-dir.y));
+}
+float environment_probability(const LightL& l, float area_total) {
+    float energy = std::fmax(std::fmax(luminance(l.color), luminance(l.horizon)), luminance(l.ground));
+    if (energy <= 0.0f) return 0.0f;
+    return clampv(4.0f * PI_F * energy / (4.0f * PI_F * energy + area_total), 0.05f, 1.0f);
+}
+// area_intersection (lights.wgsl:170-181)
+float area_intersection(F3 ro, F3 rd, const LightL& l) {
+    float den = dot(l.n, rd);
+    if (std::fabs(den) < 1e-7f) return 1e30f;
+    float t = dot(l.a - ro, l.n) / den;
+    if (t <= 1e-6f) return 1e30f;
+    F3 q = ro + rd * t - l.a;
+    float x = dot(q, l.b) / dot(l.b, l.b);
+    float y = dot(q, l.c) / dot(l.c, l.c);
+    if (std::fabs(x) > 0.5f || std::fabs(y) > 0.5f) return 1e30f;
+    return t;
+}
+// area_weight / area_total_weight (lights.wgsl:288-314)
+float area_weight(F3 p, F3 n, const LightL& l) {
+    F3 to = l.a - p;
+    float dist2 = std::fmax(dot(to, to), 1e-6f);
+    F3 dir = v3(to.x / std::sqrt(dist2), to.y / std::sqrt(dist2), to.z / std::sqrt(dist2));
+    float facing = std::fmax(dot(l.n, neg(dir)), 0.0f);
+    return luminance(l.color) * l.area * facing * std::fmax(dot(n, dir), 0.15f) / dist2;
+}
+float area_total_weight(const SceneL& S, F3 p, F3 n) {
+    float total = 0.0f;
+    for (uint32_t i = 0; i < S.light_count; ++i)
+        if (S.lights[i].kind == SYF_LIGHT_AREA) total += area_weight(p, n, S.lights[i]);
+    return total;
+}
+struct HitL {
+    Hit<float> h;
+    uint32_t light;
+};
+// trace_geometry_lights (lights.wgsl:183-209) without ground.
+HitL trace_scene(const SceneL& S, F3 ro, F3 rd, float t_end, Count* c) {
+    float nearest = t_end;
+    uint32_t light = NO_LIGHT;
+    for (uint32_t i = 0; i < S.light_count; ++i) {
+        if (S.lights[i].kind != SYF_LIGHT_AREA) continue;
+        float t = area_intersection(ro, rd, S.lights[i]);
+        if (t < nearest) {
+            nearest = t;
+            light = i;
+        }
+    }
+    HitL r;
+    r.h = march(S.F, ro, rd, S.pixel_cone, S.pixel_tol, S.max_steps, nearest, c);
+    r.light = NO_LIGHT;
+    if (!r.h.hit && !r.h.exhausted && light != NO_LIGHT) {
+        r.h.hit = true;
+        r.h.t = nearest;
+        r.light = light;
+    }
+    return r;
+}
+inline bool visible(const SceneL& S, F3 ro, F3 rd, float t_end, Count* c) {
+    HitL h = trace_scene(S, ro, rd, t_end, c);
+    return !h.h.hit && !h.h.exhausted;
+}
+// area_emission (lights.wgsl:330-357)
+F3 area_emission(const SceneL& S, F3 ro, F3 rd, float distance, float prev_pdf, bool use_mis, F3 prev_n,
+                 float prev_total) {
+    F3 L = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < S.light_count; ++i) {
+        const LightL& l = S.lights[i];
+        if (l.kind != SYF_LIGHT_AREA) continue;
+        float cos_l = dot(l.n, neg(rd));
+        if (cos_l <= 0.0f) continue;
+        float t = area_intersection(ro, rd, l);
+        if (std::fabs(t - distance) > std::fmax(1e-6f, distance * 1e-6f)) continue;
+        float w = 1.0f;
+        if (use_mis) {
+            float pl = distance * distance / std::fmax(l.area * cos_l, 1e-20f);
+            float sel = 0.0f;
+            if (prev_total > 0.0f) sel = area_weight(ro, prev_n, l) / prev_total;
+            pl *= sel;
+            w = mis_power(prev_pdf, pl);
+        }
+        L = L + l.color * w;
+    }
+    return L;
+}
+// environment_emission (pathtrace.wgsl:177-189), surfaces only.
+F3 environment_emission(const SceneL& S, F3 dir, float pdf, bool use_mis, F3 n, float area_total) {
+    F3 L = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < S.light_count; ++i) {
+        const LightL& l = S.lights[i];
+        if (l.kind != SYF_LIGHT_ENV) continue;
+        float w = 1.0f;
+        if (use_mis) {
+            float proposal = environment_probability(l, area_total) * (std::fmax(dot(n, dir), 0.0f) / PI_F);
+            w = mis_power(pdf, proposal);
+        }
+        L = L + environment_value(l, dir) * w;
+    }
+    return L;
+}
+// direct_light (pathtrace.wgsl:471-558), environment and area branches.
+F3 direct_light(const SceneL& S, F3 p, const Surf& s, F3 gn, F3 v, float p_spec, float area_total, bool last,
+                uint32_t* rng, Count* c) {
+    F3 L = v3(0.0f, 0.0f, 0.0f);
+    uint32_t selected = NO_LIGHT;
+    float selected_weight = 0.0f;
+    if (area_total > 0.0f) {
+        float threshold = rand01(rng) * area_total;
+        float sum = 0.0f;
+        for (uint32_t i = 0; i < S.light_count; ++i) {
+            if (S.lights[i].kind != SYF_LIGHT_AREA) continue;
+            float w = area_weight(p, s.n, S.lights[i]);
+            sum += w;
+            if (w > 0.0f) {
+                selected = i;
+                selected_weight = w;
+                if (threshold < sum) break;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < S.light_count; ++i) {
+        const LightL& l = S.lights[i];
+        if (l.kind == SYF_LIGHT_ENV) {
+            float mx = std::fmax(std::fmax(std::fmax(l.color.x, l.color.y), l.color.z),
+                                 std::fmax(std::fmax(std::fmax(l.horizon.x, l.horizon.y), l.horizon.z),
+                                           std::fmax(std::fmax(l.ground.x, l.ground.y), l.ground.z)));
+            if (mx <= 0.0f) continue;
+            float prob = environment_probability(l, area_total);
+            if (rand01(rng) >= prob) continue;
+            float r1 = rand01(rng);
+            float r2 = rand01(rng);
+            F3 dir = cosine_sample(s.n, r1, r2);
+            float nl = std::fmax(dot(s.n, dir), 0.0f);
+            if (dot(gn, dir) <= 0.0f || nl <= 0.0f) continue;
+            if (!visible(S, p, dir, S.t_end, c)) continue;
+            float pl = prob * nl / PI_F;
+            float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
+            float w = last ? 1.0f : mis_power(pl, pb);
+            F3 f = brdf_eval(s, v, dir);
+            F3 e = environment_value(l, dir);
+            L = L + v3(f.x * nl * e.x * 1.0f * w / pl, f.y * nl * e.y * 1.0f * w / pl, f.z * nl * e.z * 1.0f * w / pl);
+        } else if (l.kind == SYF_LIGHT_AREA) {
+            if (i != selected) continue;
+            float u1 = rand01(rng);
+            float u2 = rand01(rng);
+            F3 q = l.a + l.b * (u1 - 0.5f) + l.c * (u2 - 0.5f);  // sample_emitter_point (lights.wgsl:146-150)
+            F3 to_q = q - p;
+            float dist2 = dot(to_q, to_q);
+            if (dist2 < 1e-8f) continue;
+            float dist = std::sqrt(dist2);
+            F3 dir = v3(to_q.x / dist, to_q.y / dist, to_q.z / dist);
+            float nl = dot(s.n, dir);
+            float cos_l = dot(l.n, neg(dir));
+            if (nl <= 0.0f || cos_l <= 0.0f || dot(gn, dir) <= 0.0f) continue;
+            float end = std::fmax(0.0f, dist - std::fmax(1e-5f, dist * 1e-6f));  // shadow_endpoint
+            if (!visible(S, p, dir, end, c)) continue;
+            float pl = (selected_weight / area_total) * (dist2 / std::fmax(l.area * cos_l, 1e-20f));
+            float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
+            float w = last ? 1.0f : mis_power(pl, pb);
+            F3 f = brdf_eval(s, v, dir);
+            L = L + v3(f.x * nl * l.color.x * 1.0f * w / pl, f.y * nl * l.color.y * 1.0f * w / pl,
+                       f.z * nl * l.color.z * 1.0f * w / pl);
+        }
+    }
+    return L;
+}
+
+// radiance (pathtrace.wgsl:802-1201) for the subset; returns L and the first-hit planes.
+F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* first_depth, float* first_aux,
+            Count* c) {
+    F3 ro, rd;
+    camera_ray(S.cam, S.width, S.height, px, py, rng, &ro, &rd);
+    F3 throughput = v3(1.0f, 1.0f, 1.0f), L = v3(0.0f, 0.0f, 0.0f);
+    float prev_pdf = 0.0f, prev_total = 0.0f;
+    bool prev_from_bsdf = false;
+    F3 prev_n = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t bounce = 0; bounce <= S.max_bounces;) {
+        HitL hl = trace_scene(S, ro, rd, S.t_end, c);
+        const Hit<float>& h = hl.h;
+        F3 gn = v3(0.0f, 0.0f, 0.0f);
+        if (h.hit) {
+            if (hl.light != NO_LIGHT) {
+                gn = S.lights[hl.light].n;
+            } else {
+                gn = normal_at(S.F, ro + rd * h.t, std::fmax(h.t * S.pixel_cone * 0.5f, 1e-4f) * S.pixel_tol, c);
+            }
+            if (bounce == 0) {
+                *first_depth = h.t;
+                first_aux[0] = gn.x; first_aux[1] = gn.y; first_aux[2] = gn.z; first_aux[3] = float(h.steps);
+            }
+        }
+        if (!h.hit) {
+            if (!h.exhausted) {
+                F3 e = environment_emission(S, rd, prev_pdf, prev_from_bsdf, prev_n, prev_total);
+                L = L + mul3(throughput, e);
+            }
+            break;
+        }
+        F3 p = ro + rd * h.t;
+        if (hl.light != NO_LIGHT) {
+            F3 e = area_emission(S, ro, rd, h.t, prev_pdf, prev_from_bsdf, prev_n, prev_total);
+            L = L + mul3(throughput, e);
+            break;
+        }
+        F3 n = gn;
+        if (bounce == 0) {
+            float k = 1.0f - S.cavity * smoothstep01(float(h.steps) / 96.0f);
+            throughput = throughput * k;
+        }
+        F3 v = neg(rd);
+        Surf s = make_surface(S, n, h.sample.traps[0], h.sample.iter);
+        float offset = std::fmax(h.t * S.pixel_cone * 2.0f, 1e-3f) * S.pixel_tol;
+        F3 p_off = p + n * offset;
+        float nv = std::fmax(dot(s.n, v), 1e-4f);
+        float p_spec = spec_probability(s, nv);
+        float area_total = area_total_weight(S, p_off, s.n);
+        bool last = bounce == S.max_bounces;
+        F3 dl = direct_light(S, p_off, s, n, v, p_spec, area_total, last, rng, c);
+        L = L + mul3(throughput, dl);
+        if (last) break;
+        float r0 = rand01(rng), r1 = rand01(rng), r2 = rand01(rng);
+        F3 l;
+        if (r0 < p_spec) {
+            F3 hh = sample_ggx_h(s.n, v, s.alpha, r1, r2);
+            l = reflect3(neg(v), hh);
+        } else {
+            l = cosine_sample(s.n, r1, r2);
+        }
+        float nl = dot(s.n, l);
+        if (nl <= 0.0f || dot(n, l) <= 0.0f) break;
+        float pdf = bsdf_pdf(s, v, l, p_spec);
+        if (pdf <= 1e-6f) break;
+        F3 f = brdf_eval(s, v, l);
+        throughput = v3(throughput.x * (f.x * nl / pdf), throughput.y * (f.y * nl / pdf), throughput.z * (f.z * nl / pdf));
+        prev_pdf = pdf;
+        prev_from_bsdf = true;
+        prev_n = s.n;
+        prev_total = area_total;
+        if (bounce >= 2) {
+            float q = clampv(std::fmax(throughput.x, std::fmax(throughput.y, throughput.z)), 0.05f, 0.95f);
+            if (rand01(rng) > q) break;
+            throughput = v3(throughput.x / q, throughput.y / q, throughput.z / q);
+        }
+        ro = p_off;
+        rd = l;
+        ++bounce;
+    }
+    return L;
+}
+
 float ray_extent(const syf_job_desc& d, const Basis& b) {
     // uniforms.rs:781-782 and common.wgsl:206-208
     float p0z = std::fmax(2.0f * d.fractal.bounding_radius_hint + length(b.origin), 200.0f);
@@ -696,6 +1148,44 @@ int orc_trace_primary(const syf_job_desc* d, uint32_t sample, int precision, uin
     return 0;
 }
 
+// Megakernel accumulation (pathtrace.wgsl:1203-1276) of a pixel rectangle: samples
+// sample_start .. sample_start+sample_count-1 in ascending order, accum += (r if finite, 1);
+// depth/aux from the first sample of the call (written by the caller when sample_start == 0).
+// Outputs are rectangle-local, row-major: accum4, depth, aux4.
+int orc_render(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sample_start,
+               uint32_t sample_count, float* accum4, float* depth, float* aux4) {
+    SceneL S = lower_scene(*d);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long yy = 0; yy < long(h); ++yy) {
+        Count c;
+        for (uint32_t xx = 0; xx < w; ++xx) {
+            uint32_t px = x0 + xx, py = y0 + uint32_t(yy);
+            size_t i = size_t(yy) * w + xx;
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            float fd = -1.0f, fa[4] = {0.0f, 0.0f, 0.0f, -1.0f};
+            for (uint32_t s = 0; s < sample_count; ++s) {
+                uint32_t rng = rng_init(px, py, sample_start + s, S.seed);
+                float dd = -1.0f, aa[4] = {0.0f, 0.0f, 0.0f, -1.0f};
+                F3 r = radiance(S, px, py, &rng, &dd, aa, &c);
+                bool ok = r.x == r.x && r.y == r.y && r.z == r.z && std::fabs(r.x) < 1e30f && std::fabs(r.y) < 1e30f &&
+                          std::fabs(r.z) < 1e30f;
+                acc[0] += ok ? r.x : 0.0f;
+                acc[1] += ok ? r.y : 0.0f;
+                acc[2] += ok ? r.z : 0.0f;
+                acc[3] += 1.0f;
+                if (s == 0) {
+                    fd = dd;
+                    for (int k = 0; k < 4; ++k) fa[k] = aa[k];
+                }
+            }
+            for (int k = 0; k < 4; ++k) accum4[4 * i + k] = acc[k];
+            depth[i] = fd;
+            for (int k = 0; k < 4; ++k) aux4[4 * i + k] = fa[k];
+        }
+    }
+    return 0;
+}
+
 // Dense FP64 stepping along one ray (step `dt`) to the first sign change of the field,
 // bisected to 1e-12: the brute-force pin of the march + refine on small images.
 double orc_first_root_bruteforce(const syf_fractal* f, const double* ro3, const double* rd3, double t_max,
@@ -819,6 +1309,62 @@ void orc_escape_f64(const syf_planar_desc* d, uint32_t x0, uint32_t y0, uint32_t
             double r = std::fmin(std::sqrt(e.radius2), 1e20);
             smooth[k] = e.escaped ? float(double(e.count) + 1.0 - std::log(std::log(std::fmax(r, 1.0001))) / std::log(2.0))
                                   : float(d->iterations);
+        }
+    delete[] fx;
+    delete[] fy;
+}
+
+// FP32 direct evaluation of a pixel rectangle with the reference's f32 loop (planar.rs:
+// 107-164: |z| by sqrt against the bailout, complex_power by repeated multiplication from
+// 1, smooth count with ln of the power); c is the view point rounded to f32.
+void orc_planar_f32(const syf_planar_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t* iters,
+                    float* smooth) {
+    double* fx = new double[d->width];
+    double* fy = new double[d->height];
+    orc_planar_offsets(d, fx, fy);
+    double rot = double(d->rotation) * (3.14159265358979323846 / 180.0);
+    double cr = std::cos(rot), sr = std::sin(rot);
+    const uint32_t power = 2;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long jj = 0; jj < long(h); ++jj)
+        for (uint32_t ii = 0; ii < w; ++ii) {
+            uint32_t i = x0 + ii, j = y0 + uint32_t(jj);
+            double dx = cr * fx[i] - sr * fy[j];
+            double dy = sr * fx[i] + cr * fy[j];
+            float ptx = float(d->center[0] + dx), pty = float(d->center[1] + dy);
+            bool pp = d->family != SYF_PLANAR_JULIA;
+            float cx = pp ? ptx : float(d->julia[0]), cy = pp ? pty : float(d->julia[1]);
+            float zx = pp ? 0.0f : ptx, zy = pp ? 0.0f : pty;
+            uint32_t count = 0;
+            bool escaped = false;
+            for (uint32_t k = 0; k < d->iterations; ++k) {
+                float radius = std::sqrt(zx * zx + zy * zy);
+                if (radius > d->bailout) {
+                    escaped = true;
+                    break;
+                }
+                float bx = zx, by = zy;
+                if (d->family == SYF_PLANAR_BURNING_SHIP) {
+                    bx = std::fabs(bx);
+                    by = std::fabs(by);
+                }
+                float wx = 1.0f, wy = 0.0f;
+                for (uint32_t q = 0; q < power; ++q) {
+                    float nx = wx * bx - wy * by, ny = wx * by + wy * bx;
+                    wx = nx;
+                    wy = ny;
+                }
+                zx = wx + cx;
+                zy = wy + cy;
+                ++count;
+            }
+            float radius = std::fmin(std::sqrt(zx * zx + zy * zy), 1e20f);
+            escaped = escaped || radius > d->bailout;
+            float s = float(d->iterations);
+            if (escaped) s = float(count) + 1.0f - std::log(std::log(std::fmax(radius, 1.0001f))) / std::log(float(power));
+            size_t k = size_t(jj) * w + ii;
+            iters[k] = count;
+            smooth[k] = s;
         }
     delete[] fx;
     delete[] fy;

@@ -304,6 +304,8 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
     }
     if ((e = hipMalloc(&j->ctr, sizeof(unsigned long long) * CTR_COUNT)) != hipSuccess ||
         (e = hipMalloc(&j->t_prev, j->pixels * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&j->hit_list, j->pixels * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&j->counters, 2 * sizeof(uint32_t))) != hipSuccess ||
         (e = hipEventCreate(&j->ev0)) != hipSuccess || (e = hipEventCreate(&j->ev1)) != hipSuccess ||
         (e = hipEventCreate(&j->ev2)) != hipSuccess) {
         syf_job_destroy(j);
@@ -327,6 +329,8 @@ int syf_job_destroy(syf_job* j) {
         if (p) hipFree(p);
     if (j->ctr) hipFree(j->ctr);
     if (j->t_prev) hipFree(j->t_prev);
+    if (j->hit_list) hipFree(j->hit_list);
+    if (j->counters) hipFree(j->counters);
     if (j->ev0) hipEventDestroy(j->ev0);
     if (j->ev1) hipEventDestroy(j->ev1);
     if (j->ev2) hipEventDestroy(j->ev2);
@@ -406,14 +410,13 @@ int syf_trace_primary(syf_job* j, uint32_t sample) {
     if (!j) return fail(SYF_E_INVALID, "job must not be null");
     hipSetDevice(j->ctx->device);
     hipError_t e = hipMemsetAsync(j->ctr, 0, sizeof(unsigned long long) * CTR_COUNT, j->stream);
+    TraceBuffers B{(float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
+                   (double*)j->plane[SYF_PLANE_DEPTH64], (uint32_t*)j->plane[SYF_PLANE_STEPS], j->t_prev, j->hit_list,
+                   j->counters, j->ctr};
     if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
-    if (e == hipSuccess)
-        e = launch_trace_primary(j->P, sample, (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
-                                 (uint32_t*)j->plane[SYF_PLANE_STEPS], j->t_prev, j->ctr, j->stream);
+    if (e == hipSuccess) e = launch_trace_primary(j->P, sample, B, j->ctx->cu_count, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev1, j->stream);
-    if (e == hipSuccess)
-        e = launch_refine64(j->P, sample, (const float*)j->plane[SYF_PLANE_DEPTH], j->t_prev,
-                            (double*)j->plane[SYF_PLANE_DEPTH64], j->ctr, j->stream);
+    if (e == hipSuccess) e = launch_refine64(j->P, sample, B, j->ctx->cu_count, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev2, j->stream);
     j->timed = true;
     j->extra_ms = 0.0;

@@ -25,7 +25,9 @@ struct syf_job {
     hipStream_t stream = nullptr;
     void* plane[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     unsigned long long* ctr = nullptr;
-    float* t_prev = nullptr;  // per pixel: march position before the hit (FP64 refine bracket)
+    float* t_prev = nullptr;     // per pixel: march position before the hit (FP64 refine bracket)
+    uint32_t* hit_list = nullptr;  // per pixel capacity: hits queued for the FP64 refine
+    uint32_t* counters = nullptr;  // ray counter and hit-list length of the primary trace
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // start, after dominant kernel, end
     bool timed = false;
     double extra_ms = 0.0;

@@ -125,6 +125,20 @@ template <class T> __device__ __forceinline__ void trig_power(V3<T>& p, T radius
     p = mk((st * cp) * zr, (st * sp) * zr, ct * zr);
 }
 
+// Integer power of an orbit point closer than 1e-12 to the origin.  The reference takes its
+// trigonometric form there with the radius clamped to 1e-12 (eval.rs:462-475): the new point
+// has length 1e-12^n and dr is multiplied by n * 1e-12^(n-1).  For n >= 4 the length
+// underflows to zero in f32 and this is exact; for n = 2, 3 the point is set to the origin
+// instead of a vector of length 1e-24 or 1e-36, which leaves the distance unchanged and moves
+// a trap by less than 1e-23.  Keeping trigonometry out of the integer-power kernels keeps
+// their code small.
+template <class T> __device__ __forceinline__ void tiny_power(V3<T>& p, uint32_t n, T& dr) {
+    T prev = T(1);
+    for (uint32_t i = 1; i < n; ++i) prev = prev * T(1e-12);
+    dr = dr * (prev * T(n));
+    p = mk(T(0), T(0), T(0));
+}
+
 // Graph::evaluate for one classic family (eval.rs:98-114, :346-556, :558-599, :601-621).
 template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval(const FractalP& F, V3<T> c) {
     V3<T> z = c;
@@ -138,10 +152,14 @@ template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval(const 
             T r = len3(z);
             if (r > bail) break;
             trap = fmin(trap, r);
-            if (FAM == SYF_FAMILY_MANDELBULB && r >= T(1e-12)) {
-                T deriv;
-                spherical_power(z, r, F.ipower, deriv);
-                dr = dr * deriv;
+            if constexpr (FAM == SYF_FAMILY_MANDELBULB) {
+                if (r >= T(1e-12)) {
+                    T deriv;
+                    spherical_power(z, r, F.ipower, deriv);
+                    dr = dr * deriv;
+                } else {
+                    tiny_power(z, F.ipower, dr);
+                }
             } else {
                 trig_power(z, r, T(F.power), dr);
             }
@@ -204,6 +222,106 @@ template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval(const 
     o.runs = iter;
     return o;
 }
+
+// One field evaluation split into resumable body executions (FP32), for kernels that
+// interleave evaluations of different depths on the lanes of a wave: start() seeds the point
+// and reports whether the orbit is already done (|p| > bailout or no iterations); step() runs
+// one Iterate body plus the next bailout test and reports completion; finish() applies the
+// fence and the trap normalisation.  The arithmetic is de_eval's, in the same order, so the
+// results are identical.
+template <uint32_t FAM> struct DeStepper {
+    V3<float> c, z;
+    float dr, trap, r;  // r = |z| of the current orbit point
+    uint32_t k;
+
+    __device__ __forceinline__ bool start(const FractalP& F, V3<float> p) {
+        c = p;
+        z = p;
+        dr = 1.0f;
+        trap = INFINITY;
+        k = 0;
+        if constexpr (FAM == SYF_FAMILY_SPHERE) {
+            r = len3(z);
+            return true;
+        } else {
+            r = len3(z);
+            return F.iterations == 0u || r > F.bailout;
+        }
+    }
+
+    // One Iterate body (the caller guarantees the orbit is not done).
+    __device__ __forceinline__ bool step(const FractalP& F) {
+        if constexpr (FAM == SYF_FAMILY_MANDELBULB || FAM == FAM_BULB_TRIG) {
+            trap = fminf(trap, r);
+            if constexpr (FAM == SYF_FAMILY_MANDELBULB) {
+                if (r >= 1e-12f) {
+                    float deriv;
+                    spherical_power(z, r, F.ipower, deriv);
+                    dr = dr * deriv;
+                } else {
+                    tiny_power(z, F.ipower, dr);
+                }
+            } else {
+                trig_power(z, r, F.power, dr);
+            }
+            z = add(z, c);
+            dr = dr + 1.0f;
+        } else if constexpr (FAM == SYF_FAMILY_MANDELBOX) {
+            const float L = F.fold_limit;
+            z = mk(clampT(z.x, -L, L) * 2.0f - z.x, clampT(z.y, -L, L) * 2.0f - z.y, clampT(z.z, -L, L) * 2.0f - z.z);
+            const float r2 = dot3(z, z);
+            trap = fminf(trap, sqrtf(r2));
+            const float kf = r2 < F.mr2 ? F.fr2_over_mr2 : (r2 < F.fr2 ? F.fr2 / r2 : 1.0f);
+            z = scl(z, kf);
+            dr = dr * kf;
+            z = scl(z, F.scale);
+            dr = dr * F.abs_scale;
+            z = add(z, c);
+            dr = dr + 1.0f;
+        } else if constexpr (FAM == SYF_FAMILY_MENGER) {
+            float wx = fabsf(z.x), wy = fabsf(z.y), wz = fabsf(z.z), tmp;
+            if (wx < wy) { tmp = wx; wx = wy; wy = tmp; }
+            if (wx < wz) { tmp = wx; wx = wz; wz = tmp; }
+            if (wy < wz) { tmp = wy; wy = wz; wz = tmp; }
+            wx = wx * F.scale - F.off_s1[0];
+            wy = wy * F.scale - F.off_s1[1];
+            wz = wz * F.scale - F.off_s1[2];
+            if (wz < F.menger_shift_threshold) wz = wz + F.off_s1[2];
+            z = mk(wx, wy, wz);
+            dr = dr * F.scale;
+            trap = fminf(trap, len3(z));
+        }
+        ++k;
+        r = len3(z);
+        return k >= F.iterations || r > F.bailout;
+    }
+
+    __device__ __forceinline__ De<float> finish(const FractalP& F) const {
+        float d;
+        float tr = trap;
+        if constexpr (FAM == SYF_FAMILY_MANDELBULB || FAM == FAM_BULB_TRIG) {
+            const float rr = fmaxf(r, 1e-12f);
+            d = 0.5f * logf(rr) * rr / dr;
+        } else if constexpr (FAM == SYF_FAMILY_MANDELBOX) {
+            d = r / fabsf(dr);
+        } else if constexpr (FAM == SYF_FAMILY_MENGER) {
+            const float qx = fabsf(z.x) - F.half[0], qy = fabsf(z.y) - F.half[1], qz = fabsf(z.z) - F.half[2];
+            const V3<float> qm = mk(fmaxf(qx, 0.0f), fmaxf(qy, 0.0f), fmaxf(qz, 0.0f));
+            d = (len3(qm) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f)) / dr;
+        } else {
+            tr = r;
+            const float a = (r - F.radius) / dr;
+            const float b = (dot3(z, mk(F.plane_n[0], F.plane_n[1], F.plane_n[2])) - F.plane_offset) / dr;
+            d = fminf(a, b);
+        }
+        De<float> o;
+        o.d = d;
+        o.trap0 = isfinite(tr) ? fminf(tr / F.trap_scale, 1.0f) : 1.0f;
+        o.iter = k < F.iteration_product ? k : F.iteration_product;
+        o.runs = k;
+        return o;
+    }
+};
 
 // Family chosen at run time (uniform per job or per tile): used where several families
 // share one launch.

@@ -7,10 +7,20 @@
 
 namespace syf {
 
-hipError_t launch_trace_primary(const JobP& P, uint32_t sample, float* depth, float4* aux, uint32_t* steps,
-                                float* t_prev, unsigned long long* ctr, hipStream_t s);
-hipError_t launch_refine64(const JobP& P, uint32_t sample, const float* depth, const float* t_prev, double* depth64,
-                           unsigned long long* ctr, hipStream_t s);
+// Device buffers of a primary trace.
+struct TraceBuffers {
+    float* depth;
+    float4* aux;
+    double* depth64;
+    uint32_t* steps;
+    float* t_prev;             // per pixel: march position before the hit (refine bracket)
+    uint32_t* hit_list;        // per pixel capacity: pixels that hit, for the FP64 refine
+    uint32_t* counters;        // [0] ray counter of the march, [1] length of hit_list
+    unsigned long long* ctr;   // work counters (CTR_*)
+};
+
+hipError_t launch_trace_primary(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
+hipError_t launch_refine64(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
 hipError_t launch_de_probe(const FractalP& F, const float4* pts, uint32_t n, float4* out, float4* traps,
                            hipStream_t s);
 

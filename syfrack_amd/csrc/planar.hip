@@ -1,0 +1,597 @@
+// 2D escape-time kernels: PlanarFractal::sample (planar.rs:77-194, planar.wgsl:16-91) for
+// the parameter-plane families, in three precisions:
+//   FP32     direct iteration in f32, the reference's arithmetic (radius = |z| by sqrt);
+//   FP64     direct iteration in f64 with the squared escape test |z|^2 > bailout^2, so
+//            every operation is an exactly rounded IEEE add or multiply and the iteration
+//            counts are bit-exact against any IEEE FP64 evaluation of the same loop;
+//   PERTURB  deep zoom: an FP64 reference orbit Z_n (host) and per-pixel FP32 deltas
+//            delta <- (2 Z + delta) delta + delta_c (Burning Ship: with diffabs).  A pixel
+//            whose orbit loses precision against the reference (Pauldelbrot's criterion
+//            |Z + delta| < 1e-3 |Z|) or outlives it is a glitch: it is queued (wave64 ballot,
+//            one atomic per wave) and re-run against a new reference, the glitched pixel that
+//            got furthest; after 8 references the few left take the FP64 direct loop.
+// Each wave owns an 8x8 pixel tile (neighbours escape at similar counts).  The pixel
+// offsets are host-computed FP64 tables (one per column, one per row), so the device and
+// any checker derive the same c from the same doubles.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "device.hpp"
+
+namespace syf {
+
+struct PlanarP {
+    uint32_t family, power, iterations, width, height, tiles_x, tiles;
+    uint32_t orbit_len;  // PERTURB: index of the last orbit entry
+    double cx, cy;       // centre
+    double cr, sr;       // rotation cos, sin
+    double jx, jy;       // Julia constant
+    float bailout;
+    float b2f;           // bailout^2 in f32 (PERTURB escape test)
+    double b2;           // bailout^2 in f64
+};
+
+__device__ __forceinline__ void pixel_of(const PlanarP& P, uint32_t& px, uint32_t& py, bool& valid) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t tile = blockIdx.x * 4u + wave;
+    px = (tile % P.tiles_x) * 8u + (lane & 7u);
+    py = (tile / P.tiles_x) * 8u + (lane >> 3);
+    valid = tile < P.tiles && px < P.width && py < P.height;
+}
+
+__device__ __forceinline__ void offset_of(const PlanarP& P, const double* fx, const double* fy, uint32_t px,
+                                          uint32_t py, double& dx, double& dy) {
+    const double a = fx[px], b = fy[py];
+    dx = P.cr * a - P.sr * b;
+    dy = P.sr * a + P.cr * b;
+}
+
+__device__ __forceinline__ void flush_iters(uint32_t it, uint32_t rebases, unsigned long long* ctr) {
+    uint32_t a = wave_sum(it), b = wave_sum(rebases);
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(&ctr[0], (unsigned long long)a);
+        atomicAdd(&ctr[1], (unsigned long long)b);
+    }
+}
+
+// FP32 direct (planar.rs:107-164 restricted to the polynomial parameter-plane/Julia path,
+// twist 0, no inversion).
+template <uint32_t FAMILY>
+__global__ __launch_bounds__(256) void k_planar_fp32(const PlanarP P, const double* __restrict__ fx,
+                                                     const double* __restrict__ fy, float* __restrict__ smooth,
+                                                     uint32_t* __restrict__ iters, unsigned long long* ctr) {
+    uint32_t px, py;
+    bool valid;
+    pixel_of(P, px, py, valid);
+    uint32_t count = 0;
+    if (valid) {
+        double dx, dy;
+        offset_of(P, fx, fy, px, py, dx, dy);
+        const float ptx = float(P.cx + dx), pty = float(P.cy + dy);
+        const bool pp = FAMILY != SYF_PLANAR_JULIA;
+        const float cx = pp ? ptx : float(P.jx), cy = pp ? pty : float(P.jy);
+        float zx = pp ? 0.0f : ptx, zy = pp ? 0.0f : pty;
+        bool escaped = false;
+        for (uint32_t i = 0; i < P.iterations; ++i) {
+            float radius = sqrtf(zx * zx + zy * zy);
+            if (radius > P.bailout) {
+                escaped = true;
+                break;
+            }
+            float bx = zx, by = zy;
+            if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
+                bx = fabsf(bx);
+                by = fabsf(by);
+            }
+            float wx = 1.0f, wy = 0.0f;  // complex_power: repeated multiplication from 1
+            for (uint32_t k = 0; k < P.power; ++k) {
+                float nx = wx * bx - wy * by, ny = wx * by + wy * bx;
+                wx = nx;
+                wy = ny;
+            }
+            zx = wx + cx;
+            zy = wy + cy;
+            ++count;
+        }
+        float radius = fminf(sqrtf(zx * zx + zy * zy), 1e20f);
+        escaped = escaped || radius > P.bailout;
+        float s = float(P.iterations);
+        if (escaped) s = float(count) + 1.0f - logf(logf(fmaxf(radius, 1.0001f))) / logf(float(P.power));
+        const size_t i = size_t(py) * P.width + px;
+        smooth[i] = s;
+        iters[i] = count;
+    }
+    flush_iters(count, 0u, ctr);
+}
+
+// FP64 direct: the squared escape test, exact IEEE operations only.
+template <uint32_t FAMILY>
+__global__ __launch_bounds__(256) void k_planar_fp64(const PlanarP P, const double* __restrict__ fx,
+                                                     const double* __restrict__ fy, float* __restrict__ smooth,
+                                                     uint32_t* __restrict__ iters, unsigned long long* ctr) {
+    uint32_t px, py;
+    bool valid;
+    pixel_of(P, px, py, valid);
+    uint32_t count = 0;
+    if (valid) {
+        double dx, dy;
+        offset_of(P, fx, fy, px, py, dx, dy);
+        const double ptx = P.cx + dx, pty = P.cy + dy;
+        const bool pp = FAMILY != SYF_PLANAR_JULIA;
+        const double cx = pp ? ptx : P.jx, cy = pp ? pty : P.jy;
+        double zx = pp ? 0.0 : ptx, zy = pp ? 0.0 : pty;
+        bool escaped = false;
+        for (uint32_t i = 0; i < P.iterations; ++i) {
+            double r2 = zx * zx + zy * zy;
+            if (r2 > P.b2) {
+                escaped = true;
+                break;
+            }
+            double x = zx, y = zy;
+            if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
+                x = x < 0.0 ? -x : x;
+                y = y < 0.0 ? -y : y;
+            }
+            double nx = x * x - y * y + cx;
+            double ny = 2.0 * x * y + cy;
+            zx = nx;
+            zy = ny;
+            ++count;
+        }
+        double r2 = zx * zx + zy * zy;
+        escaped = escaped || r2 > P.b2;
+        double r = fmin(sqrt(r2), 1e20);
+        float s = float(P.iterations);
+        if (escaped) s = float(double(count) + 1.0 - log(log(fmax(r, 1.0001))) / log(2.0));
+        const size_t i = size_t(py) * P.width + px;
+        smooth[i] = s;
+        iters[i] = count;
+    }
+    flush_iters(count, 0u, ctr);
+}
+
+// |c + d| - |c| without cancellation (the Burning Ship perturbation of |x|).
+__device__ __forceinline__ float diffabs(float c, float d) {
+    const float cd = c + d;
+    if (c >= 0.0f) return cd >= 0.0f ? d : -(2.0f * c + d);
+    return cd > 0.0f ? 2.0f * c + d : -d;
+}
+
+// Glitch queue and reference choice of one perturbation pass.
+struct PerturbCtl {
+    uint32_t glitches;        // length of the output queue
+    uint32_t pad;
+    unsigned long long best;  // max over glitches of (iterations << 32 | ~pixel)
+};
+
+// Perturbation pass against one reference orbit (rounded to f32 pairs), over the pixels of
+// `list` (all pixels when list is null).  ref_dx/ref_dy: the reference's offset from the view
+// centre, so a pixel's delta_c = its offset - the reference offset (FP64, then rounded).
+template <uint32_t FAMILY>
+__global__ __launch_bounds__(256) void k_planar_perturb(const PlanarP P, const double* __restrict__ fx,
+                                                        const double* __restrict__ fy,
+                                                        const float2* __restrict__ orbit, uint32_t orbit_len,
+                                                        double ref_dx, double ref_dy,
+                                                        const uint32_t* __restrict__ list, uint32_t n_items,
+                                                        uint32_t* __restrict__ glitch_out, PerturbCtl* __restrict__ pc,
+                                                        float* __restrict__ smooth, uint32_t* __restrict__ iters,
+                                                        unsigned long long* ctr) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t done_iters = 0, glitched_total = 0;
+    // lanes of a wave stride together so the ballot below is wave-wide
+    for (uint32_t b0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b0 < n_items; b0 += stride) {
+        const uint32_t item = b0 + lane;
+        bool glitch = false;
+        uint32_t pix = 0, count = 0;
+        if (item < n_items) {
+            pix = list ? list[item] : item;
+            const uint32_t px = pix % P.width, py = pix / P.width;
+            double ddx, ddy;
+            offset_of(P, fx, fy, px, py, ddx, ddy);
+            const float dcx = float(ddx - ref_dx), dcy = float(ddy - ref_dy);
+            float ex = 0.0f, ey = 0.0f;
+            bool escaped = false;
+            float zx = 0.0f, zy = 0.0f;
+            for (uint32_t n = 0; n < P.iterations; ++n) {
+                const float2 Z = orbit[n];
+                zx = Z.x + ex;
+                zy = Z.y + ey;
+                const float r2 = zx * zx + zy * zy;
+                if (r2 > P.b2f) {
+                    escaped = true;
+                    break;
+                }
+                // precision lost against the reference, or the reference has escaped
+                if (r2 < 1e-6f * (Z.x * Z.x + Z.y * Z.y) || n == orbit_len) {
+                    glitch = true;
+                    break;
+                }
+                float nx, ny;
+                if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
+                    nx = (2.0f * Z.x + ex) * ex - (2.0f * Z.y + ey) * ey + dcx;
+                    const float XY = Z.x * Z.y;
+                    const float dXY = (Z.x + ex) * ey + ex * Z.y;  // (X+ex)(Y+ey) - XY
+                    ny = 2.0f * diffabs(XY, dXY) + dcy;
+                } else {
+                    const float tx = 2.0f * Z.x + ex, ty = 2.0f * Z.y + ey;
+                    nx = tx * ex - ty * ey + dcx;
+                    ny = tx * ey + ty * ex + dcy;
+                }
+                ex = nx;
+                ey = ny;
+                ++count;
+            }
+            if (!glitch && !escaped && count == P.iterations && P.iterations <= orbit_len) {
+                const float2 Z = orbit[count];
+                zx = Z.x + ex;
+                zy = Z.y + ey;
+                escaped = zx * zx + zy * zy > P.b2f;
+            }
+            if (!glitch) {
+                float sm = float(P.iterations);
+                if (escaped) {
+                    float r = fminf(sqrtf(zx * zx + zy * zy), 1e20f);
+                    sm = float(count) + 1.0f - logf(logf(fmaxf(r, 1.0001f))) / logf(2.0f);
+                }
+                smooth[pix] = sm;
+                iters[pix] = count;
+            }
+            done_iters += count;
+        }
+        const unsigned long long mask = __ballot(glitch);
+        const uint32_t total = uint32_t(__popcll(mask));
+        uint32_t base = 0;
+        if (lane == 0u && total) base = atomicAdd(&pc->glitches, total);
+        base = __shfl(base, 0, 64);
+        if (glitch) {
+            glitch_out[base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = pix;
+            atomicMax(&pc->best, ((unsigned long long)count << 32) | (unsigned long long)(0xffffffffu - pix));
+            ++glitched_total;
+        }
+    }
+    flush_iters(done_iters, glitched_total, ctr);
+}
+
+// FP64 direct loop over a pixel list: the last resort for glitches left after the
+// reference budget.
+template <uint32_t FAMILY>
+__global__ __launch_bounds__(256) void k_planar_fp64_list(const PlanarP P, const double* __restrict__ fx,
+                                                          const double* __restrict__ fy,
+                                                          const uint32_t* __restrict__ list, uint32_t n_items,
+                                                          float* __restrict__ smooth, uint32_t* __restrict__ iters,
+                                                          unsigned long long* ctr) {
+    uint32_t total = 0;
+    for (uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
+        const uint32_t pix = list[item];
+        const uint32_t px = pix % P.width, py = pix / P.width;
+        double dx, dy;
+        offset_of(P, fx, fy, px, py, dx, dy);
+        const double cx = P.cx + dx, cy = P.cy + dy;
+        double zx = 0.0, zy = 0.0;
+        uint32_t count = 0;
+        bool escaped = false;
+        for (uint32_t i = 0; i < P.iterations; ++i) {
+            double r2 = zx * zx + zy * zy;
+            if (r2 > P.b2) {
+                escaped = true;
+                break;
+            }
+            double x = zx, y = zy;
+            if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
+                x = x < 0.0 ? -x : x;
+                y = y < 0.0 ? -y : y;
+            }
+            double nx = x * x - y * y + cx;
+            double ny = 2.0 * x * y + cy;
+            zx = nx;
+            zy = ny;
+            ++count;
+        }
+        double r2 = zx * zx + zy * zy;
+        escaped = escaped || r2 > P.b2;
+        float sm = float(P.iterations);
+        if (escaped) sm = float(double(count) + 1.0 - log(log(fmax(fmin(sqrt(r2), 1e20), 1.0001))) / log(2.0));
+        smooth[pix] = sm;
+        iters[pix] = count;
+        total += count;
+    }
+    uint32_t a = wave_sum(total);
+    if ((threadIdx.x & 63u) == 0u) atomicAdd(&ctr[0], (unsigned long long)a);
+}
+
+}  // namespace syf
+
+using namespace syf;
+
+struct syf_planar {
+    syf_ctx* ctx = nullptr;
+    syf_planar_desc desc{};
+    PlanarP P{};
+    hipStream_t stream = nullptr;
+    double* fx = nullptr;
+    double* fy = nullptr;
+    float* smooth = nullptr;
+    uint32_t* iters = nullptr;
+    float2* orbit = nullptr;
+    size_t orbit_cap = 0;
+    uint32_t* list[2] = {nullptr, nullptr};  // PERTURB glitch queues (ping-pong)
+    PerturbCtl* pc = nullptr;
+    std::vector<double> hfx, hfy;           // host copies of the offset tables
+    uint32_t references = 0;                // references used by the last PERTURB run
+    uint32_t fallback = 0;                  // pixels left to the FP64 loop by the last run
+    unsigned long long* ctr = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+};
+
+namespace {
+
+void planar_free(syf_planar* p) {
+    if (!p) return;
+    if (p->stream) hipStreamSynchronize(p->stream);
+    hipFree(p->fx);
+    hipFree(p->fy);
+    hipFree(p->smooth);
+    hipFree(p->iters);
+    hipFree(p->orbit);
+    hipFree(p->list[0]);
+    hipFree(p->list[1]);
+    hipFree(p->pc);
+    hipFree(p->ctr);
+    if (p->ev0) hipEventDestroy(p->ev0);
+    if (p->ev1) hipEventDestroy(p->ev1);
+    if (p->stream) hipStreamDestroy(p->stream);
+    delete p;
+}
+
+// FP64 reference orbit of C = centre + (ox, oy): Z_0 = 0, Z_{n+1} = f(Z_n) + C, up to the
+// first entry with |Z|^2 > bailout^2 or the iteration budget; returns the index of its last
+// entry.
+uint32_t reference_orbit(const syf_planar_desc& d, double ox, double oy, std::vector<float2>& out) {
+    const double b2 = double(d.bailout) * double(d.bailout);
+    const double cx = d.center[0] + ox, cy = d.center[1] + oy;
+    double zx = 0.0, zy = 0.0;
+    out.clear();
+    out.reserve(size_t(d.iterations) + 1);
+    out.push_back(make_float2(0.0f, 0.0f));
+    for (uint32_t n = 0; n < d.iterations; ++n) {
+        double x = zx, y = zy;
+        if (d.family == SYF_PLANAR_BURNING_SHIP) {
+            x = std::fabs(x);
+            y = std::fabs(y);
+        }
+        double nx = x * x - y * y + cx;
+        double ny = 2.0 * x * y + cy;
+        zx = nx;
+        zy = ny;
+        out.push_back(make_float2(float(zx), float(zy)));
+        if (zx * zx + zy * zy > b2) break;
+    }
+    return uint32_t(out.size() - 1);
+}
+
+constexpr uint32_t MAX_REFERENCES = 8;
+
+// Perturbation passes: reference 0 is the view centre; each later reference is the glitched
+// pixel of the previous pass that got furthest (ties: lowest pixel index), and only that
+// pass's glitches are re-run.  Glitches left after MAX_REFERENCES take the FP64 loop.
+hipError_t run_perturbation(syf_planar* p) {
+    const syf_planar_desc& d = p->desc;
+    const PlanarP& P = p->P;
+    const uint32_t n = d.width * d.height;
+    const int cus = p->ctx->cu_count > 0 ? p->ctx->cu_count : 256;
+    const dim3 block(256);
+    double ref_dx = 0.0, ref_dy = 0.0;
+    const uint32_t* in = nullptr;  // all pixels
+    uint32_t n_in = n, which = 0;
+    std::vector<float2> orbit;
+    hipError_t e = hipSuccess;
+    p->references = 0;
+    p->fallback = 0;
+    for (uint32_t pass = 0; pass < MAX_REFERENCES && n_in > 0; ++pass) {
+        const uint32_t orbit_len = reference_orbit(d, ref_dx, ref_dy, orbit);
+        e = hipMemcpyAsync(p->orbit, orbit.data(), orbit.size() * sizeof(float2), hipMemcpyHostToDevice, p->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(p->pc, 0, sizeof(PerturbCtl), p->stream);
+        if (e != hipSuccess) return e;
+        const dim3 grid(std::min<uint32_t>((n_in + 255u) / 256u, uint32_t(cus) * 8u));
+        uint32_t* out = p->list[which];
+        if (d.family == SYF_PLANAR_MANDELBROT)
+            hipLaunchKernelGGL(k_planar_perturb<SYF_PLANAR_MANDELBROT>, grid, block, 0, p->stream, P, p->fx, p->fy,
+                               p->orbit, orbit_len, ref_dx, ref_dy, in, n_in, out, p->pc, p->smooth, p->iters, p->ctr);
+        else
+            hipLaunchKernelGGL(k_planar_perturb<SYF_PLANAR_BURNING_SHIP>, grid, block, 0, p->stream, P, p->fx, p->fy,
+                               p->orbit, orbit_len, ref_dx, ref_dy, in, n_in, out, p->pc, p->smooth, p->iters, p->ctr);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        PerturbCtl pc{};
+        e = hipMemcpyAsync(&pc, p->pc, sizeof pc, hipMemcpyDeviceToHost, p->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+        if (e != hipSuccess) return e;
+        p->references = pass + 1;
+        n_in = pc.glitches;
+        in = out;
+        which ^= 1u;
+        if (n_in == 0) break;
+        const uint32_t pix = 0xffffffffu - uint32_t(pc.best & 0xffffffffull);
+        const uint32_t px = pix % d.width, py = pix / d.width;
+        ref_dx = P.cr * p->hfx[px] - P.sr * p->hfy[py];
+        ref_dy = P.sr * p->hfx[px] + P.cr * p->hfy[py];
+    }
+    if (n_in > 0) {
+        p->fallback = n_in;
+        const dim3 grid(std::min<uint32_t>((n_in + 255u) / 256u, uint32_t(cus) * 8u));
+        if (d.family == SYF_PLANAR_MANDELBROT)
+            hipLaunchKernelGGL(k_planar_fp64_list<SYF_PLANAR_MANDELBROT>, grid, block, 0, p->stream, P, p->fx, p->fy,
+                               in, n_in, p->smooth, p->iters, p->ctr);
+        else
+            hipLaunchKernelGGL(k_planar_fp64_list<SYF_PLANAR_BURNING_SHIP>, grid, block, 0, p->stream, P, p->fx, p->fy,
+                               in, n_in, p->smooth, p->iters, p->ctr);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int syf_planar_create(syf_ctx* ctx, const syf_planar_desc* d, syf_planar** out) {
+    clear_error();
+    if (!ctx || !d || !out) return fail(SYF_E_INVALID, "ctx, desc and out must not be null");
+    if (d->family != SYF_PLANAR_MANDELBROT && d->family != SYF_PLANAR_BURNING_SHIP && d->family != SYF_PLANAR_JULIA)
+        return fail(SYF_E_INVALID, "family: unsupported planar family %u (Julia 1, Mandelbrot 2, Burning Ship 3)",
+                    d->family);
+    if (d->mode > SYF_PLANAR_PERTURB) return fail(SYF_E_INVALID, "mode: unknown mode %u", d->mode);
+    if (d->mode == SYF_PLANAR_PERTURB && d->family == SYF_PLANAR_JULIA)
+        return fail(SYF_E_INVALID, "mode: perturbation is for the parameter-plane families");
+    if (d->width < 1 || d->height < 1 || d->width > 32768 || d->height > 32768)
+        return fail(SYF_E_INVALID, "width x height (%ux%u) must be within 1..32768 per side", d->width, d->height);
+    if (d->iterations < 1 || d->iterations > (1u << 24))
+        return fail(SYF_E_INVALID, "iterations must be between 1 and 2^24, got %u", d->iterations);
+    if (!(d->bailout >= 2.0f && d->bailout <= 1e4f))
+        return fail(SYF_E_INVALID, "bailout must be between 2 and 1e4, got %g", double(d->bailout));
+    if (!(d->span > 0.0) || !std::isfinite(d->span)) return fail(SYF_E_INVALID, "span must be positive and finite");
+    if (!std::isfinite(d->center[0]) || !std::isfinite(d->center[1]))
+        return fail(SYF_E_INVALID, "center must be finite");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    syf_planar* p = new syf_planar();
+    p->ctx = ctx;
+    p->desc = *d;
+    PlanarP& P = p->P;
+    P.family = d->family;
+    P.power = 2;
+    P.iterations = d->iterations;
+    P.width = d->width;
+    P.height = d->height;
+    P.tiles_x = (d->width + 7u) / 8u;
+    P.tiles = P.tiles_x * ((d->height + 7u) / 8u);
+    P.cx = d->center[0];
+    P.cy = d->center[1];
+    const double rot = double(d->rotation) * (3.14159265358979323846 / 180.0);
+    P.cr = std::cos(rot);
+    P.sr = std::sin(rot);
+    P.jx = d->julia[0];
+    P.jy = d->julia[1];
+    P.bailout = d->bailout;
+    P.b2f = d->bailout * d->bailout;
+    P.b2 = double(d->bailout) * double(d->bailout);
+    const size_t n = size_t(d->width) * d->height;
+    std::vector<double> fx(d->width), fy(d->height);
+    for (uint32_t i = 0; i < d->width; ++i) fx[i] = ((double(i) + 0.5) / double(d->width) - 0.5) * d->span;
+    const double vspan = d->span * double(d->height) / double(d->width);
+    for (uint32_t j = 0; j < d->height; ++j) fy[j] = (0.5 - (double(j) + 0.5) / double(d->height)) * vspan;
+    if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&p->fx, fx.size() * 8)) != hipSuccess || (e = hipMalloc(&p->fy, fy.size() * 8)) != hipSuccess ||
+        (e = hipMalloc(&p->smooth, n * 4)) != hipSuccess || (e = hipMalloc(&p->iters, n * 4)) != hipSuccess ||
+        (e = hipMalloc(&p->ctr, 16)) != hipSuccess || (e = hipEventCreate(&p->ev0)) != hipSuccess ||
+        (e = hipEventCreate(&p->ev1)) != hipSuccess) {
+        planar_free(p);
+        return hip_fail(e, "syf_planar_create allocation");
+    }
+    if (d->mode == SYF_PLANAR_PERTURB) {
+        p->orbit_cap = size_t(d->iterations) + 1;
+        if ((e = hipMalloc(&p->orbit, p->orbit_cap * sizeof(float2))) != hipSuccess ||
+            (e = hipMalloc(&p->list[0], n * 4)) != hipSuccess || (e = hipMalloc(&p->list[1], n * 4)) != hipSuccess ||
+            (e = hipMalloc(&p->pc, sizeof(PerturbCtl))) != hipSuccess) {
+            planar_free(p);
+            return hip_fail(e, "syf_planar_create perturbation buffers");
+        }
+    }
+    p->hfx = fx;
+    p->hfy = fy;
+    e = hipMemcpy(p->fx, fx.data(), fx.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->fy, fy.data(), fy.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        planar_free(p);
+        return hip_fail(e, "syf_planar_create upload");
+    }
+    *out = p;
+    return SYF_OK;
+}
+
+int syf_planar_destroy(syf_planar* p) {
+    clear_error();
+    planar_free(p);
+    return SYF_OK;
+}
+
+int syf_planar_run(syf_planar* p) {
+    clear_error();
+    if (!p) return fail(SYF_E_INVALID, "planar must not be null");
+    hipSetDevice(p->ctx->device);
+    const syf_planar_desc& d = p->desc;
+    hipError_t e = hipMemsetAsync(p->ctr, 0, 16, p->stream);
+    if (e == hipSuccess) e = hipEventRecord(p->ev0, p->stream);
+    const dim3 grid((p->P.tiles + 3u) / 4u), block(256);
+    const PlanarP& P = p->P;
+    if (e == hipSuccess) {
+        switch (d.mode) {
+        case SYF_PLANAR_FP32:
+            if (d.family == SYF_PLANAR_MANDELBROT)
+                hipLaunchKernelGGL(k_planar_fp32<SYF_PLANAR_MANDELBROT>, grid, block, 0, p->stream, P, p->fx, p->fy, p->smooth, p->iters, p->ctr);
+            else if (d.family == SYF_PLANAR_BURNING_SHIP)
+                hipLaunchKernelGGL(k_planar_fp32<SYF_PLANAR_BURNING_SHIP>, grid, block, 0, p->stream, P, p->fx, p->fy, p->smooth, p->iters, p->ctr);
+            else
+                hipLaunchKernelGGL(k_planar_fp32<SYF_PLANAR_JULIA>, grid, block, 0, p->stream, P, p->fx, p->fy, p->smooth, p->iters, p->ctr);
+            break;
+        case SYF_PLANAR_FP64:
+            if (d.family == SYF_PLANAR_MANDELBROT)
+                hipLaunchKernelGGL(k_planar_fp64<SYF_PLANAR_MANDELBROT>, grid, block, 0, p->stream, P, p->fx, p->fy, p->smooth, p->iters, p->ctr);
+            else if (d.family == SYF_PLANAR_BURNING_SHIP)
+                hipLaunchKernelGGL(k_planar_fp64<SYF_PLANAR_BURNING_SHIP>, grid, block, 0, p->stream, P, p->fx, p->fy, p->smooth, p->iters, p->ctr);
+            else
+                hipLaunchKernelGGL(k_planar_fp64<SYF_PLANAR_JULIA>, grid, block, 0, p->stream, P, p->fx, p->fy, p->smooth, p->iters, p->ctr);
+            break;
+        default:
+            e = run_perturbation(p);
+            break;
+        }
+        if (e == hipSuccess) e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(p->ev1, p->stream);
+    p->timed = true;
+    return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_planar_run");
+}
+
+int syf_planar_sync(syf_planar* p) {
+    clear_error();
+    if (!p) return fail(SYF_E_INVALID, "planar must not be null");
+    hipError_t e = hipStreamSynchronize(p->stream);
+    return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_planar_sync");
+}
+
+int syf_planar_readback(syf_planar* p, uint32_t plane, void* host, size_t bytes) {
+    clear_error();
+    if (!p || !host) return fail(SYF_E_INVALID, "planar and host must not be null");
+    if (plane > SYF_PLANAR_ITER) return fail(SYF_E_INVALID, "plane: unknown planar plane %u", plane);
+    const size_t need = size_t(p->desc.width) * p->desc.height * 4;
+    if (bytes != need) return fail(SYF_E_INVALID, "bytes must be %zu, got %zu", need, bytes);
+    hipSetDevice(p->ctx->device);
+    hipError_t e = hipMemcpyAsync(host, plane == SYF_PLANAR_SMOOTH ? (void*)p->smooth : (void*)p->iters, bytes,
+                                  hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_planar_readback");
+}
+
+int syf_planar_stats(syf_planar* p, uint64_t* iterations, uint64_t* rebases, double* kernel_ms) {
+    clear_error();
+    if (!p) return fail(SYF_E_INVALID, "planar must not be null");
+    unsigned long long c[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(c, p->ctr, 16, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return hip_fail(e, "syf_planar_stats");
+    float ms = 0.0f;
+    if (p->timed && hipEventElapsedTime(&ms, p->ev0, p->ev1) != hipSuccess) ms = 0.0f;
+    if (iterations) *iterations = c[0];
+    if (rebases) *rebases = c[1];
+    if (kernel_ms) *kernel_ms = ms;
+    return SYF_OK;
+}
+
+}  // extern "C"

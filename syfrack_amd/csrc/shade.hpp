@@ -1,0 +1,410 @@
+// Device shading for the path-traced forms: the radiance estimator of pathtrace.wgsl:802-1201
+// restricted to this path's scene subset (area and environment lights, a base material with
+// an optional trap-driven SurfaceVariation giving Lambert + GGX metal; fog, sun, HDRI,
+// portals, ocean, optics, textures, guiding and phenomena are absent, so their branches are
+// inert).  The random draws happen in the reference's order so that a path is a pure function
+// of (pixel, sample, seed), whichever kernel form runs it.
+#pragma once
+
+#include "device.hpp"
+
+namespace syf {
+
+typedef V3<float> F3;
+constexpr float PI_F = 3.14159265358979f;
+constexpr uint32_t NO_LIGHT = 0xffffffffu;
+
+__device__ __forceinline__ F3 mul3(F3 a, F3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ F3 neg3(F3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float luminance(F3 c) { return dot3(c, mk(0.2126f, 0.7152f, 0.0722f)); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ F3 mix3(F3 a, F3 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)}; }
+__device__ __forceinline__ F3 safe_normalize(F3 v) {  // common.wgsl:242-247
+    float l = len3(v);
+    float rcp = 1.0f / l;
+    if (!(l > 0.0f) || !(rcp > 0.0f) || !(rcp <= 3.4028235e38f)) return mk(0.0f, 1.0f, 0.0f);
+    return scl(v, rcp);
+}
+
+struct Surf {
+    F3 n, diffuse, f0;
+    float alpha;
+};
+
+// Duff et al. 2017 orthonormal basis (pathtrace.wgsl:194-201); columns t, bt, n.
+__device__ __forceinline__ void tangent_frame(F3 n, F3& t, F3& bt) {
+    float s = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n.z);
+    float b = n.x * n.y * a;
+    t = mk(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
+    bt = mk(b, s + n.y * n.y * a, -n.y);
+}
+__device__ __forceinline__ F3 frame_mul(F3 t, F3 bt, F3 n, F3 x) { return add(add(scl(t, x.x), scl(bt, x.y)), scl(n, x.z)); }
+
+__device__ __forceinline__ F3 cosine_sample(F3 n, float r1, float r2) {  // pathtrace.wgsl:203-208
+    float phi = 2.0f * PI_F * r1;
+    float r = sqrtf(r2);
+    F3 t, bt;
+    tangent_frame(n, t, bt);
+    return frame_mul(t, bt, n, mk(r * cosf(phi), r * sinf(phi), sqrtf(fmaxf(0.0f, 1.0f - r2))));
+}
+__device__ __forceinline__ float ggx_d(float nh, float a2) {
+    float d = nh * nh * (a2 - 1.0f) + 1.0f;
+    return a2 / (PI_F * d * d);
+}
+__device__ __forceinline__ float smith_g1(float nv, float a2) { return 2.0f * nv / (nv + sqrtf(a2 + (1.0f - a2) * nv * nv)); }
+__device__ __forceinline__ F3 fresnel(F3 f0, float vh) {
+    float k = powf(1.0f - vh, 5.0f);
+    return mk(f0.x + (1.0f - f0.x) * k, f0.y + (1.0f - f0.y) * k, f0.z + (1.0f - f0.z) * k);
+}
+// Heitz 2018 GGX visible-normal sampling (pathtrace.wgsl:234-249).
+__device__ __forceinline__ F3 sample_ggx_h(F3 n, F3 v, float alpha, float r1, float r2) {
+    F3 t, bt;
+    tangent_frame(n, t, bt);
+    F3 lv = mk(dot3(t, v), dot3(bt, v), dot3(n, v));
+    F3 vh = safe_normalize(mk(alpha * lv.x, alpha * lv.y, fmaxf(lv.z, 1e-5f)));
+    float lensq = vh.x * vh.x + vh.y * vh.y;
+    F3 t1 = mk(1.0f, 0.0f, 0.0f);
+    if (lensq > 0.0f) {
+        float sl = sqrtf(lensq);
+        t1 = mk(-vh.y / sl, vh.x / sl, 0.0f);
+    }
+    F3 t2 = cross3(vh, t1);
+    float r = sqrtf(r1);
+    float phi = 2.0f * PI_F * r2;
+    float x = r * cosf(phi);
+    float blend = 0.5f * (1.0f + vh.z);
+    float y = mixf(sqrtf(fmaxf(0.0f, 1.0f - x * x)), r * sinf(phi), blend);
+    F3 nh = add(add(scl(t1, x), scl(t2, y)), scl(vh, sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y))));
+    return frame_mul(t, bt, n, safe_normalize(mk(alpha * nh.x, alpha * nh.y, fmaxf(nh.z, 0.0f))));
+}
+__device__ __forceinline__ F3 reflect3(F3 e1, F3 e2) { return sub(e1, scl(e2, 2.0f * dot3(e2, e1))); }
+
+// make_surface subset (pathtrace.wgsl:271-386).
+__device__ __forceinline__ Surf make_surface(const MaterialP& m, F3 n, float trap0) {
+    float rough = m.roughness, metallic = m.metallic, spec = m.specular;
+    if (m.surface_enabled) {
+        float value = m.surface_slot == 0u ? trap0 : 1.0f;  // the classic graphs write slot 0 only
+        float t = clampT(value, 0.0f, 1.0f);
+        rough = m.surface[0] + (m.surface[1] - m.surface[0]) * t;
+        metallic = m.surface[2] + (m.surface[3] - m.surface[2]) * t;
+    }
+    rough = clampT(rough, 0.02f, 1.0f);
+    F3 albedo = ld3(m.albedo);
+    Surf s;
+    s.n = n;
+    s.diffuse = scl(albedo, 1.0f - metallic);
+    s.f0 = mix3(mk(spec, spec, spec), albedo, metallic);
+    s.alpha = rough * rough;
+    return s;
+}
+// brdf_eval (pathtrace.wgsl:394-420) without the Oren-Nayar term.
+__device__ __forceinline__ F3 brdf_eval(const Surf& s, F3 v, F3 l) {
+    float nl = dot3(s.n, l), nv = dot3(s.n, v);
+    if (nl <= 0.0f || nv <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    F3 h = nrm(add(v, l));
+    float nh = fmaxf(dot3(s.n, h), 0.0f);
+    float vh = fmaxf(dot3(v, h), 0.0f);
+    float a2 = s.alpha * s.alpha;
+    F3 f = fresnel(s.f0, vh);
+    float k = ggx_d(nh, a2) * smith_g1(nv, a2) * smith_g1(nl, a2);
+    float den = fmaxf(4.0f * nv * nl, 1e-4f);
+    return mk(s.diffuse.x * (1.0f - f.x) * 1.0f / PI_F + k * f.x / den,
+              s.diffuse.y * (1.0f - f.y) * 1.0f / PI_F + k * f.y / den,
+              s.diffuse.z * (1.0f - f.z) * 1.0f / PI_F + k * f.z / den);
+}
+__device__ __forceinline__ float mis_power(float a, float b) {
+    float scale = fmaxf(a, b);
+    if (scale <= 0.0f) return 0.0f;
+    float pa = a / scale, pb = b / scale;
+    return pa * pa / (pa * pa + pb * pb);
+}
+__device__ __forceinline__ float spec_probability(const Surf& s, float nv) {
+    F3 f = fresnel(s.f0, nv);
+    return clampT(luminance(f) / fmaxf(luminance(f) + luminance(s.diffuse), 1e-4f), 0.05f, 0.95f);
+}
+// bsdf_pdf (pathtrace.wgsl:443-452) = continuation_pdf without guiding or transmission.
+__device__ __forceinline__ float bsdf_pdf(const Surf& s, F3 v, F3 l, float p_spec) {
+    float nl = dot3(s.n, l);
+    if (nl <= 0.0f) return 0.0f;
+    F3 hh = nrm(add(v, l));
+    float nh = fmaxf(dot3(s.n, hh), 0.0f);
+    float a2 = s.alpha * s.alpha;
+    float nv = fmaxf(dot3(s.n, v), 1e-4f);
+    float pdf_spec = ggx_d(nh, a2) * smith_g1(nv, a2) / (4.0f * nv);
+    return p_spec * pdf_spec + (1.0f - p_spec) * nl / PI_F;
+}
+__device__ __forceinline__ F3 environment_value(const LightP& l, F3 dir) {  // pathtrace.wgsl:164-167
+    if (dir.y >= 0.0f) return mix3(ld3(l.horizon), ld3(l.color), sqrtf(dir.y));
+    return mix3(ld3(l.horizon), ld3(l.ground), sqrtf(-dir.y));
+}
+__device__ __forceinline__ float environment_probability(const LightP& l, float area_total) {
+    float energy = fmaxf(fmaxf(luminance(ld3(l.color)), luminance(ld3(l.horizon))), luminance(ld3(l.ground)));
+    if (energy <= 0.0f) return 0.0f;
+    return clampT(4.0f * PI_F * energy / (4.0f * PI_F * energy + area_total), 0.05f, 1.0f);
+}
+__device__ __forceinline__ float area_intersection(F3 ro, F3 rd, const LightP& l) {  // lights.wgsl:170-181
+    F3 n = ld3(l.n);
+    float den = dot3(n, rd);
+    if (fabsf(den) < 1e-7f) return 1e30f;
+    float t = dot3(sub(ld3(l.a), ro), n) / den;
+    if (t <= 1e-6f) return 1e30f;
+    F3 q = sub(add(ro, scl(rd, t)), ld3(l.a));
+    float x = dot3(q, ld3(l.b)) / dot3(ld3(l.b), ld3(l.b));
+    float y = dot3(q, ld3(l.c)) / dot3(ld3(l.c), ld3(l.c));
+    if (fabsf(x) > 0.5f || fabsf(y) > 0.5f) return 1e30f;
+    return t;
+}
+__device__ __forceinline__ float area_weight(F3 p, F3 n, const LightP& l) {  // lights.wgsl:288-304
+    F3 to = sub(ld3(l.a), p);
+    float dist2 = fmaxf(dot3(to, to), 1e-6f);
+    float sq = sqrtf(dist2);
+    F3 dir = mk(to.x / sq, to.y / sq, to.z / sq);
+    float facing = fmaxf(dot3(ld3(l.n), neg3(dir)), 0.0f);
+    return luminance(ld3(l.color)) * l.area * facing * fmaxf(dot3(n, dir), 0.15f) / dist2;
+}
+__device__ __forceinline__ float area_total_weight(const JobP& P, F3 p, F3 n) {
+    float total = 0.0f;
+    for (uint32_t i = 0; i < P.light_count; ++i)
+        if (P.lights[i].kind == SYF_LIGHT_AREA) total += area_weight(p, n, P.lights[i]);
+    return total;
+}
+// Nearest emitter before t_end (trace_geometry_lights, lights.wgsl:183-209, no ground).
+__device__ __forceinline__ float nearest_emitter(const JobP& P, F3 ro, F3 rd, float t_end, uint32_t& light) {
+    float nearest = t_end;
+    light = NO_LIGHT;
+    for (uint32_t i = 0; i < P.light_count; ++i) {
+        if (P.lights[i].kind != SYF_LIGHT_AREA) continue;
+        float t = area_intersection(ro, rd, P.lights[i]);
+        if (t < nearest) {
+            nearest = t;
+            light = i;
+        }
+    }
+    return nearest;
+}
+__device__ __forceinline__ F3 area_emission(const JobP& P, F3 ro, F3 rd, float distance, float prev_pdf, bool use_mis,
+                                            F3 prev_n, float prev_total) {  // lights.wgsl:330-357
+    F3 L = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < P.light_count; ++i) {
+        const LightP& l = P.lights[i];
+        if (l.kind != SYF_LIGHT_AREA) continue;
+        float cos_l = dot3(ld3(l.n), neg3(rd));
+        if (cos_l <= 0.0f) continue;
+        float t = area_intersection(ro, rd, l);
+        if (fabsf(t - distance) > fmaxf(1e-6f, distance * 1e-6f)) continue;
+        float w = 1.0f;
+        if (use_mis) {
+            float pl = distance * distance / fmaxf(l.area * cos_l, 1e-20f);
+            float sel = 0.0f;
+            if (prev_total > 0.0f) sel = area_weight(ro, prev_n, l) / prev_total;
+            pl *= sel;
+            w = mis_power(prev_pdf, pl);
+        }
+        L = add(L, scl(ld3(l.color), w));
+    }
+    return L;
+}
+__device__ __forceinline__ F3 environment_emission(const JobP& P, F3 dir, float pdf, bool use_mis, F3 n,
+                                                   float area_total) {  // pathtrace.wgsl:177-189
+    F3 L = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < P.light_count; ++i) {
+        const LightP& l = P.lights[i];
+        if (l.kind != SYF_LIGHT_ENV) continue;
+        float w = 1.0f;
+        if (use_mis) w = mis_power(pdf, environment_probability(l, area_total) * (fmaxf(dot3(n, dir), 0.0f) / PI_F));
+        L = add(L, scl(environment_value(l, dir), w));
+    }
+    return L;
+}
+
+// Next-event estimation as direct_light (pathtrace.wgsl:471-558): choose one area light by
+// weight, then the environment and the chosen area light each propose one shadow ray.
+// The caller supplies the visibility test (a march to the light), so the megakernel and
+// the wavefront form share this logic.
+struct ShadowRay {
+    F3 o, d;
+    float t_end;
+    F3 contrib;  // radiance the ray carries if unoccluded (already MIS- and pdf-weighted)
+};
+
+template <class Vis>
+__device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F3 gn, F3 v, float p_spec,
+                                           float area_total, bool last, uint32_t& rng, Vis&& visible) {
+    F3 L = mk(0.0f, 0.0f, 0.0f);
+    uint32_t selected = NO_LIGHT;
+    float selected_weight = 0.0f;
+    if (area_total > 0.0f) {
+        float threshold = rand01(rng) * area_total;
+        float sum = 0.0f;
+        for (uint32_t i = 0; i < P.light_count; ++i) {
+            if (P.lights[i].kind != SYF_LIGHT_AREA) continue;
+            float w = area_weight(p, s.n, P.lights[i]);
+            sum += w;
+            if (w > 0.0f) {
+                selected = i;
+                selected_weight = w;
+                if (threshold < sum) break;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < P.light_count; ++i) {
+        const LightP& l = P.lights[i];
+        if (l.kind == SYF_LIGHT_ENV) {
+            float mx = fmaxf(fmaxf(fmaxf(l.color[0], l.color[1]), l.color[2]),
+                             fmaxf(fmaxf(fmaxf(l.horizon[0], l.horizon[1]), l.horizon[2]),
+                                   fmaxf(fmaxf(l.ground[0], l.ground[1]), l.ground[2])));
+            if (mx <= 0.0f) continue;
+            float prob = environment_probability(l, area_total);
+            if (rand01(rng) >= prob) continue;
+            float r1 = rand01(rng);
+            float r2 = rand01(rng);
+            F3 dir = cosine_sample(s.n, r1, r2);
+            float nl = fmaxf(dot3(s.n, dir), 0.0f);
+            if (dot3(gn, dir) <= 0.0f || nl <= 0.0f) continue;
+            if (!visible(p, dir, P.t_end)) continue;
+            float pl = prob * nl / PI_F;
+            float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
+            float w = last ? 1.0f : mis_power(pl, pb);
+            F3 f = brdf_eval(s, v, dir);
+            F3 e = environment_value(l, dir);
+            L = add(L, mk(f.x * nl * e.x * 1.0f * w / pl, f.y * nl * e.y * 1.0f * w / pl, f.z * nl * e.z * 1.0f * w / pl));
+        } else if (l.kind == SYF_LIGHT_AREA) {
+            if (i != selected) continue;
+            float u1 = rand01(rng);
+            float u2 = rand01(rng);
+            F3 q = add(add(ld3(l.a), scl(ld3(l.b), u1 - 0.5f)), scl(ld3(l.c), u2 - 0.5f));
+            F3 to_q = sub(q, p);
+            float dist2 = dot3(to_q, to_q);
+            if (dist2 < 1e-8f) continue;
+            float dist = sqrtf(dist2);
+            F3 dir = mk(to_q.x / dist, to_q.y / dist, to_q.z / dist);
+            float nl = dot3(s.n, dir);
+            float cos_l = dot3(ld3(l.n), neg3(dir));
+            if (nl <= 0.0f || cos_l <= 0.0f || dot3(gn, dir) <= 0.0f) continue;
+            float end = fmaxf(0.0f, dist - fmaxf(1e-5f, dist * 1e-6f));  // shadow_endpoint (lights.wgsl:253-255)
+            if (!visible(p, dir, end)) continue;
+            float pl = (selected_weight / area_total) * (dist2 / fmaxf(l.area * cos_l, 1e-20f));
+            float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
+            float w = last ? 1.0f : mis_power(pl, pb);
+            F3 f = brdf_eval(s, v, dir);
+            L = add(L, mk(f.x * nl * l.color[0] * 1.0f * w / pl, f.y * nl * l.color[1] * 1.0f * w / pl,
+                          f.z * nl * l.color[2] * 1.0f * w / pl));
+        }
+    }
+    return L;
+}
+
+// Path state carried between bounces (the wavefront form keeps it in SoA device planes).
+struct PathState {
+    F3 o, d, throughput, L, prev_n;
+    float prev_pdf, prev_total;
+    uint32_t rng, bounce;
+    bool prev_from_bsdf, alive;
+};
+
+__device__ __forceinline__ void path_start(const JobP& P, uint32_t px, uint32_t py, uint32_t sample, PathState& st) {
+    st.rng = rng_init(px, py, sample, P.seed);
+    camera_ray(P.cam, P.width, P.height, px, py, st.rng, st.o, st.d);
+    st.throughput = mk(1.0f, 1.0f, 1.0f);
+    st.L = mk(0.0f, 0.0f, 0.0f);
+    st.prev_n = mk(0.0f, 0.0f, 0.0f);
+    st.prev_pdf = 0.0f;
+    st.prev_total = 0.0f;
+    st.bounce = 0;
+    st.prev_from_bsdf = false;
+    st.alive = true;
+}
+
+// One segment's outcome as the march and emitter test leave it.
+struct SegHit {
+    bool hit, exhausted;
+    float t;
+    uint32_t steps, light;
+    float trap0;
+};
+
+// The body of one bounce after its segment was traced (pathtrace.wgsl:1012-1196 subset):
+// emission on a miss or an emitter hit, otherwise the surface, next-event estimation and the
+// continuation sample.  `normal` evaluates the field gradient at a surface point; `visible`
+// marches a shadow ray.  Sets st.alive = false when the path ends.
+template <class Normal, class Vis>
+__device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const SegHit& h, float& first_depth,
+                                            float4& first_aux, Normal&& normal, Vis&& visible) {
+    F3 gn = mk(0.0f, 0.0f, 0.0f);
+    if (h.hit) {
+        if (h.light != NO_LIGHT) gn = ld3(P.lights[h.light].n);
+        else gn = normal(add(st.o, scl(st.d, h.t)), fmaxf(h.t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance);
+        if (st.bounce == 0u) {
+            first_depth = h.t;
+            first_aux = make_float4(gn.x, gn.y, gn.z, float(h.steps));
+        }
+    }
+    if (!h.hit) {
+        if (!h.exhausted)
+            st.L = add(st.L, mul3(st.throughput,
+                                  environment_emission(P, st.d, st.prev_pdf, st.prev_from_bsdf, st.prev_n, st.prev_total)));
+        st.alive = false;
+        return;
+    }
+    const F3 p = add(st.o, scl(st.d, h.t));
+    if (h.light != NO_LIGHT) {
+        st.L = add(st.L, mul3(st.throughput, area_emission(P, st.o, st.d, h.t, st.prev_pdf, st.prev_from_bsdf,
+                                                           st.prev_n, st.prev_total)));
+        st.alive = false;
+        return;
+    }
+    const F3 n = gn;
+    if (st.bounce == 0u) {
+        float t = clampT(float(h.steps) / 96.0f, 0.0f, 1.0f);
+        st.throughput = scl(st.throughput, 1.0f - P.cavity_darkness * (t * t * (3.0f - 2.0f * t)));
+    }
+    const F3 v = neg3(st.d);
+    const Surf s = make_surface(P.mat, n, h.trap0);
+    const float offset = fmaxf(h.t * P.pixel_cone * 2.0f, 1e-3f) * P.pixel_tolerance;
+    const F3 p_off = add(p, scl(n, offset));
+    const float nv = fmaxf(dot3(s.n, v), 1e-4f);
+    const float p_spec = spec_probability(s, nv);
+    const float area_total = area_total_weight(P, p_off, s.n);
+    const bool last = st.bounce == P.max_bounces;
+    F3 dl = direct_light(P, p_off, s, n, v, p_spec, area_total, last, st.rng, visible);
+    st.L = add(st.L, mul3(st.throughput, dl));
+    if (last) {
+        st.alive = false;
+        return;
+    }
+    float r0 = rand01(st.rng), r1 = rand01(st.rng), r2 = rand01(st.rng);
+    F3 l;
+    if (r0 < p_spec) l = reflect3(neg3(v), sample_ggx_h(s.n, v, s.alpha, r1, r2));
+    else l = cosine_sample(s.n, r1, r2);
+    float nl = dot3(s.n, l);
+    if (nl <= 0.0f || dot3(n, l) <= 0.0f) {
+        st.alive = false;
+        return;
+    }
+    float pdf = bsdf_pdf(s, v, l, p_spec);
+    if (pdf <= 1e-6f) {
+        st.alive = false;
+        return;
+    }
+    F3 f = brdf_eval(s, v, l);
+    st.throughput = mk(st.throughput.x * (f.x * nl / pdf), st.throughput.y * (f.y * nl / pdf),
+                       st.throughput.z * (f.z * nl / pdf));
+    st.prev_pdf = pdf;
+    st.prev_from_bsdf = true;
+    st.prev_n = s.n;
+    st.prev_total = area_total;
+    if (st.bounce >= 2u) {
+        float q = clampT(fmaxf(st.throughput.x, fmaxf(st.throughput.y, st.throughput.z)), 0.05f, 0.95f);
+        if (rand01(st.rng) > q) {
+            st.alive = false;
+            return;
+        }
+        st.throughput = mk(st.throughput.x / q, st.throughput.y / q, st.throughput.z / q);
+    }
+    st.o = p_off;
+    st.d = l;
+    st.bounce++;
+}
+
+}  // namespace syf

