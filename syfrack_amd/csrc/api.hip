@@ -305,7 +305,7 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
     if ((e = hipMalloc(&j->ctr, sizeof(unsigned long long) * CTR_COUNT)) != hipSuccess ||
         (e = hipMalloc(&j->t_prev, j->pixels * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&j->hit_list, j->pixels * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&j->counters, 2 * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&j->counters, COUNTER_WORDS * sizeof(uint32_t))) != hipSuccess ||
         (e = hipEventCreate(&j->ev0)) != hipSuccess || (e = hipEventCreate(&j->ev1)) != hipSuccess ||
         (e = hipEventCreate(&j->ev2)) != hipSuccess) {
         syf_job_destroy(j);
@@ -413,6 +413,8 @@ int syf_trace_primary(syf_job* j, uint32_t sample) {
     TraceBuffers B{(float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
                    (double*)j->plane[SYF_PLANE_DEPTH64], (uint32_t*)j->plane[SYF_PLANE_STEPS], j->t_prev, j->hit_list,
                    j->counters, j->ctr};
+    // ray counter and hit-queue length start at zero for every call
+    if (e == hipSuccess) e = hipMemsetAsync(j->counters, 0, COUNTER_WORDS * sizeof(uint32_t), j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
     if (e == hipSuccess) e = launch_trace_primary(j->P, sample, B, j->ctx->cu_count, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev1, j->stream);

@@ -147,18 +147,19 @@ template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval(const 
     uint32_t iter = 0;
     T d;
     const T bail = T(F.bailout);
-    if constexpr (FAM == SYF_FAMILY_MANDELBULB || FAM == FAM_BULB_TRIG) {
+    if constexpr (is_bulb(FAM)) {
         for (uint32_t k = 0; k < F.iterations; ++k) {
             T r = len3(z);
             if (r > bail) break;
             trap = fmin(trap, r);
-            if constexpr (FAM == SYF_FAMILY_MANDELBULB) {
+            if constexpr (FAM != FAM_BULB_TRIG) {
+                const uint32_t n = FAM == FAM_BULB8 ? 8u : F.ipower;
                 if (r >= T(1e-12)) {
                     T deriv;
-                    spherical_power(z, r, F.ipower, deriv);
+                    spherical_power(z, r, n, deriv);
                     dr = dr * deriv;
                 } else {
-                    tiny_power(z, F.ipower, dr);
+                    tiny_power(z, n, dr);
                 }
             } else {
                 trig_power(z, r, T(F.power), dr);
@@ -251,15 +252,16 @@ template <uint32_t FAM> struct DeStepper {
 
     // One Iterate body (the caller guarantees the orbit is not done).
     __device__ __forceinline__ bool step(const FractalP& F) {
-        if constexpr (FAM == SYF_FAMILY_MANDELBULB || FAM == FAM_BULB_TRIG) {
+        if constexpr (is_bulb(FAM)) {
             trap = fminf(trap, r);
-            if constexpr (FAM == SYF_FAMILY_MANDELBULB) {
+            if constexpr (FAM != FAM_BULB_TRIG) {
+                const uint32_t n = FAM == FAM_BULB8 ? 8u : F.ipower;
                 if (r >= 1e-12f) {
                     float deriv;
-                    spherical_power(z, r, F.ipower, deriv);
+                    spherical_power(z, r, n, deriv);
                     dr = dr * deriv;
                 } else {
-                    tiny_power(z, F.ipower, dr);
+                    tiny_power(z, n, dr);
                 }
             } else {
                 trig_power(z, r, F.power, dr);
@@ -299,7 +301,7 @@ template <uint32_t FAM> struct DeStepper {
     __device__ __forceinline__ De<float> finish(const FractalP& F) const {
         float d;
         float tr = trap;
-        if constexpr (FAM == SYF_FAMILY_MANDELBULB || FAM == FAM_BULB_TRIG) {
+        if constexpr (is_bulb(FAM)) {
             const float rr = fmaxf(r, 1e-12f);
             d = 0.5f * logf(rr) * rr / dr;
         } else if constexpr (FAM == SYF_FAMILY_MANDELBOX) {
@@ -328,6 +330,7 @@ template <uint32_t FAM> struct DeStepper {
 template <class T> __device__ __forceinline__ De<T> de_eval_any(const FractalP& F, V3<T> p) {
     switch (F.family) {
     case SYF_FAMILY_MANDELBULB:
+        if (F.ipower == 8u) return de_eval<T, FAM_BULB8>(F, p);
         return F.ipower != 0u ? de_eval<T, SYF_FAMILY_MANDELBULB>(F, p) : de_eval<T, FAM_BULB_TRIG>(F, p);
     case SYF_FAMILY_MANDELBOX: return de_eval<T, SYF_FAMILY_MANDELBOX>(F, p);
     case SYF_FAMILY_MENGER: return de_eval<T, SYF_FAMILY_MENGER>(F, p);

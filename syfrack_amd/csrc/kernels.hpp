@@ -7,6 +7,10 @@
 
 namespace syf {
 
+// Layout of TraceBuffers::counters: eight ray counters 16 words apart, then the hit count.
+constexpr uint32_t COUNTER_HITS = 128;
+constexpr uint32_t COUNTER_WORDS = 144;
+
 // Device buffers of a primary trace.
 struct TraceBuffers {
     float* depth;
@@ -15,7 +19,7 @@ struct TraceBuffers {
     uint32_t* steps;
     float* t_prev;             // per pixel: march position before the hit (refine bracket)
     uint32_t* hit_list;        // per pixel capacity: pixels that hit, for the FP64 refine
-    uint32_t* counters;        // [0] ray counter of the march, [1] length of hit_list
+    uint32_t* counters;        // COUNTER_WORDS words: ray counters (one per XCD, 64 B apart), hit_list length
     unsigned long long* ctr;   // work counters (CTR_*)
 };
 

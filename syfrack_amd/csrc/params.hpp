@@ -88,10 +88,16 @@ struct JobP {
 };
 
 // Kernel specialisation of a family: the Mandelbulb with a non-integer power takes the
-// trigonometric form, instantiated separately so the integer path stays lean.
+// trigonometric form and the power-8 Mandelbulb a compile-time exponent, each instantiated
+// separately so the common paths stay lean.
 constexpr uint32_t FAM_BULB_TRIG = 4u;
+constexpr uint32_t FAM_BULB8 = 5u;
+constexpr bool is_bulb(uint32_t fam) {
+    return fam == SYF_FAMILY_MANDELBULB || fam == FAM_BULB_TRIG || fam == FAM_BULB8;
+}
 inline uint32_t kernel_family(const FractalP& F) {
-    return (F.family == SYF_FAMILY_MANDELBULB && F.ipower == 0u) ? FAM_BULB_TRIG : F.family;
+    if (F.family != SYF_FAMILY_MANDELBULB) return F.family;
+    return F.ipower == 0u ? FAM_BULB_TRIG : (F.ipower == 8u ? FAM_BULB8 : SYF_FAMILY_MANDELBULB);
 }
 
 enum : uint32_t {
