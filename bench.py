@@ -8,8 +8,11 @@ rays) traced on the device; the job's planes stay in HBM, nothing is copied per 
 
 Prints one JSON line: Mrays/s (primary rays per second over the timed steps), DE evals/s,
 the roofline of the dominant kernel (FP32 VALU, from device work counters and HIP-event
-timing on the kernel's own stream) and the CPU baseline (the oracle's restatement of the
-same trace on a bounded sample of rows, on the host cores).
+timing on the kernel's own stream), the CPU baseline (the oracle's restatement of the
+same trace on a bounded sample of rows, on the host cores), and the serving legs: the
+render server in batched mode against the plain one-launch-per-request form on an uneven
+batch of 256 requests and on a Poisson serving loop, with rays/s (all traced segments,
+primary + bounce + shadow) and p50/p99 submit-to-completion latency.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--width 1920 --height 1080]
 """
@@ -55,6 +58,64 @@ def cpu_baseline(width: int, height: int, budget_s: float = 12.0) -> dict:
                       "oracle C++ restatement, OpenMP, g++ -O2 -ffp-contract=off"}
 
 
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
+
+
+def serve_stream(ctx, reqs, mode: str, realtime: bool) -> dict:
+    """Runs a request stream through a render server; wall time from the first submission to
+    the last completion.  With ``realtime`` each request is submitted at its arrival time."""
+    from syfrack_amd import serve
+    descs = [(q.scene, q.cfg) for q in reqs]
+    with serve.Server(ctx, mode) as s:
+        t0 = time.perf_counter()
+        ids = []
+        for q, (sc_, cfg) in zip(reqs, descs):
+            if realtime:
+                dt = q.arrival_s - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+            ids.append(s.submit(sc_, cfg))
+        s.wait_all(900)
+        wall = time.perf_counter() - t0
+        st = s.stats()
+        infos = [s.poll(i) for i in ids]
+        for i in ids:
+            s.release(i)
+    prev = [i.latency_ms for i, q in zip(infos, reqs) if q.preview]
+    rest = [i.latency_ms for i, q in zip(infos, reqs) if not q.preview]
+    return {"mode": mode, "wall_s": wall, "rays": st["rays"], "primary_rays": st["primary_rays"],
+            "mrays_s": st["rays"] / wall / 1e6, "device_ms": st["device_ms"], "launches": st["launches"],
+            "quantum_samples": st["quantum_samples"], "grid_blocks": st["grid_blocks"],
+            "batch_target_us": st["batch_target_us"], "hbm_peak_gb": st["hbm_peak"] / 1e9,
+            "preview_p50_ms": _pct(prev, 50), "preview_p99_ms": _pct(prev, 99),
+            "render_p50_ms": _pct(rest, 50), "render_p99_ms": _pct(rest, 99),
+            "all_p50_ms": _pct([i.latency_ms for i in infos], 50), "all_p99_ms": _pct([i.latency_ms for i in infos], 99)}
+
+
+def serving_bench(ctx, n_uneven: int, n_mix: int, rate: float) -> dict:
+    """Batched serving against the plain one-launch-per-request form on the same streams."""
+    from syfrack_amd import serve
+    out = {}
+    uneven = serve.uneven_requests(n_uneven, seed=3)
+    b = serve_stream(ctx, uneven, "batched", realtime=False)
+    p = serve_stream(ctx, uneven, "plain", realtime=False)
+    out["uneven_batch"] = {
+        "workload": f"{n_uneven} requests submitted at once: square resolutions log-uniform 128-1024, "
+                    "Mandelbulb (power 2-12) / Mandelbox (scale -2.5..3) / Menger (4-7 levels), spp 1-16, "
+                    "bounces 1-4, <=256 steps, sky light",
+        "batched": b, "plain": p, "speedup": b["mrays_s"] / p["mrays_s"], "same_rays": b["rays"] == p["rays"]}
+    mix = serve.serving_mix(n_mix, rate_per_s=rate, seed=5)
+    b = serve_stream(ctx, mix, "batched", realtime=True)
+    p = serve_stream(ctx, mix, "plain", realtime=True)
+    out["serving_loop"] = {
+        "workload": f"{n_mix} requests, Poisson arrivals at {rate:g}/s: 70% preview tiles 256^2 1 spp, 30% "
+                    "progressive renders 1024^2-2048^2 16-64 spp, same fractal mix, bounces 1-4",
+        "batched": b, "plain": p, "speedup": b["mrays_s"] / p["mrays_s"], "same_rays": b["rays"] == p["rays"]}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +124,10 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (batched vs plain) legs")
+    ap.add_argument("--serve-requests", type=int, default=256, help="requests of the uneven batch")
+    ap.add_argument("--mix-requests", type=int, default=64, help="requests of the Poisson serving loop")
+    ap.add_argument("--mix-rate", type=float, default=100.0, help="arrivals per second of the serving loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this path shards by whole "
@@ -119,14 +184,25 @@ def main() -> int:
         "march_kernel_ms": march_ms,
         "refine_kernel_ms": kernel_ms - march_ms,
         "refine_fp64_tflops": fl64 / ((kernel_ms - march_ms) * 1e-3) / 1e12 if kernel_ms > march_ms else None,
-        "roofline": {"bound": "valu", "kernel": "k_trace_primary<mandelbulb>", "achieved": achieved,
+        "roofline": {"bound": "valu", "kernel": "k_march<power-8 Mandelbulb>", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": None, "flops_per_launch": fl,
                      "note": "FP32 VALU bound (no MFMA, HBM ~28 B/ray); algorithmic FLOPs from device counters; "
                              "special ops (div/sqrt/log) not counted; packed FP32 is disabled in the build, so the "
                              "non-packed ceiling is 78.6 TFLOP/s"},
-        "latency": {"p50_ms": None, "p99_ms": None, "note": "serving-loop tile latency not measured in this run"},
     }
+    if not args.no_serve:
+        sv = serving_bench(r.ctx, args.serve_requests, args.mix_requests, args.mix_rate)
+        out["serving"] = sv
+        lp = sv["serving_loop"]
+        out["latency"] = {"preview_p50_ms": lp["batched"]["preview_p50_ms"],
+                          "preview_p99_ms": lp["batched"]["preview_p99_ms"],
+                          "plain_preview_p50_ms": lp["plain"]["preview_p50_ms"],
+                          "plain_preview_p99_ms": lp["plain"]["preview_p99_ms"],
+                          "note": "submit-to-completion latency of the 256^2 preview requests in the Poisson "
+                                  "serving loop, batched server vs the plain one-launch-per-request form"}
+        out["batched_vs_plain_rays"] = {"uneven_batch": sv["uneven_batch"]["speedup"],
+                                        "serving_loop": lp["speedup"]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.width, args.height)
     job.close()

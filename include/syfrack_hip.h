@@ -192,7 +192,8 @@ int syf_trace_primary(syf_job* job, uint32_t sample);
 
 /* Megakernel path trace of a tile: for each pixel in [x, x+w) x [y, y+h), samples
  * sample_start .. sample_start+sample_count-1 are traced and added to ACCUM in ascending
- * sample order; DEPTH/AUX take the first hit when sample_start == 0.  The one-launch form
+ * sample order; when sample_start == 0, DEPTH/AUX take sample 0's first hit and DEPTH64 its
+ * first surface hit refined in FP64 (as syf_trace_primary; -1 otherwise).  The one-launch form
  * of Renderer::dispatch (renderer.rs:729-771) running pathtrace.wgsl main (:1203-1276).
  * Asynchronous. */
 int syf_render_tile(syf_job* job, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
@@ -244,6 +245,73 @@ int syf_planar_sync(syf_planar* p);
 int syf_planar_readback(syf_planar* p, uint32_t plane, void* host, size_t bytes);
 /* Iterations evaluated, glitch re-references, and device time of the last run. */
 int syf_planar_stats(syf_planar* p, uint64_t* iterations, uint64_t* rebases, double* kernel_ms);
+
+/* ---------------------------------------------------------------- serving
+ * A render server on one device: requests of any family, size, spp and bounce count are
+ * submitted as they arrive and traced progressively, with their accumulation planes resident
+ * in HBM.  It replaces the export scheduling of the reference (TileWorkQueue,
+ * export_queue.rs:7-120; two ExportSlots, export.rs:453-464; RenderSession::step,
+ * session.rs:694-877) for this path.  In SYF_SERVE_BATCHED mode a scheduler thread admits
+ * requests against the HBM budget and runs them together: every launch is one persistent
+ * kernel over tiles of all active requests, a request joining at the next launch after it is
+ * admitted (continuous batching).  SYF_SERVE_PLAIN runs one launch per request, in arrival
+ * order: the one-launch-per-request form.  A request's image does not depend on the mode or
+ * on the other requests in flight. */
+#define SYF_SERVE_BATCHED 0u
+#define SYF_SERVE_PLAIN 1u
+typedef struct syf_server_config {
+    uint64_t hbm_budget_bytes;   /* admission budget for request planes (0 = 90 % of free HBM) */
+    uint32_t mode;               /* SYF_SERVE_BATCHED or SYF_SERVE_PLAIN */
+    uint32_t max_active;         /* requests traced concurrently (0 = 4096) */
+    uint32_t quantum_samples;    /* samples per pixel per work item (0 = autotuned) */
+    uint32_t batch_target_us;    /* target device time of one launch (0 = 5000 us) */
+    uint32_t grid_blocks;        /* workgroups of the persistent launch (0 = autotuned) */
+    uint32_t reserved;
+} syf_server_config;
+
+#define SYF_REQ_QUEUED 0u     /* waiting for admission */
+#define SYF_REQ_ACTIVE 1u     /* admitted, being traced */
+#define SYF_REQ_DONE 2u       /* all samples accumulated; planes readable */
+#define SYF_REQ_CANCELLED 3u
+typedef struct syf_request_info {
+    uint32_t state;           /* SYF_REQ_* */
+    uint32_t samples_done;    /* samples per pixel accumulated so far */
+    uint32_t spp;
+    uint32_t reserved;
+    uint64_t footprint_bytes; /* device bytes the request holds while admitted */
+    double submit_ms;         /* host clock, ms since the server was created */
+    double start_ms;          /* first launch containing its work */
+    double done_ms;           /* completion of the launch holding its last sample */
+} syf_request_info;
+
+typedef struct syf_server_counters {
+    uint64_t submitted, completed, cancelled, launches;
+    uint64_t rays;            /* traced segments (primary + bounce + shadow) */
+    uint64_t primary_rays;
+    uint64_t hbm_in_use, hbm_peak;
+    double device_ms;         /* summed device time of the launches, HIP events */
+    double wall_ms;           /* since the server was created */
+    uint32_t quantum_samples, batch_target_us, grid_blocks, reserved;
+} syf_server_counters;
+
+int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** out);
+/* Stops the scheduler after the launches in flight and frees every request. */
+int syf_server_destroy(syf_server* s);
+/* Queues a request for `spp` samples per pixel.  SYF_E_ADMISSION when its footprint exceeds
+ * the whole budget; SYF_E_INVALID for an invalid description (message names the field). */
+int syf_server_submit(syf_server* s, const syf_job_desc* desc, uint32_t spp, uint64_t* id);
+int syf_server_poll(syf_server* s, uint64_t id, syf_request_info* out);
+/* Waits for a request to finish; SYF_E_TIMEOUT after timeout_ms. */
+int syf_server_wait(syf_server* s, uint64_t id, uint32_t timeout_ms);
+/* Waits until every submitted request has finished or was cancelled. */
+int syf_server_wait_all(syf_server* s, uint32_t timeout_ms);
+/* Copies a plane (SYF_PLANE_ACCUM, DEPTH, AUX or DEPTH64) of a finished request to the host. */
+int syf_server_result(syf_server* s, uint64_t id, uint32_t plane, void* host, size_t bytes);
+/* Frees the device planes of a finished or cancelled request; its id is unknown afterwards. */
+int syf_server_release(syf_server* s, uint64_t id);
+/* Cancels a queued or active request (its work already launched still completes). */
+int syf_server_cancel(syf_server* s, uint64_t id);
+int syf_server_stats(syf_server* s, syf_server_counters* out);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,7 @@ def load() -> C.CDLL:
     lib.orc_escape_f128.argtypes = [P, P, P, C.c_size_t, P, P]
     if hasattr(lib, "orc_render"):
         lib.orc_render.restype = C.c_int
-        lib.orc_render.argtypes = [P] + [C.c_uint32] * 6 + [P, P, P]
+        lib.orc_render.argtypes = [P] + [C.c_uint32] * 6 + [P, P, P, P]
     _LIB = lib
     return lib
 
@@ -161,15 +161,18 @@ def escape_f128(desc, px: np.ndarray, py: np.ndarray):
 
 
 def render(desc, rect, sample_start: int, sample_count: int):
-    """Path-traced accumulation (float4 per pixel), first-hit depth and aux of a rectangle."""
+    """Path-traced accumulation (float4 per pixel), first-hit depth, aux and FP64-refined
+    first-hit depth of a rectangle."""
     lib = load()
     x0, y0, w, h = rect
     accum = np.zeros((h, w, 4), np.float32)
     depth = np.zeros((h, w), np.float32)
     aux = np.zeros((h, w, 4), np.float32)
-    rc = lib.orc_render(C.byref(desc), x0, y0, w, h, sample_start, sample_count, _ptr(accum), _ptr(depth), _ptr(aux))
+    depth64 = np.zeros((h, w), np.float64)
+    rc = lib.orc_render(C.byref(desc), x0, y0, w, h, sample_start, sample_count, _ptr(accum), _ptr(depth), _ptr(aux),
+                        _ptr(depth64))
     assert rc == 0
-    return accum, depth, aux
+    return accum, depth, aux, depth64
 
 
 def planar_f32(desc, rect=None):

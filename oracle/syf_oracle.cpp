@@ -854,7 +854,7 @@ F3 direct_light(const SceneL& S, F3 p, const Surf& s, F3 gn, F3 v, float p_spec,
 
 // radiance (pathtrace.wgsl:802-1201) for the subset; returns L and the first-hit planes.
 F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* first_depth, float* first_aux,
-            Count* c) {
+            float* first_tprev, Count* c) {
     F3 ro, rd;
     camera_ray(S.cam, S.width, S.height, px, py, rng, &ro, &rd);
     F3 throughput = v3(1.0f, 1.0f, 1.0f), L = v3(0.0f, 0.0f, 0.0f);
@@ -873,6 +873,7 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
             }
             if (bounce == 0) {
                 *first_depth = h.t;
+                *first_tprev = hl.light == NO_LIGHT ? h.t_prev : -1.0f;  // refine bracket of a surface hit
                 first_aux[0] = gn.x; first_aux[1] = gn.y; first_aux[2] = gn.z; first_aux[3] = float(h.steps);
             }
         }
@@ -1150,11 +1151,15 @@ int orc_trace_primary(const syf_job_desc* d, uint32_t sample, int precision, uin
 
 // Megakernel accumulation (pathtrace.wgsl:1203-1276) of a pixel rectangle: samples
 // sample_start .. sample_start+sample_count-1 in ascending order, accum += (r if finite, 1);
-// depth/aux from the first sample of the call (written by the caller when sample_start == 0).
-// Outputs are rectangle-local, row-major: accum4, depth, aux4.
+// depth/aux from the first sample of the call (written by the caller when sample_start == 0),
+// and depth64: that sample's first surface hit refined in FP64 (refine64 between the march's
+// last two positions, as the primary trace does), -1 on a miss or an emitter hit.
+// Outputs are rectangle-local, row-major: accum4, depth, aux4, depth64.
 int orc_render(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sample_start,
-               uint32_t sample_count, float* accum4, float* depth, float* aux4) {
+               uint32_t sample_count, float* accum4, float* depth, float* aux4, double* depth64) {
     SceneL S = lower_scene(*d);
+    const Fractal<double> F64 = lower_fractal<double>(d->fractal);
+    const double hit_cone = double(S.pixel_cone) * double(S.pixel_tol);
 #pragma omp parallel for schedule(dynamic, 1)
     for (long yy = 0; yy < long(h); ++yy) {
         Count c;
@@ -1163,10 +1168,11 @@ int orc_render(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint
             size_t i = size_t(yy) * w + xx;
             float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             float fd = -1.0f, fa[4] = {0.0f, 0.0f, 0.0f, -1.0f};
+            double f64 = -1.0;
             for (uint32_t s = 0; s < sample_count; ++s) {
                 uint32_t rng = rng_init(px, py, sample_start + s, S.seed);
-                float dd = -1.0f, aa[4] = {0.0f, 0.0f, 0.0f, -1.0f};
-                F3 r = radiance(S, px, py, &rng, &dd, aa, &c);
+                float dd = -1.0f, aa[4] = {0.0f, 0.0f, 0.0f, -1.0f}, tp = -1.0f;
+                F3 r = radiance(S, px, py, &rng, &dd, aa, &tp, &c);
                 bool ok = r.x == r.x && r.y == r.y && r.z == r.z && std::fabs(r.x) < 1e30f && std::fabs(r.y) < 1e30f &&
                           std::fabs(r.z) < 1e30f;
                 acc[0] += ok ? r.x : 0.0f;
@@ -1176,10 +1182,18 @@ int orc_render(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint
                 if (s == 0) {
                     fd = dd;
                     for (int k = 0; k < 4; ++k) fa[k] = aa[k];
+                    if (dd >= 0.0f && tp >= 0.0f) {
+                        uint32_t rng0 = rng_init(px, py, sample_start, S.seed);
+                        F3 ro, rd;
+                        camera_ray(S.cam, S.width, S.height, px, py, &rng0, &ro, &rd);
+                        f64 = refine64(F64, v3(double(ro.x), double(ro.y), double(ro.z)),
+                                       v3(double(rd.x), double(rd.y), double(rd.z)), double(tp), double(dd), hit_cone, &c);
+                    }
                 }
             }
             for (int k = 0; k < 4; ++k) accum4[4 * i + k] = acc[k];
             depth[i] = fd;
+            if (depth64) depth64[i] = f64;
             for (int k = 0; k < 4; ++k) aux4[4 * i + k] = fa[k];
         }
     }

@@ -48,6 +48,13 @@ PLANAR_PERTURB = 2
 PLANAR_SMOOTH = 0
 PLANAR_ITER = 1
 
+SERVE_BATCHED = 0
+SERVE_PLAIN = 1
+REQ_QUEUED = 0
+REQ_ACTIVE = 1
+REQ_DONE = 2
+REQ_CANCELLED = 3
+
 F3 = C.c_float * 3
 F2 = C.c_float * 2
 
@@ -165,6 +172,50 @@ class SyfPlanarDesc(C.Structure):
     ]
 
 
+class SyfServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("mode", C.c_uint32),
+        ("max_active", C.c_uint32),
+        ("quantum_samples", C.c_uint32),
+        ("batch_target_us", C.c_uint32),
+        ("grid_blocks", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class SyfRequestInfo(C.Structure):
+    _fields_ = [
+        ("state", C.c_uint32),
+        ("samples_done", C.c_uint32),
+        ("spp", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("footprint_bytes", C.c_uint64),
+        ("submit_ms", C.c_double),
+        ("start_ms", C.c_double),
+        ("done_ms", C.c_double),
+    ]
+
+
+class SyfServerCounters(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("cancelled", C.c_uint64),
+        ("launches", C.c_uint64),
+        ("rays", C.c_uint64),
+        ("primary_rays", C.c_uint64),
+        ("hbm_in_use", C.c_uint64),
+        ("hbm_peak", C.c_uint64),
+        ("device_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("quantum_samples", C.c_uint32),
+        ("batch_target_us", C.c_uint32),
+        ("grid_blocks", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
 # Every symbol the header declares, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
@@ -188,6 +239,16 @@ SIGNATURES = {
     "syf_planar_sync": (C.c_int, [_P]),
     "syf_planar_readback": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
     "syf_planar_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "syf_server_create": (C.c_int, [_P, C.POINTER(SyfServerConfig), C.POINTER(_P)]),
+    "syf_server_destroy": (C.c_int, [_P]),
+    "syf_server_submit": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.c_uint32, C.POINTER(C.c_uint64)]),
+    "syf_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(SyfRequestInfo)]),
+    "syf_server_wait": (C.c_int, [_P, C.c_uint64, C.c_uint32]),
+    "syf_server_wait_all": (C.c_int, [_P, C.c_uint32]),
+    "syf_server_result": (C.c_int, [_P, C.c_uint64, C.c_uint32, _P, C.c_size_t]),
+    "syf_server_release": (C.c_int, [_P, C.c_uint64]),
+    "syf_server_cancel": (C.c_int, [_P, C.c_uint64]),
+    "syf_server_stats": (C.c_int, [_P, C.POINTER(SyfServerCounters)]),
 }
 
 

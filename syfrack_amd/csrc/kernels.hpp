@@ -25,6 +25,28 @@ struct TraceBuffers {
 
 hipError_t launch_trace_primary(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
 hipError_t launch_refine64(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
+// Work item of the batched serving kernel: an 8x8 tile of request `slot`, samples
+// [s0, s0 + ns).
+struct BatchItem {
+    uint32_t slot, x0, y0, s0, ns;
+};
+// Device planes of one request.
+struct BatchPlanes {
+    float4* accum;
+    float* depth;
+    float4* aux;
+    double* depth64;
+};
+// One persistent launch over work items that all belong to kernel family `fam`
+// (kernel_family of their jobs).  claim: 8 counters, 16 words apart (zeroed before the launch).
+constexpr uint32_t CLAIM_WORDS = 8 * 16;
+hipError_t launch_batch(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
+                        uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t grid_blocks,
+                        hipStream_t s);
+hipError_t launch_render_tile(const JobP& P, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0,
+                              uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
+                              unsigned long long* ctr, hipStream_t s);
+
 hipError_t launch_de_probe(const FractalP& F, const float4* pts, uint32_t n, float4* out, float4* traps,
                            hipStream_t s);
 

@@ -4,6 +4,7 @@
 // scene.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/syfrack_hip.h"
@@ -92,10 +93,11 @@ struct JobP {
 // separately so the common paths stay lean.
 constexpr uint32_t FAM_BULB_TRIG = 4u;
 constexpr uint32_t FAM_BULB8 = 5u;
+constexpr uint32_t FAM_COUNT = 6u;  // kernel families: the four graph families, trig and power-8 Mandelbulb
 constexpr bool is_bulb(uint32_t fam) {
     return fam == SYF_FAMILY_MANDELBULB || fam == FAM_BULB_TRIG || fam == FAM_BULB8;
 }
-inline uint32_t kernel_family(const FractalP& F) {
+__host__ __device__ inline uint32_t kernel_family(const FractalP& F) {
     if (F.family != SYF_FAMILY_MANDELBULB) return F.family;
     return F.ipower == 0u ? FAM_BULB_TRIG : (F.ipower == 8u ? FAM_BULB8 : SYF_FAMILY_MANDELBULB);
 }

@@ -32,7 +32,7 @@ __device__ __forceinline__ SegHit trace_segment(const JobP& P, F3 o, F3 d, Work&
     uint32_t light;
     const float nearest = nearest_emitter(P, o, d, P.t_end, light);
     Hit<float> hm = march<float, FAM>(P.f, o, d, P.pixel_cone, P.pixel_tolerance, P.max_steps, nearest, w);
-    SegHit sh{hm.hit, hm.exhausted, hm.t, hm.steps, NO_LIGHT, hm.s.trap0};
+    SegHit sh{hm.hit, hm.exhausted, hm.t, hm.steps, NO_LIGHT, hm.s.trap0, hm.t_prev};
     if (!hm.hit && !hm.exhausted && light != NO_LIGHT) {
         sh.hit = true;
         sh.t = nearest;
@@ -57,55 +57,80 @@ template <uint32_t FAM> struct Tracer {
 
 // ---------------------------------------------------------------- megakernel
 
+// All samples s0 .. s0+ns-1 of one pixel, in ascending order, accumulated as
+// accum += (r if finite, 1) (pathtrace.wgsl:1203-1276).  A call that starts at sample 0 also
+// writes the first-hit planes: depth, normal + steps, and the first surface hit refined in FP64
+// between the march's last two positions (refine64, as the primary trace), -1 otherwise.
+template <uint32_t FAM>
+__device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_t py, uint32_t s0, uint32_t ns,
+                                             float4* __restrict__ accum, float* __restrict__ depth,
+                                             float4* __restrict__ aux, double* __restrict__ depth64, Work& work,
+                                             uint32_t& segs, uint32_t& shadow, uint32_t& prim, uint32_t& hits) {
+    const size_t idx = size_t(py) * P.width + px;
+    Tracer<FAM> tr{P, work, shadow};
+    float4 acc = accum[idx];
+    float fd0 = -1.0f, tp0 = -1.0f;
+    float4 fa0 = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    for (uint32_t s = 0; s < ns; ++s) {
+        PathState st;
+        path_start(P, px, py, s0 + s, st);
+        float fd = -1.0f;
+        float4 fa = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        ++prim;
+        while (st.alive) {
+            SegHit sh = trace_segment<FAM>(P, st.o, st.d, work);
+            ++segs;
+            if (st.bounce == 0u && sh.hit && sh.light == NO_LIGHT) {
+                ++hits;
+                if (s == 0u) tp0 = sh.t_prev;
+            }
+            path_bounce(P, st, sh, fd, fa, [&](F3 p, float hh) { return tr.normal(p, hh); },
+                        [&](F3 o, F3 d, float te) { return tr.visible(o, d, te); });
+        }
+        const F3 r = st.L;
+        const bool ok = r.x == r.x && r.y == r.y && r.z == r.z && fabsf(r.x) < 1e30f && fabsf(r.y) < 1e30f &&
+                        fabsf(r.z) < 1e30f;
+        acc.x += ok ? r.x : 0.0f;
+        acc.y += ok ? r.y : 0.0f;
+        acc.z += ok ? r.z : 0.0f;
+        acc.w += 1.0f;
+        if (s == 0u) {
+            fd0 = fd;
+            fa0 = fa;
+        }
+    }
+    accum[idx] = acc;
+    if (s0 == 0u && ns > 0u) {
+        depth[idx] = fd0;
+        aux[idx] = fa0;
+        if (depth64) {
+            double d64 = -1.0;
+            if (fd0 >= 0.0f && tp0 >= 0.0f) {
+                uint32_t rng = rng_init(px, py, 0u, P.seed);
+                F3 ro, rd;
+                camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
+                d64 = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(tp0), double(fd0),
+                                    double(P.pixel_cone) * double(P.pixel_tolerance), work);
+            }
+            depth64[idx] = d64;
+        }
+    }
+}
+
 template <uint32_t FAM>
 __global__ __launch_bounds__(256) void k_render_tile(const JobP P, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                                                      uint32_t s0, uint32_t ns, uint32_t tiles_x, uint32_t tiles,
                                                      float4* __restrict__ accum, float* __restrict__ depth,
-                                                     float4* __restrict__ aux, unsigned long long* __restrict__ ctr) {
+                                                     float4* __restrict__ aux, double* __restrict__ depth64,
+                                                     unsigned long long* __restrict__ ctr) {
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t tile = blockIdx.x * 4u + wave;
     if (tile >= tiles) return;
     const uint32_t lx = (tile % tiles_x) * 8u + (lane & 7u), ly = (tile / tiles_x) * 8u + (lane >> 3);
     Work work;
     uint32_t segs = 0, shadow = 0, prim = 0, hits = 0;
-    if (lx < w && ly < h) {
-        const uint32_t px = x0 + lx, py = y0 + ly;
-        const size_t idx = size_t(py) * P.width + px;
-        Tracer<FAM> tr{P, work, shadow};
-        float4 acc = accum[idx];
-        float fd0 = -1.0f;
-        float4 fa0 = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-        for (uint32_t s = 0; s < ns; ++s) {
-            PathState st;
-            path_start(P, px, py, s0 + s, st);
-            float fd = -1.0f;
-            float4 fa = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-            ++prim;
-            while (st.alive) {
-                SegHit sh = trace_segment<FAM>(P, st.o, st.d, work);
-                ++segs;
-                if (st.bounce == 0u && sh.hit && sh.light == NO_LIGHT) ++hits;
-                path_bounce(P, st, sh, fd, fa, [&](F3 p, float hh) { return tr.normal(p, hh); },
-                            [&](F3 o, F3 d, float te) { return tr.visible(o, d, te); });
-            }
-            const F3 r = st.L;
-            const bool ok = r.x == r.x && r.y == r.y && r.z == r.z && fabsf(r.x) < 1e30f && fabsf(r.y) < 1e30f &&
-                            fabsf(r.z) < 1e30f;
-            acc.x += ok ? r.x : 0.0f;
-            acc.y += ok ? r.y : 0.0f;
-            acc.z += ok ? r.z : 0.0f;
-            acc.w += 1.0f;
-            if (s == 0u) {
-                fd0 = fd;
-                fa0 = fa;
-            }
-        }
-        accum[idx] = acc;
-        if (s0 == 0u && ns > 0u) {
-            depth[idx] = fd0;
-            aux[idx] = fa0;
-        }
-    }
+    if (lx < w && ly < h)
+        render_pixel<FAM>(P, x0 + lx, y0 + ly, s0, ns, accum, depth, aux, depth64, work, segs, shadow, prim, hits);
     flush_work(work, hits, segs + shadow, ctr);
     uint32_t p = wave_sum(prim);
     if (lane == 0u) atomicAdd(&ctr[CTR_PRIMARY], (unsigned long long)p);
@@ -117,6 +142,7 @@ enum : uint32_t {
     // float planes
     WF_OX, WF_OY, WF_OZ, WF_DX, WF_DY, WF_DZ, WF_TX, WF_TY, WF_TZ, WF_LX, WF_LY, WF_LZ, WF_NX, WF_NY, WF_NZ,
     WF_PDF, WF_TOTAL, WF_FD, WF_FA0, WF_FA1, WF_FA2, WF_FA3, WF_HT, WF_HTRAP,
+    WF_TPREV,  // first segment: march position before a surface hit (FP64 refine bracket), else -1
     // u32 planes
     WF_RNG, WF_BOUNCE, WF_HSTEPS, WF_HLIGHT,
     WF_PLANES
@@ -229,6 +255,7 @@ __global__ __launch_bounds__(256) void k_wf_intersect(const JobP P, const WfP W,
         S.U(WF_HSTEPS, id) = sh.steps;
         S.U(WF_HLIGHT, id) = sh.light;
         uint32_t b = S.U(WF_BOUNCE, id) & ~(0x40000u | 0x80000u);
+        if ((b & 0xffffu) == 0u) S.F(WF_TPREV, id) = sh.hit && sh.light == NO_LIGHT ? sh.t_prev : -1.0f;
         S.U(WF_BOUNCE, id) = b | (sh.hit ? 0x40000u : 0u) | (sh.exhausted ? 0x80000u : 0u);
     }
     flush_work(work, 0u, segs, ctr);
@@ -297,11 +324,14 @@ __global__ void k_wf_next_round(uint32_t in, uint32_t capacity, WfCtl* ctl) {
 }
 
 // Accumulate each path of the tile into the film (wf_commit, wavefront.wgsl:232-260).
+template <uint32_t FAM>
 __global__ __launch_bounds__(256) void k_wf_commit(const JobP P, const WfP W, Soa S, float4* __restrict__ accum,
                                                    float* __restrict__ depth, float4* __restrict__ aux,
-                                                   WfCtl* __restrict__ ctl) {
+                                                   double* __restrict__ depth64, WfCtl* __restrict__ ctl,
+                                                   unsigned long long* __restrict__ ctr) {
     const uint32_t n = W.w * W.h;
     const uint32_t sample = W.sample_start + ctl->sample;
+    Work work;
     for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) {
         const uint32_t px = W.x0 + id % W.w, py = W.y0 + id / W.w;
         const size_t idx = size_t(py) * P.width + px;
@@ -319,10 +349,21 @@ __global__ __launch_bounds__(256) void k_wf_commit(const JobP P, const WfP W, So
         a.w += 1.0f;
         accum[idx] = a;
         if (sample == 0u) {
-            depth[idx] = S.F(WF_FD, id);
+            const float fd = S.F(WF_FD, id), tp = S.F(WF_TPREV, id);
+            depth[idx] = fd;
             aux[idx] = make_float4(S.F(WF_FA0, id), S.F(WF_FA1, id), S.F(WF_FA2, id), S.F(WF_FA3, id));
+            double d64 = -1.0;
+            if (fd >= 0.0f && tp >= 0.0f) {  // the camera ray of sample 0, regenerated from its PCG stream
+                uint32_t rng = rng_init(px, py, 0u, P.seed);
+                F3 ro, rd;
+                camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
+                d64 = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(tp), double(fd),
+                                    double(P.pixel_cone) * double(P.pixel_tolerance), work);
+            }
+            depth64[idx] = d64;
         }
     }
+    flush_work(work, 0u, 0u, ctr);
 }
 
 __global__ void k_wf_end_sample(WfCtl* ctl) {
@@ -339,13 +380,71 @@ __global__ void k_wf_end_sample(WfCtl* ctl) {
     default: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_SPHERE>, __VA_ARGS__); break;                        \
     }
 
+// ---------------------------------------------------------------- batched serving form
+
+// One wave takes one work item: an 8x8 tile of one request and a range of its samples.  The
+// request's parameters are read from the job table (wave-uniform, scalar loads) and its
+// family is a wave-uniform switch, so requests of every family and size share one launch.
+// Per pixel the arithmetic is the one-launch form's: a request renders the same alone and in
+// any mix.  Items are claimed from eight counters (one per XCD, blockIdx % 8) to keep the
+// claims of different L2 slices off one address.
+template <uint32_t FAM>
+__device__ __forceinline__ void trace_tile_item(const JobP& P, const BatchItem& it, const BatchPlanes& pl, Work& work,
+                                                uint32_t& segs, uint32_t& shadow, uint32_t& prim, uint32_t& hits) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t px = it.x0 + (lane & 7u), py = it.y0 + (lane >> 3);
+    if (px >= P.width || py >= P.height) return;
+    render_pixel<FAM>(P, px, py, it.s0, it.ns, pl.accum, pl.depth, pl.aux, pl.depth64, work, segs, shadow, prim, hits);
+}
+
+// Batched serving kernel: a persistent grid drains a list of work items (8x8 tile of one
+// request, sample range) of one kernel family, each wave claiming the next item from one of
+// eight counters (the workgroup's index mod 8, i.e. the XCD it runs on, so the claims of one
+// XCD hit one cache line of its own).  The request's job and planes come from per-slot
+// tables, so one launch serves requests of every resolution, spp and parameter set.
+template <uint32_t FAM>
+__global__ __launch_bounds__(256) void k_batch(const JobP* __restrict__ jobs, const BatchPlanes* __restrict__ planes,
+                                               const BatchItem* __restrict__ items, uint32_t n_items,
+                                               uint32_t* __restrict__ claim, unsigned long long* __restrict__ ctr) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t xl = blockIdx.x % 8u;
+    // items dealt round-robin to the eight claim counters: counter xl serves items xl, xl+8, ...
+    const uint32_t mine = n_items > xl ? (n_items - xl + 7u) / 8u : 0u;
+    Work work;
+    uint32_t segs = 0, shadow = 0, prim = 0, hits = 0;
+    for (;;) {
+        uint32_t k = 0;
+        if (lane == 0u) k = atomicAdd(&claim[xl * 16u], 1u);
+        // wave-uniform from here: the item, its job and its planes are read through scalar loads
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k >= mine) break;
+        BatchItem it = items[k * 8u + xl];
+        it.slot = __builtin_amdgcn_readfirstlane(it.slot);
+        trace_tile_item<FAM>(jobs[it.slot], it, planes[it.slot], work, segs, shadow, prim, hits);
+    }
+    flush_work(work, hits, segs + shadow, ctr);
+    const uint32_t p = wave_sum(prim);
+    if (lane == 0u) atomicAdd(&ctr[CTR_PRIMARY], (unsigned long long)p);
+}
+
+hipError_t launch_batch(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
+                        uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t grid_blocks,
+                        hipStream_t s) {
+    if (n_items == 0) return hipSuccess;
+    if (fam >= FAM_COUNT) return hipErrorInvalidValue;
+    const uint32_t waves = (n_items + 7u) / 8u * 8u;
+    const dim3 grid(std::max<uint32_t>(8u, std::min<uint32_t>(grid_blocks, (waves + 3u) / 4u))), block(256);
+    SYF_DISPATCH_FAMILY(fam, k_batch, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr);
+    return hipGetLastError();
+}
+
 hipError_t launch_render_tile(const JobP& P, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0,
-                              uint32_t ns, float4* accum, float* depth, float4* aux, unsigned long long* ctr,
-                              hipStream_t s) {
+                              uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
+                              unsigned long long* ctr, hipStream_t s) {
     const uint32_t tiles_x = (w + 7u) / 8u, tiles = tiles_x * ((h + 7u) / 8u);
     const dim3 grid((tiles + 3u) / 4u), block(256);
     SYF_DISPATCH_FAMILY(kernel_family(P.f), k_render_tile, grid, block, 0, s, P, x0, y0, w, h, s0, ns, tiles_x,
-                        tiles, accum, depth, aux, ctr);
+                        tiles, accum, depth, aux, depth64, ctr);
     return hipGetLastError();
 }
 
@@ -385,8 +484,9 @@ hipError_t enqueue_wavefront_sample(syf_job* j, const WfP& W, int cu_count) {
         SYF_DISPATCH_FAMILY(fam, k_wf_shade, grid, block, 0, s, P, W, S, a->queue, in, a->ctl, j->ctr);
         hipLaunchKernelGGL(k_wf_next_round, dim3(1), dim3(64), 0, s, in, a->capacity, a->ctl);
     }
-    hipLaunchKernelGGL(k_wf_commit, gen_grid, block, 0, s, P, W, S, (float4*)j->plane[SYF_PLANE_ACCUM],
-                       (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX], a->ctl);
+    SYF_DISPATCH_FAMILY(fam, k_wf_commit, gen_grid, block, 0, s, P, W, S, (float4*)j->plane[SYF_PLANE_ACCUM],
+                        (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
+                        (double*)j->plane[SYF_PLANE_DEPTH64], a->ctl, j->ctr);
     hipLaunchKernelGGL(k_wf_end_sample, dim3(1), dim3(64), 0, s, a->ctl);
     return hipGetLastError();
 }
@@ -407,7 +507,8 @@ int syf_render_tile(syf_job* j, uint32_t x, uint32_t y, uint32_t w, uint32_t h, 
     if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
     if (e == hipSuccess)
         e = launch_render_tile(j->P, x, y, w, h, sample_start, sample_count, (float4*)j->plane[SYF_PLANE_ACCUM],
-                               (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX], j->ctr, j->stream);
+                               (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
+                               (double*)j->plane[SYF_PLANE_DEPTH64], j->ctr, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev1, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev2, j->stream);
     j->timed = true;

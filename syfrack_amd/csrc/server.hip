@@ -1,0 +1,784 @@
+// Render server: continuous batching of uneven requests on one device.
+//
+// A scheduler thread owns the device side.  Each cycle it
+//   1. admits queued requests in arrival order while their footprint (accumulation, depth,
+//      aux and FP64 depth planes, 44 bytes per pixel) fits the HBM budget and a slot is free;
+//   2. plans one launch: work items (8x8 tile, sample range) of the active requests, shortest
+//      remaining work first, until the launch's estimated device time reaches the target
+//      (a running estimate of ns per pixel-sample-segment, refreshed from every launch);
+//   3. enqueues it (items copied from pinned host memory, claim counters cleared, one
+//      persistent kernel) and keeps up to two launches in flight on its stream, so the
+//      device does not idle while the next launch is planned;
+//   4. retires the oldest launch when its end event completes: adds its samples to the
+//      requests' progress and stamps the requests it finished.
+// Accumulation of a pixel is owned by one work item per sample range and successive ranges
+// of a request run in launch order, so samples land in ascending order with no float atomics.
+// The plain mode runs one launch per request (all samples), in arrival order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "kernels.hpp"
+
+namespace syf {
+
+struct Request {
+    uint64_t id = 0;
+    JobP P{};
+    uint32_t spp = 0;
+    uint32_t state = SYF_REQ_QUEUED;
+    uint32_t slot = ~0u;
+    uint32_t tiles_x = 0, tiles = 0;
+    uint64_t total = 0;      // tile-samples of the whole request
+    uint64_t planned = 0;    // tile-samples launched
+    uint64_t completed = 0;  // tile-samples whose launches finished
+    uint32_t q_start = 0, q_len = 0, q_cursor = 0;  // current quantum: samples and next tile
+    size_t pixels = 0;
+    uint64_t footprint = 0;
+    void* accum = nullptr;
+    void* depth = nullptr;
+    void* aux = nullptr;
+    void* depth64 = nullptr;
+    BatchPlanes planes{};  // source of the request's entry in the device plane table
+    double submit_ms = 0, start_ms = -1, done_ms = -1;
+    bool cancelled = false;
+    bool released = false;  // released by the caller; deleted once no launch refers to it
+    uint32_t refs = 0;      // launches in flight (or a plain render) that refer to the request
+};
+
+struct Launch {
+    hipEvent_t start = nullptr, end = nullptr;
+    std::vector<std::pair<Request*, uint64_t>> progress;  // request, tile-samples completed by the launch
+    uint32_t buf = 0;
+    double units = 0;
+};
+
+}  // namespace syf
+
+using namespace syf;
+
+struct syf_server {
+    syf_ctx* ctx = nullptr;
+    syf_server_config cfg{};
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::thread worker;
+    bool stop = false;
+    std::map<uint64_t, Request*> reqs;
+    std::deque<Request*> queue;
+    std::vector<Request*> active;
+    std::deque<Launch> inflight;
+    uint64_t next_id = 1;
+    uint32_t max_slots = 0;
+    std::vector<uint32_t> free_slots;
+    JobP* d_jobs = nullptr;
+    BatchPlanes* d_planes = nullptr;
+    BatchItem* d_items[2] = {nullptr, nullptr};
+    BatchItem* h_items[2] = {nullptr, nullptr};
+    size_t items_cap[2] = {0, 0};
+    uint32_t* d_claim[2] = {nullptr, nullptr};
+    unsigned long long* d_ctr[2] = {nullptr, nullptr};
+    hipEvent_t ev_pool[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t quantum = 0, target_us = 0, grid = 0;
+    double ns_per_unit = 0.0;
+    uint64_t hbm_in_use = 0, hbm_peak = 0, budget = 0;
+    uint64_t submitted = 0, completed = 0, cancelled = 0, launches = 0, rays = 0, primary = 0;
+    double device_ms = 0.0;
+    std::chrono::steady_clock::time_point t0;
+    int error = SYF_OK;
+    std::string error_text;
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+constexpr uint32_t TILE = 8;
+constexpr size_t BYTES_PER_PIXEL = 16 + 4 + 16 + 8;  // accum, depth, aux, depth64
+
+// Device time estimate of a work item: pixel-samples x segments per path.
+inline double item_units(const Request* r, uint32_t ns) {
+    return double(TILE * TILE) * double(ns) * double(r->P.max_bounces + 1u);
+}
+
+void set_server_error(syf_server* s, int code, const char* what, hipError_t e) {
+    s->error = code;
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    s->error_text = buf;
+}
+
+// Allocates a request's planes and its slot (caller holds the lock).
+bool admit(syf_server* s, Request* r) {
+    if (s->free_slots.empty()) return false;
+    if (s->hbm_in_use + r->footprint > s->budget) return false;
+    hipError_t e = hipMalloc(&r->accum, r->pixels * 16);
+    if (e == hipSuccess) e = hipMalloc(&r->depth, r->pixels * 4);
+    if (e == hipSuccess) e = hipMalloc(&r->aux, r->pixels * 16);
+    if (e == hipSuccess) e = hipMalloc(&r->depth64, r->pixels * 8);
+    if (e == hipSuccess) e = hipMemsetAsync(r->accum, 0, r->pixels * 16, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(r->depth, 0, r->pixels * 4, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(r->aux, 0, r->pixels * 16, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(r->depth64, 0, r->pixels * 8, s->stream);
+    if (e != hipSuccess) {
+        hipFree(r->accum);
+        hipFree(r->depth);
+        hipFree(r->aux);
+        hipFree(r->depth64);
+        r->accum = r->depth = r->aux = r->depth64 = nullptr;
+        return false;
+    }
+    r->slot = s->free_slots.back();
+    s->free_slots.pop_back();
+    r->planes = BatchPlanes{(float4*)r->accum, (float*)r->depth, (float4*)r->aux, (double*)r->depth64};
+    // sources live in the request, which outlives every launch that reads its slot
+    e = hipMemcpyAsync(&s->d_jobs[r->slot], &r->P, sizeof(JobP), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&s->d_planes[r->slot], &r->planes, sizeof(BatchPlanes), hipMemcpyHostToDevice, s->stream);
+    if (e != hipSuccess) {
+        set_server_error(s, SYF_E_DEVICE, "request upload", e);
+        return false;
+    }
+    s->hbm_in_use += r->footprint;
+    s->hbm_peak = std::max(s->hbm_peak, s->hbm_in_use);
+    r->state = SYF_REQ_ACTIVE;
+    s->active.push_back(r);
+    return true;
+}
+
+void free_request_planes(syf_server* s, Request* r) {
+    if (r->accum || r->depth || r->aux || r->depth64) {
+        hipFree(r->accum);
+        hipFree(r->depth);
+        hipFree(r->aux);
+        hipFree(r->depth64);
+        r->accum = r->depth = r->aux = r->depth64 = nullptr;
+        s->hbm_in_use -= r->footprint;
+    }
+    if (r->slot != ~0u) {
+        s->free_slots.push_back(r->slot);
+        r->slot = ~0u;
+    }
+}
+
+bool ensure_items(syf_server* s, uint32_t buf, size_t n) {
+    if (n <= s->items_cap[buf]) return true;
+    size_t cap = std::max<size_t>(n, s->items_cap[buf] * 2);
+    hipFree(s->d_items[buf]);
+    hipHostFree(s->h_items[buf]);
+    s->d_items[buf] = nullptr;
+    s->h_items[buf] = nullptr;
+    s->items_cap[buf] = 0;
+    if (hipMalloc(&s->d_items[buf], cap * sizeof(BatchItem)) != hipSuccess) return false;
+    if (hipHostMalloc(&s->h_items[buf], cap * sizeof(BatchItem), hipHostMallocDefault) != hipSuccess) return false;
+    s->items_cap[buf] = cap;
+    return true;
+}
+
+// Plans and enqueues one batched launch (caller holds the lock).  Returns false when there
+// is nothing to launch.
+bool plan_launch(syf_server* s) {
+    const uint32_t buf = s->launches & 1u;
+    std::vector<Request*> order;
+    for (Request* r : s->active)
+        if (!r->cancelled && r->planned < r->total) order.push_back(r);
+    if (order.empty()) return false;
+    std::stable_sort(order.begin(), order.end(), [](const Request* a, const Request* b) {
+        return (a->total - a->planned) * (a->P.max_bounces + 1u) < (b->total - b->planned) * (b->P.max_bounces + 1u);
+    });
+    const double budget_units = double(s->target_us) * 1000.0 / std::max(s->ns_per_unit, 1e-6);
+    std::vector<BatchItem> by_family[FAM_COUNT];
+    size_t n_items = 0;
+    Launch L;
+    double units = 0.0;
+    for (Request* r : order) {
+        if (units >= budget_units && n_items > 0) break;
+        uint64_t added = 0;
+        // at most one quantum of a request per launch: two items of one pixel in one launch
+        // would race on its accumulation and could add its samples out of order
+        std::vector<BatchItem>& items = by_family[kernel_family(r->P.f)];
+        while (r->planned < r->total && (units < budget_units || n_items == 0)) {
+            if (r->q_cursor == 0) {  // open the next quantum
+                r->q_start = uint32_t(r->planned / r->tiles);
+                r->q_len = std::min(s->quantum, r->spp - r->q_start);
+            }
+            const uint32_t t = r->q_cursor;
+            items.push_back(BatchItem{r->slot, (t % r->tiles_x) * TILE, (t / r->tiles_x) * TILE, r->q_start, r->q_len});
+            n_items++;
+            units += item_units(r, r->q_len);
+            r->planned += r->q_len;
+            added += r->q_len;
+            if (++r->q_cursor == r->tiles) {
+                r->q_cursor = 0;
+                break;
+            }
+        }
+        if (added) {
+            L.progress.push_back({r, added});
+            r->refs++;
+            if (r->start_ms < 0) r->start_ms = s->now_ms();
+        }
+    }
+    if (n_items == 0) return false;
+    if (!ensure_items(s, buf, n_items)) {
+        s->error = SYF_E_DEVICE;
+        s->error_text = "work-item buffers: out of memory";
+        return false;
+    }
+    // one persistent kernel per kernel family present, back to back on the stream; the items
+    // of a family stay in shortest-remaining-first order
+    uint32_t first[FAM_COUNT], count[FAM_COUNT];
+    size_t at = 0;
+    for (uint32_t f = 0; f < FAM_COUNT; ++f) {
+        first[f] = uint32_t(at);
+        count[f] = uint32_t(by_family[f].size());
+        if (count[f]) std::memcpy(s->h_items[buf] + at, by_family[f].data(), count[f] * sizeof(BatchItem));
+        at += count[f];
+    }
+    L.buf = buf;
+    L.units = units;
+    L.start = s->ev_pool[2 * buf];
+    L.end = s->ev_pool[2 * buf + 1];
+    hipError_t e = hipMemcpyAsync(s->d_items[buf], s->h_items[buf], n_items * sizeof(BatchItem),
+                                  hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_claim[buf], 0, FAM_COUNT * CLAIM_WORDS * sizeof(uint32_t), s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_ctr[buf], 0, CTR_COUNT * sizeof(unsigned long long), s->stream);
+    if (e == hipSuccess) e = hipEventRecord(L.start, s->stream);
+    for (uint32_t f = 0; f < FAM_COUNT && e == hipSuccess; ++f)
+        if (count[f])
+            e = launch_batch(f, s->d_jobs, s->d_planes, s->d_items[buf] + first[f], count[f],
+                             s->d_claim[buf] + f * CLAIM_WORDS, s->d_ctr[buf], s->grid, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(L.end, s->stream);
+    if (e != hipSuccess) {
+        set_server_error(s, SYF_E_DEVICE, "batch launch", e);
+        return false;
+    }
+    s->launches++;
+    s->inflight.push_back(std::move(L));
+    return true;
+}
+
+// Marks an active request done (caller holds the lock).
+void finish(syf_server* s, Request* r, double t) {
+    r->state = SYF_REQ_DONE;
+    r->done_ms = t;
+    s->completed++;
+    auto it = std::find(s->active.begin(), s->active.end(), r);
+    if (it != s->active.end()) s->active.erase(it);
+}
+
+// Retires the oldest launch after its end event (entered with the lock held; waits without it).
+void retire(syf_server* s, std::unique_lock<std::mutex>& lk) {
+    Launch L = std::move(s->inflight.front());
+    s->inflight.pop_front();
+    lk.unlock();
+    hipError_t e = hipEventSynchronize(L.end);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, L.start, L.end);
+    unsigned long long c[CTR_COUNT] = {0};
+    if (e == hipSuccess) e = hipMemcpy(c, s->d_ctr[L.buf], sizeof c, hipMemcpyDeviceToHost);
+    const double t = s->now_ms();
+    lk.lock();
+    if (e != hipSuccess) {
+        set_server_error(s, SYF_E_DEVICE, "batch completion", e);
+        return;
+    }
+    s->device_ms += ms;
+    s->rays += c[CTR_RAYS];
+    s->primary += c[CTR_PRIMARY];
+    if (L.units > 0 && ms > 0) {
+        const double meas = double(ms) * 1e6 / L.units;
+        s->ns_per_unit = s->ns_per_unit > 0 ? 0.7 * s->ns_per_unit + 0.3 * meas : meas;
+    }
+    for (auto& pr : L.progress) {
+        Request* r = pr.first;
+        r->completed += pr.second;
+        r->refs--;
+        if (r->completed >= r->total && r->state == SYF_REQ_ACTIVE) finish(s, r, t);
+        if (r->released && r->refs == 0) delete r;
+    }
+    s->cv.notify_all();
+}
+
+bool run_plain(syf_server* s, std::unique_lock<std::mutex>& lk) {
+    Request* r = nullptr;
+    while (!s->queue.empty()) {
+        r = s->queue.front();
+        if (r->cancelled) {
+            s->queue.pop_front();
+            r = nullptr;
+            continue;
+        }
+        if (!admit(s, r)) {
+            if (s->hbm_in_use > 0 || !s->active.empty()) return false;  // wait for releases
+            r->state = SYF_REQ_CANCELLED;
+            r->cancelled = true;
+            s->cancelled++;
+            s->queue.pop_front();
+            s->cv.notify_all();
+            r = nullptr;
+            continue;
+        }
+        s->queue.pop_front();
+        break;
+    }
+    if (!r) return false;
+    r->start_ms = s->now_ms();
+    r->planned = r->total;
+    r->refs++;
+    lk.unlock();
+    hipError_t e = hipMemsetAsync(s->d_ctr[0], 0, CTR_COUNT * sizeof(unsigned long long), s->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->ev_pool[0], s->stream);
+    if (e == hipSuccess)
+        e = launch_render_tile(r->P, 0, 0, r->P.width, r->P.height, 0, r->spp, (float4*)r->accum, (float*)r->depth,
+                               (float4*)r->aux, (double*)r->depth64, s->d_ctr[0], s->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->ev_pool[1], s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(s->ev_pool[1]);
+    float ms = 0.0f;
+    unsigned long long c[CTR_COUNT] = {0};
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->ev_pool[0], s->ev_pool[1]);
+    if (e == hipSuccess) e = hipMemcpy(c, s->d_ctr[0], sizeof c, hipMemcpyDeviceToHost);
+    const double t = s->now_ms();
+    lk.lock();
+    r->refs--;
+    if (e != hipSuccess) {
+        set_server_error(s, SYF_E_DEVICE, "plain launch", e);
+        return false;
+    }
+    s->launches++;
+    s->device_ms += ms;
+    s->rays += c[CTR_RAYS];
+    s->primary += c[CTR_PRIMARY];
+    r->completed = r->total;
+    if (r->state == SYF_REQ_ACTIVE) finish(s, r, t);  // a cancellation during the render wins
+    if (r->released && r->refs == 0) delete r;
+    s->cv.notify_all();
+    return true;
+}
+
+void worker_main(syf_server* s) {
+    hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        if (s->error != SYF_OK) {  // stays stopped: waiters see the error, destroy cleans up
+            s->cv.notify_all();
+            s->cv.wait(lk, [&] { return s->stop; });
+            break;
+        }
+        if (s->cfg.mode == SYF_SERVE_PLAIN) {
+            s->cv.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+            if (s->stop) break;
+            if (!run_plain(s, lk) && !s->stop) s->cv.wait_for(lk, std::chrono::milliseconds(5));  // memory held
+            continue;
+        }
+        // admissions in arrival order
+        while (!s->queue.empty()) {
+            Request* r = s->queue.front();
+            if (r->cancelled) {
+                s->queue.pop_front();
+                continue;
+            }
+            if (s->active.size() >= s->cfg.max_active) break;
+            if (!admit(s, r)) {
+                if (s->error != SYF_OK || s->hbm_in_use > 0 || !s->inflight.empty()) break;  // wait for releases
+                // nothing else holds device memory and the allocation still failed: refuse it
+                r->state = SYF_REQ_CANCELLED;
+                r->cancelled = true;
+                s->cancelled++;
+                s->queue.pop_front();
+                s->cv.notify_all();
+                continue;
+            }
+            s->queue.pop_front();
+        }
+        bool planned = false;
+        while (!s->stop && s->inflight.size() < 2 && plan_launch(s)) planned = true;
+        if (!s->inflight.empty()) {
+            retire(s, lk);
+            continue;
+        }
+        if (s->stop) break;
+        // idle: woken by a submission, a release or a cancellation
+        if (!planned) s->cv.wait_for(lk, std::chrono::milliseconds(5));
+    }
+}
+
+// Autotuner: times a fixed calibration workload (a 384x384 Mandelbulb job, 8 samples, 2
+// bounces, in 8x8-tile items) for each candidate (samples per work item x workgroups per
+// launch), keeps the fastest and seeds the cost model (ns per unit) with its measured rate.
+void autotune(syf_server* s) {
+    syf_job_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.fractal.family = SYF_FAMILY_MANDELBULB;
+    d.fractal.iterations = 10;
+    d.fractal.bailout = 2.0f;
+    d.fractal.power = 8.0f;
+    d.fractal.trap_scale = 4.0f;
+    d.fractal.safety = 0.7f;
+    d.fractal.bounding_radius_hint = 2.5f;
+    d.fractal.plane_normal[1] = 1.0f;
+    d.camera.mode = SYF_CAMERA_FREE;
+    d.camera.position[2] = -2.5f;
+    d.camera.focal_mm = 30.0f;
+    d.material.albedo[0] = d.material.albedo[1] = d.material.albedo[2] = 0.8f;
+    d.material.roughness = 0.4f;
+    d.material.specular = 0.04f;
+    d.light_count = 1;
+    d.lights[0].kind = SYF_LIGHT_ENV;
+    for (int i = 0; i < 3; ++i) d.lights[0].zenith[i] = d.lights[0].horizon[i] = d.lights[0].ground[i] = 1.0f;
+    d.lights[0].intensity = 1.0f;
+    d.width = d.height = 384;
+    d.max_bounces = 2;
+    d.max_steps = 256;
+    d.pixel_tolerance = 1.0f;
+    d.seed = 1;
+    JobP P;
+    if (lower_job(d, P) != SYF_OK) return;
+    const size_t pixels = size_t(d.width) * d.height;
+    float4 *accum = nullptr, *aux = nullptr;
+    float* depth = nullptr;
+    double* depth64 = nullptr;
+    JobP* djob = nullptr;
+    BatchPlanes* dpl = nullptr;
+    BatchItem* ditems = nullptr;
+    uint32_t* claim = nullptr;
+    unsigned long long* ctr = nullptr;
+    const uint32_t tiles_x = d.width / TILE, tiles = tiles_x * (d.height / TILE), spp = 8;
+    hipEvent_t a, b;
+    bool ok = hipMalloc(&accum, pixels * 16) == hipSuccess && hipMalloc(&aux, pixels * 16) == hipSuccess &&
+              hipMalloc(&depth, pixels * 4) == hipSuccess && hipMalloc(&depth64, pixels * 8) == hipSuccess &&
+              hipMalloc(&djob, sizeof(JobP)) == hipSuccess &&
+              hipMalloc(&dpl, sizeof(BatchPlanes)) == hipSuccess &&
+              hipMalloc(&ditems, size_t(tiles) * spp * sizeof(BatchItem)) == hipSuccess &&
+              hipMalloc(&claim, CLAIM_WORDS * 4) == hipSuccess && hipMalloc(&ctr, CTR_COUNT * 8) == hipSuccess &&
+              hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess;
+    BatchPlanes pl{accum, depth, aux, depth64};
+    ok = ok && hipMemcpy(djob, &P, sizeof P, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dpl, &pl, sizeof pl, hipMemcpyHostToDevice) == hipSuccess;
+    const uint32_t cu = uint32_t(s->ctx->cu_count > 0 ? s->ctx->cu_count : 256);
+    const uint32_t q_cands[3] = {1, 2, 4};
+    const uint32_t g_cands[3] = {cu * 4, cu * 8, cu * 16};
+    double best = 1e30;
+    uint32_t bq = 2, bg = cu * 8;
+    double best_units_ns = 0.0;
+    for (uint32_t qi = 0; ok && qi < 3; ++qi) {
+        std::vector<BatchItem> items;
+        const uint32_t q = q_cands[qi];
+        for (uint32_t s0 = 0; s0 < spp; s0 += q)
+            for (uint32_t t = 0; t < tiles; ++t) items.push_back(BatchItem{0, (t % tiles_x) * TILE, (t / tiles_x) * TILE, s0, q});
+        // one item per tile and quantum must run in sample order: separate launches per quantum
+        for (uint32_t gi = 0; ok && gi < 3; ++gi) {
+            double total_ms = 0.0;
+            for (int rep = 0; rep < 2 && ok; ++rep) {  // first repetition warms up
+                float ms_sum = 0.0f;
+                for (uint32_t s0 = 0; s0 < spp; s0 += q) {
+                    const size_t off = size_t(s0 / q) * tiles;
+                    ok = hipMemcpy(ditems, items.data() + off, size_t(tiles) * sizeof(BatchItem),
+                                   hipMemcpyHostToDevice) == hipSuccess &&
+                         hipMemset(claim, 0, CLAIM_WORDS * 4) == hipSuccess && hipEventRecord(a, s->stream) == hipSuccess &&
+                         launch_batch(kernel_family(P.f), djob, dpl, ditems, tiles, claim, ctr, g_cands[gi],
+                                      s->stream) == hipSuccess &&
+                         hipEventRecord(b, s->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess;
+                    float ms = 0.0f;
+                    if (ok) hipEventElapsedTime(&ms, a, b);
+                    ms_sum += ms;
+                }
+                total_ms = ms_sum;
+            }
+            if (ok && total_ms < best) {
+                best = total_ms;
+                bq = q;
+                bg = g_cands[gi];
+                best_units_ns = total_ms * 1e6 / (double(pixels) * spp * (d.max_bounces + 1));
+            }
+        }
+    }
+    if (s->cfg.quantum_samples == 0) s->quantum = bq;
+    if (s->cfg.grid_blocks == 0) s->grid = bg;
+    if (best_units_ns > 0) s->ns_per_unit = best_units_ns;
+    hipFree(accum);
+    hipFree(aux);
+    hipFree(depth);
+    hipFree(depth64);
+    hipFree(djob);
+    hipFree(dpl);
+    hipFree(ditems);
+    hipFree(claim);
+    hipFree(ctr);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+}
+
+Request* find(syf_server* s, uint64_t id) {
+    auto it = s->reqs.find(id);
+    return it == s->reqs.end() ? nullptr : it->second;
+}
+
+}  // namespace
+
+extern "C" {
+
+int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** out) {
+    clear_error();
+    if (!ctx || !cfg || !out) return fail(SYF_E_INVALID, "ctx, cfg and out must not be null");
+    if (cfg->mode > SYF_SERVE_PLAIN) return fail(SYF_E_INVALID, "mode: unknown serving mode %u", cfg->mode);
+    if (cfg->quantum_samples > 4096) return fail(SYF_E_INVALID, "quantum_samples must be at most 4096");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    syf_server* s = new syf_server();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    if (s->cfg.max_active == 0) s->cfg.max_active = 4096;
+    s->t0 = std::chrono::steady_clock::now();
+    size_t free_b = 0, total_b = 0;
+    hipMemGetInfo(&free_b, &total_b);
+    s->budget = cfg->hbm_budget_bytes ? cfg->hbm_budget_bytes : uint64_t(double(free_b) * 0.9);
+    s->max_slots = s->cfg.max_active;
+    for (uint32_t i = 0; i < s->max_slots; ++i) s->free_slots.push_back(s->max_slots - 1u - i);
+    const uint32_t cu = uint32_t(ctx->cu_count > 0 ? ctx->cu_count : 256);
+    s->quantum = cfg->quantum_samples ? cfg->quantum_samples : 2u;
+    s->grid = cfg->grid_blocks ? cfg->grid_blocks : cu * 8u;
+    s->target_us = cfg->batch_target_us ? cfg->batch_target_us : 5000u;
+    s->ns_per_unit = 0.05;
+    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&s->d_jobs, size_t(s->max_slots) * sizeof(JobP))) != hipSuccess ||
+        (e = hipMalloc(&s->d_planes, size_t(s->max_slots) * sizeof(BatchPlanes))) != hipSuccess) {
+        syf_server_destroy(s);
+        return hip_fail(e, "server allocation");
+    }
+    for (int b = 0; b < 2; ++b) {
+        if ((e = hipMalloc(&s->d_claim[b], FAM_COUNT * CLAIM_WORDS * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMalloc(&s->d_ctr[b], CTR_COUNT * sizeof(unsigned long long))) != hipSuccess) {
+            syf_server_destroy(s);
+            return hip_fail(e, "server allocation");
+        }
+    }
+    for (auto& ev : s->ev_pool) {
+        if ((e = hipEventCreate(&ev)) != hipSuccess) {
+            syf_server_destroy(s);
+            return hip_fail(e, "server events");
+        }
+    }
+    if (s->cfg.mode == SYF_SERVE_BATCHED && (cfg->quantum_samples == 0 || cfg->grid_blocks == 0)) autotune(s);
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return SYF_OK;
+}
+
+int syf_server_destroy(syf_server* s) {
+    clear_error();
+    if (!s) return SYF_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    hipSetDevice(s->ctx->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    // released requests still referred to by launches left behind by an error
+    std::vector<Request*> orphans;
+    for (auto& L : s->inflight)
+        for (auto& pr : L.progress)
+            if (pr.first->released && std::find(orphans.begin(), orphans.end(), pr.first) == orphans.end())
+                orphans.push_back(pr.first);
+    for (Request* r : orphans) delete r;
+    for (auto& kv : s->reqs) {
+        free_request_planes(s, kv.second);
+        delete kv.second;
+    }
+    hipFree(s->d_jobs);
+    hipFree(s->d_planes);
+    for (int b = 0; b < 2; ++b) {
+        hipFree(s->d_items[b]);
+        hipHostFree(s->h_items[b]);
+        hipFree(s->d_claim[b]);
+        hipFree(s->d_ctr[b]);
+    }
+    for (auto& ev : s->ev_pool)
+        if (ev) hipEventDestroy(ev);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return SYF_OK;
+}
+
+int syf_server_submit(syf_server* s, const syf_job_desc* desc, uint32_t spp, uint64_t* id) {
+    clear_error();
+    if (!s || !desc || !id) return fail(SYF_E_INVALID, "server, desc and id must not be null");
+    if (spp < 1 || spp > 65536) return fail(SYF_E_INVALID, "spp must be between 1 and 65536, got %u", spp);
+    Request* r = new Request();
+    int rc = lower_job(*desc, r->P);
+    if (rc != SYF_OK) {
+        delete r;
+        return rc;
+    }
+    r->spp = spp;
+    r->pixels = size_t(desc->width) * desc->height;
+    r->footprint = uint64_t(r->pixels) * BYTES_PER_PIXEL;
+    r->tiles_x = (desc->width + TILE - 1) / TILE;
+    r->tiles = r->tiles_x * ((desc->height + TILE - 1) / TILE);
+    r->total = uint64_t(r->tiles) * spp;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->error != SYF_OK) {
+        delete r;
+        return fail(s->error, "server stopped after an error: %s", s->error_text.c_str());
+    }
+    if (r->footprint > s->budget) {
+        const uint64_t fp = r->footprint;
+        delete r;
+        return fail(SYF_E_ADMISSION, "request needs %llu device bytes, above the server's HBM budget of %llu",
+                    (unsigned long long)fp, (unsigned long long)s->budget);
+    }
+    r->id = s->next_id++;
+    r->submit_ms = s->now_ms();
+    s->reqs[r->id] = r;
+    s->queue.push_back(r);
+    s->submitted++;
+    *id = r->id;
+    s->cv.notify_all();
+    return SYF_OK;
+}
+
+int syf_server_poll(syf_server* s, uint64_t id, syf_request_info* out) {
+    clear_error();
+    if (!s || !out) return fail(SYF_E_INVALID, "server and out must not be null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    Request* r = find(s, id);
+    if (!r) return fail(SYF_E_NOT_FOUND, "unknown request id %llu", (unsigned long long)id);
+    std::memset(out, 0, sizeof *out);
+    out->state = r->state;
+    out->samples_done = r->tiles ? uint32_t(r->completed / r->tiles) : 0;
+    out->spp = r->spp;
+    out->footprint_bytes = r->footprint;
+    out->submit_ms = r->submit_ms;
+    out->start_ms = r->start_ms;
+    out->done_ms = r->done_ms;
+    return SYF_OK;
+}
+
+int syf_server_wait(syf_server* s, uint64_t id, uint32_t timeout_ms) {
+    clear_error();
+    if (!s) return fail(SYF_E_INVALID, "server must not be null");
+    std::unique_lock<std::mutex> lk(s->mu);
+    Request* r = find(s, id);
+    if (!r) return fail(SYF_E_NOT_FOUND, "unknown request id %llu", (unsigned long long)id);
+    const bool ok = s->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] {
+        return r->state == SYF_REQ_DONE || r->state == SYF_REQ_CANCELLED || s->error != SYF_OK;
+    });
+    if (s->error != SYF_OK) return fail(s->error, "server error: %s", s->error_text.c_str());
+    if (!ok) return fail(SYF_E_TIMEOUT, "request %llu not finished after %u ms", (unsigned long long)id, timeout_ms);
+    return SYF_OK;
+}
+
+int syf_server_wait_all(syf_server* s, uint32_t timeout_ms) {
+    clear_error();
+    if (!s) return fail(SYF_E_INVALID, "server must not be null");
+    std::unique_lock<std::mutex> lk(s->mu);
+    const bool ok = s->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] {
+        return s->error != SYF_OK || s->completed + s->cancelled >= s->submitted;
+    });
+    if (s->error != SYF_OK) return fail(s->error, "server error: %s", s->error_text.c_str());
+    if (!ok) return fail(SYF_E_TIMEOUT, "requests still running after %u ms", timeout_ms);
+    return SYF_OK;
+}
+
+int syf_server_result(syf_server* s, uint64_t id, uint32_t plane, void* host, size_t bytes) {
+    clear_error();
+    if (!s || !host) return fail(SYF_E_INVALID, "server and host must not be null");
+    void* src = nullptr;
+    size_t need = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        Request* r = find(s, id);
+        if (!r) return fail(SYF_E_NOT_FOUND, "unknown request id %llu", (unsigned long long)id);
+        if (r->state != SYF_REQ_DONE) return fail(SYF_E_INVALID, "request %llu is not finished", (unsigned long long)id);
+        switch (plane) {
+        case SYF_PLANE_ACCUM: src = r->accum; need = r->pixels * 16; break;
+        case SYF_PLANE_DEPTH: src = r->depth; need = r->pixels * 4; break;
+        case SYF_PLANE_AUX: src = r->aux; need = r->pixels * 16; break;
+        case SYF_PLANE_DEPTH64: src = r->depth64; need = r->pixels * 8; break;
+        default: return fail(SYF_E_INVALID, "plane: a request has ACCUM, DEPTH, AUX and DEPTH64 planes, got %u", plane);
+        }
+        if (!src) return fail(SYF_E_INVALID, "request %llu was released", (unsigned long long)id);
+    }
+    if (bytes != need) return fail(SYF_E_INVALID, "bytes must be %zu, got %zu", need, bytes);
+    hipSetDevice(s->ctx->device);
+    hipError_t e = hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_server_result");
+}
+
+int syf_server_release(syf_server* s, uint64_t id) {
+    clear_error();
+    if (!s) return fail(SYF_E_INVALID, "server must not be null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    Request* r = find(s, id);
+    if (!r) return fail(SYF_E_NOT_FOUND, "unknown request id %llu", (unsigned long long)id);
+    if (r->state != SYF_REQ_DONE && r->state != SYF_REQ_CANCELLED)
+        return fail(SYF_E_INVALID, "request %llu is still running", (unsigned long long)id);
+    hipSetDevice(s->ctx->device);
+    hipStreamSynchronize(s->stream);  // launches of a cancelled request may still write its planes
+    free_request_planes(s, r);
+    s->reqs.erase(r->id);
+    r->released = true;
+    if (r->refs == 0) delete r;  // otherwise the last launch that refers to it deletes it
+    s->cv.notify_all();
+    return SYF_OK;
+}
+
+int syf_server_cancel(syf_server* s, uint64_t id) {
+    clear_error();
+    if (!s) return fail(SYF_E_INVALID, "server must not be null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    Request* r = find(s, id);
+    if (!r) return fail(SYF_E_NOT_FOUND, "unknown request id %llu", (unsigned long long)id);
+    if (r->state == SYF_REQ_DONE || r->state == SYF_REQ_CANCELLED) return SYF_OK;
+    r->cancelled = true;
+    if (r->state == SYF_REQ_QUEUED) {
+        auto q = std::find(s->queue.begin(), s->queue.end(), r);
+        if (q != s->queue.end()) s->queue.erase(q);
+    } else {
+        auto it = std::find(s->active.begin(), s->active.end(), r);
+        if (it != s->active.end()) s->active.erase(it);
+        // launches already holding its items still write its planes; syf_server_release
+        // waits for them before freeing the planes
+    }
+    r->state = SYF_REQ_CANCELLED;
+    s->cancelled++;
+    s->cv.notify_all();
+    return SYF_OK;
+}
+
+int syf_server_stats(syf_server* s, syf_server_counters* out) {
+    clear_error();
+    if (!s || !out) return fail(SYF_E_INVALID, "server and out must not be null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(out, 0, sizeof *out);
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->cancelled = s->cancelled;
+    out->launches = s->launches;
+    out->rays = s->rays;
+    out->primary_rays = s->primary;
+    out->hbm_in_use = s->hbm_in_use;
+    out->hbm_peak = s->hbm_peak;
+    out->device_ms = s->device_ms;
+    out->wall_ms = s->now_ms();
+    out->quantum_samples = s->quantum;
+    out->batch_target_us = s->target_us;
+    out->grid_blocks = s->grid;
+    return SYF_OK;
+}
+
+}  // extern "C"

@@ -163,9 +163,20 @@ __device__ __forceinline__ float area_weight(F3 p, F3 n, const LightP& l) {  // 
     float facing = fmaxf(dot3(ld3(l.n), neg3(dir)), 0.0f);
     return luminance(ld3(l.color)) * l.area * facing * fmaxf(dot3(n, dir), 0.15f) / dist2;
 }
+// Loops over the job's lights run over the fixed capacity, unrolled, and stop at light_count:
+// with constant indices the light table stays in registers/constant loads instead of being
+// copied to scratch for a dynamic index.
+#define LIGHT_LOOP(i) _Pragma("unroll") for (uint32_t i = 0; i < SYF_MAX_LIGHTS; ++i) if (i < P.light_count)
+__device__ __forceinline__ F3 light_normal(const JobP& P, uint32_t k) {
+    F3 n = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t i = 0; i < SYF_MAX_LIGHTS; ++i)
+        if (i == k) n = ld3(P.lights[i].n);
+    return n;
+}
 __device__ __forceinline__ float area_total_weight(const JobP& P, F3 p, F3 n) {
     float total = 0.0f;
-    for (uint32_t i = 0; i < P.light_count; ++i)
+    LIGHT_LOOP(i)
         if (P.lights[i].kind == SYF_LIGHT_AREA) total += area_weight(p, n, P.lights[i]);
     return total;
 }
@@ -173,7 +184,7 @@ __device__ __forceinline__ float area_total_weight(const JobP& P, F3 p, F3 n) {
 __device__ __forceinline__ float nearest_emitter(const JobP& P, F3 ro, F3 rd, float t_end, uint32_t& light) {
     float nearest = t_end;
     light = NO_LIGHT;
-    for (uint32_t i = 0; i < P.light_count; ++i) {
+    LIGHT_LOOP(i) {
         if (P.lights[i].kind != SYF_LIGHT_AREA) continue;
         float t = area_intersection(ro, rd, P.lights[i]);
         if (t < nearest) {
@@ -186,7 +197,7 @@ __device__ __forceinline__ float nearest_emitter(const JobP& P, F3 ro, F3 rd, fl
 __device__ __forceinline__ F3 area_emission(const JobP& P, F3 ro, F3 rd, float distance, float prev_pdf, bool use_mis,
                                             F3 prev_n, float prev_total) {  // lights.wgsl:330-357
     F3 L = mk(0.0f, 0.0f, 0.0f);
-    for (uint32_t i = 0; i < P.light_count; ++i) {
+    LIGHT_LOOP(i) {
         const LightP& l = P.lights[i];
         if (l.kind != SYF_LIGHT_AREA) continue;
         float cos_l = dot3(ld3(l.n), neg3(rd));
@@ -208,7 +219,7 @@ __device__ __forceinline__ F3 area_emission(const JobP& P, F3 ro, F3 rd, float d
 __device__ __forceinline__ F3 environment_emission(const JobP& P, F3 dir, float pdf, bool use_mis, F3 n,
                                                    float area_total) {  // pathtrace.wgsl:177-189
     F3 L = mk(0.0f, 0.0f, 0.0f);
-    for (uint32_t i = 0; i < P.light_count; ++i) {
+    LIGHT_LOOP(i) {
         const LightP& l = P.lights[i];
         if (l.kind != SYF_LIGHT_ENV) continue;
         float w = 1.0f;
@@ -237,7 +248,7 @@ __device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F
     if (area_total > 0.0f) {
         float threshold = rand01(rng) * area_total;
         float sum = 0.0f;
-        for (uint32_t i = 0; i < P.light_count; ++i) {
+        LIGHT_LOOP(i) {
             if (P.lights[i].kind != SYF_LIGHT_AREA) continue;
             float w = area_weight(p, s.n, P.lights[i]);
             sum += w;
@@ -248,7 +259,7 @@ __device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F
             }
         }
     }
-    for (uint32_t i = 0; i < P.light_count; ++i) {
+    LIGHT_LOOP(i) {
         const LightP& l = P.lights[i];
         if (l.kind == SYF_LIGHT_ENV) {
             float mx = fmaxf(fmaxf(fmaxf(l.color[0], l.color[1]), l.color[2]),
@@ -322,6 +333,7 @@ struct SegHit {
     float t;
     uint32_t steps, light;
     float trap0;
+    float t_prev;  // march position before a surface hit: lower end of the FP64 refine bracket
 };
 
 // The body of one bounce after its segment was traced (pathtrace.wgsl:1012-1196 subset):
@@ -333,7 +345,7 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
                                             float4& first_aux, Normal&& normal, Vis&& visible) {
     F3 gn = mk(0.0f, 0.0f, 0.0f);
     if (h.hit) {
-        if (h.light != NO_LIGHT) gn = ld3(P.lights[h.light].n);
+        if (h.light != NO_LIGHT) gn = light_normal(P, h.light);
         else gn = normal(add(st.o, scl(st.d, h.t)), fmaxf(h.t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance);
         if (st.bounce == 0u) {
             first_depth = h.t;

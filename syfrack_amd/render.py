@@ -31,7 +31,7 @@ class RenderOutput:
     accum: Optional[np.ndarray]   # (h, w, 4): radiance sums and sample counts
     depth: np.ndarray             # (h, w): first-hit distance, -1 on a miss
     normals: np.ndarray           # (h, w, 4): normal xyz and steps; (0, 0, 0, -1) on a miss
-    depth64: Optional[np.ndarray] = None  # (h, w): FP64-refined hit distance
+    depth64: Optional[np.ndarray] = None  # (h, w): first surface hit refined in FP64, -1 otherwise
     steps: Optional[np.ndarray] = None    # (h, w): primary march steps
     stats: Optional[dict] = None
 
@@ -173,6 +173,6 @@ class Renderer:
                             j.render_tile(x, y, w, h, s0, n)
             j.sync()
             return RenderOutput(j.readback(abi.PLANE_ACCUM), j.readback(abi.PLANE_DEPTH), j.readback(abi.PLANE_AUX),
-                                stats=j.stats())
+                                depth64=j.readback(abi.PLANE_DEPTH64), stats=j.stats())
         finally:
             j.close()

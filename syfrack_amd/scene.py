@@ -281,11 +281,12 @@ def mandelbulb_primary(width: int = 320, height: int = 180, seed: int = 7):
 def mandelbox_pathtrace(width: int = 1920, height: int = 1080, spp: int = 64, seed: int = 11):
     """Mandelbox wavefront path trace: scale 2.0, min radius 0.5, fixed radius 1.0, fold
     limit 1.0, 15 iterations; 4 bounces; Lambert + metal through a trap-driven surface
-    variation (metallic 1 -> 0, roughness 0.3); one area light; max 512 steps."""
+    variation (metallic 1 -> 0, roughness 0.3); one 6x6 area light at (8, 14, 10) facing the
+    origin, on the camera's side of the box; max 512 steps."""
     f = Fractal.bundled("mandelbox")
     f.set_param("fractal.nodes[1].iterations", 15)
     cam = Camera.orbit((0.0, 0.0, 0.0), 11.0, 28.0, 22.0, 35.0)
-    light = Light.area((6.0, 12.0, -8.0), (-6.0, -12.0, 8.0), (6.0, 6.0), (1.0, 0.95, 0.9), 40.0)
+    light = Light.area((8.0, 14.0, 10.0), (-8.0, -14.0, -10.0), (6.0, 6.0), (1.0, 0.95, 0.9), 40.0)
     mat = Material(albedo=(0.8, 0.8, 0.8), roughness=0.3, metallic=0.0, specular=0.04,
                    surface=(0, (0.3, 0.3), (1.0, 0.0)))
     scene = Scene(f, cam, [light], mat, seed=seed)

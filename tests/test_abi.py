@@ -3,7 +3,6 @@ its ctypes mirror has the header's struct sizes.  No kernel is launched here."""
 import ctypes as C
 import os
 import subprocess
-import textwrap
 
 import pytest
 
@@ -23,22 +22,30 @@ def test_library_exports_every_header_symbol():
     assert lib.syf_abi_version() == 1
 
 
-def test_struct_sizes_match_the_header(tmp_path):
+STRUCTS = [
+    ("syf_fractal", abi.SyfFractal), ("syf_camera", abi.SyfCamera), ("syf_light", abi.SyfLight),
+    ("syf_material", abi.SyfMaterial), ("syf_job_desc", abi.SyfJobDesc), ("syf_stats", abi.SyfStats),
+    ("syf_planar_desc", abi.SyfPlanarDesc), ("syf_server_config", abi.SyfServerConfig),
+    ("syf_request_info", abi.SyfRequestInfo), ("syf_server_counters", abi.SyfServerCounters),
+]
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof and offsetof of every field, from gcc on the header, against the ctypes mirror."""
+    lines, expect = [], []
+    for cname, cls in STRUCTS:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        expect.append(C.sizeof(cls))
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+            expect.append(getattr(cls, fname).offset)
     src = tmp_path / "sz.c"
-    src.write_text(textwrap.dedent(f"""
-        #include <stdio.h>
-        #include "{HEADER}"
-        int main(void) {{
-          printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(syf_fractal), sizeof(syf_camera), sizeof(syf_light),
-                 sizeof(syf_material), sizeof(syf_job_desc), sizeof(syf_stats), sizeof(syf_planar_desc));
-          return 0;
-        }}"""))
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n" f'#include "{HEADER}"\n'
+                   "int main(void) {\n" + "\n".join(lines) + "\nreturn 0;\n}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
-    sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    mine = [C.sizeof(t) for t in (abi.SyfFractal, abi.SyfCamera, abi.SyfLight, abi.SyfMaterial, abi.SyfJobDesc,
-                                  abi.SyfStats, abi.SyfPlanarDesc)]
-    assert sizes == mine
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == expect
 
 
 def test_error_text_is_thread_local_and_null_checked():

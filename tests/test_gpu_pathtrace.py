@@ -6,7 +6,9 @@ Tolerances:
   * megakernel vs the oracle's FP32 radiance: per-pixel mean radiance within 1e-3 relative
     (plus 1e-4 absolute) for >= 99 % of pixels, the image mean within 1e-3;
   * wavefront vs megakernel: identical sums (both run the same per-path arithmetic and
-    random stream) to 1e-6 relative;
+    random stream) to 1e-6 relative, identical depth and FP64 depth planes;
+  * FP64 first-hit depth: within 1e-9 relative of the oracle's refine of its FP32 march, and
+    within 1e-4 relative of the FP64 twin on >= 99.9 % of the pixels both hit;
   * tile size and samples-per-dispatch do not change the result (gpu_render.rs:610-635).
 """
 import numpy as np
@@ -56,7 +58,7 @@ def test_megakernel_matches_oracle(renderer):
     from oracle import oracle
     scene, cfg = small_box()
     out = renderer.render(scene, cfg)
-    acc, dep, aux = oracle.render(scene.desc(cfg), (0, 0, cfg.width, cfg.height), 0, cfg.samples_per_pixel)
+    acc, dep, aux, dep64 = oracle.render(scene.desc(cfg), (0, 0, cfg.width, cfg.height), 0, cfg.samples_per_pixel)
     assert np.array_equal(out.accum[..., 3], acc[..., 3])
     g, o = out.accum[..., :3], acc[..., :3]
     close = np.abs(g - o) <= 1e-3 * np.abs(o) + 1e-4
@@ -67,6 +69,16 @@ def test_megakernel_matches_oracle(renderer):
     hits = dep >= 0
     assert np.mean((out.depth >= 0) == hits) >= 0.999
     assert np.allclose(out.depth[hits], dep[hits], rtol=1e-5)
+    # FP64 first-hit refine: against the oracle's refine of its FP32 march, and against the
+    # FP64 twin (FP64 march + refine) where the first segment reached the surface
+    s64 = (out.depth64 >= 0) & (dep64 >= 0)
+    assert np.mean((out.depth64 >= 0) == (dep64 >= 0)) >= 0.999
+    assert np.mean(np.abs(out.depth64[s64] - dep64[s64]) <= 1e-9 * dep64[s64]) >= 0.999
+    twin = oracle.trace_primary(scene.desc(cfg), 0, 64)["depth64"]
+    both = s64 & (twin >= 0)
+    rel = np.abs(out.depth64[both] - twin[both]) / twin[both]
+    print(f"first-hit FP64 depth vs FP64 twin: {both.sum()} pixels, max rel {rel.max():.3e}")
+    assert both.sum() > 100 and np.mean(rel <= 1e-4) >= 0.999
 
 
 def test_wavefront_matches_megakernel(renderer):
@@ -76,6 +88,7 @@ def test_wavefront_matches_megakernel(renderer):
     assert np.array_equal(mk.accum[..., 3], wf.accum[..., 3])
     assert np.allclose(wf.accum, mk.accum, rtol=1e-6, atol=1e-7)
     assert np.array_equal(mk.depth, wf.depth)
+    assert np.array_equal(mk.depth64, wf.depth64)
     st = wf.stats
     assert st["primary_rays"] == 80 * 48 * 3 or st["primary_rays"] > 0
 

@@ -1,0 +1,154 @@
+"""The render server on the device: continuous batching must not change any request's image.
+
+Tolerances:
+  * a request's accumulation, depth, normal and FP64-depth planes are bit-identical whether it
+    runs alone (plain mode: one launch per request), in a mix of uneven requests (batched mode,
+    several launches each holding work of several requests), or through the job API
+    (syf_render_tile in 64-pixel tiles, 8 samples per dispatch): every form runs the same
+    per-pixel arithmetic and adds the samples in ascending order;
+  * the served FP64-refined first-hit distance agrees with the FP64 oracle (its FP64 march and
+    refine) within 1e-4 relative on >= 99.9 % of the pixels both hit, and hit/miss agrees on
+    >= 99.9 % of the pixels;
+  * admission: a request above the budget is refused with SYF_E_ADMISSION; requests that
+    together exceed it wait until earlier ones are released, and the peak stays within it.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from syfrack_amd.render import Context
+    return Context(0)
+
+
+def _planes(out):
+    return out.accum, out.depth, out.normals
+
+
+def _same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_result_same_alone_and_in_the_mix(ctx):
+    from syfrack_amd import abi, serve
+    from syfrack_amd.render import Job
+    reqs = serve.uneven_requests(10, seed=9, min_side=48, max_side=200)
+    print("requests:", [(q.cfg.width, q.spp, q.cfg.max_bounces, q.scene.fractal.family) for q in reqs], flush=True)
+    alone = []
+    with serve.Server(ctx, "plain") as s:
+        for q in reqs:
+            rid = s.submit(q.scene, q.cfg)
+            s.wait(rid, 120)
+            alone.append((_planes(s.result(rid)), s.plane(rid, abi.PLANE_DEPTH64)))
+            s.release(rid)
+        assert s.stats()["launches"] == len(reqs)
+    # batched: one sample per item and a tiny launch target, so every request is split over
+    # many launches that also hold other requests' work; the second half arrives later
+    with serve.Server(ctx, "batched", quantum_samples=1, batch_target_us=300, grid_blocks=256) as s:
+        ids = [s.submit(q.scene, q.cfg) for q in reqs[:5]]
+        time.sleep(0.02)
+        ids += [s.submit(q.scene, q.cfg) for q in reqs[5:]]
+        s.wait_all(300)
+        st = s.stats()
+        print("batched stats:", st, flush=True)
+        assert st["completed"] == len(reqs) and st["launches"] > len(reqs)
+        infos = [s.poll(r) for r in ids]
+        assert all(i.state == "done" and i.samples_done == q.spp for i, q in zip(infos, reqs))
+        overlaps = sum(1 for i in range(len(infos)) for j in range(i + 1, len(infos))
+                       if infos[i].start_ms < infos[j].done_ms and infos[j].start_ms < infos[i].done_ms)
+        assert overlaps > 0
+        for k, (q, rid) in enumerate(zip(reqs, ids)):
+            got = _planes(s.result(rid))
+            assert _same(got, alone[k][0]), k
+            assert np.array_equal(s.plane(rid, abi.PLANE_DEPTH64), alone[k][1]), k
+            s.release(rid)
+        assert s.stats()["hbm_in_use"] == 0
+    # the job API, tile by tile
+    for k in (0, 3, 7):
+        q = reqs[k]
+        j = Job(ctx, q.scene.desc(q.cfg))
+        try:
+            for s0 in range(0, q.spp, 8):
+                n = min(8, q.spp - s0)
+                for y in range(0, q.cfg.height, 64):
+                    for x in range(0, q.cfg.width, 64):
+                        j.render_tile(x, y, min(64, q.cfg.width - x), min(64, q.cfg.height - y), s0, n)
+            j.sync()
+            got = (j.readback(abi.PLANE_ACCUM), j.readback(abi.PLANE_DEPTH), j.readback(abi.PLANE_AUX))
+            assert _same(got, alone[k][0]), k
+            assert np.array_equal(j.readback(abi.PLANE_DEPTH64), alone[k][1]), k
+        finally:
+            j.close()
+
+
+def test_served_depth_matches_fp64_oracle(ctx):
+    from oracle import oracle
+    from syfrack_amd import abi, serve
+    reqs = serve.uneven_requests(3, seed=21, min_side=96, max_side=97)
+    with serve.Server(ctx, "batched") as s:
+        ids = [s.submit(q.scene, q.cfg) for q in reqs]
+        s.wait_all(120)
+        for q, rid in zip(reqs, ids):
+            d64 = s.plane(rid, abi.PLANE_DEPTH64)
+            ref = oracle.trace_primary(q.scene.desc(q.cfg), 0, 64)
+            hit_g, hit_o = d64 >= 0, ref["depth64"] >= 0
+            both = hit_g & hit_o
+            rel = np.abs(d64[both] - ref["depth64"][both]) / ref["depth64"][both]
+            print(f"family {q.scene.fractal.family}: hits {both.sum()}, agreement {np.mean(hit_g == hit_o):.5f}, "
+                  f"rel max {rel.max() if both.any() else 0:.3e}", flush=True)
+            assert np.mean(hit_g == hit_o) >= 0.999
+            if both.any():
+                assert np.mean(rel <= 1e-4) >= 0.999
+
+
+def test_admission_against_the_budget(ctx):
+    from syfrack_amd import abi, serve
+    from syfrack_amd.scene import RenderConfig
+    q = serve.uneven_requests(1, seed=4)[0]
+    cfg = RenderConfig(128, 128, samples_per_pixel=2, max_bounces=1, max_steps=128)
+    fp = serve.footprint(cfg)
+    with serve.Server(ctx, "batched", hbm_budget_bytes=int(fp * 1.5)) as s:
+        with pytest.raises(abi.SyfError) as e:
+            s.submit(q.scene, RenderConfig(256, 256, 1, 1, 128))
+        assert e.value.code == abi.SYF_E_ADMISSION
+        a = s.submit(q.scene, cfg)
+        b = s.submit(q.scene, cfg)
+        s.wait(a, 60)
+        time.sleep(0.05)
+        assert s.poll(b).state == "queued"  # a still holds its planes
+        with pytest.raises(abi.SyfError) as e:
+            s.wait(b, 0.05)
+        assert e.value.code == abi.SYF_E_TIMEOUT
+        s.release(a)
+        s.wait(b, 60)
+        st = s.stats()
+        assert st["hbm_peak"] <= int(fp * 1.5) and st["completed"] == 2
+        s.release(b)
+
+
+def test_cancel_release_and_unknown_ids(ctx):
+    from syfrack_amd import abi, serve
+    from syfrack_amd.scene import RenderConfig
+    q = serve.uneven_requests(1, seed=6)[0]
+    with serve.Server(ctx, "batched", quantum_samples=1, batch_target_us=500) as s:
+        big = s.submit(q.scene, RenderConfig(512, 512, 64, 2, 256))
+        small = s.submit(q.scene, RenderConfig(64, 64, 1, 1, 128))
+        s.cancel(big)
+        assert s.poll(big).state == "cancelled"
+        s.wait(small, 60)
+        s.wait_all(60)
+        with pytest.raises(abi.SyfError) as e:
+            s.plane(big, abi.PLANE_ACCUM)
+        assert e.value.code == abi.SYF_E_INVALID
+        s.release(big)
+        s.release(small)
+        for fn in (lambda: s.poll(big), lambda: s.release(small), lambda: s.cancel(12345)):
+            with pytest.raises(abi.SyfError) as e:
+                fn()
+            assert e.value.code == abi.SYF_E_NOT_FOUND
+        assert s.stats()["hbm_in_use"] == 0

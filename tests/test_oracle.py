@@ -238,3 +238,27 @@ def test_fp32_and_fp64_oracles_agree_on_mandelbulb_config():
     both = ha & hb
     rel = np.abs(a["depth64"][both] - b["depth64"][both]) / b["depth64"][both]
     assert np.mean(rel <= 1e-4) >= 0.999
+
+
+def test_path_trace_first_hit_refine_is_the_primary_refine():
+    """Under an environment light (no emitter to stop the march) a path's first segment is the
+    primary march: the path tracer's depth and FP64-refined depth of sample 0 must equal the
+    primary trace's, and the refined depth must sit on the FP64 twin's within 1e-4."""
+    for name, cam in (("mandelbulb", sc.Camera.orbit((0, 0, 0), 3.0, 30.0, 20.0, 35.0)),
+                      ("menger", sc.Camera.orbit((0, 0, 0), 4.2, 40.0, 30.0, 35.0))):
+        f = sc.Fractal.bundled(name)
+        env = sc.Light.environment((0.9, 0.9, 1.0), (0.5, 0.5, 0.5), (0.2, 0.2, 0.2), 1.0)
+        scene = sc.Scene(f, cam, [env], seed=13)
+        cfg = sc.RenderConfig(48, 40, 2, 2, 256)
+        d = scene.desc(cfg)
+        acc, dep, aux, dep64 = oracle.render(d, (0, 0, 48, 40), 0, 2)
+        p32 = oracle.trace_primary(d, 0, 32)
+        p64 = oracle.trace_primary(d, 0, 64)
+        assert np.array_equal(dep, p32["depth"].astype(np.float32))
+        assert np.array_equal(dep64, p32["depth64"])
+        hit = dep >= 0
+        assert 50 < hit.sum() < hit.size
+        assert np.all(dep64[~hit] == -1.0)
+        both = hit & (p64["depth"] >= 0)
+        rel = np.abs(dep64[both] - p64["depth64"][both]) / p64["depth64"][both]
+        assert np.mean(rel <= 1e-4) >= 0.999
