@@ -54,7 +54,7 @@ def load() -> C.CDLL:
     lib.orc_escape_f64.restype = None
     lib.orc_escape_f64.argtypes = [P] + [C.c_uint32] * 4 + [P, P]
     lib.orc_escape_f128.restype = None
-    lib.orc_escape_f128.argtypes = [P, P, P, C.c_size_t, P, P]
+    lib.orc_escape_f128.argtypes = [P, P, P, C.c_size_t, C.c_double, P, P]
     if hasattr(lib, "orc_render"):
         lib.orc_render.restype = C.c_int
         lib.orc_render.argtypes = [P] + [C.c_uint32] * 6 + [P, P, P, P]
@@ -151,13 +151,26 @@ def escape_f64(desc, rect=None):
     return iters.reshape(h, w), smooth.reshape(h, w)
 
 
-def escape_f128(desc, px: np.ndarray, py: np.ndarray):
+def escape_f128(desc, px: np.ndarray, py: np.ndarray, nudge: float = 0.0):
+    """binary128 counts at c = centre + f32 offset (+ nudge f32 ulps of the offset)."""
     px = np.ascontiguousarray(px, np.uint32)
     py = np.ascontiguousarray(py, np.uint32)
     iters = np.zeros(len(px), np.uint32)
     smooth = np.zeros(len(px), np.float32)
-    load().orc_escape_f128(C.byref(desc), _ptr(px), _ptr(py), len(px), _ptr(iters), _ptr(smooth))
+    load().orc_escape_f128(C.byref(desc), _ptr(px), _ptr(py), C.c_size_t(len(px)), C.c_double(nudge), _ptr(iters),
+                           _ptr(smooth))
     return iters, smooth
+
+
+def escape_f64_points(desc, px: np.ndarray, py: np.ndarray):
+    """FP64 iteration counts at the points escape_f128 evaluates (centre + f32 offset)."""
+    px = np.ascontiguousarray(px, np.uint32)
+    py = np.ascontiguousarray(py, np.uint32)
+    iters = np.zeros(len(px), np.uint32)
+    lib = load()
+    lib.orc_escape_f64_points.restype = None
+    lib.orc_escape_f64_points(C.byref(desc), _ptr(px), _ptr(py), C.c_size_t(len(px)), _ptr(iters))
+    return iters
 
 
 def render(desc, rect, sample_start: int, sample_count: int):

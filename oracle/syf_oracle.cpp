@@ -1387,8 +1387,8 @@ void orc_planar_f32(const syf_planar_desc* d, uint32_t x0, uint32_t y0, uint32_t
 // Quad-precision (binary128) direct evaluation at listed pixels, the deep-zoom pin of the
 // perturbation mode: the pixel's c is centre + (float) offset, exactly the point whose
 // orbit the FP32 deltas follow.
-void orc_escape_f128(const syf_planar_desc* d, const uint32_t* px, const uint32_t* py, size_t n, uint32_t* iters,
-                     float* smooth) {
+void orc_escape_f128(const syf_planar_desc* d, const uint32_t* px, const uint32_t* py, size_t n, double nudge,
+                     uint32_t* iters, float* smooth) {
     double* fx = new double[d->width];
     double* fy = new double[d->height];
     orc_planar_offsets(d, fx, fy);
@@ -1399,13 +1399,39 @@ void orc_escape_f128(const syf_planar_desc* d, const uint32_t* px, const uint32_
     for (long k = 0; k < long(n); ++k) {
         double dx = cr * fx[px[k]] - sr * fy[py[k]];
         double dy = sr * fx[px[k]] + cr * fy[py[k]];
-        __float128 cx = (__float128)d->center[0] + (__float128)(float)dx;
-        __float128 cy = (__float128)d->center[1] + (__float128)(float)dy;
+        // the f32 offset, optionally moved by `nudge` of its own ulps (a conditioning probe)
+        const float fxo = float(dx), fyo = float(dy);
+        const double ux = double(std::nextafter(std::fabs(fxo), INFINITY)) - std::fabs(double(fxo));
+        const double uy = double(std::nextafter(std::fabs(fyo), INFINITY)) - std::fabs(double(fyo));
+        __float128 cx = (__float128)d->center[0] + (__float128)fxo + (__float128)(nudge * ux);
+        __float128 cy = (__float128)d->center[1] + (__float128)fyo + (__float128)(nudge * uy);
         Escape<__float128> e = escape_loop<__float128>(d->family, cx, cy, (__float128)0, (__float128)0, d->iterations, b2);
         iters[k] = e.count;
         double r = std::fmin(std::sqrt((double)e.radius2), 1e20);
         smooth[k] = e.escaped ? float(double(e.count) + 1.0 - std::log(std::log(std::fmax(r, 1.0001))) / std::log(2.0))
                               : float(d->iterations);
+    }
+    delete[] fx;
+    delete[] fy;
+}
+
+// The same points as orc_escape_f128 (c = centre + the f32-rounded offset) iterated in FP64:
+// how far a plain FP64 loop resolves them, the bar a perturbation method is held to.
+void orc_escape_f64_points(const syf_planar_desc* d, const uint32_t* px, const uint32_t* py, size_t n,
+                           uint32_t* iters) {
+    double* fx = new double[d->width];
+    double* fy = new double[d->height];
+    orc_planar_offsets(d, fx, fy);
+    double rot = double(d->rotation) * (3.14159265358979323846 / 180.0);
+    double cr = std::cos(rot), sr = std::sin(rot);
+    double b2 = double(d->bailout) * double(d->bailout);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long k = 0; k < long(n); ++k) {
+        double dx = cr * fx[px[k]] - sr * fy[py[k]];
+        double dy = sr * fx[px[k]] + cr * fy[py[k]];
+        double cx = d->center[0] + double(float(dx));
+        double cy = d->center[1] + double(float(dy));
+        iters[k] = escape_loop<double>(d->family, cx, cy, 0.0, 0.0, d->iterations, b2).count;
     }
     delete[] fx;
     delete[] fy;

@@ -4,8 +4,9 @@
 //   1. admits queued requests in arrival order while their footprint (accumulation, depth,
 //      aux and FP64 depth planes, 44 bytes per pixel) fits the HBM budget and a slot is free;
 //   2. plans one launch: work items (8x8 tile, sample range) of the active requests, shortest
-//      remaining work first, until the launch's estimated device time reaches the target
-//      (a running estimate of ns per pixel-sample-segment, refreshed from every launch);
+//      remaining work first, until the launch holds enough items to fill the device and its
+//      estimated device time reaches the target (a running estimate of ns per
+//      pixel-sample-segment, refreshed from every launch that filled the device);
 //   3. enqueues it (items copied from pinned host memory, claim counters cleared, one
 //      persistent kernel) and keeps up to two launches in flight on its stream, so the
 //      device does not idle while the next launch is planned;
@@ -60,6 +61,7 @@ struct Launch {
     std::vector<std::pair<Request*, uint64_t>> progress;  // request, tile-samples completed by the launch
     uint32_t buf = 0;
     double units = 0;
+    bool full = false;  // enough items to fill the device: its time measures throughput
 };
 
 }  // namespace syf
@@ -90,6 +92,7 @@ struct syf_server {
     unsigned long long* d_ctr[2] = {nullptr, nullptr};
     hipEvent_t ev_pool[4] = {nullptr, nullptr, nullptr, nullptr};
     uint32_t quantum = 0, target_us = 0, grid = 0;
+    uint32_t fill_items = 0;  // work items (one wave each) a launch needs to fill the device
     double ns_per_unit = 0.0;
     uint64_t hbm_in_use = 0, hbm_peak = 0, budget = 0;
     uint64_t submitted = 0, completed = 0, cancelled = 0, launches = 0, rays = 0, primary = 0;
@@ -204,12 +207,12 @@ bool plan_launch(syf_server* s) {
     Launch L;
     double units = 0.0;
     for (Request* r : order) {
-        if (units >= budget_units && n_items > 0) break;
+        if (units >= budget_units && n_items >= s->fill_items) break;
         uint64_t added = 0;
         // at most one quantum of a request per launch: two items of one pixel in one launch
         // would race on its accumulation and could add its samples out of order
         std::vector<BatchItem>& items = by_family[kernel_family(r->P.f)];
-        while (r->planned < r->total && (units < budget_units || n_items == 0)) {
+        while (r->planned < r->total && (units < budget_units || n_items < s->fill_items)) {
             if (r->q_cursor == 0) {  // open the next quantum
                 r->q_start = uint32_t(r->planned / r->tiles);
                 r->q_len = std::min(s->quantum, r->spp - r->q_start);
@@ -249,6 +252,7 @@ bool plan_launch(syf_server* s) {
     }
     L.buf = buf;
     L.units = units;
+    L.full = n_items >= s->fill_items;
     L.start = s->ev_pool[2 * buf];
     L.end = s->ev_pool[2 * buf + 1];
     hipError_t e = hipMemcpyAsync(s->d_items[buf], s->h_items[buf], n_items * sizeof(BatchItem),
@@ -298,7 +302,9 @@ void retire(syf_server* s, std::unique_lock<std::mutex>& lk) {
     s->device_ms += ms;
     s->rays += c[CTR_RAYS];
     s->primary += c[CTR_PRIMARY];
-    if (L.units > 0 && ms > 0) {
+    // a launch too small to fill the device is latency-bound: its time per unit says nothing
+    // about throughput and would shrink the next launches
+    if (L.full && L.units > 0 && ms > 0) {
         const double meas = double(ms) * 1e6 / L.units;
         s->ns_per_unit = s->ns_per_unit > 0 ? 0.7 * s->ns_per_unit + 0.3 * meas : meas;
     }
@@ -415,8 +421,8 @@ void worker_main(syf_server* s) {
     }
 }
 
-// Autotuner: times a fixed calibration workload (a 384x384 Mandelbulb job, 8 samples, 2
-// bounces, in 8x8-tile items) for each candidate (samples per work item x workgroups per
+// Autotuner: times a fixed calibration workload (a 768x768 Mandelbulb job, 4 samples, 2
+// bounces, in 8x8-tile items: enough to fill the device) for each candidate (samples per work item x workgroups per
 // launch), keeps the fastest and seeds the cost model (ns per unit) with its measured rate.
 void autotune(syf_server* s) {
     syf_job_desc d;
@@ -439,7 +445,7 @@ void autotune(syf_server* s) {
     d.lights[0].kind = SYF_LIGHT_ENV;
     for (int i = 0; i < 3; ++i) d.lights[0].zenith[i] = d.lights[0].horizon[i] = d.lights[0].ground[i] = 1.0f;
     d.lights[0].intensity = 1.0f;
-    d.width = d.height = 384;
+    d.width = d.height = 768;
     d.max_bounces = 2;
     d.max_steps = 256;
     d.pixel_tolerance = 1.0f;
@@ -455,7 +461,7 @@ void autotune(syf_server* s) {
     BatchItem* ditems = nullptr;
     uint32_t* claim = nullptr;
     unsigned long long* ctr = nullptr;
-    const uint32_t tiles_x = d.width / TILE, tiles = tiles_x * (d.height / TILE), spp = 8;
+    const uint32_t tiles_x = d.width / TILE, tiles = tiles_x * (d.height / TILE), spp = 4;
     hipEvent_t a, b;
     bool ok = hipMalloc(&accum, pixels * 16) == hipSuccess && hipMalloc(&aux, pixels * 16) == hipSuccess &&
               hipMalloc(&depth, pixels * 4) == hipSuccess && hipMalloc(&depth64, pixels * 8) == hipSuccess &&
@@ -550,7 +556,8 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     const uint32_t cu = uint32_t(ctx->cu_count > 0 ? ctx->cu_count : 256);
     s->quantum = cfg->quantum_samples ? cfg->quantum_samples : 2u;
     s->grid = cfg->grid_blocks ? cfg->grid_blocks : cu * 8u;
-    s->target_us = cfg->batch_target_us ? cfg->batch_target_us : 5000u;
+    s->target_us = cfg->batch_target_us ? cfg->batch_target_us : 10000u;
+    s->fill_items = cu * 16u;  // two workgroups of four waves per CU, twice over
     s->ns_per_unit = 0.05;
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc(&s->d_jobs, size_t(s->max_slots) * sizeof(JobP))) != hipSuccess ||

@@ -10,8 +10,8 @@ GPU (marked gpu):
   * FP64 mode: iteration counts bit-exact against the oracle's FP64 loop;
   * FP32 mode: counts equal for >= 99.9 % of pixels against the f32 restatement;
   * perturbation: against binary128 evaluations of the same points at span 1e-6 (calm and
-    chaotic views) and 1e-12 (a boundary-band centre), and against FP64 direct on a whole-
-    set view.
+    chaotic views, on the points whose count survives a 16-ulp move of the f32 offset) and
+    1e-12 (a boundary-band centre), and against FP64 direct on a whole-set view.
 """
 import math
 
@@ -115,26 +115,36 @@ class TestPlanarGPU:
         esc = (ref_it < 512) & (it == ref_it)
         assert np.allclose(sm[esc], ref_sm[esc], rtol=1e-4, atol=1e-4)
 
-    @pytest.mark.parametrize("family,centre,span,min_agree", [
-        (MB, (-0.743643887, 0.131825904), 1e-6, 0.95),   # Seahorse valley
-        (BS, (-1.7548, -0.0284), 1e-6, 0.99),            # the ship in the antenna
-        (BS, (-1.62, -0.01), 1e-6, 0.85),                # chaotic antenna band
+    @pytest.mark.parametrize("family,centre,span", [
+        (MB, (-0.743643887, 0.131825904), 1e-6),   # Seahorse valley
+        (BS, (-1.7548, -0.0284), 1e-6),            # the ship in the antenna
+        (BS, (-1.62, -0.01), 1e-6),                # chaotic antenna band
     ])
-    def test_perturbation_matches_quad_on_the_same_points(self, ctx, family, centre, span, min_agree):
+    def test_perturbation_matches_quad_on_the_same_points(self, ctx, family, centre, span):
         """FP32 deltas against a binary128 evaluation of c = centre + (f32) offset, the point
-        the first reference's deltas represent.  Near the set boundary escape counts are
-        chaotic below ~1e-13 in c, where no FP32-delta method (nor FP64 direct) resolves
-        them; the bar per view is what such a method reaches there."""
+        the first reference's deltas represent, on the points an FP32-delta method can resolve.
+        An FP32 delta carries a rounding error of about one ulp of itself per iteration; where
+        the dynamics stretch errors (|delta_k| growing as the amplification), each iteration's
+        rounding reaches the escape as about one ulp of the initial offset, so after n
+        iterations the orbit is known to about sqrt(n) ulps of the offset: 16 for the few
+        hundred iterations these points run.  The resolvable points are those whose binary128
+        count moves by at most 1 when the f32 offset is moved by 16 of its ulps either way
+        (near the set boundary, counts are chaotic below that: nearly half of the chaotic band
+        view).  On them: within 1 of binary128 for >= 99 %."""
         v = PlanarView(family=family, mode=abi.PLANAR_PERTURB, width=128, height=128, iterations=2000,
                        center=centre, span=span, bailout=16.0)
         it, sm, st = self._run(ctx, v)
         ys, xs = np.mgrid[0:128:4, 0:128:4]
-        q_it, _ = oracle.escape_f128(v.c_struct(), xs.ravel(), ys.ravel())
-        g = it[ys.ravel(), xs.ravel()]
-        agree = np.mean(np.abs(g.astype(int) - q_it.astype(int)) <= 1)
-        print(f"family {family} centre {centre} span {span}: within 1 of quad {agree:.4f}, glitch restarts "
-              f"{st['rebases']}")
-        assert agree >= min_agree
+        xs, ys = xs.ravel(), ys.ravel()
+        q = oracle.escape_f128(v.c_struct(), xs, ys)[0].astype(int)
+        stable = np.ones(len(q), bool)
+        for nudge in (16.0, -16.0):
+            stable &= np.abs(oracle.escape_f128(v.c_struct(), xs, ys, nudge)[0].astype(int) - q) <= 1
+        near = np.abs(it[ys, xs].astype(int) - q) <= 1
+        print(f"family {family} centre {centre} span {span}: resolvable {stable.mean():.4f}, within 1 of quad there "
+              f"{near[stable].mean():.4f} (all points {near.mean():.4f}), glitch restarts {st['rebases']}")
+        assert stable.mean() >= 0.5
+        assert near[stable].mean() >= 0.99
 
     def test_perturbation_deep_zoom_matches_quad(self, ctx):
         """Mandelbrot at 1e-12 around a boundary-band centre (seed 5, the deep-zoom case)."""
