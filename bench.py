@@ -58,17 +58,21 @@ def cpu_baseline(width: int, height: int, budget_s: float = 12.0) -> dict:
                       "oracle C++ restatement, OpenMP, g++ -O2 -ffp-contract=off"}
 
 
+def progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _pct(xs, q):
     import numpy as np
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
 
 
-def serve_stream(ctx, reqs, mode: str, realtime: bool) -> dict:
+def serve_stream(ctx, reqs, mode: str, realtime: bool, budget: int = 200 * 10**9) -> dict:
     """Runs a request stream through a render server; wall time from the first submission to
     the last completion.  With ``realtime`` each request is submitted at its arrival time."""
     from syfrack_amd import serve
     descs = [(q.scene, q.cfg) for q in reqs]
-    with serve.Server(ctx, mode) as s:
+    with serve.Server(ctx, mode, hbm_budget_bytes=budget) as s:
         t0 = time.perf_counter()
         ids = []
         for q, (sc_, cfg) in zip(reqs, descs):
@@ -77,7 +81,10 @@ def serve_stream(ctx, reqs, mode: str, realtime: bool) -> dict:
                 if dt > 0:
                     time.sleep(dt)
             ids.append(s.submit(sc_, cfg))
-        s.wait_all(900)
+        for k, i in enumerate(ids):
+            s.wait(i, 900)
+            if (k + 1) % max(1, len(ids) // 4) == 0:
+                progress(f"{mode}: {k + 1}/{len(ids)} requests done at {time.perf_counter() - t0:.1f} s")
         wall = time.perf_counter() - t0
         st = s.stats()
         infos = [s.poll(i) for i in ids]
@@ -99,6 +106,7 @@ def serving_bench(ctx, n_uneven: int, n_mix: int, rate: float) -> dict:
     from syfrack_amd import serve
     out = {}
     uneven = serve.uneven_requests(n_uneven, seed=3)
+    progress(f"uneven batch of {n_uneven} requests")
     b = serve_stream(ctx, uneven, "batched", realtime=False)
     p = serve_stream(ctx, uneven, "plain", realtime=False)
     out["uneven_batch"] = {
@@ -107,11 +115,13 @@ def serving_bench(ctx, n_uneven: int, n_mix: int, rate: float) -> dict:
                     "bounces 1-4, <=256 steps, sky light",
         "batched": b, "plain": p, "speedup": b["mrays_s"] / p["mrays_s"], "same_rays": b["rays"] == p["rays"]}
     mix = serve.serving_mix(n_mix, rate_per_s=rate, seed=5)
+    progress(f"serving loop of {n_mix} requests at {rate:g}/s")
     b = serve_stream(ctx, mix, "batched", realtime=True)
     p = serve_stream(ctx, mix, "plain", realtime=True)
     out["serving_loop"] = {
         "workload": f"{n_mix} requests, Poisson arrivals at {rate:g}/s: 70% preview tiles 256^2 1 spp, 30% "
-                    "progressive renders 1024^2-2048^2 16-64 spp, same fractal mix, bounces 1-4",
+                    "progressive renders 1024^2-2048^2 16-64 spp, same fractal mix, bounces 1-4, "
+                    "200 GB admission budget",
         "batched": b, "plain": p, "speedup": b["mrays_s"] / p["mrays_s"], "same_rays": b["rays"] == p["rays"]}
     return out
 
@@ -126,7 +136,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (batched vs plain) legs")
     ap.add_argument("--serve-requests", type=int, default=256, help="requests of the uneven batch")
-    ap.add_argument("--mix-requests", type=int, default=64, help="requests of the Poisson serving loop")
+    ap.add_argument("--mix-requests", type=int, default=32, help="requests of the Poisson serving loop")
     ap.add_argument("--mix-rate", type=float, default=100.0, help="arrivals per second of the serving loop")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -138,6 +148,7 @@ def main() -> int:
     from syfrack_amd import scene as sc
     from syfrack_amd.render import Renderer
 
+    progress("primary trace")
     scene, cfg = sc.mandelbulb_primary(args.width, args.height)
     r = Renderer(0)
     job = r.job(scene, cfg)
@@ -204,6 +215,7 @@ def main() -> int:
         out["batched_vs_plain_rays"] = {"uneven_batch": sv["uneven_batch"]["speedup"],
                                         "serving_loop": lp["speedup"]}
     if not args.no_cpu:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.width, args.height)
     job.close()
     print(json.dumps(out), flush=True)
