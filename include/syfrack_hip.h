@@ -263,8 +263,8 @@ typedef struct syf_server_config {
     uint64_t hbm_budget_bytes;   /* admission budget for request planes (0 = 90 % of free HBM) */
     uint32_t mode;               /* SYF_SERVE_BATCHED or SYF_SERVE_PLAIN */
     uint32_t max_active;         /* requests traced concurrently (0 = 4096) */
-    uint32_t quantum_samples;    /* samples per pixel per work item (0 = autotuned) */
-    uint32_t batch_target_us;    /* target device time of one launch (0 = 10000 us) */
+    uint32_t quantum_samples;    /* samples per pixel per work item of a long request (0 = 1) */
+    uint32_t batch_target_us;    /* target device time of one launch (0 = 30000 us) */
     uint32_t grid_blocks;        /* workgroups of the persistent launch (0 = autotuned) */
     uint32_t reserved;
 } syf_server_config;

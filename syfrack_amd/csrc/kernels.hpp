@@ -37,12 +37,17 @@ struct BatchPlanes {
     float4* aux;
     double* depth64;
 };
-// One persistent launch over work items that all belong to kernel family `fam`
-// (kernel_family of their jobs).  claim: 8 counters, 16 words apart (zeroed before the launch).
+// One persistent launch of the flattened path tracer over work items that all belong to
+// kernel family `fam` (kernel_family of their jobs).  claim: 8 counters, 16 words apart
+// (zeroed before the launch).  Items starting at sample 0 leave the refine bracket in their
+// DEPTH64 plane; launch_refine_first_hits over the same items replaces it by the FP64 hit.
 constexpr uint32_t CLAIM_WORDS = 8 * 16;
-hipError_t launch_batch(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
+hipError_t launch_paths(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
                         uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t grid_blocks,
                         hipStream_t s);
+hipError_t launch_refine_first_hits(uint32_t fam, const JobP* jobs, const BatchPlanes* planes,
+                                    const BatchItem* items, uint32_t n_items, unsigned long long* ctr,
+                                    hipStream_t s);
 hipError_t launch_render_tile(const JobP& P, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0,
                               uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
                               unsigned long long* ctr, hipStream_t s);

@@ -388,56 +388,6 @@ __global__ void k_wf_end_sample(WfCtl* ctl) {
 // Per pixel the arithmetic is the one-launch form's: a request renders the same alone and in
 // any mix.  Items are claimed from eight counters (one per XCD, blockIdx % 8) to keep the
 // claims of different L2 slices off one address.
-template <uint32_t FAM>
-__device__ __forceinline__ void trace_tile_item(const JobP& P, const BatchItem& it, const BatchPlanes& pl, Work& work,
-                                                uint32_t& segs, uint32_t& shadow, uint32_t& prim, uint32_t& hits) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t px = it.x0 + (lane & 7u), py = it.y0 + (lane >> 3);
-    if (px >= P.width || py >= P.height) return;
-    render_pixel<FAM>(P, px, py, it.s0, it.ns, pl.accum, pl.depth, pl.aux, pl.depth64, work, segs, shadow, prim, hits);
-}
-
-// Batched serving kernel: a persistent grid drains a list of work items (8x8 tile of one
-// request, sample range) of one kernel family, each wave claiming the next item from one of
-// eight counters (the workgroup's index mod 8, i.e. the XCD it runs on, so the claims of one
-// XCD hit one cache line of its own).  The request's job and planes come from per-slot
-// tables, so one launch serves requests of every resolution, spp and parameter set.
-template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_batch(const JobP* __restrict__ jobs, const BatchPlanes* __restrict__ planes,
-                                               const BatchItem* __restrict__ items, uint32_t n_items,
-                                               uint32_t* __restrict__ claim, unsigned long long* __restrict__ ctr) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t xl = blockIdx.x % 8u;
-    // items dealt round-robin to the eight claim counters: counter xl serves items xl, xl+8, ...
-    const uint32_t mine = n_items > xl ? (n_items - xl + 7u) / 8u : 0u;
-    Work work;
-    uint32_t segs = 0, shadow = 0, prim = 0, hits = 0;
-    for (;;) {
-        uint32_t k = 0;
-        if (lane == 0u) k = atomicAdd(&claim[xl * 16u], 1u);
-        // wave-uniform from here: the item, its job and its planes are read through scalar loads
-        k = __builtin_amdgcn_readfirstlane(k);
-        if (k >= mine) break;
-        BatchItem it = items[k * 8u + xl];
-        it.slot = __builtin_amdgcn_readfirstlane(it.slot);
-        trace_tile_item<FAM>(jobs[it.slot], it, planes[it.slot], work, segs, shadow, prim, hits);
-    }
-    flush_work(work, hits, segs + shadow, ctr);
-    const uint32_t p = wave_sum(prim);
-    if (lane == 0u) atomicAdd(&ctr[CTR_PRIMARY], (unsigned long long)p);
-}
-
-hipError_t launch_batch(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
-                        uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t grid_blocks,
-                        hipStream_t s) {
-    if (n_items == 0) return hipSuccess;
-    if (fam >= FAM_COUNT) return hipErrorInvalidValue;
-    const uint32_t waves = (n_items + 7u) / 8u * 8u;
-    const dim3 grid(std::max<uint32_t>(8u, std::min<uint32_t>(grid_blocks, (waves + 3u) / 4u))), block(256);
-    SYF_DISPATCH_FAMILY(fam, k_batch, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr);
-    return hipGetLastError();
-}
-
 hipError_t launch_render_tile(const JobP& P, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0,
                               uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
                               unsigned long long* ctr, hipStream_t s) {

@@ -8,8 +8,9 @@
 //      estimated device time reaches the target (a running estimate of ns per
 //      pixel-sample-segment, refreshed from every launch that filled the device);
 //   3. enqueues it (items copied from pinned host memory, claim counters cleared, one
-//      persistent kernel) and keeps up to two launches in flight on its stream, so the
-//      device does not idle while the next launch is planned;
+//      persistent kernel per kernel family present, the families on streams of their own so
+//      that they run concurrently) and keeps up to two launches in flight, so the device does
+//      not idle while the next launch is planned;
 //   4. retires the oldest launch when its end event completes: adds its samples to the
 //      requests' progress and stamps the requests it finished.
 // Accumulation of a pixel is owned by one work item per sample range and successive ranges
@@ -71,7 +72,16 @@ using namespace syf;
 struct syf_server {
     syf_ctx* ctx = nullptr;
     syf_server_config cfg{};
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // admissions, plain-mode launches, calibration
+    // Batched launches: launch b (b = launch index & 1) runs on stream ls[b] and, per kernel
+    // family present, on fam_stream[b][f]: the family groups of a launch run concurrently (they
+    // serve different requests), and so do two consecutive launches (the planner keeps their
+    // pixels disjoint).  Events order each launch after the admissions before it.
+    hipStream_t ls[2] = {};
+    hipStream_t fam_stream[2][FAM_COUNT] = {};
+    hipEvent_t ev_fam[2][FAM_COUNT] = {};
+    hipEvent_t ev_ready[2] = {};
+    hipEvent_t ev_admit = nullptr;
     std::mutex mu;
     std::condition_variable cv;
     std::thread worker;
@@ -214,8 +224,16 @@ bool plan_launch(syf_server* s) {
         std::vector<BatchItem>& items = by_family[kernel_family(r->P.f)];
         while (r->planned < r->total && (units < budget_units || n_items < s->fill_items)) {
             if (r->q_cursor == 0) {  // open the next quantum
+                // only once no item of the previous one is in flight: the launches in flight
+                // then hold disjoint pixels, and each pixel's samples still land in order
+                if (r->refs > 0) break;
                 r->q_start = uint32_t(r->planned / r->tiles);
                 r->q_len = std::min(s->quantum, r->spp - r->q_start);
+                // a request whose remaining work fits in a launch goes in one piece: it then
+                // needs one launch, not one per quantum
+                if (double(r->total - r->planned) * double(TILE * TILE) * double(r->P.max_bounces + 1u) <=
+                    budget_units)
+                    r->q_len = r->spp - r->q_start;
             }
             const uint32_t t = r->q_cursor;
             items.push_back(BatchItem{r->slot, (t % r->tiles_x) * TILE, (t / r->tiles_x) * TILE, r->q_start, r->q_len});
@@ -243,10 +261,13 @@ bool plan_launch(syf_server* s) {
     // one persistent kernel per kernel family present, back to back on the stream; the items
     // of a family stay in shortest-remaining-first order
     uint32_t first[FAM_COUNT], count[FAM_COUNT];
+    bool first_quantum[FAM_COUNT];
     size_t at = 0;
     for (uint32_t f = 0; f < FAM_COUNT; ++f) {
         first[f] = uint32_t(at);
         count[f] = uint32_t(by_family[f].size());
+        first_quantum[f] = std::any_of(by_family[f].begin(), by_family[f].end(),
+                                       [](const BatchItem& it) { return it.s0 == 0u; });
         if (count[f]) std::memcpy(s->h_items[buf] + at, by_family[f].data(), count[f] * sizeof(BatchItem));
         at += count[f];
     }
@@ -255,16 +276,29 @@ bool plan_launch(syf_server* s) {
     L.full = n_items >= s->fill_items;
     L.start = s->ev_pool[2 * buf];
     L.end = s->ev_pool[2 * buf + 1];
-    hipError_t e = hipMemcpyAsync(s->d_items[buf], s->h_items[buf], n_items * sizeof(BatchItem),
-                                  hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(s->d_claim[buf], 0, FAM_COUNT * CLAIM_WORDS * sizeof(uint32_t), s->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(s->d_ctr[buf], 0, CTR_COUNT * sizeof(unsigned long long), s->stream);
-    if (e == hipSuccess) e = hipEventRecord(L.start, s->stream);
+    hipStream_t ls = s->ls[buf];
+    hipError_t e = hipEventRecord(s->ev_admit, s->stream);  // planes zeroed, slot tables written
+    if (e == hipSuccess) e = hipStreamWaitEvent(ls, s->ev_admit, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_items[buf], s->h_items[buf], n_items * sizeof(BatchItem), hipMemcpyHostToDevice, ls);
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_claim[buf], 0, FAM_COUNT * CLAIM_WORDS * sizeof(uint32_t), ls);
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_ctr[buf], 0, CTR_COUNT * sizeof(unsigned long long), ls);
+    if (e == hipSuccess) e = hipEventRecord(L.start, ls);
+    if (e == hipSuccess) e = hipEventRecord(s->ev_ready[buf], ls);
     for (uint32_t f = 0; f < FAM_COUNT && e == hipSuccess; ++f)
-        if (count[f])
-            e = launch_batch(f, s->d_jobs, s->d_planes, s->d_items[buf] + first[f], count[f],
-                             s->d_claim[buf] + f * CLAIM_WORDS, s->d_ctr[buf], s->grid, s->stream);
-    if (e == hipSuccess) e = hipEventRecord(L.end, s->stream);
+        if (count[f]) {
+            hipStream_t fs = s->fam_stream[buf][f];
+            e = hipStreamWaitEvent(fs, s->ev_ready[buf], 0);
+            if (e == hipSuccess)
+                e = launch_paths(f, s->d_jobs, s->d_planes, s->d_items[buf] + first[f], count[f],
+                                 s->d_claim[buf] + f * CLAIM_WORDS, s->d_ctr[buf], s->grid, fs);
+            if (e == hipSuccess && first_quantum[f])
+                e = launch_refine_first_hits(f, s->d_jobs, s->d_planes, s->d_items[buf] + first[f], count[f],
+                                             s->d_ctr[buf], fs);
+            if (e == hipSuccess) e = hipEventRecord(s->ev_fam[buf][f], fs);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ls, s->ev_fam[buf][f], 0);
+        }
+    if (e == hipSuccess) e = hipEventRecord(L.end, ls);
     if (e != hipSuccess) {
         set_server_error(s, SYF_E_DEVICE, "batch launch", e);
         return false;
@@ -422,8 +456,12 @@ void worker_main(syf_server* s) {
 }
 
 // Autotuner: times a fixed calibration workload (a 768x768 Mandelbulb job, 4 samples, 2
-// bounces, in 8x8-tile items: enough to fill the device) for each candidate (samples per work item x workgroups per
-// launch), keeps the fastest and seeds the cost model (ns per unit) with its measured rate.
+// bounces, in 8x8-tile items: enough to fill the device) for each candidate size of the
+// persistent grid (4, 8 or 16 workgroups per CU), keeps the fastest and seeds the cost model
+// (ns per pixel-sample-segment) with its measured rate.  The sample quantum is not tuned here:
+// one sample per work item with whole-request items for requests that fit in a launch
+// measured best on both request streams (long renders need short per-lane work so that
+// launches end together; small requests need few launches).
 void autotune(syf_server* s) {
     syf_job_desc d;
     std::memset(&d, 0, sizeof d);
@@ -474,44 +512,29 @@ void autotune(syf_server* s) {
     ok = ok && hipMemcpy(djob, &P, sizeof P, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(dpl, &pl, sizeof pl, hipMemcpyHostToDevice) == hipSuccess;
     const uint32_t cu = uint32_t(s->ctx->cu_count > 0 ? s->ctx->cu_count : 256);
-    const uint32_t q_cands[3] = {1, 2, 4};
     const uint32_t g_cands[3] = {cu * 4, cu * 8, cu * 16};
     double best = 1e30;
-    uint32_t bq = 2, bg = cu * 8;
+    uint32_t bg = cu * 8;
     double best_units_ns = 0.0;
-    for (uint32_t qi = 0; ok && qi < 3; ++qi) {
-        std::vector<BatchItem> items;
-        const uint32_t q = q_cands[qi];
-        for (uint32_t s0 = 0; s0 < spp; s0 += q)
-            for (uint32_t t = 0; t < tiles; ++t) items.push_back(BatchItem{0, (t % tiles_x) * TILE, (t / tiles_x) * TILE, s0, q});
-        // one item per tile and quantum must run in sample order: separate launches per quantum
-        for (uint32_t gi = 0; ok && gi < 3; ++gi) {
-            double total_ms = 0.0;
-            for (int rep = 0; rep < 2 && ok; ++rep) {  // first repetition warms up
-                float ms_sum = 0.0f;
-                for (uint32_t s0 = 0; s0 < spp; s0 += q) {
-                    const size_t off = size_t(s0 / q) * tiles;
-                    ok = hipMemcpy(ditems, items.data() + off, size_t(tiles) * sizeof(BatchItem),
-                                   hipMemcpyHostToDevice) == hipSuccess &&
-                         hipMemset(claim, 0, CLAIM_WORDS * 4) == hipSuccess && hipEventRecord(a, s->stream) == hipSuccess &&
-                         launch_batch(kernel_family(P.f), djob, dpl, ditems, tiles, claim, ctr, g_cands[gi],
-                                      s->stream) == hipSuccess &&
-                         hipEventRecord(b, s->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess;
-                    float ms = 0.0f;
-                    if (ok) hipEventElapsedTime(&ms, a, b);
-                    ms_sum += ms;
-                }
-                total_ms = ms_sum;
-            }
-            if (ok && total_ms < best) {
-                best = total_ms;
-                bq = q;
-                bg = g_cands[gi];
-                best_units_ns = total_ms * 1e6 / (double(pixels) * spp * (d.max_bounces + 1));
-            }
+    // every tile with all its samples in one item: one launch per candidate and repetition
+    std::vector<BatchItem> items;
+    for (uint32_t t = 0; t < tiles; ++t) items.push_back(BatchItem{0, (t % tiles_x) * TILE, (t / tiles_x) * TILE, 0, spp});
+    ok = ok && hipMemcpy(ditems, items.data(), size_t(tiles) * sizeof(BatchItem), hipMemcpyHostToDevice) == hipSuccess;
+    for (uint32_t gi = 0; ok && gi < 3; ++gi) {
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && ok; ++rep) {  // the first repetition warms up
+            ok = hipMemset(claim, 0, CLAIM_WORDS * 4) == hipSuccess && hipEventRecord(a, s->stream) == hipSuccess &&
+                 launch_paths(kernel_family(P.f), djob, dpl, ditems, tiles, claim, ctr, g_cands[gi], s->stream) ==
+                     hipSuccess &&
+                 hipEventRecord(b, s->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+                 hipEventElapsedTime(&ms, a, b) == hipSuccess;
+        }
+        if (ok && ms < best) {
+            best = ms;
+            bg = g_cands[gi];
+            best_units_ns = double(ms) * 1e6 / (double(pixels) * spp * (d.max_bounces + 1));
         }
     }
-    if (s->cfg.quantum_samples == 0) s->quantum = bq;
     if (s->cfg.grid_blocks == 0) s->grid = bg;
     if (best_units_ns > 0) s->ns_per_unit = best_units_ns;
     hipFree(accum);
@@ -554,9 +577,11 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     s->max_slots = s->cfg.max_active;
     for (uint32_t i = 0; i < s->max_slots; ++i) s->free_slots.push_back(s->max_slots - 1u - i);
     const uint32_t cu = uint32_t(ctx->cu_count > 0 ? ctx->cu_count : 256);
-    s->quantum = cfg->quantum_samples ? cfg->quantum_samples : 2u;
+    s->quantum = cfg->quantum_samples ? cfg->quantum_samples : 1u;
     s->grid = cfg->grid_blocks ? cfg->grid_blocks : cu * 8u;
-    s->target_us = cfg->batch_target_us ? cfg->batch_target_us : 10000u;
+    // each launch ends with a tail as long as its slowest pixel's quantum: launches of a few
+    // tens of ms keep that below a few per cent, and a preview waits at most two of them
+    s->target_us = cfg->batch_target_us ? cfg->batch_target_us : 30000u;
     s->fill_items = cu * 16u;  // two workgroups of four waves per CU, twice over
     s->ns_per_unit = 0.05;
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
@@ -578,7 +603,26 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
             return hip_fail(e, "server events");
         }
     }
-    if (s->cfg.mode == SYF_SERVE_BATCHED && (cfg->quantum_samples == 0 || cfg->grid_blocks == 0)) autotune(s);
+    // streams created alternately for the two launches, so that a family's streams of two
+    // consecutive launches land on different hardware queues
+    for (uint32_t f = 0; f < FAM_COUNT; ++f)
+        for (int b = 0; b < 2; ++b)
+            if ((e = hipStreamCreateWithFlags(&s->fam_stream[b][f], hipStreamNonBlocking)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&s->ev_fam[b][f], hipEventDisableTiming)) != hipSuccess) {
+                syf_server_destroy(s);
+                return hip_fail(e, "server streams");
+            }
+    for (int b = 0; b < 2; ++b)
+        if ((e = hipStreamCreateWithFlags(&s->ls[b], hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&s->ev_ready[b], hipEventDisableTiming)) != hipSuccess) {
+            syf_server_destroy(s);
+            return hip_fail(e, "server streams");
+        }
+    if ((e = hipEventCreateWithFlags(&s->ev_admit, hipEventDisableTiming)) != hipSuccess) {
+        syf_server_destroy(s);
+        return hip_fail(e, "server events");
+    }
+    if (s->cfg.mode == SYF_SERVE_BATCHED && cfg->grid_blocks == 0) autotune(s);
     s->worker = std::thread(worker_main, s);
     *out = s;
     return SYF_OK;
@@ -616,6 +660,21 @@ int syf_server_destroy(syf_server* s) {
     }
     for (auto& ev : s->ev_pool)
         if (ev) hipEventDestroy(ev);
+    for (int b = 0; b < 2; ++b) {
+        for (uint32_t f = 0; f < FAM_COUNT; ++f) {
+            if (s->fam_stream[b][f]) {
+                hipStreamSynchronize(s->fam_stream[b][f]);
+                hipStreamDestroy(s->fam_stream[b][f]);
+            }
+            if (s->ev_fam[b][f]) hipEventDestroy(s->ev_fam[b][f]);
+        }
+        if (s->ls[b]) {
+            hipStreamSynchronize(s->ls[b]);
+            hipStreamDestroy(s->ls[b]);
+        }
+        if (s->ev_ready[b]) hipEventDestroy(s->ev_ready[b]);
+    }
+    if (s->ev_admit) hipEventDestroy(s->ev_admit);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return SYF_OK;
@@ -735,7 +794,10 @@ int syf_server_release(syf_server* s, uint64_t id) {
     if (r->state != SYF_REQ_DONE && r->state != SYF_REQ_CANCELLED)
         return fail(SYF_E_INVALID, "request %llu is still running", (unsigned long long)id);
     hipSetDevice(s->ctx->device);
-    hipStreamSynchronize(s->stream);  // launches of a cancelled request may still write its planes
+    // launches of a cancelled request may still write its planes
+    hipStreamSynchronize(s->stream);
+    for (int b = 0; b < 2; ++b)
+        if (s->ls[b]) hipStreamSynchronize(s->ls[b]);
     free_request_planes(s, r);
     s->reqs.erase(r->id);
     r->released = true;
