@@ -147,6 +147,8 @@ typedef struct syf_stats {
     uint64_t steps;            /* march steps over all segments */
     double kernel_ms;          /* device time of the last call's launches, HIP events */
     double march_ms;           /* device time of its dominant (FP32 march) kernel alone */
+    uint64_t hit_de_evals;     /* of de_evals: those of the primary trace's hit kernel (normals) */
+    uint64_t hit_de_iterations;
 } syf_stats;
 
 /* ---------------------------------------------------------------- opaque handles */

@@ -153,6 +153,8 @@ class SyfStats(C.Structure):
         ("steps", C.c_uint64),
         ("kernel_ms", C.c_double),
         ("march_ms", C.c_double),
+        ("hit_de_evals", C.c_uint64),
+        ("hit_de_iterations", C.c_uint64),
     ]
 
 

@@ -374,8 +374,10 @@ int syf_job_stats(syf_job* j, syf_stats* out) {
     out->rays = c[CTR_RAYS];
     out->primary_rays = c[CTR_PRIMARY];
     out->hits = c[CTR_HITS];
-    out->de_evals = c[CTR_DE_EVALS];
-    out->de_iterations = c[CTR_DE_ITERS];
+    out->de_evals = c[CTR_DE_EVALS] + c[CTR_HIT_EVALS];
+    out->de_iterations = c[CTR_DE_ITERS] + c[CTR_HIT_ITERS];
+    out->hit_de_evals = c[CTR_HIT_EVALS];
+    out->hit_de_iterations = c[CTR_HIT_ITERS];
     out->de64_evals = c[CTR_DE64_EVALS];
     out->de64_iterations = c[CTR_DE64_ITERS];
     out->steps = c[CTR_STEPS];

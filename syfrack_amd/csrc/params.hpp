@@ -111,7 +111,9 @@ enum : uint32_t {
     CTR_HITS = 5,
     CTR_RAYS = 6,
     CTR_PRIMARY = 7,
-    CTR_COUNT = 8,
+    CTR_HIT_EVALS = 8,  // FP32 evaluations of the hit kernel (normal taps of the primary trace)
+    CTR_HIT_ITERS = 9,
+    CTR_COUNT = 10,
 };
 
 }  // namespace syf

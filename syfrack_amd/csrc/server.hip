@@ -457,7 +457,7 @@ void worker_main(syf_server* s) {
 
 // Autotuner: times a fixed calibration workload (a 768x768 Mandelbulb job, 4 samples, 2
 // bounces, in 8x8-tile items: enough to fill the device) for each candidate size of the
-// persistent grid (4, 8 or 16 workgroups per CU), keeps the fastest and seeds the cost model
+// persistent grid (2, 4 or 8 workgroups per CU), keeps the fastest and seeds the cost model
 // (ns per pixel-sample-segment) with its measured rate.  The sample quantum is not tuned here:
 // one sample per work item with whole-request items for requests that fit in a launch
 // measured best on both request streams (long renders need short per-lane work so that
@@ -512,7 +512,7 @@ void autotune(syf_server* s) {
     ok = ok && hipMemcpy(djob, &P, sizeof P, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(dpl, &pl, sizeof pl, hipMemcpyHostToDevice) == hipSuccess;
     const uint32_t cu = uint32_t(s->ctx->cu_count > 0 ? s->ctx->cu_count : 256);
-    const uint32_t g_cands[3] = {cu * 4, cu * 8, cu * 16};
+    const uint32_t g_cands[3] = {cu * 2, cu * 4, cu * 8};
     double best = 1e30;
     uint32_t bg = cu * 8;
     double best_units_ns = 0.0;

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_hits(const JobP P, uint32_t sample, con
                                               float4* __restrict__ aux, double* __restrict__ depth64,
                                               unsigned long long* __restrict__ ctr) {
     const uint32_t n = *hit_count;
-    Work w;
+    Work w, wn;  // refine; normal taps (counted apart from the march's evaluations)
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t pix = hit_list[k];
         const uint32_t px = pix % P.width, py = pix / P.width;
@@ -244,12 +244,17 @@ __global__ __launch_bounds__(256) void k_hits(const JobP P, uint32_t sample, con
         camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
         const float t = depth[pix];
         const V3<float> p = add(ro, scl(rd, t));
-        const V3<float> nv = normal_at<float, FAM>(P.f, p, fmaxf(t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, w);
+        const V3<float> nv = normal_at<float, FAM>(P.f, p, fmaxf(t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, wn);
         aux[pix] = make_float4(nv.x, nv.y, nv.z, float(steps[pix]));
         depth64[pix] = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(t_prev[pix]), double(t),
                                      double(P.pixel_cone) * double(P.pixel_tolerance), w);
     }
     flush_work(w, 0u, 0u, ctr);
+    const uint32_t a = wave_sum(wn.de_evals), b = wave_sum(wn.de_iters);
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(&ctr[CTR_HIT_EVALS], (unsigned long long)a);
+        atomicAdd(&ctr[CTR_HIT_ITERS], (unsigned long long)b);
+    }
 }
 
 // Misses of the FP64 plane: -1 wherever the march found no hit.
@@ -283,7 +288,11 @@ __global__ __launch_bounds__(256) void k_de_probe(const FractalP F, const float4
 hipError_t launch_trace_primary(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s) {
     const uint32_t tiles_x = (P.width + 7u) / 8u, tiles_y = (P.height + 7u) / 8u;
     const uint32_t n_slots = tiles_x * tiles_y * 64u;
-    const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * 8u;
+    // Two workgroups (eight waves) per CU.  A frame's time is set by its longest rays (up to
+    // max_steps evaluations in sequence on one lane); more resident waves share each SIMD's
+    // issue slots with them and stretch that critical path: measured at 1080p, 2 per CU
+    // 0.80 ms, 1 per CU 1.00 ms, 3 per CU 0.88 ms, 8 per CU 1.65 ms.
+    const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * 2u;
     // at least one workgroup per ray lane, or that lane's tiles would never be traced
     const dim3 grid(std::max<uint32_t>(std::min<uint32_t>((n_slots + 255u) / 256u, fill), RAY_LANES)), block(256);
     SYF_DISPATCH_FAMILY(kernel_family(P.f), k_march, grid, block, 0, s, P, sample, tiles_x, n_slots, B.counters,
@@ -293,7 +302,10 @@ hipError_t launch_trace_primary(const JobP& P, uint32_t sample, const TraceBuffe
 
 hipError_t launch_refine64(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s) {
     const uint32_t n = P.width * P.height;
-    const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * 8u;
+    // two workgroups per CU, each lane striding over several hits: measured 2.3x faster than
+    // one hit per lane (eight workgroups per CU) at 1080p; a lane's successive hits lie far
+    // apart, so the expensive refines of silhouette regions spread over all lanes
+    const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * 2u;
     const dim3 grid(std::min<uint32_t>((n + 255u) / 256u, fill)), block(256);
     hipLaunchKernelGGL(k_depth64_init, grid, block, 0, s, B.depth, B.depth64, n);
     hipError_t e = hipGetLastError();

@@ -35,9 +35,15 @@ REFINE_EVAL = 9
 
 
 def march_flops(family: int, stats: dict) -> float:
-    """FP32 FLOPs of the march kernel (march + normals) from its device counters."""
-    return (BODY[family] * stats["de_iterations"] + EVAL[family] * stats["de_evals"] + STEP * stats["steps"]
-            + NORMAL * stats["hits"])
+    """FP32 FLOPs of the primary trace's march kernel alone (its evaluations and steps; the
+    normal taps of the hit kernel are counted apart, in hit_de_*)."""
+    return (BODY[family] * (stats["de_iterations"] - stats["hit_de_iterations"])
+            + EVAL[family] * (stats["de_evals"] - stats["hit_de_evals"]) + STEP * stats["steps"])
+
+
+def hit_fp32_flops(family: int, stats: dict) -> float:
+    """FP32 FLOPs of the primary trace's hit kernel: the four-tap normals."""
+    return BODY[family] * stats["hit_de_iterations"] + EVAL[family] * stats["hit_de_evals"] + NORMAL * stats["hits"]
 
 
 def refine_flops(family: int, stats: dict) -> float:
