@@ -90,6 +90,79 @@ def calibrate_copy() -> int:
     return 0
 
 
+def pathtrace_leg(spp: int) -> int:
+    """Mandelbox path trace at 1920x1080 (15 iterations, 4 bounces, one area light, <= 512
+    steps) in its three forms: the megakernel (syf_render_tile over 256-pixel tiles, 8 samples
+    per dispatch), the wavefront stages (syf_render_wavefront) and the flattened lane-persistent
+    kernel (one request through the batched server).  Prints one JSON line of Mrays/s (all
+    traced segments: primary, bounce and shadow) and device time per form."""
+    from syfrack_amd import scene as sc, serve
+    from syfrack_amd.render import Renderer
+    scene, cfg = sc.mandelbox_pathtrace(1920, 1080, spp)
+    r = Renderer(0)
+    out = {"workload": f"Mandelbox path trace 1920x1080, {spp} spp, 4 bounces, one area light, <=512 steps"}
+    for name, wf in (("megakernel", False), ("wavefront", True)):
+        progress(f"path trace: {name}")
+        job = r.job(scene, cfg)
+        rays, dev_ms = 0, 0.0
+        t0 = time.perf_counter()
+        for s0 in range(0, spp, 8):  # the whole frame per dispatch, 8 samples at a time
+            n = min(8, spp - s0)
+            if wf:
+                job.render_wavefront(0, 0, cfg.width, cfg.height, s0, n)
+            else:
+                job.render_tile(0, 0, cfg.width, cfg.height, s0, n)
+            st = job.stats()  # waits; counters and device time of this dispatch
+            rays += st["rays"]
+            dev_ms += st["kernel_ms"]
+        el = time.perf_counter() - t0
+        job.close()
+        out[name] = {"s": el, "device_ms": dev_ms, "rays": rays, "mrays_s": rays / el / 1e6}
+    progress("path trace: flattened (server)")
+    with serve.Server(r.ctx, "batched") as srv:
+        t0 = time.perf_counter()
+        rid = srv.submit(scene, cfg)
+        srv.wait(rid, 900)
+        el = time.perf_counter() - t0
+        st = srv.stats()
+        srv.release(rid)
+    out["flattened"] = {"s": el, "rays": st["rays"], "mrays_s": st["rays"] / el / 1e6, "launches": st["launches"]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def planar_leg(size: int, seeds: int) -> int:
+    """2D deep zoom: size x size Mandelbrot and Burning Ship images at centres drawn in the
+    boundary band (seeded), spans 1e-6 and 1e-12, 10000 iterations, FP64 reference orbit + FP32
+    perturbation with glitch re-reference.  Prints one JSON line of Mpixel/s and iterations/s."""
+    from syfrack_amd import abi
+    from syfrack_amd.planar import Planar, PlanarView, boundary_centre
+    from syfrack_amd.render import Context
+    ctx = Context(0)
+    rows = []
+    for fam, name in ((abi.PLANAR_MANDELBROT, "mandelbrot"), (abi.PLANAR_BURNING_SHIP, "burning_ship")):
+        for seed in range(seeds):
+            c = boundary_centre(ctx, fam, seed=seed, iterations=2000)
+            for span in (1e-6, 1e-12):
+                progress(f"planar: {name} seed {seed} span {span:g}")
+                v = PlanarView(family=fam, mode=abi.PLANAR_PERTURB, width=size, height=size, iterations=10000,
+                               center=c, span=span, bailout=16.0)
+                p = Planar(ctx, v)
+                t0 = time.perf_counter()
+                p.run()
+                p.sync()
+                el = time.perf_counter() - t0
+                st = p.stats()
+                p.close()
+                rows.append({"family": name, "seed": seed, "centre": list(c), "span": span, "s": el,
+                             "mpix_s": size * size / el / 1e6, "giter_s": st["iterations"] / el / 1e9,
+                             "iterations": st["iterations"], "glitch_restarts": st["rebases"],
+                             "kernel_ms": st["kernel_ms"]})
+    print(json.dumps({"workload": f"{size}x{size} deep zoom, 10000 iterations, perturbation", "views": rows}),
+          flush=True)
+    return 0
+
+
 def progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -172,9 +245,19 @@ def main() -> int:
     ap.add_argument("--mix-rate", type=float, default=100.0, help="arrivals per second of the serving loop")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy (reference for the byte counters)")
+    ap.add_argument("--pathtrace", action="store_true",
+                    help="run only the Mandelbox 1080p path-trace leg (three kernel forms)")
+    ap.add_argument("--spp", type=int, default=64, help="samples per pixel of the path-trace leg")
+    ap.add_argument("--planar", action="store_true", help="run only the 2D deep-zoom leg")
+    ap.add_argument("--planar-size", type=int, default=4096)
+    ap.add_argument("--planar-seeds", type=int, default=2)
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.pathtrace:
+        return pathtrace_leg(args.spp)
+    if args.planar:
+        return planar_leg(args.planar_size, args.planar_seeds)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this path shards by whole "
               "requests or tiles with no exchange, see the design notes", file=sys.stderr)
