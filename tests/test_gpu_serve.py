@@ -156,3 +156,29 @@ def test_cancel_release_and_unknown_ids(ctx):
                 fn()
             assert e.value.code == abi.SYF_E_NOT_FOUND
         assert s.stats()["hbm_in_use"] == 0
+
+
+def test_area_light_scene_served_equals_megakernel(ctx):
+    """The Mandelbox path-trace scene (an area light: light sampling, shadow rays an emitter
+    decides, emitter hits) served by the flattened kernel, split over several launches, gives
+    the megakernel's planes bit for bit."""
+    from syfrack_amd import abi, serve
+    from syfrack_amd import scene as sc
+    from syfrack_amd.render import Job
+    scene, cfg = sc.mandelbox_pathtrace(160, 96, 6)
+    j = Job(ctx, scene.desc(cfg))
+    try:
+        j.render_tile(0, 0, cfg.width, cfg.height, 0, cfg.samples_per_pixel)
+        j.sync()
+        ref = [j.readback(p) for p in (abi.PLANE_ACCUM, abi.PLANE_DEPTH, abi.PLANE_AUX, abi.PLANE_DEPTH64)]
+    finally:
+        j.close()
+    assert ref[0][..., :3].mean() > 1e-3  # the light reaches the visible surface
+    with serve.Server(ctx, "batched", quantum_samples=1, batch_target_us=1, grid_blocks=256) as s:
+        rid = s.submit(scene, cfg)
+        s.wait(rid, 120)
+        got = [s.plane(rid, p) for p in (abi.PLANE_ACCUM, abi.PLANE_DEPTH, abi.PLANE_AUX, abi.PLANE_DEPTH64)]
+        assert s.stats()["launches"] >= cfg.samples_per_pixel
+        s.release(rid)
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
