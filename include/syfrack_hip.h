@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SYF_ABI_VERSION 1
+#define SYF_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- status codes
  * Replaces anyhow::Result at the seams: config errors (renderer.rs:81-138),
@@ -146,9 +146,15 @@ typedef struct syf_stats {
     uint64_t de64_iterations;
     uint64_t steps;            /* march steps over all segments */
     double kernel_ms;          /* device time of the last call's launches, HIP events */
-    double march_ms;           /* device time of its dominant (FP32 march) kernel alone */
-    uint64_t hit_de_evals;     /* of de_evals: those of the primary trace's hit kernel (normals) */
+    double march_ms;           /* device time of its dominant kernel alone (primary trace: the fused
+                                  march + hit-shading kernel, without the trailing hit pass) */
+    uint64_t hit_de_evals;     /* of de_evals: the primary trace's normal taps */
     uint64_t hit_de_iterations;
+    uint64_t tail_hits;        /* primary trace: hits shaded by the trailing pass, not the fused kernel */
+    uint64_t tail_hit_de_evals;     /* of hit_de_evals: those of the trailing pass */
+    uint64_t tail_hit_de_iterations;
+    uint64_t tail_de64_evals;       /* of de64_evals: those of the trailing pass */
+    uint64_t tail_de64_iterations;
 } syf_stats;
 
 /* ---------------------------------------------------------------- opaque handles */

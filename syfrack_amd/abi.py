@@ -155,6 +155,11 @@ class SyfStats(C.Structure):
         ("march_ms", C.c_double),
         ("hit_de_evals", C.c_uint64),
         ("hit_de_iterations", C.c_uint64),
+        ("tail_hits", C.c_uint64),
+        ("tail_hit_de_evals", C.c_uint64),
+        ("tail_hit_de_iterations", C.c_uint64),
+        ("tail_de64_evals", C.c_uint64),
+        ("tail_de64_iterations", C.c_uint64),
     ]
 
 

@@ -282,6 +282,7 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
     j->desc = *desc;
     j->P = P;
     j->pixels = size_t(desc->width) * desc->height;
+    j->hitq_cap = hitq_capacity(desc->width, desc->height, ctx->cu_count);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
         size_t need = 0;
@@ -303,8 +304,7 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
         }
     }
     if ((e = hipMalloc(&j->ctr, sizeof(unsigned long long) * CTR_COUNT)) != hipSuccess ||
-        (e = hipMalloc(&j->t_prev, j->pixels * sizeof(float))) != hipSuccess ||
-        (e = hipMalloc(&j->hit_list, j->pixels * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&j->hitq, (4 * j->hitq_cap + j->hitq_cap / 64) * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&j->counters, COUNTER_WORDS * sizeof(uint32_t))) != hipSuccess ||
         (e = hipEventCreate(&j->ev0)) != hipSuccess || (e = hipEventCreate(&j->ev1)) != hipSuccess ||
         (e = hipEventCreate(&j->ev2)) != hipSuccess) {
@@ -328,8 +328,7 @@ int syf_job_destroy(syf_job* j) {
     for (void*& p : j->plane)
         if (p) hipFree(p);
     if (j->ctr) hipFree(j->ctr);
-    if (j->t_prev) hipFree(j->t_prev);
-    if (j->hit_list) hipFree(j->hit_list);
+    if (j->hitq) hipFree(j->hitq);
     if (j->counters) hipFree(j->counters);
     if (j->ev0) hipEventDestroy(j->ev0);
     if (j->ev1) hipEventDestroy(j->ev1);
@@ -364,9 +363,15 @@ int syf_job_stats(syf_job* j, syf_stats* out) {
     if (!j || !out) return fail(SYF_E_INVALID, "job and out must not be null");
     hipSetDevice(j->ctx->device);
     unsigned long long c[CTR_COUNT];
+    uint32_t qerr = 0;
     hipError_t e = hipMemcpyAsync(c, j->ctr, sizeof c, hipMemcpyDeviceToHost, j->stream);
+    if (e == hipSuccess && j->traced)
+        e = hipMemcpyAsync(&qerr, j->counters + COUNTER_QERR, sizeof qerr, hipMemcpyDeviceToHost, j->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
     if (e != hipSuccess) return hip_fail(e, "syf_job_stats");
+    if (qerr)
+        return fail(SYF_E_QUEUE, "primary trace hit queue invariant broken: %u blocks above its capacity of %zu entries",
+                    qerr, j->hitq_cap);
     float ms = 0.0f, ms_dom = 0.0f;
     if (j->timed && hipEventElapsedTime(&ms, j->ev0, j->ev2) != hipSuccess) ms = 0.0f;
     if (j->timed && hipEventElapsedTime(&ms_dom, j->ev0, j->ev1) != hipSuccess) ms_dom = 0.0f;
@@ -378,6 +383,11 @@ int syf_job_stats(syf_job* j, syf_stats* out) {
     out->de_iterations = c[CTR_DE_ITERS] + c[CTR_HIT_ITERS];
     out->hit_de_evals = c[CTR_HIT_EVALS];
     out->hit_de_iterations = c[CTR_HIT_ITERS];
+    out->tail_hits = c[CTR_TAIL_HITS];
+    out->tail_hit_de_evals = c[CTR_TAIL_EVALS];
+    out->tail_hit_de_iterations = c[CTR_TAIL_ITERS];
+    out->tail_de64_evals = c[CTR_TAIL_DE64_EVALS];
+    out->tail_de64_iterations = c[CTR_TAIL_DE64_ITERS];
     out->de64_evals = c[CTR_DE64_EVALS];
     out->de64_iterations = c[CTR_DE64_ITERS];
     out->steps = c[CTR_STEPS];
@@ -411,16 +421,26 @@ int syf_trace_primary(syf_job* j, uint32_t sample) {
     clear_error();
     if (!j) return fail(SYF_E_INVALID, "job must not be null");
     hipSetDevice(j->ctx->device);
+    const size_t cap = j->hitq_cap;
+    TraceBuffers B{(float*)j->plane[SYF_PLANE_DEPTH],
+                   (float4*)j->plane[SYF_PLANE_AUX],
+                   (double*)j->plane[SYF_PLANE_DEPTH64],
+                   (uint32_t*)j->plane[SYF_PLANE_STEPS],
+                   j->counters,
+                   HitQueue{j->hitq, j->hitq + cap, j->hitq + 2 * cap, j->hitq + 3 * cap, j->counters + COUNTER_HITS,
+                            j->counters + COUNTER_CURSOR, j->hitq + 4 * cap, j->counters + COUNTER_QERR, uint32_t(cap)},
+                   j->ctr};
     hipError_t e = hipMemsetAsync(j->ctr, 0, sizeof(unsigned long long) * CTR_COUNT, j->stream);
-    TraceBuffers B{(float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
-                   (double*)j->plane[SYF_PLANE_DEPTH64], (uint32_t*)j->plane[SYF_PLANE_STEPS], j->t_prev, j->hit_list,
-                   j->counters, j->ctr};
-    // ray counter and hit-queue length start at zero for every call
+    // ray counters, queue count, cursor and error flag start at zero, every queue field empty,
+    // every done flag clear
     if (e == hipSuccess) e = hipMemsetAsync(j->counters, 0, COUNTER_WORDS * sizeof(uint32_t), j->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(j->hitq, 0xff, 4 * cap * sizeof(uint32_t), j->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(j->hitq + 4 * cap, 0, cap / 64 * sizeof(uint32_t), j->stream);
+    j->traced = true;
     if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
     if (e == hipSuccess) e = launch_trace_primary(j->P, sample, B, j->ctx->cu_count, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev1, j->stream);
-    if (e == hipSuccess) e = launch_refine64(j->P, sample, B, j->ctx->cu_count, j->stream);
+    if (e == hipSuccess) e = launch_hits_tail(j->P, sample, B, j->ctx->cu_count, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev2, j->stream);
     j->timed = true;
     j->extra_ms = 0.0;

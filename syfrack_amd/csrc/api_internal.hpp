@@ -25,11 +25,12 @@ struct syf_job {
     hipStream_t stream = nullptr;
     void* plane[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     unsigned long long* ctr = nullptr;
-    float* t_prev = nullptr;     // per pixel: march position before the hit (FP64 refine bracket)
-    uint32_t* hit_list = nullptr;  // per pixel capacity: hits queued for the FP64 refine
-    uint32_t* counters = nullptr;  // ray counter and hit-list length of the primary trace
+    uint32_t* hitq = nullptr;      // hit queue of the primary trace: 4 fields x hitq_cap entries, then done flags
+    size_t hitq_cap = 0;
+    uint32_t* counters = nullptr;  // ray counters, hit-queue count and cursor of the primary trace
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // start, after dominant kernel, end
     bool timed = false;
+    bool traced = false;  // a primary trace ran: its hit-queue error flag is checked by syf_job_stats
     double extra_ms = 0.0;
     syf::WavefrontArena* wf = nullptr;
 };

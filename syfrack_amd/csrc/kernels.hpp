@@ -7,9 +7,29 @@
 
 namespace syf {
 
-// Layout of TraceBuffers::counters: eight ray counters 16 words apart, then the hit count.
+// Layout of TraceBuffers::counters: eight ray counters 16 words apart, then the hit queue's
+// reserved count, its claim cursor and its error flag, 16 words apart.
 constexpr uint32_t COUNTER_HITS = 128;
-constexpr uint32_t COUNTER_WORDS = 144;
+constexpr uint32_t COUNTER_CURSOR = 144;
+constexpr uint32_t COUNTER_QERR = 160;
+constexpr uint32_t COUNTER_WORDS = 176;
+
+// Hit queue of a primary trace: four fields per entry in separate arrays of `capacity` words
+// (room for every pixel plus one 64-entry block of padding per wave), every word HITQ_EMPTY
+// before the launch; one done flag per 64-entry block, zero before the launch.
+constexpr uint32_t HITQ_EMPTY = 0xffffffffu;
+constexpr uint32_t HITQ_SKIP = 0xfffffffeu;  // pixel of a padding entry
+struct HitQueue {
+    uint32_t* pix;
+    uint32_t* t;       // march distance of the hit (float bits)
+    uint32_t* t_prev;  // march position before the hit (float bits): lower end of the refine bracket
+    uint32_t* steps;
+    uint32_t* count;   // entries reserved by the march
+    uint32_t* cursor;  // blocks claimed by shading waves
+    uint32_t* done;    // per block: 1 once a shading wave of the fused kernel has shaded it
+    uint32_t* error;   // nonzero if a block did not fit (never at hitq_capacity())
+    uint32_t capacity; // entries, a multiple of 64
+};
 
 // Device buffers of a primary trace.
 struct TraceBuffers {
@@ -17,14 +37,16 @@ struct TraceBuffers {
     float4* aux;
     double* depth64;
     uint32_t* steps;
-    float* t_prev;             // per pixel: march position before the hit (refine bracket)
-    uint32_t* hit_list;        // per pixel capacity: pixels that hit, for the FP64 refine
-    uint32_t* counters;        // COUNTER_WORDS words: ray counters (one per XCD, 64 B apart), hit_list length
+    uint32_t* counters;        // COUNTER_WORDS words: ray counters (one per XCD, 64 B apart), queue count, cursor
+    HitQueue q;
     unsigned long long* ctr;   // work counters (CTR_*)
 };
 
+// Entries of the hit queue a width x height primary trace needs on a device of cu_count CUs.
+size_t hitq_capacity(uint32_t width, uint32_t height, int cu_count);
+// The fused march + hit-shading kernel, then the trailing pass over the hits it left.
 hipError_t launch_trace_primary(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
-hipError_t launch_refine64(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
+hipError_t launch_hits_tail(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
 // Work item of the batched serving kernel: an 8x8 tile of request `slot`, samples
 // [s0, s0 + ns).
 struct BatchItem {

@@ -113,7 +113,13 @@ enum : uint32_t {
     CTR_PRIMARY = 7,
     CTR_HIT_EVALS = 8,  // FP32 evaluations of the hit kernel (normal taps of the primary trace)
     CTR_HIT_ITERS = 9,
-    CTR_COUNT = 10,
+    // of the above: the work of the primary trace's trailing hit pass
+    CTR_TAIL_HITS = 10,
+    CTR_TAIL_EVALS = 11,
+    CTR_TAIL_ITERS = 12,
+    CTR_TAIL_DE64_EVALS = 13,
+    CTR_TAIL_DE64_ITERS = 14,
+    CTR_COUNT = 15,
 };
 
 }  // namespace syf

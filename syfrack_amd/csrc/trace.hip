@@ -1,13 +1,16 @@
 // Primary-ray sphere trace and distance probe kernels.
 //
-// The primary trace runs in two kernels:
-//   k_march  FP32 march of every camera ray (march_segment, pathtrace.wgsl:78-145) as a
-//            lane-persistent state machine; hits are queued;
-//   k_hits   for every queued hit, the tetrahedral normal (common.wgsl:376-387, FP32) and
-//            the FP64 refine of the hit distance.
+// The primary trace runs in one persistent kernel and a short trailing pass:
+//   k_primary    FP32 march of every camera ray (march_segment, pathtrace.wgsl:78-145) as a
+//                lane-persistent state machine; hits go to a hit queue.  A wave whose lanes
+//                have no rays left lowers its issue priority and takes queued hits: the
+//                tetrahedral normal (common.wgsl:376-387, FP32) and the FP64 refine of the hit
+//                distance.  The hit work thus fills the SIMDs the march's tail leaves idle
+//                instead of running after it.
+//   k_hits_tail  the hits no wave took (those queued after the last consumer left).
 // The march reads nothing from HBM: the job's constants sit in the kernel argument segment and
 // each ray is derived from (pixel, sample, seed) by the PCG stream; HBM traffic is the output
-// planes and the hit queue (about 32 bytes per ray).
+// planes and the hit queue (about 40 bytes per ray).
 #include <algorithm>
 
 #include "device.hpp"
@@ -33,32 +36,91 @@ constexpr uint32_t HIT_BATCH = 64;
 // A wave generates new camera rays when at least this many of its lanes are idle (or none is
 // marching): ray generation is a branch every lane of the wave pays for, so it is batched.
 constexpr uint32_t REFILL_BATCH = 8;
+// Issue priorities: marching waves are served first; hit consumers take the slots left over.
+constexpr short PRIO_MARCH = 1, PRIO_HITS = 0;
 
 __device__ __forceinline__ uint64_t lanes_below() {
     const uint32_t lane = threadIdx.x & 63u;
     return lane == 0u ? 0ull : (~0ull >> (64u - lane));
 }
 
-// FP32 march of all primary rays.  Every trip of the loop runs at most one Iterate body on
-// each lane; a lane whose field evaluation completes in the trip applies the fence and
-// advances its march (the arithmetic of march(), in the same order), queueing a hit or writing
-// a miss when the ray ends.  Idle lanes take new rays from a wave-owned chunk of the launch's
-// ray counter.  Deep and shallow evaluations, long and short rays share a wave without waiting
-// for each other.
+// The hit queue's entries cross XCDs, whose L2 caches are not coherent with each other:
+// fields are written and read with agent-scope atomic accesses (coherent at the device), and
+// every field starts as HITQ_EMPTY (the queue is filled with it before the launch).  A field
+// is thus either empty or final, and an entry whose four fields all differ from HITQ_EMPTY is
+// complete; no fence is needed.  No field can be HITQ_EMPTY when written: pixel < 2^32 - 2,
+// distances are finite floats, steps <= max_steps.  Entries are reserved in blocks of 64; a
+// wave's last, partial block is padded with HITQ_SKIP entries.
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_agent(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Normal and FP64 refine of one hit.  The camera ray is regenerated from its PCG stream (the
+// FP32 ray the march followed); the normal takes four FP32 taps at the hit point
+// (common.wgsl:376-387, h = max(t * cone / 2, 1e-4) * pixel_tolerance); the refine locates the
+// hit rule's crossing in FP64 between the march's last two positions.
+template <uint32_t FAM>
+__device__ __forceinline__ void shade_hit(const JobP& P, uint32_t sample, uint32_t pix, float t, float t_prev,
+                                          uint32_t steps, float4* __restrict__ aux, double* __restrict__ depth64,
+                                          Work& w64, Work& wn) {
+    const uint32_t px = pix % P.width, py = pix / P.width;
+    uint32_t rng = rng_init(px, py, sample, P.seed);
+    V3<float> ro, rd;
+    camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
+    const V3<float> p = add(ro, scl(rd, t));
+    const V3<float> nv = normal_at<float, FAM>(P.f, p, fmaxf(t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, wn);
+    aux[pix] = make_float4(nv.x, nv.y, nv.z, float(steps));
+    depth64[pix] = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(t_prev), double(t),
+                                 double(P.pixel_cone) * double(P.pixel_tolerance), w64);
+}
+
 // Global ray index of slot j of ray lane `xl` (tiles dealt round-robin to the lanes).
 __device__ __forceinline__ uint32_t lane_ray(uint32_t j, uint32_t xl) {
     return (((j >> 6) * RAY_LANES + xl) << 6) | (j & 63u);
 }
 
+// Publishes the first `n` staged hits of a wave (n <= 64) as one block of the hit queue: one
+// atomic reserves the block, each lane writes one entry's four fields (lanes >= n a skip entry).
+__device__ __forceinline__ void publish_hits(const uint32_t (*stage)[2 * HIT_BATCH], uint32_t n, const HitQueue& Q,
+                                             uint32_t lane) {
+    uint32_t base = 0;
+    if (lane == 0u) base = atomicAdd(Q.count, HIT_BATCH);
+    base = __shfl(base, 0, 64);
+    if (base + HIT_BATCH > Q.capacity) {  // cannot happen at the capacity hitq_capacity() gives
+        if (lane == 0u) atomicAdd(Q.error, 1u);
+        return;
+    }
+    const uint32_t k = base + lane;
+    const bool real = lane < n;
+    st_agent(Q.t + k, real ? stage[1][lane] : 0u);
+    st_agent(Q.t_prev + k, real ? stage[2][lane] : 0u);
+    st_agent(Q.steps + k, real ? stage[3][lane] : 0u);
+    st_agent(Q.pix + k, real ? stage[0][lane] : HITQ_SKIP);
+}
+
+// FP32 march of all primary rays, then hit shading.  March: every trip of the loop runs at
+// most one Iterate body on each lane; a lane whose field evaluation completes in the trip
+// applies the fence and advances its march (the arithmetic of march(), in the same order),
+// queueing a hit or writing a miss when the ray ends.  Idle lanes take new rays from a
+// wave-owned chunk of the launch's ray counter.  Deep and shallow evaluations, long and short
+// rays share a wave without waiting for each other.  Hits: once no ray is left for any of its
+// lanes, the wave claims runs of complete queue entries (one per lane) at the queue's cursor
+// and shades them, until it finds none complete; it never waits for one.
 template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_march(const JobP P, uint32_t sample, uint32_t tiles_x, uint32_t n_slots,
-                                               uint32_t* __restrict__ next, float* __restrict__ depth,
-                                               float4* __restrict__ aux, uint32_t* __restrict__ steps_out,
-                                               float* __restrict__ t_prev_out, uint32_t* __restrict__ hit_list,
-                                               uint32_t* __restrict__ hit_count, unsigned long long* __restrict__ ctr) {
+__global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, uint32_t tiles_x, uint32_t n_slots,
+                                                 uint32_t* __restrict__ next, float* __restrict__ depth,
+                                                 float4* __restrict__ aux, double* __restrict__ depth64,
+                                                 uint32_t* __restrict__ steps_out, const HitQueue Q,
+                                                 unsigned long long* __restrict__ ctr) {
     enum : uint32_t { IDLE = 0, MARCH = 1, DONE = 2 };
-    __shared__ uint32_t hit_stage[4][2 * HIT_BATCH];
+    // per wave: staged hits' pixel, t, t before the hit and steps
+    __shared__ uint32_t hit_stage[4][4][2 * HIT_BATCH];
+    __builtin_amdgcn_s_setprio(PRIO_MARCH);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t (*stage)[2 * HIT_BATCH] = hit_stage[wave];
     const uint32_t xl = blockIdx.x % RAY_LANES;
     const uint32_t lane_slots = ((n_slots >> 6) + RAY_LANES - 1u) / RAY_LANES * 64u;  // slots per ray lane
     uint32_t staged = 0;  // wave-uniform count of staged hits
@@ -126,6 +188,8 @@ __global__ __launch_bounds__(256) void k_march(const JobP P, uint32_t sample, ui
         const bool active = mode == MARCH;
         if (active && !de_done) de_done = de.step(F);
         bool hit_now = false;
+        uint32_t hit_steps = 0;
+        float t_hit = 0.0f, tp_hit = 0.0f;
         if (active && de_done) {
             const De<float> s = de.finish(F);
             w.de_evals++;
@@ -147,7 +211,8 @@ __global__ __launch_bounds__(256) void k_march(const JobP P, uint32_t sample, ui
                 cand_err = err;
             }
             uint32_t outcome = 0;  // 0 continue, 1 hit, 2 miss
-            float t_hit = t, tp_hit = t_last;
+            t_hit = t;
+            tp_hit = t_last;
             if (!sor_fail && err < hit_cone) {
                 outcome = 1;
             } else {
@@ -179,11 +244,12 @@ __global__ __launch_bounds__(256) void k_march(const JobP P, uint32_t sample, ui
                 steps_out[pix] = nsteps;
                 if (outcome == 1u) {
                     depth[pix] = t_hit;
-                    t_prev_out[pix] = tp_hit;
                     hit_now = true;
+                    hit_steps = nsteps;
                     ++hits;
                 } else {
                     depth[pix] = -1.0f;
+                    depth64[pix] = -1.0;
                     aux[pix] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
                 }
                 mode = IDLE;
@@ -192,20 +258,28 @@ __global__ __launch_bounds__(256) void k_march(const JobP P, uint32_t sample, ui
         // stage this trip's hits in LDS; publish them 64 at a time (one atomic per batch)
         const uint64_t hm = __ballot(hit_now);
         if (hm) {
-            if (hit_now) hit_stage[wave][staged + uint32_t(__popcll(hm & lanes_below()))] = pix;
+            if (hit_now) {
+                const uint32_t at = staged + uint32_t(__popcll(hm & lanes_below()));
+                stage[0][at] = pix;
+                stage[1][at] = __float_as_uint(t_hit);
+                stage[2][at] = __float_as_uint(tp_hit);
+                stage[3][at] = hit_steps;
+            }
             staged += uint32_t(__popcll(hm));
             if (staged >= HIT_BATCH) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                uint32_t base = 0;
-                if (lane == 0u) base = atomicAdd(hit_count, HIT_BATCH);
-                base = __shfl(base, 0, 64);
-                hit_list[base + lane] = hit_stage[wave][lane];
+                publish_hits(stage, HIT_BATCH, Q, lane);
                 const uint32_t rest = staged - HIT_BATCH;
-                const uint32_t carry = lane < rest ? hit_stage[wave][HIT_BATCH + lane] : 0u;
+                uint32_t carry[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) carry[f] = lane < rest ? stage[f][HIT_BATCH + lane] : 0u;
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                if (lane < rest) hit_stage[wave][lane] = carry;
+                if (lane < rest) {
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) stage[f][lane] = carry[f];
+                }
                 staged = rest;
             }
         }
@@ -213,55 +287,75 @@ __global__ __launch_bounds__(256) void k_march(const JobP P, uint32_t sample, ui
     if (staged) {  // the wave's last hits
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        uint32_t base = 0;
-        if (lane == 0u) base = atomicAdd(hit_count, staged);
-        base = __shfl(base, 0, 64);
-        if (lane < staged) hit_list[base + lane] = hit_stage[wave][lane];
+        publish_hits(stage, staged, Q, lane);
+    }
+    // Hit shading, at the lower priority.  A claim takes the block at the cursor with one
+    // fetch-and-add (never retried); a wave stops when the cursor has caught up with the blocks
+    // reserved.  A claimed block that is not reserved yet, or whose entries are not all
+    // complete, is left without its done flag: the trailing pass shades it.
+    __builtin_amdgcn_s_setprio(PRIO_HITS);
+    Work wn;  // normal taps (counted apart from the march's evaluations)
+    const uint32_t cap_blocks = Q.capacity / HIT_BATCH;
+    for (uint32_t guard = 0; guard < cap_blocks; ++guard) {  // every trip claims a distinct block
+        uint32_t b = 0, reserved = 0;
+        if (lane == 0u) {
+            reserved = ld_agent(Q.count) / HIT_BATCH;
+            b = ld_agent(Q.cursor) < reserved ? atomicAdd(Q.cursor, 1u) : HITQ_EMPTY;
+            if (b != HITQ_EMPTY && b >= reserved) reserved = ld_agent(Q.count) / HIT_BATCH;
+        }
+        b = __shfl(b, 0, 64);
+        reserved = __shfl(reserved, 0, 64);
+        if (b >= reserved || b >= cap_blocks) break;
+        const uint32_t k = b * HIT_BATCH + lane;
+        const uint32_t e_pix = ld_agent(Q.pix + k), e_t = ld_agent(Q.t + k), e_tp = ld_agent(Q.t_prev + k),
+                       e_st = ld_agent(Q.steps + k);
+        const bool ready = e_pix != HITQ_EMPTY && e_t != HITQ_EMPTY && e_tp != HITQ_EMPTY && e_st != HITQ_EMPTY;
+        if (__ballot(ready) != ~0ull) continue;  // its producer is still writing it
+        if (e_pix < P.width * P.height)  // not a padding entry
+            shade_hit<FAM>(P, sample, e_pix, __uint_as_float(e_t), __uint_as_float(e_tp), e_st, aux, depth64, w, wn);
+        if (lane == 0u) Q.done[b] = 1u;
     }
     flush_work(w, hits, rays, ctr);
-    const uint32_t pr = wave_sum(rays);
-    if (lane == 0u) atomicAdd(&ctr[CTR_PRIMARY], (unsigned long long)pr);
-}
-
-// Normal and FP64 refine of every queued hit.  The camera ray is regenerated from its PCG
-// stream (the FP32 ray the march followed); the normal takes four FP32 taps at the hit point
-// (common.wgsl:376-387, h = max(t * cone / 2, 1e-4) * pixel_tolerance); the refine locates the
-// hit rule's crossing in FP64 between the march's last two positions.  Every lane holds a hit,
-// and all four taps of a lane sit at the same depth of the field.
-template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_hits(const JobP P, uint32_t sample, const uint32_t* __restrict__ hit_list,
-                                              const uint32_t* __restrict__ hit_count, const float* __restrict__ depth,
-                                              const float* __restrict__ t_prev, const uint32_t* __restrict__ steps,
-                                              float4* __restrict__ aux, double* __restrict__ depth64,
-                                              unsigned long long* __restrict__ ctr) {
-    const uint32_t n = *hit_count;
-    Work w, wn;  // refine; normal taps (counted apart from the march's evaluations)
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const uint32_t pix = hit_list[k];
-        const uint32_t px = pix % P.width, py = pix / P.width;
-        uint32_t rng = rng_init(px, py, sample, P.seed);
-        V3<float> ro, rd;
-        camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
-        const float t = depth[pix];
-        const V3<float> p = add(ro, scl(rd, t));
-        const V3<float> nv = normal_at<float, FAM>(P.f, p, fmaxf(t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, wn);
-        aux[pix] = make_float4(nv.x, nv.y, nv.z, float(steps[pix]));
-        depth64[pix] = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(t_prev[pix]), double(t),
-                                     double(P.pixel_cone) * double(P.pixel_tolerance), w);
-    }
-    flush_work(w, 0u, 0u, ctr);
-    const uint32_t a = wave_sum(wn.de_evals), b = wave_sum(wn.de_iters);
-    if ((threadIdx.x & 63u) == 0u) {
+    const uint32_t pr = wave_sum(rays), a = wave_sum(wn.de_evals), b = wave_sum(wn.de_iters);
+    if (lane == 0u) {
+        atomicAdd(&ctr[CTR_PRIMARY], (unsigned long long)pr);
         atomicAdd(&ctr[CTR_HIT_EVALS], (unsigned long long)a);
         atomicAdd(&ctr[CTR_HIT_ITERS], (unsigned long long)b);
     }
 }
 
-// Misses of the FP64 plane: -1 wherever the march found no hit.
-__global__ __launch_bounds__(256) void k_depth64_init(const float* __restrict__ depth, double* __restrict__ depth64,
-                                                      uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        if (depth[i] < 0.0f) depth64[i] = -1.0;
+// The hits no wave of k_primary shaded: the entries of every reserved block without its done
+// flag, one per lane, each lane striding over several (a lane's successive hits lie far apart,
+// so the expensive refines of silhouette regions spread over all lanes).  The queue is complete
+// and coherent at this launch's start.  Counted apart (CTR_TAIL_*) so that the fused kernel's
+// own work is known.
+template <uint32_t FAM>
+__global__ __launch_bounds__(256) void k_hits_tail(const JobP P, uint32_t sample, const HitQueue Q,
+                                                   float4* __restrict__ aux, double* __restrict__ depth64,
+                                                   unsigned long long* __restrict__ ctr) {
+    const uint32_t n = min(*Q.count, Q.capacity);
+    Work w, wn;
+    uint32_t tail = 0;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t pix = Q.pix[k];
+        if (Q.done[k / HIT_BATCH] || pix >= P.width * P.height) continue;  // shaded, or a padding entry
+        shade_hit<FAM>(P, sample, pix, __uint_as_float(Q.t[k]), __uint_as_float(Q.t_prev[k]), Q.steps[k], aux,
+                       depth64, w, wn);
+        ++tail;
+    }
+    const uint32_t a = wave_sum(wn.de_evals), b = wave_sum(wn.de_iters), c = wave_sum(w.de64_evals),
+                   d = wave_sum(w.de64_iters), h = wave_sum(tail);
+    if ((threadIdx.x & 63u) == 0u && h) {
+        atomicAdd(&ctr[CTR_HIT_EVALS], (unsigned long long)a);
+        atomicAdd(&ctr[CTR_HIT_ITERS], (unsigned long long)b);
+        atomicAdd(&ctr[CTR_DE64_EVALS], (unsigned long long)c);
+        atomicAdd(&ctr[CTR_DE64_ITERS], (unsigned long long)d);
+        atomicAdd(&ctr[CTR_TAIL_HITS], (unsigned long long)h);
+        atomicAdd(&ctr[CTR_TAIL_EVALS], (unsigned long long)a);
+        atomicAdd(&ctr[CTR_TAIL_ITERS], (unsigned long long)b);
+        atomicAdd(&ctr[CTR_TAIL_DE64_EVALS], (unsigned long long)c);
+        atomicAdd(&ctr[CTR_TAIL_DE64_ITERS], (unsigned long long)d);
+    }
 }
 
 template <uint32_t FAM>
@@ -285,33 +379,39 @@ __global__ __launch_bounds__(256) void k_de_probe(const FractalP F, const float4
     default: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_SPHERE>, __VA_ARGS__); break;                        \
     }
 
+// Workgroups of a primary-trace launch.  Two workgroups (eight waves) per CU: a frame's time is
+// set by its longest rays (up to max_steps evaluations in sequence on one lane); more resident
+// waves share each SIMD's issue slots with them and stretch that critical path (measured at
+// 1080p, march alone: 2 per CU 0.80 ms, 1 per CU 1.00 ms, 3 per CU 0.88 ms, 8 per CU 1.65 ms).
+// At least one workgroup per ray lane, or that lane's tiles would never be traced.
+static uint32_t primary_blocks(uint32_t width, uint32_t height, int cu_count) {
+    const uint32_t n_slots = ((width + 7u) / 8u) * ((height + 7u) / 8u) * 64u;
+    const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * 2u;
+    return std::max<uint32_t>(std::min<uint32_t>((n_slots + 255u) / 256u, fill), RAY_LANES);
+}
+
+size_t hitq_capacity(uint32_t width, uint32_t height, int cu_count) {
+    // every pixel, plus one padded block per wave (4 waves per workgroup)
+    const size_t pixels = size_t(width) * height;
+    return (pixels + HIT_BATCH - 1) / HIT_BATCH * HIT_BATCH + size_t(primary_blocks(width, height, cu_count)) * 4u * HIT_BATCH;
+}
+
 hipError_t launch_trace_primary(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s) {
     const uint32_t tiles_x = (P.width + 7u) / 8u, tiles_y = (P.height + 7u) / 8u;
     const uint32_t n_slots = tiles_x * tiles_y * 64u;
-    // Two workgroups (eight waves) per CU.  A frame's time is set by its longest rays (up to
-    // max_steps evaluations in sequence on one lane); more resident waves share each SIMD's
-    // issue slots with them and stretch that critical path: measured at 1080p, 2 per CU
-    // 0.80 ms, 1 per CU 1.00 ms, 3 per CU 0.88 ms, 8 per CU 1.65 ms.
-    const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * 2u;
-    // at least one workgroup per ray lane, or that lane's tiles would never be traced
-    const dim3 grid(std::max<uint32_t>(std::min<uint32_t>((n_slots + 255u) / 256u, fill), RAY_LANES)), block(256);
-    SYF_DISPATCH_FAMILY(kernel_family(P.f), k_march, grid, block, 0, s, P, sample, tiles_x, n_slots, B.counters,
-                        B.depth, B.aux, B.steps, B.t_prev, B.hit_list, B.counters + COUNTER_HITS, B.ctr);
+    const dim3 grid(primary_blocks(P.width, P.height, cu_count)), block(256);
+    SYF_DISPATCH_FAMILY(kernel_family(P.f), k_primary, grid, block, 0, s, P, sample, tiles_x, n_slots, B.counters,
+                        B.depth, B.aux, B.depth64, B.steps, B.q, B.ctr);
     return hipGetLastError();
 }
 
-hipError_t launch_refine64(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s) {
-    const uint32_t n = P.width * P.height;
+hipError_t launch_hits_tail(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s) {
     // two workgroups per CU, each lane striding over several hits: measured 2.3x faster than
-    // one hit per lane (eight workgroups per CU) at 1080p; a lane's successive hits lie far
-    // apart, so the expensive refines of silhouette regions spread over all lanes
+    // one hit per lane (eight workgroups per CU) for a whole 1080p frame's hits
+    const uint32_t n = P.width * P.height;
     const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * 2u;
     const dim3 grid(std::min<uint32_t>((n + 255u) / 256u, fill)), block(256);
-    hipLaunchKernelGGL(k_depth64_init, grid, block, 0, s, B.depth, B.depth64, n);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    SYF_DISPATCH_FAMILY(kernel_family(P.f), k_hits, grid, block, 0, s, P, sample, B.hit_list, B.counters + COUNTER_HITS, B.depth,
-                        B.t_prev, B.steps, B.aux, B.depth64, B.ctr);
+    SYF_DISPATCH_FAMILY(kernel_family(P.f), k_hits_tail, grid, block, 0, s, P, sample, B.q, B.aux, B.depth64, B.ctr);
     return hipGetLastError();
 }
 

@@ -35,17 +35,37 @@ REFINE_EVAL = 9
 
 
 def march_flops(family: int, stats: dict) -> float:
-    """FP32 FLOPs of the primary trace's march kernel alone (its evaluations and steps; the
-    normal taps of the hit kernel are counted apart, in hit_de_*)."""
+    """FP32 FLOPs of the primary trace's march (its evaluations and steps; the normal taps are
+    counted apart, in hit_de_*)."""
     return (BODY[family] * (stats["de_iterations"] - stats["hit_de_iterations"])
             + EVAL[family] * (stats["de_evals"] - stats["hit_de_evals"]) + STEP * stats["steps"])
 
 
 def hit_fp32_flops(family: int, stats: dict) -> float:
-    """FP32 FLOPs of the primary trace's hit kernel: the four-tap normals."""
+    """FP32 FLOPs of the primary trace's hit shading: the four-tap normals."""
     return BODY[family] * stats["hit_de_iterations"] + EVAL[family] * stats["hit_de_evals"] + NORMAL * stats["hits"]
 
 
 def refine_flops(family: int, stats: dict) -> float:
-    """FP64 FLOPs of the final-hit refine kernel."""
+    """FP64 FLOPs of the final-hit refines."""
     return (BODY[family] * stats["de64_iterations"] + (EVAL[family] + REFINE_EVAL) * stats["de64_evals"])
+
+
+def tail_fp32_flops(family: int, stats: dict) -> float:
+    """FP32 FLOPs of the normals shaded by the primary trace's trailing hit pass."""
+    return (BODY[family] * stats["tail_hit_de_iterations"] + EVAL[family] * stats["tail_hit_de_evals"]
+            + NORMAL * stats["tail_hits"])
+
+
+def tail_fp64_flops(family: int, stats: dict) -> float:
+    """FP64 FLOPs of the refines run by the primary trace's trailing hit pass."""
+    return (BODY[family] * stats["tail_de64_iterations"]
+            + (EVAL[family] + REFINE_EVAL) * stats["tail_de64_evals"])
+
+
+def fused_flops(family: int, stats: dict) -> tuple:
+    """(FP32, FP64) FLOPs of the fused march + hit-shading kernel alone: the march, and the
+    normals and refines of the hits it shaded (all hits but the trailing pass's)."""
+    fp32 = march_flops(family, stats) + hit_fp32_flops(family, stats) - tail_fp32_flops(family, stats)
+    fp64 = refine_flops(family, stats) - tail_fp64_flops(family, stats)
+    return fp32, fp64

@@ -154,6 +154,23 @@ def test_mandelbulb_1080p_properties_and_centre_rows(renderer):
     assert st["hits"] == int(hits.sum())
     assert st["steps"] == int(out.steps.astype(np.int64).sum())
     assert st["de_evals"] == st["steps"] + 4 * st["hits"]
+    # every hit was shaded once, by the fused kernel or the trailing pass: its normal carries
+    # its step count and its FP64 depth is set
+    assert 0 <= st["tail_hits"] <= st["hits"]
+    assert st["tail_hit_de_evals"] == 4 * st["tail_hits"]
+    assert np.all(out.normals[hits][:, 3] == out.steps[hits].astype(np.float32))
+    assert np.all(out.depth64[hits] > 0.0)
+    print(f"1080p: {st['hits']} hits, {st['tail_hits']} shaded by the trailing pass")
+
+
+def test_primary_trace_is_deterministic(renderer):
+    """Which wave shades which hit varies from run to run; the planes do not."""
+    from syfrack_amd import scene as sc
+    scene, cfg = sc.mandelbulb_primary(1920, 1080)
+    a = renderer.trace_primary(scene, cfg)
+    b = renderer.trace_primary(scene, cfg)
+    for x, y in ((a.depth, b.depth), (a.depth64, b.depth64), (a.normals, b.normals), (a.steps, b.steps)):
+        assert np.array_equal(x, y)
 
 
 def test_sphere_scene_centre_depth_and_tolerance(renderer):
