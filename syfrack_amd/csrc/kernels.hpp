@@ -60,12 +60,19 @@ struct BatchPlanes {
     double* depth64;
 };
 // One persistent launch of the flattened path tracer over work items that all belong to
-// kernel family `fam` (kernel_family of their jobs).  claim: 8 counters, 16 words apart
-// (zeroed before the launch).  Items starting at sample 0 leave the refine bracket in their
-// DEPTH64 plane; launch_refine_first_hits over the same items replaces it by the FP64 hit.
+// kernel family `fam` (kernel_family of their jobs).  claim: 8 counters, 16 words apart; ctr:
+// CTR_COUNT work counters; exit_count: one word.  All three are zero before the launch and are
+// left zero by it: the launch's last wave copies the work counters to host_out (host memory)
+// and clears them.  Items starting at sample 0 begin the pixels' accumulation and leave the
+// refine bracket in their DEPTH64 plane; launch_refine_first_hits over the same items replaces
+// it by the FP64 hit.
 constexpr uint32_t CLAIM_WORDS = 8 * 16;
 hipError_t launch_paths(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
-                        uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t grid_blocks,
+                        uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t* exit_count,
+                        unsigned long long* host_out, uint32_t grid_blocks, hipStream_t s);
+// Writes slot `slot` of the batched launches' job and plane tables (one small kernel: it runs on
+// the stream of the launch that first reads the slot).
+hipError_t launch_admit(JobP* jobs, BatchPlanes* planes, uint32_t slot, const JobP& P, const BatchPlanes& pl,
                         hipStream_t s);
 hipError_t launch_refine_first_hits(uint32_t fam, const JobP* jobs, const BatchPlanes* planes,
                                     const BatchItem* items, uint32_t n_items, unsigned long long* ctr,

@@ -332,7 +332,9 @@ constexpr int PATHS_WAVES = 3;
 template <uint32_t FAM>
 __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restrict__ jobs, const BatchPlanes* __restrict__ planes,
                                                const BatchItem* __restrict__ items, uint32_t n_items,
-                                               uint32_t* __restrict__ claim, unsigned long long* __restrict__ ctr) {
+                                               uint32_t* __restrict__ claim, unsigned long long* __restrict__ ctr,
+                                               uint32_t* __restrict__ exit_count,
+                                               unsigned long long* __restrict__ host_out) {
     const uint32_t lane = threadIdx.x & 63u;
     // The items are cut into eight contiguous ranges, one per XCD (a workgroup starts on the
     // range of blockIdx % 8, the XCD the dispatcher places it on): consecutive claims of a
@@ -391,7 +393,9 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
                         L.s = pool.s0;
                         L.s_first = pool.s0;
                         L.s_end = pool.s0 + pool.ns;
-                        L.acc = planes[slot].accum[size_t(py) * P.width + px];
+                        // the first pass starts the accumulation (the planes are not cleared)
+                        L.acc = pool.s0 == 0u ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
+                                              : planes[slot].accum[size_t(py) * P.width + px];
                         L.fd0 = -1.0f;
                         L.tp0 = -1.0f;
                         L.fa0 = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
@@ -525,7 +529,20 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
     }
     flush_work(work, hits, segs + shadow, ctr);
     const uint32_t p = wave_sum(prim);
-    if (lane == 0u) atomicAdd(&ctr[CTR_PRIMARY], (unsigned long long)p);
+    if (lane == 0u) {
+        atomicAdd(&ctr[CTR_PRIMARY], (unsigned long long)p);
+        // The launch's last wave (every wave of the grid counts itself out once its counts are
+        // in) hands the counters to the host and clears them, the claim counters and the exit
+        // count for the launch that reuses them: no clearing kernel runs between launches.
+        const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+        if (__hip_atomic_fetch_add(exit_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == waves - 1u) {
+            for (uint32_t c = 0; c < CTR_COUNT; ++c)
+                host_out[c] = __hip_atomic_exchange(&ctr[c], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (uint32_t x = 0; x < 8u; ++x) __hip_atomic_store(&claim[x * 16u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(exit_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // system scope: the host reads host_out
+        }
+    }
 }
 
 // FP64 refine of the first surface hits of the items that start at sample 0: k_paths left the
@@ -577,20 +594,37 @@ hipError_t launch_refine_first_hits(uint32_t fam, const JobP* jobs, const BatchP
     return hipGetLastError();
 }
 
-hipError_t launch_paths(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
-                        uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t grid_blocks,
+// Writes a request's entries of the slot tables; the entries travel as kernel arguments.
+__global__ __launch_bounds__(64) void k_admit(JobP* __restrict__ jobs, BatchPlanes* __restrict__ planes, uint32_t slot,
+                                              const JobP P, const BatchPlanes pl) {
+    constexpr uint32_t words = sizeof(JobP) / 4u;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&P);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(jobs + slot);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+    if (threadIdx.x == 0u) planes[slot] = pl;
+}
+
+hipError_t launch_admit(JobP* jobs, BatchPlanes* planes, uint32_t slot, const JobP& P, const BatchPlanes& pl,
                         hipStream_t s) {
+    static_assert(sizeof(JobP) % 4u == 0u, "JobP is copied in words");
+    hipLaunchKernelGGL(k_admit, dim3(1), dim3(64), 0, s, jobs, planes, slot, P, pl);
+    return hipGetLastError();
+}
+
+hipError_t launch_paths(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
+                        uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t* exit_count,
+                        unsigned long long* host_out, uint32_t grid_blocks, hipStream_t s) {
     if (n_items == 0) return hipSuccess;
     if (fam >= FAM_COUNT) return hipErrorInvalidValue;
     const uint32_t waves = (n_items + 7u) / 8u * 8u;
     const dim3 grid(std::max<uint32_t>(8u, std::min<uint32_t>(grid_blocks, (waves + 3u) / 4u))), block(256);
     switch (fam) {
-    case SYF_FAMILY_MANDELBULB: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MANDELBULB>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr); break;
-    case SYF_FAMILY_MANDELBOX: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MANDELBOX>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr); break;
-    case SYF_FAMILY_MENGER: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MENGER>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr); break;
-    case FAM_BULB_TRIG: hipLaunchKernelGGL(k_paths<FAM_BULB_TRIG>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr); break;
-    case FAM_BULB8: hipLaunchKernelGGL(k_paths<FAM_BULB8>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr); break;
-    default: hipLaunchKernelGGL(k_paths<SYF_FAMILY_SPHERE>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr); break;
+    case SYF_FAMILY_MANDELBULB: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MANDELBULB>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    case SYF_FAMILY_MANDELBOX: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MANDELBOX>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    case SYF_FAMILY_MENGER: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MENGER>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    case FAM_BULB_TRIG: hipLaunchKernelGGL(k_paths<FAM_BULB_TRIG>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    case FAM_BULB8: hipLaunchKernelGGL(k_paths<FAM_BULB8>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    default: hipLaunchKernelGGL(k_paths<SYF_FAMILY_SPHERE>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
     }
     return hipGetLastError();
 }

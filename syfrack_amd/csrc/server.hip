@@ -7,14 +7,18 @@
 //      remaining work first, until the launch holds enough items to fill the device and its
 //      estimated device time reaches the target (a running estimate of ns per
 //      pixel-sample-segment, refreshed from every launch that filled the device);
-//   3. enqueues it (items copied from pinned host memory, claim counters cleared, one
-//      persistent kernel per kernel family present, the families on streams of their own so
-//      that they run concurrently) and keeps up to two launches in flight, so the device does
-//      not idle while the next launch is planned;
+//   3. enqueues it (one persistent kernel per kernel family present, the families on streams
+//      of their own so that they run concurrently) and keeps up to three launches in flight, so
+//      that the next launch's workgroups take the CUs the running launch's tail frees.  Nothing else is enqueued
+//      between launches: the kernels read their work items from pinned host memory, and each
+//      launch's last wave hands its counters to the host and clears them.  (A copy or clear
+//      enqueued between two launches runs as a kernel of its own, which waits for the running
+//      launch's workgroups to leave the CUs and so kept consecutive launches from overlapping.)
 //   4. retires the oldest launch when its end event completes: adds its samples to the
 //      requests' progress and stamps the requests it finished.
-// Accumulation of a pixel is owned by one work item per sample range and successive ranges
-// of a request run in launch order, so samples land in ascending order with no float atomics.
+// Accumulation of a pixel is owned by one work item per sample range, and a tile's next range
+// is planned only after the launch holding its previous range retired, so samples land in
+// ascending order with no float atomics; the other tiles of the request go on meanwhile.
 // The plain mode runs one launch per request (all samples), in arrival order.
 #include <hip/hip_runtime.h>
 
@@ -43,7 +47,13 @@ struct Request {
     uint64_t total = 0;      // tile-samples of the whole request
     uint64_t planned = 0;    // tile-samples launched
     uint64_t completed = 0;  // tile-samples whose launches finished
-    uint32_t q_start = 0, q_len = 0, q_cursor = 0;  // current quantum: samples and next tile
+    // Passes: the request's samples go in quanta of q_len samples starting at q_start; a pass
+    // plans the tiles in ascending order (q_cursor = next tile).  Tile t of a pass may be
+    // planned only once the launch holding tile t of the previous pass has retired: ret_prev
+    // and ret_cur are the retired tile prefixes of passes `pass - 1` and `pass`.
+    uint32_t q_start = 0, q_len = 0, q_cursor = 0;
+    uint32_t pass = 0, ret_prev = 0, ret_cur = 0;
+    bool pass_open = false;
     size_t pixels = 0;
     uint64_t footprint = 0;
     void* accum = nullptr;
@@ -51,16 +61,29 @@ struct Request {
     void* aux = nullptr;
     void* depth64 = nullptr;
     BatchPlanes planes{};  // source of the request's entry in the device plane table
+    // Batched mode: the slot-table entries are written by the first launch that holds items of
+    // the request; until that launch retires, no other launch takes its items.
+    bool uploaded = false, upload_in_flight = false;
     double submit_ms = 0, start_ms = -1, done_ms = -1;
     bool cancelled = false;
     bool released = false;  // released by the caller; deleted once no launch refers to it
     uint32_t refs = 0;      // launches in flight (or a plain render) that refer to the request
 };
 
+// A launch's share of one request: tile-samples, and the pass and end of the tile range it
+// holds (tiles of one pass, ascending).
+struct Share {
+    Request* r;
+    uint64_t samples;
+    uint32_t pass, tile_end;
+};
+
 struct Launch {
     hipEvent_t start = nullptr, end = nullptr;
-    std::vector<std::pair<Request*, uint64_t>> progress;  // request, tile-samples completed by the launch
+    std::vector<Share> progress;
+    std::vector<Request*> uploads;  // requests whose slot-table entries this launch writes
     uint32_t buf = 0;
+    uint32_t families = 0;  // bit f: a kernel of family f ran
     double units = 0;
     bool full = false;  // enough items to fill the device: its time measures throughput
 };
@@ -69,19 +92,19 @@ struct Launch {
 
 using namespace syf;
 
+// Batched launches in flight at once; each has its own work-item and counter buffers, streams
+// and events.
+constexpr int NBUF = 3;
+
 struct syf_server {
     syf_ctx* ctx = nullptr;
     syf_server_config cfg{};
     hipStream_t stream = nullptr;  // admissions, plain-mode launches, calibration
-    // Batched launches: launch b (b = launch index & 1) runs on stream ls[b] and, per kernel
-    // family present, on fam_stream[b][f]: the family groups of a launch run concurrently (they
-    // serve different requests), and so do two consecutive launches (the planner keeps their
-    // pixels disjoint).  Events order each launch after the admissions before it.
-    hipStream_t ls[2] = {};
-    hipStream_t fam_stream[2][FAM_COUNT] = {};
-    hipEvent_t ev_fam[2][FAM_COUNT] = {};
-    hipEvent_t ev_ready[2] = {};
-    hipEvent_t ev_admit = nullptr;
+    // Batched launches: launch b (b = launch index % NBUF) runs all its kernels on stream ls[b].
+    // Consecutive launches run concurrently (the planner keeps their pixels disjoint), and no
+    // launch waits on another stream: a wait on a stream that shares a hardware queue with a
+    // running launch would hold it back until that launch ends.
+    hipStream_t ls[NBUF] = {};
     std::mutex mu;
     std::condition_variable cv;
     std::thread worker;
@@ -95,12 +118,17 @@ struct syf_server {
     std::vector<uint32_t> free_slots;
     JobP* d_jobs = nullptr;
     BatchPlanes* d_planes = nullptr;
-    BatchItem* d_items[2] = {nullptr, nullptr};
-    BatchItem* h_items[2] = {nullptr, nullptr};
-    size_t items_cap[2] = {0, 0};
-    uint32_t* d_claim[2] = {nullptr, nullptr};
-    unsigned long long* d_ctr[2] = {nullptr, nullptr};
-    hipEvent_t ev_pool[4] = {nullptr, nullptr, nullptr, nullptr};
+    BatchItem* h_items[NBUF] = {};  // pinned, coherent: read by the kernels in place
+    size_t items_cap[NBUF] = {};
+    // per launch buffer and kernel family: claim counters, work counters and exit count (zero
+    // between launches), and the host copy of the work counters
+    uint32_t* d_claim[NBUF] = {};
+    unsigned long long* d_ctr[NBUF] = {};
+    uint32_t* d_exit[NBUF] = {};
+    unsigned long long* h_out[NBUF] = {};
+    unsigned long long* d_rctr = nullptr;  // counters of the refine kernels (not reported)
+    unsigned long long* d_pctr = nullptr;  // counters of plain-mode launches
+    hipEvent_t ev_pool[2 * NBUF] = {};
     uint32_t quantum = 0, target_us = 0, grid = 0;
     uint32_t fill_items = 0;  // work items (one wave each) a launch needs to fill the device
     double ns_per_unit = 0.0;
@@ -141,10 +169,14 @@ bool admit(syf_server* s, Request* r) {
     if (e == hipSuccess) e = hipMalloc(&r->depth, r->pixels * 4);
     if (e == hipSuccess) e = hipMalloc(&r->aux, r->pixels * 16);
     if (e == hipSuccess) e = hipMalloc(&r->depth64, r->pixels * 8);
-    if (e == hipSuccess) e = hipMemsetAsync(r->accum, 0, r->pixels * 16, s->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(r->depth, 0, r->pixels * 4, s->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(r->aux, 0, r->pixels * 16, s->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(r->depth64, 0, r->pixels * 8, s->stream);
+    // batched launches write every pixel of every plane in the request's first pass; the
+    // megakernel of the plain mode adds to the accumulation
+    if (s->cfg.mode == SYF_SERVE_PLAIN) {
+        if (e == hipSuccess) e = hipMemsetAsync(r->accum, 0, r->pixels * 16, s->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(r->depth, 0, r->pixels * 4, s->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(r->aux, 0, r->pixels * 16, s->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(r->depth64, 0, r->pixels * 8, s->stream);
+    }
     if (e != hipSuccess) {
         hipFree(r->accum);
         hipFree(r->depth);
@@ -156,10 +188,10 @@ bool admit(syf_server* s, Request* r) {
     r->slot = s->free_slots.back();
     s->free_slots.pop_back();
     r->planes = BatchPlanes{(float4*)r->accum, (float*)r->depth, (float4*)r->aux, (double*)r->depth64};
-    // sources live in the request, which outlives every launch that reads its slot
-    e = hipMemcpyAsync(&s->d_jobs[r->slot], &r->P, sizeof(JobP), hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(&s->d_planes[r->slot], &r->planes, sizeof(BatchPlanes), hipMemcpyHostToDevice, s->stream);
+    // plain mode: the megakernel takes the job by value; batched mode: the first launch that
+    // holds the request's items writes its slot-table entries
+    r->uploaded = false;
+    r->upload_in_flight = false;
     if (e != hipSuccess) {
         set_server_error(s, SYF_E_DEVICE, "request upload", e);
         return false;
@@ -189,13 +221,11 @@ void free_request_planes(syf_server* s, Request* r) {
 bool ensure_items(syf_server* s, uint32_t buf, size_t n) {
     if (n <= s->items_cap[buf]) return true;
     size_t cap = std::max<size_t>(n, s->items_cap[buf] * 2);
-    hipFree(s->d_items[buf]);
+    // the launches of this buffer have retired: nothing reads the old items
     hipHostFree(s->h_items[buf]);
-    s->d_items[buf] = nullptr;
     s->h_items[buf] = nullptr;
     s->items_cap[buf] = 0;
-    if (hipMalloc(&s->d_items[buf], cap * sizeof(BatchItem)) != hipSuccess) return false;
-    if (hipHostMalloc(&s->h_items[buf], cap * sizeof(BatchItem), hipHostMallocDefault) != hipSuccess) return false;
+    if (hipHostMalloc(&s->h_items[buf], cap * sizeof(BatchItem), hipHostMallocCoherent) != hipSuccess) return false;
     s->items_cap[buf] = cap;
     return true;
 }
@@ -203,10 +233,10 @@ bool ensure_items(syf_server* s, uint32_t buf, size_t n) {
 // Plans and enqueues one batched launch (caller holds the lock).  Returns false when there
 // is nothing to launch.
 bool plan_launch(syf_server* s) {
-    const uint32_t buf = s->launches & 1u;
+    const uint32_t buf = uint32_t(s->launches % NBUF);
     std::vector<Request*> order;
     for (Request* r : s->active)
-        if (!r->cancelled && r->planned < r->total) order.push_back(r);
+        if (!r->cancelled && r->planned < r->total && !r->upload_in_flight) order.push_back(r);
     if (order.empty()) return false;
     std::stable_sort(order.begin(), order.end(), [](const Request* a, const Request* b) {
         return (a->total - a->planned) * (a->P.max_bounces + 1u) < (b->total - b->planned) * (b->P.max_bounces + 1u);
@@ -218,15 +248,17 @@ bool plan_launch(syf_server* s) {
     double units = 0.0;
     for (Request* r : order) {
         if (units >= budget_units && n_items >= s->fill_items) break;
-        uint64_t added = 0;
-        // at most one quantum of a request per launch: two items of one pixel in one launch
-        // would race on its accumulation and could add its samples out of order
+        // A launch may hold the end of one pass of a request and the start of the next: the
+        // frontier keeps every pixel in at most one of its items.
+        uint64_t seg = 0;  // tile-samples of the current pass in this launch
         std::vector<BatchItem>& items = by_family[kernel_family(r->P.f)];
+        auto close_segment = [&](uint32_t tile_end) {
+            L.progress.push_back(Share{r, seg, r->pass, tile_end});
+            r->refs++;
+            seg = 0;
+        };
         while (r->planned < r->total && (units < budget_units || n_items < s->fill_items)) {
-            if (r->q_cursor == 0) {  // open the next quantum
-                // only once no item of the previous one is in flight: the launches in flight
-                // then hold disjoint pixels, and each pixel's samples still land in order
-                if (r->refs > 0) break;
+            if (!r->pass_open) {  // open the next pass
                 r->q_start = uint32_t(r->planned / r->tiles);
                 r->q_len = std::min(s->quantum, r->spp - r->q_start);
                 // a request whose remaining work fits in a launch goes in one piece: it then
@@ -234,22 +266,35 @@ bool plan_launch(syf_server* s) {
                 if (double(r->total - r->planned) * double(TILE * TILE) * double(r->P.max_bounces + 1u) <=
                     budget_units)
                     r->q_len = r->spp - r->q_start;
+                if (r->q_start == 0u) {
+                    r->pass = 0;
+                    r->ret_prev = r->tiles;  // no previous pass
+                } else {
+                    r->pass++;
+                    r->ret_prev = r->ret_cur;
+                }
+                r->ret_cur = 0;
+                r->q_cursor = 0;
+                r->pass_open = true;
             }
             const uint32_t t = r->q_cursor;
+            // the previous pass's item of this tile is still in flight: its samples must land first
+            if (t >= r->ret_prev) break;
             items.push_back(BatchItem{r->slot, (t % r->tiles_x) * TILE, (t / r->tiles_x) * TILE, r->q_start, r->q_len});
             n_items++;
             units += item_units(r, r->q_len);
             r->planned += r->q_len;
-            added += r->q_len;
+            seg += r->q_len;
             if (++r->q_cursor == r->tiles) {
-                r->q_cursor = 0;
-                break;
+                r->pass_open = false;
+                close_segment(r->tiles);
             }
         }
-        if (added) {
-            L.progress.push_back({r, added});
-            r->refs++;
-            if (r->start_ms < 0) r->start_ms = s->now_ms();
+        if (seg) close_segment(r->q_cursor);
+        if (r->refs && r->start_ms < 0) r->start_ms = s->now_ms();
+        if (r->refs && !r->uploaded && !r->upload_in_flight) {
+            r->upload_in_flight = true;
+            L.uploads.push_back(r);
         }
     }
     if (n_items == 0) return false;
@@ -277,26 +322,18 @@ bool plan_launch(syf_server* s) {
     L.start = s->ev_pool[2 * buf];
     L.end = s->ev_pool[2 * buf + 1];
     hipStream_t ls = s->ls[buf];
-    hipError_t e = hipEventRecord(s->ev_admit, s->stream);  // planes zeroed, slot tables written
-    if (e == hipSuccess) e = hipStreamWaitEvent(ls, s->ev_admit, 0);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(s->d_items[buf], s->h_items[buf], n_items * sizeof(BatchItem), hipMemcpyHostToDevice, ls);
-    if (e == hipSuccess) e = hipMemsetAsync(s->d_claim[buf], 0, FAM_COUNT * CLAIM_WORDS * sizeof(uint32_t), ls);
-    if (e == hipSuccess) e = hipMemsetAsync(s->d_ctr[buf], 0, CTR_COUNT * sizeof(unsigned long long), ls);
-    if (e == hipSuccess) e = hipEventRecord(L.start, ls);
-    if (e == hipSuccess) e = hipEventRecord(s->ev_ready[buf], ls);
+    hipError_t e = hipEventRecord(L.start, ls);
+    for (Request* r : L.uploads)
+        if (e == hipSuccess) e = launch_admit(s->d_jobs, s->d_planes, r->slot, r->P, r->planes, ls);
     for (uint32_t f = 0; f < FAM_COUNT && e == hipSuccess; ++f)
         if (count[f]) {
-            hipStream_t fs = s->fam_stream[buf][f];
-            e = hipStreamWaitEvent(fs, s->ev_ready[buf], 0);
-            if (e == hipSuccess)
-                e = launch_paths(f, s->d_jobs, s->d_planes, s->d_items[buf] + first[f], count[f],
-                                 s->d_claim[buf] + f * CLAIM_WORDS, s->d_ctr[buf], s->grid, fs);
+            e = launch_paths(f, s->d_jobs, s->d_planes, s->h_items[buf] + first[f], count[f],
+                             s->d_claim[buf] + f * CLAIM_WORDS, s->d_ctr[buf] + f * CTR_COUNT, s->d_exit[buf] + f * 16u,
+                             s->h_out[buf] + f * CTR_COUNT, s->grid, ls);
             if (e == hipSuccess && first_quantum[f])
-                e = launch_refine_first_hits(f, s->d_jobs, s->d_planes, s->d_items[buf] + first[f], count[f],
-                                             s->d_ctr[buf], fs);
-            if (e == hipSuccess) e = hipEventRecord(s->ev_fam[buf][f], fs);
-            if (e == hipSuccess) e = hipStreamWaitEvent(ls, s->ev_fam[buf][f], 0);
+                e = launch_refine_first_hits(f, s->d_jobs, s->d_planes, s->h_items[buf] + first[f], count[f],
+                                             s->d_rctr, ls);
+            L.families |= 1u << f;
         }
     if (e == hipSuccess) e = hipEventRecord(L.end, ls);
     if (e != hipSuccess) {
@@ -325,8 +362,12 @@ void retire(syf_server* s, std::unique_lock<std::mutex>& lk) {
     hipError_t e = hipEventSynchronize(L.end);
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, L.start, L.end);
+    // the counters each family's kernel left in host memory
     unsigned long long c[CTR_COUNT] = {0};
-    if (e == hipSuccess) e = hipMemcpy(c, s->d_ctr[L.buf], sizeof c, hipMemcpyDeviceToHost);
+    for (uint32_t f = 0; f < FAM_COUNT; ++f)
+        if (L.families >> f & 1u)
+            for (uint32_t k = 0; k < CTR_COUNT; ++k)
+                c[k] += __atomic_load_n(&s->h_out[L.buf][f * CTR_COUNT + k], __ATOMIC_ACQUIRE);
     const double t = s->now_ms();
     lk.lock();
     if (e != hipSuccess) {
@@ -342,9 +383,16 @@ void retire(syf_server* s, std::unique_lock<std::mutex>& lk) {
         const double meas = double(ms) * 1e6 / L.units;
         s->ns_per_unit = s->ns_per_unit > 0 ? 0.7 * s->ns_per_unit + 0.3 * meas : meas;
     }
-    for (auto& pr : L.progress) {
-        Request* r = pr.first;
-        r->completed += pr.second;
+    for (Request* r : L.uploads) {
+        r->uploaded = true;
+        r->upload_in_flight = false;
+    }
+    for (const Share& sh : L.progress) {
+        Request* r = sh.r;
+        r->completed += sh.samples;
+        // launches retire in order, so a pass's retired tiles form a growing prefix
+        if (sh.pass == r->pass) r->ret_cur = sh.tile_end;
+        else if (sh.pass + 1u == r->pass) r->ret_prev = sh.tile_end;
         r->refs--;
         if (r->completed >= r->total && r->state == SYF_REQ_ACTIVE) finish(s, r, t);
         if (r->released && r->refs == 0) delete r;
@@ -379,17 +427,17 @@ bool run_plain(syf_server* s, std::unique_lock<std::mutex>& lk) {
     r->planned = r->total;
     r->refs++;
     lk.unlock();
-    hipError_t e = hipMemsetAsync(s->d_ctr[0], 0, CTR_COUNT * sizeof(unsigned long long), s->stream);
+    hipError_t e = hipMemsetAsync(s->d_pctr, 0, CTR_COUNT * sizeof(unsigned long long), s->stream);
     if (e == hipSuccess) e = hipEventRecord(s->ev_pool[0], s->stream);
     if (e == hipSuccess)
         e = launch_render_tile(r->P, 0, 0, r->P.width, r->P.height, 0, r->spp, (float4*)r->accum, (float*)r->depth,
-                               (float4*)r->aux, (double*)r->depth64, s->d_ctr[0], s->stream);
+                               (float4*)r->aux, (double*)r->depth64, s->d_pctr, s->stream);
     if (e == hipSuccess) e = hipEventRecord(s->ev_pool[1], s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(s->ev_pool[1]);
     float ms = 0.0f;
     unsigned long long c[CTR_COUNT] = {0};
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->ev_pool[0], s->ev_pool[1]);
-    if (e == hipSuccess) e = hipMemcpy(c, s->d_ctr[0], sizeof c, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(c, s->d_pctr, sizeof c, hipMemcpyDeviceToHost);
     const double t = s->now_ms();
     lk.lock();
     r->refs--;
@@ -444,7 +492,7 @@ void worker_main(syf_server* s) {
             s->queue.pop_front();
         }
         bool planned = false;
-        while (!s->stop && s->inflight.size() < 2 && plan_launch(s)) planned = true;
+        while (!s->stop && s->inflight.size() < NBUF && plan_launch(s)) planned = true;
         if (!s->inflight.empty()) {
             retire(s, lk);
             continue;
@@ -457,7 +505,7 @@ void worker_main(syf_server* s) {
 
 // Autotuner: times a fixed calibration workload (a 768x768 Mandelbulb job, 4 samples, 2
 // bounces, in 8x8-tile items: enough to fill the device) for each candidate size of the
-// persistent grid (2, 4 or 8 workgroups per CU), keeps the fastest and seeds the cost model
+// persistent grid (1, 2 or 3 workgroups per CU), keeps the fastest and seeds the cost model
 // (ns per pixel-sample-segment) with its measured rate.  The sample quantum is not tuned here:
 // one sample per work item with whole-request items for requests that fit in a launch
 // measured best on both request streams (long renders need short per-lane work so that
@@ -498,7 +546,9 @@ void autotune(syf_server* s) {
     BatchPlanes* dpl = nullptr;
     BatchItem* ditems = nullptr;
     uint32_t* claim = nullptr;
+    uint32_t* exit_count = nullptr;
     unsigned long long* ctr = nullptr;
+    unsigned long long* host_out = nullptr;
     const uint32_t tiles_x = d.width / TILE, tiles = tiles_x * (d.height / TILE), spp = 4;
     hipEvent_t a, b;
     bool ok = hipMalloc(&accum, pixels * 16) == hipSuccess && hipMalloc(&aux, pixels * 16) == hipSuccess &&
@@ -507,14 +557,21 @@ void autotune(syf_server* s) {
               hipMalloc(&dpl, sizeof(BatchPlanes)) == hipSuccess &&
               hipMalloc(&ditems, size_t(tiles) * spp * sizeof(BatchItem)) == hipSuccess &&
               hipMalloc(&claim, CLAIM_WORDS * 4) == hipSuccess && hipMalloc(&ctr, CTR_COUNT * 8) == hipSuccess &&
+              hipMalloc(&exit_count, 16 * 4) == hipSuccess &&
+              hipHostMalloc(&host_out, CTR_COUNT * 8, hipHostMallocCoherent) == hipSuccess &&
+              hipMemset(claim, 0, CLAIM_WORDS * 4) == hipSuccess && hipMemset(ctr, 0, CTR_COUNT * 8) == hipSuccess &&
+              hipMemset(exit_count, 0, 16 * 4) == hipSuccess &&
               hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess;
     BatchPlanes pl{accum, depth, aux, depth64};
     ok = ok && hipMemcpy(djob, &P, sizeof P, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(dpl, &pl, sizeof pl, hipMemcpyHostToDevice) == hipSuccess;
     const uint32_t cu = uint32_t(s->ctx->cu_count > 0 ? s->ctx->cu_count : 256);
-    const uint32_t g_cands[3] = {cu * 2, cu * 4, cu * 8};
+    // every candidate grid is resident at once (k_paths holds 3 waves per SIMD: 3 workgroups
+    // per CU); a larger grid leaves workgroups waiting for a CU until the launch's end, which
+    // keeps the next launch off the CUs its tail frees
+    const uint32_t g_cands[3] = {cu * 1, cu * 2, cu * 3};
     double best = 1e30;
-    uint32_t bg = cu * 8;
+    uint32_t bg = cu * 3;
     double best_units_ns = 0.0;
     // every tile with all its samples in one item: one launch per candidate and repetition
     std::vector<BatchItem> items;
@@ -523,9 +580,10 @@ void autotune(syf_server* s) {
     for (uint32_t gi = 0; ok && gi < 3; ++gi) {
         float ms = 0.0f;
         for (int rep = 0; rep < 2 && ok; ++rep) {  // the first repetition warms up
-            ok = hipMemset(claim, 0, CLAIM_WORDS * 4) == hipSuccess && hipEventRecord(a, s->stream) == hipSuccess &&
-                 launch_paths(kernel_family(P.f), djob, dpl, ditems, tiles, claim, ctr, g_cands[gi], s->stream) ==
-                     hipSuccess &&
+            // the claim and work counters and the exit count are left zero by each launch
+            ok = hipEventRecord(a, s->stream) == hipSuccess &&
+                 launch_paths(kernel_family(P.f), djob, dpl, ditems, tiles, claim, ctr, exit_count, host_out,
+                              g_cands[gi], s->stream) == hipSuccess &&
                  hipEventRecord(b, s->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
                  hipEventElapsedTime(&ms, a, b) == hipSuccess;
         }
@@ -546,6 +604,8 @@ void autotune(syf_server* s) {
     hipFree(ditems);
     hipFree(claim);
     hipFree(ctr);
+    hipFree(exit_count);
+    hipHostFree(host_out);
     hipEventDestroy(a);
     hipEventDestroy(b);
 }
@@ -578,7 +638,7 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     for (uint32_t i = 0; i < s->max_slots; ++i) s->free_slots.push_back(s->max_slots - 1u - i);
     const uint32_t cu = uint32_t(ctx->cu_count > 0 ? ctx->cu_count : 256);
     s->quantum = cfg->quantum_samples ? cfg->quantum_samples : 1u;
-    s->grid = cfg->grid_blocks ? cfg->grid_blocks : cu * 8u;
+    s->grid = cfg->grid_blocks ? cfg->grid_blocks : cu * 3u;
     // each launch ends with a tail as long as its slowest pixel's quantum: launches of a few
     // tens of ms keep that below a few per cent, and a preview waits at most two of them
     s->target_us = cfg->batch_target_us ? cfg->batch_target_us : 30000u;
@@ -590,12 +650,25 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
         syf_server_destroy(s);
         return hip_fail(e, "server allocation");
     }
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NBUF; ++b) {
         if ((e = hipMalloc(&s->d_claim[b], FAM_COUNT * CLAIM_WORDS * sizeof(uint32_t))) != hipSuccess ||
-            (e = hipMalloc(&s->d_ctr[b], CTR_COUNT * sizeof(unsigned long long))) != hipSuccess) {
+            (e = hipMalloc(&s->d_ctr[b], FAM_COUNT * CTR_COUNT * sizeof(unsigned long long))) != hipSuccess ||
+            (e = hipMalloc(&s->d_exit[b], FAM_COUNT * 16 * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipHostMalloc(&s->h_out[b], FAM_COUNT * CTR_COUNT * sizeof(unsigned long long), hipHostMallocCoherent)) !=
+                hipSuccess ||
+            (e = hipMemset(s->d_claim[b], 0, FAM_COUNT * CLAIM_WORDS * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMemset(s->d_ctr[b], 0, FAM_COUNT * CTR_COUNT * sizeof(unsigned long long))) != hipSuccess ||
+            (e = hipMemset(s->d_exit[b], 0, FAM_COUNT * 16 * sizeof(uint32_t))) != hipSuccess) {
             syf_server_destroy(s);
             return hip_fail(e, "server allocation");
         }
+    }
+    if ((e = hipMalloc(&s->d_rctr, CTR_COUNT * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc(&s->d_pctr, CTR_COUNT * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMemset(s->d_rctr, 0, CTR_COUNT * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipDeviceSynchronize()) != hipSuccess) {
+        syf_server_destroy(s);
+        return hip_fail(e, "server allocation");
     }
     for (auto& ev : s->ev_pool) {
         if ((e = hipEventCreate(&ev)) != hipSuccess) {
@@ -603,25 +676,11 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
             return hip_fail(e, "server events");
         }
     }
-    // streams created alternately for the two launches, so that a family's streams of two
-    // consecutive launches land on different hardware queues
-    for (uint32_t f = 0; f < FAM_COUNT; ++f)
-        for (int b = 0; b < 2; ++b)
-            if ((e = hipStreamCreateWithFlags(&s->fam_stream[b][f], hipStreamNonBlocking)) != hipSuccess ||
-                (e = hipEventCreateWithFlags(&s->ev_fam[b][f], hipEventDisableTiming)) != hipSuccess) {
-                syf_server_destroy(s);
-                return hip_fail(e, "server streams");
-            }
-    for (int b = 0; b < 2; ++b)
-        if ((e = hipStreamCreateWithFlags(&s->ls[b], hipStreamNonBlocking)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&s->ev_ready[b], hipEventDisableTiming)) != hipSuccess) {
+    for (int b = 0; b < NBUF; ++b)
+        if ((e = hipStreamCreateWithFlags(&s->ls[b], hipStreamNonBlocking)) != hipSuccess) {
             syf_server_destroy(s);
             return hip_fail(e, "server streams");
         }
-    if ((e = hipEventCreateWithFlags(&s->ev_admit, hipEventDisableTiming)) != hipSuccess) {
-        syf_server_destroy(s);
-        return hip_fail(e, "server events");
-    }
     if (s->cfg.mode == SYF_SERVE_BATCHED && cfg->grid_blocks == 0) autotune(s);
     s->worker = std::thread(worker_main, s);
     *out = s;
@@ -642,9 +701,9 @@ int syf_server_destroy(syf_server* s) {
     // released requests still referred to by launches left behind by an error
     std::vector<Request*> orphans;
     for (auto& L : s->inflight)
-        for (auto& pr : L.progress)
-            if (pr.first->released && std::find(orphans.begin(), orphans.end(), pr.first) == orphans.end())
-                orphans.push_back(pr.first);
+        for (const Share& sh : L.progress)
+            if (sh.r->released && std::find(orphans.begin(), orphans.end(), sh.r) == orphans.end())
+                orphans.push_back(sh.r);
     for (Request* r : orphans) delete r;
     for (auto& kv : s->reqs) {
         free_request_planes(s, kv.second);
@@ -652,29 +711,22 @@ int syf_server_destroy(syf_server* s) {
     }
     hipFree(s->d_jobs);
     hipFree(s->d_planes);
-    for (int b = 0; b < 2; ++b) {
-        hipFree(s->d_items[b]);
+    for (int b = 0; b < NBUF; ++b) {
         hipHostFree(s->h_items[b]);
         hipFree(s->d_claim[b]);
         hipFree(s->d_ctr[b]);
+        hipFree(s->d_exit[b]);
+        hipHostFree(s->h_out[b]);
     }
+    hipFree(s->d_rctr);
+    hipFree(s->d_pctr);
     for (auto& ev : s->ev_pool)
         if (ev) hipEventDestroy(ev);
-    for (int b = 0; b < 2; ++b) {
-        for (uint32_t f = 0; f < FAM_COUNT; ++f) {
-            if (s->fam_stream[b][f]) {
-                hipStreamSynchronize(s->fam_stream[b][f]);
-                hipStreamDestroy(s->fam_stream[b][f]);
-            }
-            if (s->ev_fam[b][f]) hipEventDestroy(s->ev_fam[b][f]);
-        }
+    for (int b = 0; b < NBUF; ++b)
         if (s->ls[b]) {
             hipStreamSynchronize(s->ls[b]);
             hipStreamDestroy(s->ls[b]);
         }
-        if (s->ev_ready[b]) hipEventDestroy(s->ev_ready[b]);
-    }
-    if (s->ev_admit) hipEventDestroy(s->ev_admit);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return SYF_OK;
@@ -796,7 +848,7 @@ int syf_server_release(syf_server* s, uint64_t id) {
     hipSetDevice(s->ctx->device);
     // launches of a cancelled request may still write its planes
     hipStreamSynchronize(s->stream);
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < NBUF; ++b)
         if (s->ls[b]) hipStreamSynchronize(s->ls[b]);
     free_request_planes(s, r);
     s->reqs.erase(r->id);
