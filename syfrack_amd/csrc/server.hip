@@ -506,10 +506,9 @@ void worker_main(syf_server* s) {
 // Autotuner: times a fixed calibration workload (a 768x768 Mandelbulb job, 4 samples, 2
 // bounces, in 8x8-tile items: enough to fill the device) for each candidate size of the
 // persistent grid (1, 2 or 3 workgroups per CU), keeps the fastest and seeds the cost model
-// (ns per pixel-sample-segment) with its measured rate.  The sample quantum is not tuned here:
-// one sample per work item with whole-request items for requests that fit in a launch
-// measured best on both request streams (long renders need short per-lane work so that
-// launches end together; small requests need few launches).
+// (ns per pixel-sample-segment) with its measured rate.  The sample quantum is not tuned here
+// (a fixed default, measured on both families of streams; requests that fit in a launch go in
+// one piece whatever it is).
 void autotune(syf_server* s) {
     syf_job_desc d;
     std::memset(&d, 0, sizeof d);
@@ -637,6 +636,9 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     s->max_slots = s->cfg.max_active;
     for (uint32_t i = 0; i < s->max_slots; ++i) s->free_slots.push_back(s->max_slots - 1u - i);
     const uint32_t cu = uint32_t(ctx->cu_count > 0 ? ctx->cu_count : 256);
+    // one sample per item of a long request.  Two gained 8-15 % on single 1024^2 Mandelbulb and
+    // Mandelbox requests, but not on the 256-request uneven batch (233 against 244-259 Mrays/s),
+    // and it doubled the preview latency of the Poisson stream (p50 142 against 74 ms).
     s->quantum = cfg->quantum_samples ? cfg->quantum_samples : 1u;
     s->grid = cfg->grid_blocks ? cfg->grid_blocks : cu * 3u;
     // each launch ends with a tail as long as its slowest pixel's quantum: launches of a few
