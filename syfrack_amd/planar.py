@@ -43,6 +43,7 @@ class Planar:
     """A device-resident 2D escape-time image (syf_planar)."""
 
     def __init__(self, ctx, view: PlanarView):
+        self.ctx = ctx  # the context must outlive the image
         self.lib = ctx.lib
         self.view = view
         self.desc = view.c_struct()
