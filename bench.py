@@ -172,12 +172,12 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
 
 
-def serve_stream(ctx, reqs, mode: str, realtime: bool, budget: int = 200 * 10**9) -> dict:
+def serve_stream(ctx, reqs, mode: str, realtime: bool, budget: int = 200 * 10**9, **server_kw) -> dict:
     """Runs a request stream through a render server; wall time from the first submission to
     the last completion.  With ``realtime`` each request is submitted at its arrival time."""
     from syfrack_amd import serve
     descs = [(q.scene, q.cfg) for q in reqs]
-    with serve.Server(ctx, mode, hbm_budget_bytes=budget) as s:
+    with serve.Server(ctx, mode, hbm_budget_bytes=budget, **server_kw) as s:
         t0 = time.perf_counter()
         ids = []
         for q, (sc_, cfg) in zip(reqs, descs):
@@ -208,16 +208,19 @@ def serve_stream(ctx, reqs, mode: str, realtime: bool, budget: int = 200 * 10**9
 
 def serving_bench(ctx, n_uneven: int, n_mix: int, rate: float) -> dict:
     """Batched serving against the plain one-launch-per-request form on the same streams."""
-    from syfrack_amd import serve
+    from syfrack_amd import abi, serve
     out = {}
     uneven = serve.uneven_requests(n_uneven, seed=3)
     progress(f"uneven batch of {n_uneven} requests")
-    b = serve_stream(ctx, uneven, "batched", realtime=False)
+    # a batch submitted at once: once the submissions stop, everything left goes in one
+    # persistent launch (quiet_target_us unbounded)
+    b = serve_stream(ctx, uneven, "batched", realtime=False, quiet_target_us=abi.QUIET_UNBOUNDED)
     p = serve_stream(ctx, uneven, "plain", realtime=False)
     out["uneven_batch"] = {
         "workload": f"{n_uneven} requests submitted at once: square resolutions log-uniform 128-1024, "
                     "Mandelbulb (power 2-12) / Mandelbox (scale -2.5..3) / Menger (4-7 levels), spp 1-16, "
-                    "bounces 1-4, <=256 steps, sky light",
+                    "bounces 1-4, <=256 steps, sky light; batched: 30 ms launches while requests arrive, then "
+                    "all remaining work in one persistent launch",
         "batched": b, "plain": p, "speedup": b["mrays_s"] / p["mrays_s"], "same_rays": b["rays"] == p["rays"]}
     mix = serve.serving_mix(n_mix, rate_per_s=rate, seed=5)
     progress(f"serving loop of {n_mix} requests at {rate:g}/s")

@@ -271,11 +271,16 @@ typedef struct syf_server_config {
     uint64_t hbm_budget_bytes;   /* admission budget for request planes (0 = 90 % of free HBM) */
     uint32_t mode;               /* SYF_SERVE_BATCHED or SYF_SERVE_PLAIN */
     uint32_t max_active;         /* requests traced concurrently (0 = 4096) */
-    uint32_t quantum_samples;    /* samples per pixel per work item of a long request (0 = 1) */
-    uint32_t batch_target_us;    /* target device time of one launch (0 = 30000 us) */
+    uint32_t quantum_samples;    /* samples per pixel per work item of a long request (0 = 1 while
+                                    requests arrive, 16 once they have stopped) */
+    uint32_t batch_target_us;    /* target device time of one launch while requests arrive
+                                    (0 = 30000 us) */
     uint32_t grid_blocks;        /* workgroups of the persistent launch (0 = autotuned) */
-    uint32_t reserved;
+    uint32_t quiet_target_us;    /* target device time of one launch once no request has arrived
+                                    for 200 ms (0 = 32 x batch_target_us; SYF_QUIET_UNBOUNDED =
+                                    all remaining work in one launch) */
 } syf_server_config;
+#define SYF_QUIET_UNBOUNDED 0xffffffffu
 
 #define SYF_REQ_QUEUED 0u     /* waiting for admission */
 #define SYF_REQ_ACTIVE 1u     /* admitted, being traced */

@@ -187,8 +187,11 @@ class SyfServerConfig(C.Structure):
         ("quantum_samples", C.c_uint32),
         ("batch_target_us", C.c_uint32),
         ("grid_blocks", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("quiet_target_us", C.c_uint32),
     ]
+
+
+QUIET_UNBOUNDED = 0xFFFFFFFF
 
 
 class SyfRequestInfo(C.Structure):

@@ -95,9 +95,13 @@ using namespace syf;
 // Batched launches in flight at once; each has its own work-item and counter buffers, streams
 // and events.
 constexpr int NBUF = 3;
-// Launch size and samples per pass once only long renders remain (see plan_launch).
-constexpr double LONG_FACTOR = 4.0;
-constexpr uint32_t LONG_QUANTUM = 4;
+// Quiet mode (see plan_launch): after LATENCY_HOLD_MS without a submission, launches of
+// QUIET_FACTOR x the latency-mode target with QUIET_QUANTUM samples per pass.  Measured on
+// the 256-request uneven batch (Mrays/s; the plain form 142): 4x / 4 samples 336, 8x / 4
+// 359, 16x / 8 404, 32x / 16 427, everything in one launch 526; the Poisson stream's rays/s
+// and preview latency did not change (its requests arrive within 0.3 s, in latency mode).
+constexpr double QUIET_FACTOR = 32.0;
+constexpr uint32_t QUIET_QUANTUM = 16;
 constexpr double LATENCY_HOLD_MS = 200.0;
 
 struct syf_server {
@@ -133,7 +137,7 @@ struct syf_server {
     unsigned long long* d_rctr = nullptr;  // counters of the refine kernels (not reported)
     unsigned long long* d_pctr = nullptr;  // counters of plain-mode launches
     hipEvent_t ev_pool[2 * NBUF] = {};
-    uint32_t quantum = 0, quantum_long = 0, target_us = 0, grid = 0;
+    uint32_t quantum = 0, quantum_long = 0, target_us = 0, quiet_us = 0, grid = 0;
     double last_submit_ms = -1e30;  // host clock of the latest submission
     uint32_t fill_items = 0;  // work items (one wave each) a launch needs to fill the device
     double ns_per_unit = 0.0;
@@ -246,16 +250,17 @@ bool plan_launch(syf_server* s) {
     std::stable_sort(order.begin(), order.end(), [](const Request* a, const Request* b) {
         return (a->total - a->planned) * (a->P.max_bounces + 1u) < (b->total - b->planned) * (b->P.max_bounces + 1u);
     });
-    // Latency mode (launches of the target time, s->quantum samples per pass) while requests
-    // arrive (a submission within the last LATENCY_HOLD_MS) or one waits for admission.
-    // Otherwise nothing new has come in for a while: launches grow LONG_FACTOR times with
-    // s->quantum_long samples per pass, for fewer launch tails and less per-sample bookkeeping;
-    // a request arriving then waits for those longer launches.  Measured on one 1080p
-    // Mandelbox request at 16 spp: 141 Mrays/s at (30 ms, 1 sample), 205 at (100 ms, 4
-    // samples), 215 for the whole request in one launch.
-    const double short_units = double(s->target_us) * 1000.0 / std::max(s->ns_per_unit, 1e-6);
+    // Latency mode (launches of target_us, s->quantum samples per pass) while requests arrive
+    // (a submission within the last LATENCY_HOLD_MS) or one waits for admission.  Otherwise
+    // nothing new has come in for a while: launches of quiet_us with s->quantum_long samples
+    // per pass, for fewer launch tails and less per-sample bookkeeping; a request arriving then
+    // waits for those longer launches.  Measured on one 1080p Mandelbox request at 16 spp: 141
+    // Mrays/s at (30 ms, 1 sample), 205 at (100 ms, 4 samples), 215 for the whole request in
+    // one launch.
     const bool latency = !s->queue.empty() || s->now_ms() - s->last_submit_ms < LATENCY_HOLD_MS;
-    const double budget_units = latency ? short_units : short_units * LONG_FACTOR;
+    const double ns = std::max(s->ns_per_unit, 1e-6);
+    const double budget_units = latency ? double(s->target_us) * 1000.0 / ns
+                                : s->quiet_us == SYF_QUIET_UNBOUNDED ? 1e300 : double(s->quiet_us) * 1000.0 / ns;
     const uint32_t quantum = latency ? s->quantum : s->quantum_long;
     std::vector<BatchItem> by_family[FAM_COUNT];
     size_t n_items = 0;
@@ -656,11 +661,13 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     // and it doubled the preview latency of the Poisson stream (p50 142 against 74 ms).
     s->quantum = cfg->quantum_samples ? cfg->quantum_samples : 1u;
     // a quantum the caller chose holds in both modes
-    s->quantum_long = cfg->quantum_samples ? cfg->quantum_samples : LONG_QUANTUM;
+    s->quantum_long = cfg->quantum_samples ? cfg->quantum_samples : QUIET_QUANTUM;
     s->grid = cfg->grid_blocks ? cfg->grid_blocks : cu * 3u;
     // each launch ends with a tail as long as its slowest pixel's quantum: launches of a few
     // tens of ms keep that below a few per cent, and a preview waits at most two of them
     s->target_us = cfg->batch_target_us ? cfg->batch_target_us : 30000u;
+    s->quiet_us = cfg->quiet_target_us ? cfg->quiet_target_us
+                                        : uint32_t(std::min(double(s->target_us) * QUIET_FACTOR, 4.0e9));
     s->fill_items = cu * 16u;  // two workgroups of four waves per CU, twice over
     s->ns_per_unit = 0.05;
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||

@@ -56,7 +56,8 @@ class Server:
     """syf_server: requests in, resident accumulation planes out."""
 
     def __init__(self, ctx: Context, mode: str = "batched", hbm_budget_bytes: int = 0, max_active: int = 0,
-                 quantum_samples: int = 0, batch_target_us: int = 0, grid_blocks: int = 0):
+                 quantum_samples: int = 0, batch_target_us: int = 0, grid_blocks: int = 0,
+                 quiet_target_us: int = 0):
         if mode not in ("batched", "plain"):
             raise ValueError(f"mode must be 'batched' or 'plain', got {mode!r}")
         self.ctx = ctx
@@ -68,6 +69,7 @@ class Server:
         cfg.quantum_samples = quantum_samples
         cfg.batch_target_us = batch_target_us
         cfg.grid_blocks = grid_blocks
+        cfg.quiet_target_us = quiet_target_us
         h = C.c_void_p()
         abi.check(self.lib.syf_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
         self.handle = h

@@ -110,6 +110,30 @@ def test_served_depth_matches_fp64_oracle(ctx):
                 assert np.mean(rel <= 1e-4) >= 0.999
 
 
+def test_quiet_mode_launches_equal_plain(ctx):
+    """Once submissions stop (200 ms), the batched server plans everything left in one launch
+    with whole-request items; every image still equals its plain render bit for bit."""
+    from syfrack_amd import abi, serve
+    reqs = serve.uneven_requests(8, seed=13, min_side=64, max_side=256)
+    alone = []
+    with serve.Server(ctx, "plain") as s:
+        for q in reqs:
+            rid = s.submit(q.scene, q.cfg)
+            s.wait(rid, 120)
+            alone.append((_planes(s.result(rid)), s.plane(rid, abi.PLANE_DEPTH64)))
+            s.release(rid)
+    for quiet in (abi.QUIET_UNBOUNDED, 1):  # everything at once; the smallest quiet launches
+        with serve.Server(ctx, "batched", quiet_target_us=quiet) as s:
+            ids = [s.submit(q.scene, q.cfg) for q in reqs]
+            s.wait_all(300)
+            st = s.stats()
+            print(f"quiet_target_us {quiet}: {st['launches']} launches", flush=True)
+            for k, rid in enumerate(ids):
+                assert _same(_planes(s.result(rid)), alone[k][0]), (quiet, k)
+                assert np.array_equal(s.plane(rid, abi.PLANE_DEPTH64), alone[k][1]), (quiet, k)
+                s.release(rid)
+
+
 def test_admission_against_the_budget(ctx):
     from syfrack_amd import abi, serve
     from syfrack_amd.scene import RenderConfig
