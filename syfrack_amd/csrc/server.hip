@@ -6,7 +6,8 @@
 //   2. plans one launch: work items (8x8 tile, sample range) of the active requests, shortest
 //      remaining work first, until the launch holds enough items to fill the device and its
 //      estimated device time reaches the target (a running estimate of ns per
-//      pixel-sample-segment, refreshed from every launch that filled the device);
+//      pixel-sample-segment, refreshed from every launch that filled the device).  The target
+//      is short while requests arrive and long once they have stopped (quiet mode);
 //   3. enqueues it (one persistent kernel per kernel family present, the families on streams
 //      of their own so that they run concurrently) and keeps up to three launches in flight, so
 //      that the next launch's workgroups take the CUs the running launch's tail frees.  Nothing else is enqueued
