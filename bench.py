@@ -76,6 +76,19 @@ def read_traffic(kernel: str, width: int, height: int):
     return None
 
 
+def read_counters(kernel: str, width: int, height: int) -> dict:
+    """VALU counters of a kernel at this frame size, from the committed rocprofv3 summary
+    (``<name>: <kernel> <width>x<height> <value>`` lines); empty when absent."""
+    path = os.path.join(ROOT, "profiles", "counters.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path, encoding="utf-8"):
+            f = line.split()
+            if len(f) == 4 and f[0].endswith(":") and f[1] == kernel and f[2] == f"{width}x{height}":
+                out[f[0][:-1]] = float(f[3])
+    return out
+
+
 def calibrate_copy() -> int:
     """A plain copy of 256 MiB (torch's device copy kernel), three times: the reference
     reading for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters into bytes."""
@@ -341,8 +354,12 @@ def main() -> int:
                              "issue at the FP32 non-packed rate on MI355X), special ops (div/sqrt/log) not counted; "
                              "packed FP32 is disabled in the build, so the reachable ceiling is 78.6 TFLOP/s; "
                              "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a "
-                             "plain copy kernel"},
+                             "plain copy kernel; valu_counters = VALU busy, lane utilisation and resident waves per "
+                             "CU of the same kernel from a committed rocprofv3 --pmc run"},
     }
+    counters = read_counters("k_primary", args.width, args.height)
+    if counters:
+        out["roofline"]["valu_counters"] = counters
     if not args.no_serve:
         sv = serving_bench(r.ctx, args.serve_requests, args.mix_requests, args.mix_rate)
         out["serving"] = sv

@@ -548,7 +548,7 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
 // FP64 refine of the first surface hits of the items that start at sample 0: k_paths left the
 // march's last position before the hit (the lower end of the bracket) in the FP64 plane, or -1
 // where sample 0 hit no surface.  One wave per item, one lane per pixel; the camera ray of
-// sample 0 is regenerated from its PCG stream (as k_hits does for the primary trace).
+// sample 0 is regenerated from its PCG stream (as the primary trace does for its hits).
 template <uint32_t FAM>
 __global__ __launch_bounds__(256) void k_refine_first_hits(const JobP* __restrict__ jobs,
                                                            const BatchPlanes* __restrict__ planes,
