@@ -483,42 +483,90 @@ __device__ __forceinline__ V3<T> normal_at(const FractalP& F, V3<T> p, T h, Work
 // and the hit (g <= 0), by Illinois regula falsi to a bracket of 1e-12 * t (at most 40
 // evaluations after the two end points); the bracket midpoint is returned.  Without an
 // FP64 sign change between the end points the march t is kept.
+//
+// Refine64 holds the refine between two evaluations of g, so that kernels can run it one
+// evaluation at a time (begin, then g at x and update until it reports the result);
+// refine64() runs it in one loop.  Both take the same steps in the same order.
 constexpr int REFINE_ITERS = 40;
+struct Refine64 {
+    double a, b, ga, gb, x;  // bracket, g at its ends, next point to evaluate
+    int side;
+    uint32_t it, phase;      // iterates done; 0: evaluating g(a), 1: g(b), 2: an iterate
+
+    // false: no bracket (t_prev >= t_hit, a best-candidate hit), the result is t_hit
+    __device__ __forceinline__ bool begin(double t_prev, double t_hit) {
+        if (!(t_prev < t_hit)) return false;
+        a = t_prev;
+        b = t_hit;
+        x = a;
+        phase = 0;
+        return true;
+    }
+
+    // Takes g(x); returns true with the refined t in out once the refine has ended.
+    __device__ __forceinline__ bool update(double gx, double& out) {
+        if (phase == 0u) {
+            ga = gx;
+            x = b;
+            phase = 1;
+            return false;
+        }
+        if (phase == 1u) {
+            gb = gx;
+            if (!(ga > 0.0 && gb <= 0.0)) {  // no FP64 sign change: the march t
+                out = b;
+                return true;
+            }
+            side = 0;
+            it = 0;
+            phase = 2;
+        } else {
+            if (gx > 0.0) {
+                a = x;
+                ga = gx;
+                if (side == 1) gb *= 0.5;
+                side = 1;
+            } else {
+                b = x;
+                gb = gx;
+                if (side == -1) ga *= 0.5;
+                side = -1;
+            }
+            ++it;
+        }
+        if (it >= uint32_t(REFINE_ITERS) || b - a <= 1e-12 * b) {
+            out = 0.5 * (a + b);
+            return true;
+        }
+        double c = b - gb * (b - a) / (gb - ga);
+        if (!(c > a && c < b)) c = 0.5 * (a + b);
+        x = c;
+        return false;
+    }
+};
+
+// g(t) of the refine at a point of the ray.
+template <uint32_t FAM>
+__device__ __forceinline__ double refine_g(const FractalP& F, V3<double> ro, V3<double> rd, double t, double hit_cone,
+                                           Work& w) {
+    De<double> s;
+    if constexpr (FAM == 0xffffffffu) s = de_eval_any<double>(F, add(ro, scl(rd, t)));
+    else s = de_eval<double, FAM>(F, add(ro, scl(rd, t)));
+    w.de64_evals++;
+    w.de64_iters += s.runs;
+    return fabs(s.d) * double(F.safety) - hit_cone * t;
+}
+
 template <uint32_t FAM>
 __device__ __forceinline__ double refine64(const FractalP& F, V3<double> ro, V3<double> rd, double t_prev,
                                            double t_hit, double hit_cone, Work& w) {
-    const double safety = double(F.safety);
-    auto g = [&](double t) {
-        De<double> s;
-        if constexpr (FAM == 0xffffffffu) s = de_eval_any<double>(F, add(ro, scl(rd, t)));
-        else s = de_eval<double, FAM>(F, add(ro, scl(rd, t)));
-        w.de64_evals++;
-        w.de64_iters += s.runs;
-        return fabs(s.d) * safety - hit_cone * t;
-    };
-    if (!(t_prev < t_hit)) return t_hit;
-    double a = t_prev, b = t_hit;
-    double ga = g(a), gb = g(b);
-    if (!(ga > 0.0 && gb <= 0.0)) return t_hit;
-    int side = 0;
-    for (int i = 0; i < REFINE_ITERS; ++i) {
-        if (b - a <= 1e-12 * b) break;
-        double c = b - gb * (b - a) / (gb - ga);
-        if (!(c > a && c < b)) c = 0.5 * (a + b);
-        double gc = g(c);
-        if (gc > 0.0) {
-            a = c;
-            ga = gc;
-            if (side == 1) gb *= 0.5;
-            side = 1;
-        } else {
-            b = c;
-            gb = gc;
-            if (side == -1) ga *= 0.5;
-            side = -1;
-        }
-    }
-    return 0.5 * (a + b);
+    Refine64 r;
+    if (!r.begin(t_prev, t_hit)) return t_hit;
+    double out = t_hit;
+    // at most 2 + REFINE_ITERS evaluations: update() ends the refine by then
+    for (int e = 0; e < REFINE_ITERS + 2; ++e)
+        if (r.update(refine_g<FAM>(F, ro, rd, r.x, hit_cone, w), out)) break;
+    return out;
 }
 
 // ---------------------------------------------------------------- wave-level helpers

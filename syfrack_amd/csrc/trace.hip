@@ -38,6 +38,10 @@ constexpr uint32_t HIT_BATCH = 64;
 constexpr uint32_t REFILL_BATCH = 8;
 // Issue priorities: marching waves are served first; hit consumers take the slots left over.
 constexpr short PRIO_MARCH = 1, PRIO_HITS = 0;
+// Queue blocks a shading wave claims at once.  Measured at 1080p (frame ms): 1 (one refine
+// per lane) 1.10, 2 1.05, 4 1.08, 8 1.47 (claims run past the reserved blocks at the end and
+// leave them to the trailing pass).
+constexpr uint32_t SHADE_CLAIM = 2;
 
 __device__ __forceinline__ uint64_t lanes_below() {
     const uint32_t lane = threadIdx.x & 63u;
@@ -295,25 +299,80 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
     // complete, is left without its done flag: the trailing pass shades it.
     __builtin_amdgcn_s_setprio(PRIO_HITS);
     Work wn;  // normal taps (counted apart from the march's evaluations)
-    const uint32_t cap_blocks = Q.capacity / HIT_BATCH;
-    for (uint32_t guard = 0; guard < cap_blocks; ++guard) {  // every trip claims a distinct block
-        uint32_t b = 0, reserved = 0;
+    const uint32_t cap_blocks = Q.capacity / HIT_BATCH, npix = P.width * P.height;
+    const double cone64 = double(P.pixel_cone) * double(P.pixel_tolerance);
+    for (uint32_t guard = 0; guard < cap_blocks; ++guard) {  // every trip claims distinct blocks
+        // SHADE_CLAIM blocks per claim, one hit of each per lane.  All lanes first take the
+        // normals of their hits (the same work on every lane); then each lane runs the FP64
+        // refines of its hits back to back, one field evaluation per trip, so a wave takes the
+        // longest lane's sum of refines rather than the sum of each block's longest refine
+        // (a refine takes 2 to 42 FP64 evaluations).
+        uint32_t b0 = 0, reserved = 0;
         if (lane == 0u) {
             reserved = ld_agent(Q.count) / HIT_BATCH;
-            b = ld_agent(Q.cursor) < reserved ? atomicAdd(Q.cursor, 1u) : HITQ_EMPTY;
-            if (b != HITQ_EMPTY && b >= reserved) reserved = ld_agent(Q.count) / HIT_BATCH;
+            b0 = ld_agent(Q.cursor) < reserved ? atomicAdd(Q.cursor, SHADE_CLAIM) : HITQ_EMPTY;
+            if (b0 != HITQ_EMPTY && b0 + SHADE_CLAIM > reserved) reserved = ld_agent(Q.count) / HIT_BATCH;
         }
-        b = __shfl(b, 0, 64);
+        b0 = __shfl(b0, 0, 64);
         reserved = __shfl(reserved, 0, 64);
-        if (b >= reserved || b >= cap_blocks) break;
-        const uint32_t k = b * HIT_BATCH + lane;
-        const uint32_t e_pix = ld_agent(Q.pix + k), e_t = ld_agent(Q.t + k), e_tp = ld_agent(Q.t_prev + k),
-                       e_st = ld_agent(Q.steps + k);
-        const bool ready = e_pix != HITQ_EMPTY && e_t != HITQ_EMPTY && e_tp != HITQ_EMPTY && e_st != HITQ_EMPTY;
-        if (__ballot(ready) != ~0ull) continue;  // its producer is still writing it
-        if (e_pix < P.width * P.height)  // not a padding entry
-            shade_hit<FAM>(P, sample, e_pix, __uint_as_float(e_t), __uint_as_float(e_tp), e_st, aux, depth64, w, wn);
-        if (lane == 0u) Q.done[b] = 1u;
+        if (b0 >= reserved || b0 >= cap_blocks) break;
+        uint32_t h_pix[SHADE_CLAIM];
+        float h_t[SHADE_CLAIM], h_tp[SHADE_CLAIM];
+#pragma unroll
+        for (uint32_t j = 0; j < SHADE_CLAIM; ++j) {
+            h_pix[j] = HITQ_SKIP;
+            const uint32_t b = b0 + j;
+            // a block not reserved yet, or still being written, is left to the trailing pass
+            if (b >= reserved || b >= cap_blocks) continue;
+            const uint32_t k = b * HIT_BATCH + lane;
+            const uint32_t e_pix = ld_agent(Q.pix + k), e_t = ld_agent(Q.t + k), e_tp = ld_agent(Q.t_prev + k),
+                           e_st = ld_agent(Q.steps + k);
+            const bool ready = e_pix != HITQ_EMPTY && e_t != HITQ_EMPTY && e_tp != HITQ_EMPTY && e_st != HITQ_EMPTY;
+            if (__ballot(ready) != ~0ull) continue;
+            if (lane == 0u) Q.done[b] = 1u;
+            if (e_pix >= npix) continue;  // a padding entry
+            h_pix[j] = e_pix;
+            h_t[j] = __uint_as_float(e_t);
+            h_tp[j] = __uint_as_float(e_tp);
+            const uint32_t px = e_pix % P.width, py = e_pix / P.width;
+            uint32_t rng = rng_init(px, py, sample, P.seed);
+            V3<float> ro, rd;
+            camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
+            const V3<float> nv = normal_at<float, FAM>(P.f, add(ro, scl(rd, h_t[j])),
+                                                       fmaxf(h_t[j] * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, wn);
+            aux[e_pix] = make_float4(nv.x, nv.y, nv.z, float(e_st));
+        }
+        // the refines, one evaluation per trip (Refine64: refine64's steps)
+        uint32_t j = 0;
+        bool refining = false;
+        Refine64 rf;
+        V3<double> ro64 = mk(0.0, 0.0, 0.0), rd64 = ro64;
+        for (uint32_t trip = 0; trip < SHADE_CLAIM * uint32_t(REFINE_ITERS + 2); ++trip) {
+            // a lane between hits starts its next one
+            while (!refining && j < SHADE_CLAIM) {
+                const uint32_t pix = h_pix[j];
+                if (pix != HITQ_SKIP && rf.begin(double(h_tp[j]), double(h_t[j]))) {
+                    const uint32_t px = pix % P.width, py = pix / P.width;
+                    uint32_t rng = rng_init(px, py, sample, P.seed);
+                    V3<float> ro, rd;
+                    camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
+                    ro64 = cvt<double>(ro);
+                    rd64 = cvt<double>(rd);
+                    refining = true;
+                } else {
+                    if (pix != HITQ_SKIP) depth64[pix] = double(h_t[j]);  // no bracket: the march t
+                    ++j;
+                }
+            }
+            if (__ballot(refining) == 0ull) break;
+            if (!refining) continue;
+            double out;
+            if (rf.update(refine_g<FAM>(F, ro64, rd64, rf.x, cone64, w), out)) {
+                depth64[h_pix[j]] = out;
+                ++j;
+                refining = false;
+            }
+        }
     }
     flush_work(w, hits, rays, ctr);
     const uint32_t pr = wave_sum(rays), a = wave_sum(wn.de_evals), b = wave_sum(wn.de_iters);
