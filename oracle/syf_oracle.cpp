@@ -445,7 +445,7 @@ template <class T> V3<T> normal_at(const Fractal<T>& F, V3<T> p, T h, Count* cnt
 // The refine finds, in FP64, where along the ray that rule first holds: the crossing of
 //     g(t) = |d(t)| * safety - hit_cone * t
 // between the last march position before the hit (g > 0) and the hit (g <= 0), by Illinois
-// regula falsi until the bracket is <= 1e-12 * t wide (at most 40 evaluations after the two
+// regula falsi until the bracket is <= 1e-9 * t wide (at most 40 evaluations after the two
 // end points); the result is the bracket midpoint.  When the end points do not bracket a
 // crossing in FP64 (a preceding over-relaxation back-step, or a best-candidate hit after the
 // step budget), the march t is returned unchanged.
@@ -464,7 +464,7 @@ double refine64(const Fractal<double>& F, V3<double> ro, V3<double> rd, double t
     if (!(ga > 0.0 && gb <= 0.0)) return t_hit;
     int side = 0;
     for (int i = 0; i < REFINE_ITERS; ++i) {
-        if (b - a <= 1e-12 * b) break;
+        if (b - a <= 1e-9 * b) break;
         double c = b - gb * (b - a) / (gb - ga);
         if (!(c > a && c < b)) c = 0.5 * (a + b);
         double gc = g(c);
