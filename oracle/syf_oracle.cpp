@@ -355,6 +355,7 @@ template <class T> struct Hit {
     bool hit, exhausted, candidate;
     T t;
     T t_prev;  // march position evaluated before the hit (bracket of the FP64 refine)
+    T r_prev, r_hit;  // |d| * safety at t_prev and at t: g at the bracket's ends
     uint32_t steps;
     DeOut<T> sample;
 };
@@ -392,6 +393,7 @@ Hit<T> march(const Fractal<T>& F, V3<T> ro, V3<T> rd, T pixel_cone, T pixel_tol,
         } else {
             step_len = radius * omega;
         }
+        const T r_last = prev_radius;  // the radius at t_last
         prev_radius = radius;
         T err = radius / fmax_(t, T(1e-6));
         if (!sor_fail && err < cand_err) {
@@ -403,6 +405,8 @@ Hit<T> march(const Fractal<T>& F, V3<T> ro, V3<T> rd, T pixel_cone, T pixel_tol,
             out.hit = true;
             out.t = t;
             out.t_prev = t_last;
+            out.r_prev = r_last;
+            out.r_hit = radius;
             out.sample = s;
             return out;
         }
@@ -445,13 +449,15 @@ template <class T> V3<T> normal_at(const Fractal<T>& F, V3<T> p, T h, Count* cnt
 // The refine finds, in FP64, where along the ray that rule first holds: the crossing of
 //     g(t) = |d(t)| * safety - hit_cone * t
 // between the last march position before the hit (g > 0) and the hit (g <= 0), by Illinois
-// regula falsi until the bracket is <= 1e-9 * t wide (at most 40 evaluations after the two
-// end points); the result is the bracket midpoint.  When the end points do not bracket a
-// crossing in FP64 (a preceding over-relaxation back-step, or a best-candidate hit after the
-// step budget), the march t is returned unchanged.
+// regula falsi until the bracket is <= 1e-9 * t wide (at most 40 evaluations); the result is
+// the bracket midpoint.  g at the two ends comes from the march, which evaluated the field
+// there (g_prev, g_hit: the march's radius minus hit_cone * t); every interior point is
+// evaluated in FP64.  When the ends do not bracket a crossing (a preceding over-relaxation
+// back-step, or a best-candidate hit after the step budget), the march t is returned
+// unchanged.
 constexpr int REFINE_ITERS = 40;
 double refine64(const Fractal<double>& F, V3<double> ro, V3<double> rd, double t_prev, double t_hit,
-                double hit_cone, Count* cnt) {
+                double g_prev, double g_hit, double hit_cone, Count* cnt) {
     auto g = [&](double t) {
         DeOut<double> s = de_eval(F, ro + rd * t);
         cnt->de64_evals++;
@@ -460,7 +466,7 @@ double refine64(const Fractal<double>& F, V3<double> ro, V3<double> rd, double t
     };
     if (!(t_prev < t_hit)) return t_hit;
     double a = t_prev, b = t_hit;
-    double ga = g(a), gb = g(b);
+    double ga = g_prev, gb = g_hit;
     if (!(ga > 0.0 && gb <= 0.0)) return t_hit;
     int side = 0;
     for (int i = 0; i < REFINE_ITERS; ++i) {
@@ -481,6 +487,21 @@ double refine64(const Fractal<double>& F, V3<double> ro, V3<double> rd, double t
         }
     }
     return 0.5 * (a + b);
+}
+
+// g at a bracket end from a march's radius there.
+template <class T> double end_g(T radius, T t, double hit_cone) { return double(radius) - hit_cone * double(t); }
+
+// refine64 of an FP32 march's hit known by its t values only: the radii at the ends are
+// evaluated again in FP32 (the same positions and evaluations as the march's; not counted).
+double refine64_of_march(const Fractal<float>& F32, const Fractal<double>& F64, V3<float> ro, V3<float> rd,
+                         float t_prev, float t_hit, double hit_cone, Count* cnt) {
+    if (!(t_prev < t_hit)) return double(t_hit);
+    const float r_prev = std::fabs(de_eval(F32, ro + rd * t_prev).d) * F32.safety;
+    const float r_hit = std::fabs(de_eval(F32, ro + rd * t_hit).d) * F32.safety;
+    return refine64(F64, v3(double(ro.x), double(ro.y), double(ro.z)), v3(double(rd.x), double(rd.y), double(rd.z)),
+                    double(t_prev), double(t_hit), end_g(r_prev, t_prev, hit_cone), end_g(r_hit, t_hit, hit_cone),
+                    hit_cone, cnt);
 }
 
 // ---------------------------------------------------------------- path tracing (FP32)
@@ -1088,7 +1109,8 @@ int orc_trace_primary(const syf_job_desc* d, uint32_t sample, int precision, uin
                 V3<double> ro64 = v3(double(ro.x), double(ro.y), double(ro.z));
                 V3<double> rd64 = v3(double(rd.x), double(rd.y), double(rd.z));
                 bool hit;
-                double t, t_prev = 0.0;
+                double t, t_prev = 0.0, g_prev = 0.0, g_hit = 0.0;
+                const double cone64 = double(pixel_cone) * double(d->pixel_tolerance);
                 uint32_t st;
                 V3<double> n;
                 if (precision == 32) {
@@ -1096,6 +1118,8 @@ int orc_trace_primary(const syf_job_desc* d, uint32_t sample, int precision, uin
                     hit = hh.hit;
                     t = hh.t;
                     t_prev = hh.t_prev;
+                    g_prev = end_g(hh.r_prev, hh.t_prev, cone64);
+                    g_hit = end_g(hh.r_hit, hh.t, cone64);
                     st = hh.steps;
                     if (hit) {
                         V3<float> p = ro + rd * hh.t;
@@ -1108,6 +1132,8 @@ int orc_trace_primary(const syf_job_desc* d, uint32_t sample, int precision, uin
                     hit = hh.hit;
                     t = hh.t;
                     t_prev = hh.t_prev;
+                    g_prev = end_g(hh.r_prev, hh.t_prev, cone64);
+                    g_hit = end_g(hh.r_hit, hh.t, cone64);
                     st = hh.steps;
                     if (hit) {
                         V3<double> p = ro64 + rd64 * hh.t;
@@ -1117,7 +1143,7 @@ int orc_trace_primary(const syf_job_desc* d, uint32_t sample, int precision, uin
                 steps[i] = st;
                 if (hit) {
                     depth[i] = t;
-                    depth64[i] = refine64(F64, ro64, rd64, t_prev, t, double(pixel_cone) * double(d->pixel_tolerance), &c);
+                    depth64[i] = refine64(F64, ro64, rd64, t_prev, t, g_prev, g_hit, cone64, &c);
                     aux4[4 * i] = float(n.x);
                     aux4[4 * i + 1] = float(n.y);
                     aux4[4 * i + 2] = float(n.z);
@@ -1186,8 +1212,7 @@ int orc_render(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint
                         uint32_t rng0 = rng_init(px, py, sample_start, S.seed);
                         F3 ro, rd;
                         camera_ray(S.cam, S.width, S.height, px, py, &rng0, &ro, &rd);
-                        f64 = refine64(F64, v3(double(ro.x), double(ro.y), double(ro.z)),
-                                       v3(double(rd.x), double(rd.y), double(rd.z)), double(tp), double(dd), hit_cone, &c);
+                        f64 = refine64_of_march(S.F, F64, ro, rd, tp, dd, hit_cone, &c);
                     }
                 }
             }

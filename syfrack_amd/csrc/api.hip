@@ -304,7 +304,7 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
         }
     }
     if ((e = hipMalloc(&j->ctr, sizeof(unsigned long long) * CTR_COUNT)) != hipSuccess ||
-        (e = hipMalloc(&j->hitq, (4 * j->hitq_cap + j->hitq_cap / 64) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&j->hitq, (HITQ_FIELDS * j->hitq_cap + j->hitq_cap / 64) * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&j->counters, COUNTER_WORDS * sizeof(uint32_t))) != hipSuccess ||
         (e = hipEventCreate(&j->ev0)) != hipSuccess || (e = hipEventCreate(&j->ev1)) != hipSuccess ||
         (e = hipEventCreate(&j->ev2)) != hipSuccess) {
@@ -323,18 +323,19 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
 int syf_job_destroy(syf_job* j) {
     clear_error();
     if (!j) return SYF_OK;
-    hipSetDevice(j->ctx->device);
-    if (j->stream) hipStreamSynchronize(j->stream);
+    // teardown: every release is attempted, whatever an earlier one returned
+    (void)hipSetDevice(j->ctx->device);
+    if (j->stream) (void)hipStreamSynchronize(j->stream);
     for (void*& p : j->plane)
-        if (p) hipFree(p);
-    if (j->ctr) hipFree(j->ctr);
-    if (j->hitq) hipFree(j->hitq);
-    if (j->counters) hipFree(j->counters);
-    if (j->ev0) hipEventDestroy(j->ev0);
-    if (j->ev1) hipEventDestroy(j->ev1);
-    if (j->ev2) hipEventDestroy(j->ev2);
+        if (p) (void)hipFree(p);
+    if (j->ctr) (void)hipFree(j->ctr);
+    if (j->hitq) (void)hipFree(j->hitq);
+    if (j->counters) (void)hipFree(j->counters);
+    if (j->ev0) (void)hipEventDestroy(j->ev0);
+    if (j->ev1) (void)hipEventDestroy(j->ev1);
+    if (j->ev2) (void)hipEventDestroy(j->ev2);
     release_wavefront(j);
-    if (j->stream) hipStreamDestroy(j->stream);
+    if (j->stream) (void)hipStreamDestroy(j->stream);
     delete j;
     return SYF_OK;
 }
@@ -427,15 +428,16 @@ int syf_trace_primary(syf_job* j, uint32_t sample) {
                    (double*)j->plane[SYF_PLANE_DEPTH64],
                    (uint32_t*)j->plane[SYF_PLANE_STEPS],
                    j->counters,
-                   HitQueue{j->hitq, j->hitq + cap, j->hitq + 2 * cap, j->hitq + 3 * cap, j->counters + COUNTER_HITS,
-                            j->counters + COUNTER_CURSOR, j->hitq + 4 * cap, j->counters + COUNTER_QERR, uint32_t(cap)},
+                   HitQueue{j->hitq, j->hitq + cap, j->hitq + 2 * cap, j->hitq + 3 * cap, j->hitq + 4 * cap,
+                            j->hitq + 5 * cap, j->counters + COUNTER_HITS, j->counters + COUNTER_CURSOR,
+                            j->hitq + HITQ_FIELDS * cap, j->counters + COUNTER_QERR, uint32_t(cap)},
                    j->ctr};
     hipError_t e = hipMemsetAsync(j->ctr, 0, sizeof(unsigned long long) * CTR_COUNT, j->stream);
     // ray counters, queue count, cursor and error flag start at zero, every queue field empty,
     // every done flag clear
     if (e == hipSuccess) e = hipMemsetAsync(j->counters, 0, COUNTER_WORDS * sizeof(uint32_t), j->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(j->hitq, 0xff, 4 * cap * sizeof(uint32_t), j->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(j->hitq + 4 * cap, 0, cap / 64 * sizeof(uint32_t), j->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(j->hitq, 0xff, HITQ_FIELDS * cap * sizeof(uint32_t), j->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(j->hitq + HITQ_FIELDS * cap, 0, cap / 64 * sizeof(uint32_t), j->stream);
     j->traced = true;
     if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
     if (e == hipSuccess) e = launch_trace_primary(j->P, sample, B, j->ctx->cu_count, j->stream);

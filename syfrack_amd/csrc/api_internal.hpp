@@ -25,7 +25,7 @@ struct syf_job {
     hipStream_t stream = nullptr;
     void* plane[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     unsigned long long* ctr = nullptr;
-    uint32_t* hitq = nullptr;      // hit queue of the primary trace: 4 fields x hitq_cap entries, then done flags
+    uint32_t* hitq = nullptr;      // hit queue of the primary trace: HITQ_FIELDS fields x hitq_cap entries, then done flags
     size_t hitq_cap = 0;
     uint32_t* counters = nullptr;  // ray counters, hit-queue count and cursor of the primary trace
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // start, after dominant kernel, end

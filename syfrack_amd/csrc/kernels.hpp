@@ -14,16 +14,19 @@ constexpr uint32_t COUNTER_CURSOR = 144;
 constexpr uint32_t COUNTER_QERR = 160;
 constexpr uint32_t COUNTER_WORDS = 176;
 
-// Hit queue of a primary trace: four fields per entry in separate arrays of `capacity` words
-// (room for every pixel plus one 64-entry block of padding per wave), every word HITQ_EMPTY
-// before the launch; one done flag per 64-entry block, zero before the launch.
+// Hit queue of a primary trace: HITQ_FIELDS fields per entry in separate arrays of `capacity`
+// words (room for every pixel plus one 64-entry block of padding per wave), every word
+// HITQ_EMPTY before the launch; one done flag per 64-entry block, zero before the launch.
 constexpr uint32_t HITQ_EMPTY = 0xffffffffu;
 constexpr uint32_t HITQ_SKIP = 0xfffffffeu;  // pixel of a padding entry
+constexpr uint32_t HITQ_FIELDS = 6;
 struct HitQueue {
     uint32_t* pix;
     uint32_t* t;       // march distance of the hit (float bits)
     uint32_t* t_prev;  // march position before the hit (float bits): lower end of the refine bracket
     uint32_t* steps;
+    uint32_t* r_prev;  // the march's radius |d| * safety at t_prev (float bits)
+    uint32_t* r_hit;   // the march's radius at t (float bits)
     uint32_t* count;   // entries reserved by the march
     uint32_t* cursor;  // blocks claimed by shading waves
     uint32_t* done;    // per block: 1 once a shading wave of the fused kernel has shaded it

@@ -571,8 +571,9 @@ __global__ __launch_bounds__(256) void k_refine_first_hits(const JobP* __restric
         uint32_t rng = rng_init(px, py, 0u, P.seed);
         F3 ro, rd;
         camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
-        pl.depth64[idx] = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), tp, double(pl.depth[idx]),
-                                        double(P.pixel_cone) * double(P.pixel_tolerance), w);
+        // tp is the float t_prev the flattened march stored (exact in a double)
+        pl.depth64[idx] = refine64_of_march<FAM>(P.f, ro, rd, float(tp), pl.depth[idx],
+                                                 double(P.pixel_cone) * double(P.pixel_tolerance), w);
     }
     flush_work(w, 0u, 0u, ctr);
 }

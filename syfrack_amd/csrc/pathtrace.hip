@@ -109,8 +109,8 @@ __device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_
                 uint32_t rng = rng_init(px, py, 0u, P.seed);
                 F3 ro, rd;
                 camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
-                d64 = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(tp0), double(fd0),
-                                    double(P.pixel_cone) * double(P.pixel_tolerance), work);
+                d64 = refine64_of_march<FAM>(P.f, ro, rd, tp0, fd0, double(P.pixel_cone) * double(P.pixel_tolerance),
+                                             work);
             }
             depth64[idx] = d64;
         }
@@ -357,8 +357,8 @@ __global__ __launch_bounds__(256) void k_wf_commit(const JobP P, const WfP W, So
                 uint32_t rng = rng_init(px, py, 0u, P.seed);
                 F3 ro, rd;
                 camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
-                d64 = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(tp), double(fd),
-                                    double(P.pixel_cone) * double(P.pixel_tolerance), work);
+                d64 = refine64_of_march<FAM>(P.f, ro, rd, tp, fd, double(P.pixel_cone) * double(P.pixel_tolerance),
+                                             work);
             }
             depth64[idx] = d64;
         }

@@ -38,10 +38,10 @@ constexpr uint32_t HIT_BATCH = 64;
 constexpr uint32_t REFILL_BATCH = 8;
 // Issue priorities: marching waves are served first; hit consumers take the slots left over.
 constexpr short PRIO_MARCH = 1, PRIO_HITS = 0;
-// Queue blocks a shading wave claims at once.  Measured at 1080p (frame ms): 1 (one refine
-// per lane) 1.10, 2 1.05, 4 1.08, 8 1.47 (claims run past the reserved blocks at the end and
-// leave them to the trailing pass).
-constexpr uint32_t SHADE_CLAIM = 2;
+// Queue blocks a shading wave claims at once.  Measured at 1080p (ms per frame, frames back to
+// back): 2 0.973, 3 0.911, 4 0.950 (with 4, claims run past the reserved blocks at the end
+// and leave 45-57 hits to the trailing pass; at 8 that took the frame to 1.47 ms).
+constexpr uint32_t SHADE_CLAIM = 3;
 
 __device__ __forceinline__ uint64_t lanes_below() {
     const uint32_t lane = threadIdx.x & 63u;
@@ -51,10 +51,10 @@ __device__ __forceinline__ uint64_t lanes_below() {
 // The hit queue's entries cross XCDs, whose L2 caches are not coherent with each other:
 // fields are written and read with agent-scope atomic accesses (coherent at the device), and
 // every field starts as HITQ_EMPTY (the queue is filled with it before the launch).  A field
-// is thus either empty or final, and an entry whose four fields all differ from HITQ_EMPTY is
+// is thus either empty or final, and an entry whose fields all differ from HITQ_EMPTY is
 // complete; no fence is needed.  No field can be HITQ_EMPTY when written: pixel < 2^32 - 2,
-// distances are finite floats, steps <= max_steps.  Entries are reserved in blocks of 64; a
-// wave's last, partial block is padded with HITQ_SKIP entries.
+// distances are finite floats, radii have a clear sign bit, steps <= max_steps.  Entries are
+// reserved in blocks of 64; a wave's last, partial block is padded with HITQ_SKIP entries.
 __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -65,11 +65,12 @@ __device__ __forceinline__ uint32_t ld_agent(uint32_t* p) {
 // Normal and FP64 refine of one hit.  The camera ray is regenerated from its PCG stream (the
 // FP32 ray the march followed); the normal takes four FP32 taps at the hit point
 // (common.wgsl:376-387, h = max(t * cone / 2, 1e-4) * pixel_tolerance); the refine locates the
-// hit rule's crossing in FP64 between the march's last two positions.
+// hit rule's crossing in FP64 between the march's last two positions, whose radii the march
+// queued with the hit.
 template <uint32_t FAM>
 __device__ __forceinline__ void shade_hit(const JobP& P, uint32_t sample, uint32_t pix, float t, float t_prev,
-                                          uint32_t steps, float4* __restrict__ aux, double* __restrict__ depth64,
-                                          Work& w64, Work& wn) {
+                                          float r_prev, float r_hit, uint32_t steps, float4* __restrict__ aux,
+                                          double* __restrict__ depth64, Work& w64, Work& wn) {
     const uint32_t px = pix % P.width, py = pix / P.width;
     uint32_t rng = rng_init(px, py, sample, P.seed);
     V3<float> ro, rd;
@@ -77,8 +78,9 @@ __device__ __forceinline__ void shade_hit(const JobP& P, uint32_t sample, uint32
     const V3<float> p = add(ro, scl(rd, t));
     const V3<float> nv = normal_at<float, FAM>(P.f, p, fmaxf(t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, wn);
     aux[pix] = make_float4(nv.x, nv.y, nv.z, float(steps));
+    const double cone64 = double(P.pixel_cone) * double(P.pixel_tolerance);
     depth64[pix] = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(t_prev), double(t),
-                                 double(P.pixel_cone) * double(P.pixel_tolerance), w64);
+                                 refine_end_g(r_prev, t_prev, cone64), refine_end_g(r_hit, t, cone64), cone64, w64);
 }
 
 // Global ray index of slot j of ray lane `xl` (tiles dealt round-robin to the lanes).
@@ -87,7 +89,7 @@ __device__ __forceinline__ uint32_t lane_ray(uint32_t j, uint32_t xl) {
 }
 
 // Publishes the first `n` staged hits of a wave (n <= 64) as one block of the hit queue: one
-// atomic reserves the block, each lane writes one entry's four fields (lanes >= n a skip entry).
+// atomic reserves the block, each lane writes one entry's fields (lanes >= n a skip entry).
 __device__ __forceinline__ void publish_hits(const uint32_t (*stage)[2 * HIT_BATCH], uint32_t n, const HitQueue& Q,
                                              uint32_t lane) {
     uint32_t base = 0;
@@ -102,6 +104,8 @@ __device__ __forceinline__ void publish_hits(const uint32_t (*stage)[2 * HIT_BAT
     st_agent(Q.t + k, real ? stage[1][lane] : 0u);
     st_agent(Q.t_prev + k, real ? stage[2][lane] : 0u);
     st_agent(Q.steps + k, real ? stage[3][lane] : 0u);
+    st_agent(Q.r_prev + k, real ? stage[4][lane] : 0u);
+    st_agent(Q.r_hit + k, real ? stage[5][lane] : 0u);
     st_agent(Q.pix + k, real ? stage[0][lane] : HITQ_SKIP);
 }
 
@@ -120,8 +124,8 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
                                                  uint32_t* __restrict__ steps_out, const HitQueue Q,
                                                  unsigned long long* __restrict__ ctr) {
     enum : uint32_t { IDLE = 0, MARCH = 1, DONE = 2 };
-    // per wave: staged hits' pixel, t, t before the hit and steps
-    __shared__ uint32_t hit_stage[4][4][2 * HIT_BATCH];
+    // per wave: staged hits' pixel, t, t before the hit, steps and the radii at both
+    __shared__ uint32_t hit_stage[4][HITQ_FIELDS][2 * HIT_BATCH];
     __builtin_amdgcn_s_setprio(PRIO_MARCH);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t (*stage)[2 * HIT_BATCH] = hit_stage[wave];
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
         if (active && !de_done) de_done = de.step(F);
         bool hit_now = false;
         uint32_t hit_steps = 0;
-        float t_hit = 0.0f, tp_hit = 0.0f;
+        float t_hit = 0.0f, tp_hit = 0.0f, r_hit = 0.0f, rp_hit = 0.0f;
         if (active && de_done) {
             const De<float> s = de.finish(F);
             w.de_evals++;
@@ -208,6 +212,8 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
             } else {
                 step_len = radius * omega;
             }
+            rp_hit = prev_radius;  // the radius at t_last
+            r_hit = radius;
             prev_radius = radius;
             const float err = radius / fmaxf(t, 1e-6f);
             if (!sor_fail && err < cand_err) {
@@ -268,6 +274,8 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
                 stage[1][at] = __float_as_uint(t_hit);
                 stage[2][at] = __float_as_uint(tp_hit);
                 stage[3][at] = hit_steps;
+                stage[4][at] = __float_as_uint(rp_hit);
+                stage[5][at] = __float_as_uint(r_hit);
             }
             staged += uint32_t(__popcll(hm));
             if (staged >= HIT_BATCH) {
@@ -275,14 +283,14 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
                 __builtin_amdgcn_wave_barrier();
                 publish_hits(stage, HIT_BATCH, Q, lane);
                 const uint32_t rest = staged - HIT_BATCH;
-                uint32_t carry[4];
+                uint32_t carry[HITQ_FIELDS];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) carry[f] = lane < rest ? stage[f][HIT_BATCH + lane] : 0u;
+                for (int f = 0; f < int(HITQ_FIELDS); ++f) carry[f] = lane < rest ? stage[f][HIT_BATCH + lane] : 0u;
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 if (lane < rest) {
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) stage[f][lane] = carry[f];
+                    for (int f = 0; f < int(HITQ_FIELDS); ++f) stage[f][lane] = carry[f];
                 }
                 staged = rest;
             }
@@ -306,7 +314,7 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
         // normals of their hits (the same work on every lane); then each lane runs the FP64
         // refines of its hits back to back, one field evaluation per trip, so a wave takes the
         // longest lane's sum of refines rather than the sum of each block's longest refine
-        // (a refine takes 2 to 42 FP64 evaluations).
+        // (a refine takes 0 to 40 FP64 evaluations).
         uint32_t b0 = 0, reserved = 0;
         if (lane == 0u) {
             reserved = ld_agent(Q.count) / HIT_BATCH;
@@ -317,7 +325,7 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
         reserved = __shfl(reserved, 0, 64);
         if (b0 >= reserved || b0 >= cap_blocks) break;
         uint32_t h_pix[SHADE_CLAIM];
-        float h_t[SHADE_CLAIM], h_tp[SHADE_CLAIM];
+        float h_t[SHADE_CLAIM], h_tp[SHADE_CLAIM], h_rp[SHADE_CLAIM], h_rh[SHADE_CLAIM];
 #pragma unroll
         for (uint32_t j = 0; j < SHADE_CLAIM; ++j) {
             h_pix[j] = HITQ_SKIP;
@@ -326,14 +334,17 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
             if (b >= reserved || b >= cap_blocks) continue;
             const uint32_t k = b * HIT_BATCH + lane;
             const uint32_t e_pix = ld_agent(Q.pix + k), e_t = ld_agent(Q.t + k), e_tp = ld_agent(Q.t_prev + k),
-                           e_st = ld_agent(Q.steps + k);
-            const bool ready = e_pix != HITQ_EMPTY && e_t != HITQ_EMPTY && e_tp != HITQ_EMPTY && e_st != HITQ_EMPTY;
+                           e_st = ld_agent(Q.steps + k), e_rp = ld_agent(Q.r_prev + k), e_rh = ld_agent(Q.r_hit + k);
+            const bool ready = e_pix != HITQ_EMPTY && e_t != HITQ_EMPTY && e_tp != HITQ_EMPTY && e_st != HITQ_EMPTY &&
+                               e_rp != HITQ_EMPTY && e_rh != HITQ_EMPTY;
             if (__ballot(ready) != ~0ull) continue;
             if (lane == 0u) Q.done[b] = 1u;
             if (e_pix >= npix) continue;  // a padding entry
             h_pix[j] = e_pix;
             h_t[j] = __uint_as_float(e_t);
             h_tp[j] = __uint_as_float(e_tp);
+            h_rp[j] = __uint_as_float(e_rp);
+            h_rh[j] = __uint_as_float(e_rh);
             const uint32_t px = e_pix % P.width, py = e_pix / P.width;
             uint32_t rng = rng_init(px, py, sample, P.seed);
             V3<float> ro, rd;
@@ -347,11 +358,14 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
         bool refining = false;
         Refine64 rf;
         V3<double> ro64 = mk(0.0, 0.0, 0.0), rd64 = ro64;
-        for (uint32_t trip = 0; trip < SHADE_CLAIM * uint32_t(REFINE_ITERS + 2); ++trip) {
+        for (uint32_t trip = 0; trip <= SHADE_CLAIM * uint32_t(REFINE_ITERS); ++trip) {
             // a lane between hits starts its next one
             while (!refining && j < SHADE_CLAIM) {
                 const uint32_t pix = h_pix[j];
-                if (pix != HITQ_SKIP && rf.begin(double(h_tp[j]), double(h_t[j]))) {
+                double out = 0.0;
+                if (pix != HITQ_SKIP &&
+                    !rf.begin(double(h_tp[j]), double(h_t[j]), refine_end_g(h_rp[j], h_tp[j], cone64),
+                              refine_end_g(h_rh[j], h_t[j], cone64), out)) {
                     const uint32_t px = pix % P.width, py = pix / P.width;
                     uint32_t rng = rng_init(px, py, sample, P.seed);
                     V3<float> ro, rd;
@@ -360,7 +374,7 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
                     rd64 = cvt<double>(rd);
                     refining = true;
                 } else {
-                    if (pix != HITQ_SKIP) depth64[pix] = double(h_t[j]);  // no bracket: the march t
+                    if (pix != HITQ_SKIP) depth64[pix] = out;  // no evaluation needed (no bracket: the march t)
                     ++j;
                 }
             }
@@ -398,8 +412,8 @@ __global__ __launch_bounds__(256) void k_hits_tail(const JobP P, uint32_t sample
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t pix = Q.pix[k];
         if (Q.done[k / HIT_BATCH] || pix >= P.width * P.height) continue;  // shaded, or a padding entry
-        shade_hit<FAM>(P, sample, pix, __uint_as_float(Q.t[k]), __uint_as_float(Q.t_prev[k]), Q.steps[k], aux,
-                       depth64, w, wn);
+        shade_hit<FAM>(P, sample, pix, __uint_as_float(Q.t[k]), __uint_as_float(Q.t_prev[k]),
+                       __uint_as_float(Q.r_prev[k]), __uint_as_float(Q.r_hit[k]), Q.steps[k], aux, depth64, w, wn);
         ++tail;
     }
     const uint32_t a = wave_sum(wn.de_evals), b = wave_sum(wn.de_iters), c = wave_sum(w.de64_evals),
