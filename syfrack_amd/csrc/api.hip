@@ -303,14 +303,15 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
             return hip_fail(e, "hipMalloc plane");
         }
     }
-    if ((e = hipMalloc(&j->ctr, sizeof(unsigned long long) * CTR_COUNT)) != hipSuccess ||
+    // the primary trace's counters and the work counters share one allocation (one clear per trace)
+    if ((e = hipMalloc(&j->counters, COUNTER_BYTES)) != hipSuccess ||
         (e = hipMalloc(&j->hitq, (HITQ_FIELDS * j->hitq_cap + j->hitq_cap / 64) * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&j->counters, COUNTER_WORDS * sizeof(uint32_t))) != hipSuccess ||
         (e = hipEventCreate(&j->ev0)) != hipSuccess || (e = hipEventCreate(&j->ev1)) != hipSuccess ||
         (e = hipEventCreate(&j->ev2)) != hipSuccess) {
         syf_job_destroy(j);
         return hip_fail(e, "job allocation");
     }
+    j->ctr = reinterpret_cast<unsigned long long*>(j->counters + COUNTER_WORDS);
     rc = syf_job_clear(j);
     if (rc != SYF_OK) {
         syf_job_destroy(j);
@@ -328,9 +329,8 @@ int syf_job_destroy(syf_job* j) {
     if (j->stream) (void)hipStreamSynchronize(j->stream);
     for (void*& p : j->plane)
         if (p) (void)hipFree(p);
-    if (j->ctr) (void)hipFree(j->ctr);
     if (j->hitq) (void)hipFree(j->hitq);
-    if (j->counters) (void)hipFree(j->counters);
+    if (j->counters) (void)hipFree(j->counters);  // holds ctr too
     if (j->ev0) (void)hipEventDestroy(j->ev0);
     if (j->ev1) (void)hipEventDestroy(j->ev1);
     if (j->ev2) (void)hipEventDestroy(j->ev2);
@@ -430,19 +430,30 @@ int syf_trace_primary(syf_job* j, uint32_t sample) {
                    j->counters,
                    HitQueue{j->hitq, j->hitq + cap, j->hitq + 2 * cap, j->hitq + 3 * cap, j->hitq + 4 * cap,
                             j->hitq + 5 * cap, j->counters + COUNTER_HITS, j->counters + COUNTER_CURSOR,
-                            j->hitq + HITQ_FIELDS * cap, j->counters + COUNTER_QERR, uint32_t(cap)},
+                            j->hitq + HITQ_FIELDS * cap, j->counters + COUNTER_QERR, uint32_t(cap), 0u},
                    j->ctr};
-    hipError_t e = hipMemsetAsync(j->ctr, 0, sizeof(unsigned long long) * CTR_COUNT, j->stream);
-    // ray counters, queue count, cursor and error flag start at zero, every queue field empty,
-    // every done flag clear
-    if (e == hipSuccess) e = hipMemsetAsync(j->counters, 0, COUNTER_WORDS * sizeof(uint32_t), j->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(j->hitq, 0xff, HITQ_FIELDS * cap * sizeof(uint32_t), j->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(j->hitq + HITQ_FIELDS * cap, 0, cap / 64 * sizeof(uint32_t), j->stream);
+    // ray counters, queue count, cursor, error flag and work counters start at zero
+    hipError_t e = hipMemsetAsync(j->counters, 0, COUNTER_BYTES, j->stream);
+    // Every queue field must be empty.  The trailing pass of the previous trace emptied the
+    // entries it used; after a trace whose trailing pass was not enqueued, or when the done-flag
+    // epoch wraps, the whole queue is cleared.  A done flag counts only if it holds this trace's
+    // epoch, so the flags are never cleared otherwise.
+    if (++j->hitq_epoch == 0u) {
+        j->hitq_epoch = 1u;
+        j->hitq_empty = false;
+    }
+    if (!j->hitq_empty) {
+        if (e == hipSuccess) e = hipMemsetAsync(j->hitq, 0xff, HITQ_FIELDS * cap * sizeof(uint32_t), j->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(j->hitq + HITQ_FIELDS * cap, 0, cap / 64 * sizeof(uint32_t), j->stream);
+    }
+    B.q.epoch = j->hitq_epoch;
+    j->hitq_empty = false;
     j->traced = true;
     if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
     if (e == hipSuccess) e = launch_trace_primary(j->P, sample, B, j->ctx->cu_count, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev1, j->stream);
     if (e == hipSuccess) e = launch_hits_tail(j->P, sample, B, j->ctx->cu_count, j->stream);
+    if (e == hipSuccess) j->hitq_empty = true;  // the trailing pass empties the entries the trace used
     if (e == hipSuccess) e = hipEventRecord(j->ev2, j->stream);
     j->timed = true;
     j->extra_ms = 0.0;

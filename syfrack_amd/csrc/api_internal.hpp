@@ -24,10 +24,12 @@ struct syf_job {
     size_t pixels = 0;
     hipStream_t stream = nullptr;
     void* plane[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    unsigned long long* ctr = nullptr;
+    unsigned long long* ctr = nullptr;  // work counters: CTR_COUNT words inside the counters allocation
     uint32_t* hitq = nullptr;      // hit queue of the primary trace: HITQ_FIELDS fields x hitq_cap entries, then done flags
     size_t hitq_cap = 0;
-    uint32_t* counters = nullptr;  // ray counters, hit-queue count and cursor of the primary trace
+    uint32_t hitq_epoch = 0;       // done-flag value of the latest primary trace (never 0)
+    bool hitq_empty = false;       // every queue field is HITQ_EMPTY once the stream reaches the next trace
+    uint32_t* counters = nullptr;  // ray counters, hit-queue count and cursor of the primary trace, then ctr
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // start, after dominant kernel, end
     bool timed = false;
     bool traced = false;  // a primary trace ran: its hit-queue error flag is checked by syf_job_stats

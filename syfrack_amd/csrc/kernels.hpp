@@ -13,10 +13,13 @@ constexpr uint32_t COUNTER_HITS = 128;
 constexpr uint32_t COUNTER_CURSOR = 144;
 constexpr uint32_t COUNTER_QERR = 160;
 constexpr uint32_t COUNTER_WORDS = 176;
+// One allocation holds the counters and, after them, the CTR_COUNT work counters.
+constexpr size_t COUNTER_BYTES = COUNTER_WORDS * sizeof(uint32_t) + CTR_COUNT * sizeof(unsigned long long);
 
 // Hit queue of a primary trace: HITQ_FIELDS fields per entry in separate arrays of `capacity`
 // words (room for every pixel plus one 64-entry block of padding per wave), every word
-// HITQ_EMPTY before the launch; one done flag per 64-entry block, zero before the launch.
+// HITQ_EMPTY before the launch (the trailing pass empties the entries it covered); one done
+// flag per 64-entry block, which counts only when it holds the launch's epoch.
 constexpr uint32_t HITQ_EMPTY = 0xffffffffu;
 constexpr uint32_t HITQ_SKIP = 0xfffffffeu;  // pixel of a padding entry
 constexpr uint32_t HITQ_FIELDS = 6;
@@ -29,9 +32,10 @@ struct HitQueue {
     uint32_t* r_hit;   // the march's radius at t (float bits)
     uint32_t* count;   // entries reserved by the march
     uint32_t* cursor;  // blocks claimed by shading waves
-    uint32_t* done;    // per block: 1 once a shading wave of the fused kernel has shaded it
+    uint32_t* done;    // per block: `epoch` once a shading wave of the fused kernel has shaded it
     uint32_t* error;   // nonzero if a block did not fit (never at hitq_capacity())
     uint32_t capacity; // entries, a multiple of 64
+    uint32_t epoch;    // nonzero, distinct from every stale done flag
 };
 
 // Device buffers of a primary trace.

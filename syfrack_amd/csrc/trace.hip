@@ -50,9 +50,9 @@ __device__ __forceinline__ uint64_t lanes_below() {
 
 // The hit queue's entries cross XCDs, whose L2 caches are not coherent with each other:
 // fields are written and read with agent-scope atomic accesses (coherent at the device), and
-// every field starts as HITQ_EMPTY (the queue is filled with it before the launch).  A field
-// is thus either empty or final, and an entry whose fields all differ from HITQ_EMPTY is
-// complete; no fence is needed.  No field can be HITQ_EMPTY when written: pixel < 2^32 - 2,
+// every field starts as HITQ_EMPTY (filled once, then emptied by each trace's trailing pass).
+// A field is thus either empty or final, and an entry whose fields all differ from HITQ_EMPTY
+// is complete; no fence is needed.  No field can be HITQ_EMPTY when written: pixel < 2^32 - 2,
 // distances are finite floats, radii have a clear sign bit, steps <= max_steps.  Entries are
 // reserved in blocks of 64; a wave's last, partial block is padded with HITQ_SKIP entries.
 __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
             const bool ready = e_pix != HITQ_EMPTY && e_t != HITQ_EMPTY && e_tp != HITQ_EMPTY && e_st != HITQ_EMPTY &&
                                e_rp != HITQ_EMPTY && e_rh != HITQ_EMPTY;
             if (__ballot(ready) != ~0ull) continue;
-            if (lane == 0u) Q.done[b] = 1u;
+            if (lane == 0u) Q.done[b] = Q.epoch;
             if (e_pix >= npix) continue;  // a padding entry
             h_pix[j] = e_pix;
             h_t[j] = __uint_as_float(e_t);
@@ -397,11 +397,12 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
     }
 }
 
-// The hits no wave of k_primary shaded: the entries of every reserved block without its done
-// flag, one per lane, each lane striding over several (a lane's successive hits lie far apart,
-// so the expensive refines of silhouette regions spread over all lanes).  The queue is complete
-// and coherent at this launch's start.  Counted apart (CTR_TAIL_*) so that the fused kernel's
-// own work is known.
+// The hits no wave of k_primary shaded: the entries of every reserved block without this
+// launch's done flag, one per lane, each lane striding over several (a lane's successive hits
+// lie far apart, so the expensive refines of silhouette regions spread over all lanes).  The
+// queue is complete and coherent at this launch's start.  Every reserved entry is then emptied
+// for the next trace, so the queue needs no clearing between traces.  Counted apart
+// (CTR_TAIL_*) so that the fused kernel's own work is known.
 template <uint32_t FAM>
 __global__ __launch_bounds__(256) void k_hits_tail(const JobP P, uint32_t sample, const HitQueue Q,
                                                    float4* __restrict__ aux, double* __restrict__ depth64,
@@ -411,10 +412,19 @@ __global__ __launch_bounds__(256) void k_hits_tail(const JobP P, uint32_t sample
     uint32_t tail = 0;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t pix = Q.pix[k];
-        if (Q.done[k / HIT_BATCH] || pix >= P.width * P.height) continue;  // shaded, or a padding entry
-        shade_hit<FAM>(P, sample, pix, __uint_as_float(Q.t[k]), __uint_as_float(Q.t_prev[k]),
-                       __uint_as_float(Q.r_prev[k]), __uint_as_float(Q.r_hit[k]), Q.steps[k], aux, depth64, w, wn);
-        ++tail;
+        // not shaded by k_primary, and not a padding entry
+        if (Q.done[k / HIT_BATCH] != Q.epoch && pix < P.width * P.height) {
+            shade_hit<FAM>(P, sample, pix, __uint_as_float(Q.t[k]), __uint_as_float(Q.t_prev[k]),
+                           __uint_as_float(Q.r_prev[k]), __uint_as_float(Q.r_hit[k]), Q.steps[k], aux, depth64, w,
+                           wn);
+            ++tail;
+        }
+        Q.pix[k] = HITQ_EMPTY;
+        Q.t[k] = HITQ_EMPTY;
+        Q.t_prev[k] = HITQ_EMPTY;
+        Q.steps[k] = HITQ_EMPTY;
+        Q.r_prev[k] = HITQ_EMPTY;
+        Q.r_hit[k] = HITQ_EMPTY;
     }
     const uint32_t a = wave_sum(wn.de_evals), b = wave_sum(wn.de_iters), c = wave_sum(w.de64_evals),
                    d = wave_sum(w.de64_iters), h = wave_sum(tail);

@@ -173,6 +173,32 @@ def test_primary_trace_is_deterministic(renderer):
         assert np.array_equal(x, y)
 
 
+def test_repeated_traces_on_one_job_reuse_the_hit_queue(renderer):
+    """A job's hit queue is cleared once; each trace's trailing pass empties the entries it used
+    and done flags count only for their own trace.  Traces of several samples on one job, then
+    the first sample again, must give the planes of a fresh job, with no queue error."""
+    from syfrack_amd import abi, scene as sc
+    scene, cfg = sc.mandelbulb_primary(640, 360)
+    job = renderer.job(scene, cfg)
+    try:
+        planes = (abi.PLANE_DEPTH, abi.PLANE_DEPTH64, abi.PLANE_AUX, abi.PLANE_STEPS)
+        runs = []
+        for sample in (0, 1, 2, 0):
+            job.trace_primary(sample)
+            job.sync()
+            st = job.stats()
+            assert st["hits"] > 0
+            runs.append([job.readback(p).copy() for p in planes])
+        fresh = renderer.trace_primary(scene, cfg, sample=1)
+    finally:
+        job.close()
+    for x, y in zip(runs[0], runs[3]):
+        assert np.array_equal(x, y)
+    assert not np.array_equal(runs[0][0], runs[1][0])  # another sample jitters the rays
+    for x, y in zip(runs[1], (fresh.depth, fresh.depth64, fresh.normals, fresh.steps)):
+        assert np.array_equal(x.reshape(y.shape), y)
+
+
 def test_sphere_scene_centre_depth_and_tolerance(renderer):
     """gpu_render.rs:479-490 and :113-145 on the device: a unit sphere seen from distance 3
     has centre depth 2 +- 0.05, and a finer pixel tolerance lowers the hit-radius error."""
