@@ -29,6 +29,8 @@ __device__ __forceinline__ void ray_pixel(uint32_t idx, uint32_t tiles_x, uint32
 // Rays a wave claims at a time: two 8x8 tiles, one atomic per claim.  The tiles are dealt to
 // eight counters, one per XCD (a workgroup uses counter blockIdx % 8, the XCD the dispatcher
 // places it on), so the claims of the chip's 8 L2 slices never contend for one address.
+// Measured at 1080p (ms per frame): 32 rays 1.00, 64 0.907, 128 0.882, 256 0.94 (256 also
+// leaves 650-700 hits to the trailing pass).
 constexpr uint32_t RAY_CHUNK = 128;
 constexpr uint32_t RAY_LANES = 8;
 // Hits are staged per wave in LDS and published to the hit queue 64 at a time.
