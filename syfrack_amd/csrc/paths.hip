@@ -325,8 +325,9 @@ template <uint32_t FAM> __device__ __forceinline__ void shade_surface(const JobP
 
 }  // namespace
 
-// Waves per SIMD the register allocation of k_paths is held to (3: 168 VGPRs, a couple
-// spilled; measured 1.2x faster than the unconstrained 2-wave allocation).
+// Waves per SIMD the register allocation of k_paths is held to (3: 168 VGPRs, no vector
+// spills except one in the trigonometric Mandelbulb; measured 1.2x faster than the
+// unconstrained 2-wave allocation).
 constexpr int PATHS_WAVES = 3;
 
 template <uint32_t FAM>
