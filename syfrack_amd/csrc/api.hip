@@ -343,7 +343,7 @@ int syf_job_destroy(syf_job* j) {
 int syf_job_clear(syf_job* j) {
     clear_error();
     if (!j) return fail(SYF_E_INVALID, "job must not be null");
-    hipSetDevice(j->ctx->device);
+    if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     hipError_t e = hipSuccess;
     for (uint32_t p = 0; p <= SYF_PLANE_STEPS && e == hipSuccess; ++p)
         e = hipMemsetAsync(j->plane[p], 0, plane_bytes(p, j->pixels), j->stream);
@@ -362,7 +362,7 @@ int syf_job_sync(syf_job* j) {
 int syf_job_stats(syf_job* j, syf_stats* out) {
     clear_error();
     if (!j || !out) return fail(SYF_E_INVALID, "job and out must not be null");
-    hipSetDevice(j->ctx->device);
+    if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     unsigned long long c[CTR_COUNT];
     uint32_t qerr = 0;
     hipError_t e = hipMemcpyAsync(c, j->ctr, sizeof c, hipMemcpyDeviceToHost, j->stream);
@@ -402,7 +402,7 @@ int syf_de_probe(syf_job* j, const float* xyz4, size_t n, float* out4, float* tr
     if (!j || (n && (!xyz4 || !out4 || !traps4))) return fail(SYF_E_INVALID, "job and buffers must not be null");
     if (n == 0) return SYF_OK;
     if (n > (1u << 30)) return fail(SYF_E_INVALID, "n must be below 2^30, got %zu", n);
-    hipSetDevice(j->ctx->device);
+    if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     float4 *dp = nullptr, *dout = nullptr, *dtr = nullptr;
     hipError_t e = hipMalloc(&dp, n * 16);
     if (e == hipSuccess) e = hipMalloc(&dout, n * 16);
@@ -421,7 +421,7 @@ int syf_de_probe(syf_job* j, const float* xyz4, size_t n, float* out4, float* tr
 int syf_trace_primary(syf_job* j, uint32_t sample) {
     clear_error();
     if (!j) return fail(SYF_E_INVALID, "job must not be null");
-    hipSetDevice(j->ctx->device);
+    if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     const size_t cap = j->hitq_cap;
     TraceBuffers B{(float*)j->plane[SYF_PLANE_DEPTH],
                    (float4*)j->plane[SYF_PLANE_AUX],
@@ -466,7 +466,7 @@ int syf_readback(syf_job* j, uint32_t plane, void* host, size_t bytes) {
     if (plane > SYF_PLANE_STEPS) return fail(SYF_E_INVALID, "plane: unknown plane %u", plane);
     size_t need = plane_bytes(plane, j->pixels);
     if (bytes != need) return fail(SYF_E_INVALID, "bytes must be %zu for plane %u, got %zu", need, plane, bytes);
-    hipSetDevice(j->ctx->device);
+    if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     hipError_t e = hipMemcpyAsync(host, j->plane[plane], bytes, hipMemcpyDeviceToHost, j->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
     return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_readback");

@@ -452,7 +452,7 @@ int syf_render_tile(syf_job* j, uint32_t x, uint32_t y, uint32_t w, uint32_t h, 
     if (w == 0 || h == 0 || x >= j->P.width || y >= j->P.height || w > j->P.width - x || h > j->P.height - y)
         return fail(SYF_E_INVALID, "tile (%u,%u %ux%u) must lie inside the %ux%u image", x, y, w, h, j->P.width,
                     j->P.height);
-    hipSetDevice(j->ctx->device);
+    if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     hipError_t e = hipMemsetAsync(j->ctr, 0, sizeof(unsigned long long) * CTR_COUNT, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
     if (e == hipSuccess)
@@ -473,7 +473,7 @@ int syf_render_wavefront(syf_job* j, uint32_t x, uint32_t y, uint32_t w, uint32_
     if (w == 0 || h == 0 || x >= j->P.width || y >= j->P.height || w > j->P.width - x || h > j->P.height - y)
         return fail(SYF_E_INVALID, "tile (%u,%u %ux%u) must lie inside the %ux%u image", x, y, w, h, j->P.width,
                     j->P.height);
-    hipSetDevice(j->ctx->device);
+    if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     const uint32_t n = w * h;
     hipError_t e = hipSuccess;
     if (!j->wf || j->wf->capacity < n) {

@@ -169,6 +169,10 @@ struct PerturbCtl {
     unsigned long long best;  // max over glitches of (iterations << 32 | ~pixel)
 };
 
+// Reference-orbit entries a perturbation pass loads at once (one scalar load of 8 float2 each
+// 8): the loop otherwise waits for a load at every iteration.
+constexpr uint32_t ORBIT_CHUNK = 16;
+
 // Perturbation pass against one reference orbit (rounded to f32 pairs), over the pixels of
 // `list` (all pixels when list is null).  ref_dx/ref_dy: the reference's offset from the view
 // centre, so a pixel's delta_c = its offset - the reference offset (FP64, then rounded).
@@ -198,34 +202,49 @@ __global__ __launch_bounds__(256) void k_planar_perturb(const PlanarP P, const d
             float ex = 0.0f, ey = 0.0f;
             bool escaped = false;
             float zx = 0.0f, zy = 0.0f;
-            for (uint32_t n = 0; n < P.iterations; ++n) {
-                const float2 Z = orbit[n];
-                zx = Z.x + ex;
-                zy = Z.y + ey;
-                const float r2 = zx * zx + zy * zy;
-                if (r2 > P.b2f) {
-                    escaped = true;
-                    break;
+            // The orbit index is the same on every lane still iterating, so the reference values
+            // are wave-uniform loads; they are fetched ORBIT_CHUNK at a time, and a lane that
+            // stops (escape or glitch) idles to the end of the chunk.
+            bool active = true;
+            for (uint32_t n0 = 0; n0 < P.iterations; n0 += ORBIT_CHUNK) {
+                float2 Zc[ORBIT_CHUNK];
+#pragma unroll
+                for (uint32_t k = 0; k < ORBIT_CHUNK; ++k) Zc[k] = orbit[min(n0 + k, P.iterations)];
+#pragma unroll
+                for (uint32_t k = 0; k < ORBIT_CHUNK; ++k) {
+                    const uint32_t n = n0 + k;
+                    if (!active || n >= P.iterations) continue;
+                    const float2 Z = Zc[k];
+                    zx = Z.x + ex;
+                    zy = Z.y + ey;
+                    const float r2 = zx * zx + zy * zy;
+                    if (r2 > P.b2f) {
+                        escaped = true;
+                        active = false;
+                        continue;
+                    }
+                    // precision lost against the reference, or the reference has escaped
+                    if (r2 < 1e-6f * (Z.x * Z.x + Z.y * Z.y) || n == orbit_len) {
+                        glitch = true;
+                        active = false;
+                        continue;
+                    }
+                    float nx, ny;
+                    if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
+                        nx = (2.0f * Z.x + ex) * ex - (2.0f * Z.y + ey) * ey + dcx;
+                        const float XY = Z.x * Z.y;
+                        const float dXY = (Z.x + ex) * ey + ex * Z.y;  // (X+ex)(Y+ey) - XY
+                        ny = 2.0f * diffabs(XY, dXY) + dcy;
+                    } else {
+                        const float tx = 2.0f * Z.x + ex, ty = 2.0f * Z.y + ey;
+                        nx = tx * ex - ty * ey + dcx;
+                        ny = tx * ey + ty * ex + dcy;
+                    }
+                    ex = nx;
+                    ey = ny;
+                    ++count;
                 }
-                // precision lost against the reference, or the reference has escaped
-                if (r2 < 1e-6f * (Z.x * Z.x + Z.y * Z.y) || n == orbit_len) {
-                    glitch = true;
-                    break;
-                }
-                float nx, ny;
-                if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
-                    nx = (2.0f * Z.x + ex) * ex - (2.0f * Z.y + ey) * ey + dcx;
-                    const float XY = Z.x * Z.y;
-                    const float dXY = (Z.x + ex) * ey + ex * Z.y;  // (X+ex)(Y+ey) - XY
-                    ny = 2.0f * diffabs(XY, dXY) + dcy;
-                } else {
-                    const float tx = 2.0f * Z.x + ex, ty = 2.0f * Z.y + ey;
-                    nx = tx * ex - ty * ey + dcx;
-                    ny = tx * ey + ty * ex + dcy;
-                }
-                ex = nx;
-                ey = ny;
-                ++count;
+                if (__ballot(active) == 0ull) break;
             }
             if (!glitch && !escaped && count == P.iterations && P.iterations <= orbit_len) {
                 const float2 Z = orbit[count];
@@ -334,19 +353,20 @@ namespace {
 
 void planar_free(syf_planar* p) {
     if (!p) return;
-    if (p->stream) hipStreamSynchronize(p->stream);
-    hipFree(p->fx);
-    hipFree(p->fy);
-    hipFree(p->smooth);
-    hipFree(p->iters);
-    hipFree(p->orbit);
-    hipFree(p->list[0]);
-    hipFree(p->list[1]);
-    hipFree(p->pc);
-    hipFree(p->ctr);
-    if (p->ev0) hipEventDestroy(p->ev0);
-    if (p->ev1) hipEventDestroy(p->ev1);
-    if (p->stream) hipStreamDestroy(p->stream);
+    // teardown: every release is attempted, whatever an earlier one returned
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(p->fx);
+    (void)hipFree(p->fy);
+    (void)hipFree(p->smooth);
+    (void)hipFree(p->iters);
+    (void)hipFree(p->orbit);
+    (void)hipFree(p->list[0]);
+    (void)hipFree(p->list[1]);
+    (void)hipFree(p->pc);
+    (void)hipFree(p->ctr);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
 
@@ -524,7 +544,7 @@ int syf_planar_destroy(syf_planar* p) {
 int syf_planar_run(syf_planar* p) {
     clear_error();
     if (!p) return fail(SYF_E_INVALID, "planar must not be null");
-    hipSetDevice(p->ctx->device);
+    if (hipError_t de = hipSetDevice(p->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     const syf_planar_desc& d = p->desc;
     hipError_t e = hipMemsetAsync(p->ctr, 0, 16, p->stream);
     if (e == hipSuccess) e = hipEventRecord(p->ev0, p->stream);
@@ -572,7 +592,7 @@ int syf_planar_readback(syf_planar* p, uint32_t plane, void* host, size_t bytes)
     if (plane > SYF_PLANAR_ITER) return fail(SYF_E_INVALID, "plane: unknown planar plane %u", plane);
     const size_t need = size_t(p->desc.width) * p->desc.height * 4;
     if (bytes != need) return fail(SYF_E_INVALID, "bytes must be %zu, got %zu", need, bytes);
-    hipSetDevice(p->ctx->device);
+    if (hipError_t de = hipSetDevice(p->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     hipError_t e = hipMemcpyAsync(host, plane == SYF_PLANAR_SMOOTH ? (void*)p->smooth : (void*)p->iters, bytes,
                                   hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);

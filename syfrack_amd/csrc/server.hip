@@ -478,7 +478,7 @@ bool run_plain(syf_server* s, std::unique_lock<std::mutex>& lk) {
 }
 
 void worker_main(syf_server* s) {
-    hipSetDevice(s->ctx->device);
+    (void)hipSetDevice(s->ctx->device);  // a failure shows in the calls that follow
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         if (s->error != SYF_OK) {  // stays stopped: waiters see the error, destroy cleans up
@@ -723,7 +723,7 @@ int syf_server_destroy(syf_server* s) {
     }
     s->cv.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    hipSetDevice(s->ctx->device);
+    (void)hipSetDevice(s->ctx->device);  // a failure shows in the calls that follow
     if (s->stream) hipStreamSynchronize(s->stream);
     // released requests still referred to by launches left behind by an error
     std::vector<Request*> orphans;
@@ -860,7 +860,7 @@ int syf_server_result(syf_server* s, uint64_t id, uint32_t plane, void* host, si
         if (!src) return fail(SYF_E_INVALID, "request %llu was released", (unsigned long long)id);
     }
     if (bytes != need) return fail(SYF_E_INVALID, "bytes must be %zu, got %zu", need, bytes);
-    hipSetDevice(s->ctx->device);
+    if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     hipError_t e = hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost);
     return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_server_result");
 }
@@ -873,7 +873,7 @@ int syf_server_release(syf_server* s, uint64_t id) {
     if (!r) return fail(SYF_E_NOT_FOUND, "unknown request id %llu", (unsigned long long)id);
     if (r->state != SYF_REQ_DONE && r->state != SYF_REQ_CANCELLED)
         return fail(SYF_E_INVALID, "request %llu is still running", (unsigned long long)id);
-    hipSetDevice(s->ctx->device);
+    if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     // launches of a cancelled request may still write its planes
     hipStreamSynchronize(s->stream);
     for (int b = 0; b < NBUF; ++b)
