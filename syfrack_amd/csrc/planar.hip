@@ -203,8 +203,10 @@ __global__ __launch_bounds__(256) void k_planar_perturb(const PlanarP P, const d
             bool escaped = false;
             float zx = 0.0f, zy = 0.0f;
             // The orbit index is the same on every lane still iterating, so the reference values
-            // are wave-uniform loads; they are fetched ORBIT_CHUNK at a time, and a lane that
-            // stops (escape or glitch) idles to the end of the chunk.
+            // are wave-uniform loads; they are fetched ORBIT_CHUNK at a time.  The iteration is
+            // branch-free: every lane computes it, and a lane that has stopped (escape or
+            // glitch) keeps its state through selects (measured 1.5x faster than branching out
+            // of the loop, whose mask updates took most of the issue slots).
             bool active = true;
             for (uint32_t n0 = 0; n0 < P.iterations; n0 += ORBIT_CHUNK) {
                 float2 Zc[ORBIT_CHUNK];
@@ -213,22 +215,13 @@ __global__ __launch_bounds__(256) void k_planar_perturb(const PlanarP P, const d
 #pragma unroll
                 for (uint32_t k = 0; k < ORBIT_CHUNK; ++k) {
                     const uint32_t n = n0 + k;
-                    if (!active || n >= P.iterations) continue;
                     const float2 Z = Zc[k];
-                    zx = Z.x + ex;
-                    zy = Z.y + ey;
-                    const float r2 = zx * zx + zy * zy;
-                    if (r2 > P.b2f) {
-                        escaped = true;
-                        active = false;
-                        continue;
-                    }
+                    const float zx1 = Z.x + ex, zy1 = Z.y + ey;
+                    const float r2 = zx1 * zx1 + zy1 * zy1;
+                    const bool live = active && n < P.iterations;
+                    const bool esc = r2 > P.b2f;
                     // precision lost against the reference, or the reference has escaped
-                    if (r2 < 1e-6f * (Z.x * Z.x + Z.y * Z.y) || n == orbit_len) {
-                        glitch = true;
-                        active = false;
-                        continue;
-                    }
+                    const bool gl = r2 < 1e-6f * (Z.x * Z.x + Z.y * Z.y) || n == orbit_len;
                     float nx, ny;
                     if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
                         nx = (2.0f * Z.x + ex) * ex - (2.0f * Z.y + ey) * ey + dcx;
@@ -240,9 +233,15 @@ __global__ __launch_bounds__(256) void k_planar_perturb(const PlanarP P, const d
                         nx = tx * ex - ty * ey + dcx;
                         ny = tx * ey + ty * ex + dcy;
                     }
-                    ex = nx;
-                    ey = ny;
-                    ++count;
+                    const bool step = live && !esc && !gl;
+                    zx = live ? zx1 : zx;
+                    zy = live ? zy1 : zy;
+                    escaped = escaped || (live && esc);
+                    glitch = glitch || (live && !esc && gl);
+                    active = step;
+                    ex = step ? nx : ex;
+                    ey = step ? ny : ey;
+                    count += step ? 1u : 0u;
                 }
                 if (__ballot(active) == 0ull) break;
             }
