@@ -162,9 +162,11 @@ class TestPlanarGPU:
         print(f"deep zoom 1e-12 centre {c}: within 1 of quad: {near:.4f}; glitch restarts {st['rebases']}")
         assert near >= 0.95
 
-    def test_perturbation_agrees_with_fp64_where_fp64_resolves(self, ctx):
-        """A view of the whole set: FP64 direct resolves every pixel, perturbation must match."""
-        v = PlanarView(family=MB, mode=abi.PLANAR_PERTURB, width=256, height=192, iterations=1000,
+    @pytest.mark.parametrize("iterations", [7, 37, 1000])
+    def test_perturbation_agrees_with_fp64_where_fp64_resolves(self, ctx, iterations):
+        """A view of the whole set: FP64 direct resolves every pixel, perturbation must match.
+        Iteration counts below and between the kernel's 16-entry orbit chunks included."""
+        v = PlanarView(family=MB, mode=abi.PLANAR_PERTURB, width=256, height=192, iterations=iterations,
                        center=(-0.5, 0.0), span=3.0, bailout=16.0)
         it, sm, st = self._run(ctx, v)
         v64 = PlanarView(**{**v.__dict__, "mode": abi.PLANAR_FP64})
