@@ -104,11 +104,17 @@ def calibrate_copy() -> int:
 
 
 def pathtrace_leg(spp: int) -> int:
+    """The path-trace leg alone: prints its JSON line."""
+    print(json.dumps(pathtrace_results(spp)), flush=True)
+    return 0
+
+
+def pathtrace_results(spp: int) -> dict:
     """Mandelbox path trace at 1920x1080 (15 iterations, 4 bounces, one area light, <= 512
     steps) in its three forms: the megakernel (syf_render_tile over 256-pixel tiles, 8 samples
     per dispatch), the wavefront stages (syf_render_wavefront) and the flattened lane-persistent
-    kernel (one request through the batched server).  Prints one JSON line of Mrays/s (all
-    traced segments: primary, bounce and shadow) and device time per form."""
+    kernel (one request through the batched server).  Mrays/s (all traced segments: primary,
+    bounce and shadow) and device time per form."""
     from syfrack_amd import scene as sc, serve
     from syfrack_amd.render import Renderer
     scene, cfg = sc.mandelbox_pathtrace(1920, 1080, spp)
@@ -140,14 +146,19 @@ def pathtrace_leg(spp: int) -> int:
         st = srv.stats()
         srv.release(rid)
     out["flattened"] = {"s": el, "rays": st["rays"], "mrays_s": st["rays"] / el / 1e6, "launches": st["launches"]}
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 def planar_leg(size: int, seeds: int) -> int:
+    """The deep-zoom leg alone: prints its JSON line."""
+    print(json.dumps(planar_results(size, seeds)), flush=True)
+    return 0
+
+
+def planar_results(size: int, seeds: int) -> dict:
     """2D deep zoom: size x size Mandelbrot and Burning Ship images at centres drawn in the
     boundary band (seeded), spans 1e-6 and 1e-12, 10000 iterations, FP64 reference orbit + FP32
-    perturbation with glitch re-reference.  Prints one JSON line of Mpixel/s and iterations/s."""
+    perturbation with glitch re-reference.  Mpixel/s and iterations/s per view."""
     from syfrack_amd import abi
     from syfrack_amd.planar import Planar, PlanarView, boundary_centre
     from syfrack_amd.render import Context
@@ -171,9 +182,7 @@ def planar_leg(size: int, seeds: int) -> int:
                              "mpix_s": size * size / el / 1e6, "giter_s": st["iterations"] / el / 1e9,
                              "iterations": st["iterations"], "glitch_restarts": st["rebases"],
                              "kernel_ms": st["kernel_ms"]})
-    print(json.dumps({"workload": f"{size}x{size} deep zoom, 10000 iterations, perturbation", "views": rows}),
-          flush=True)
-    return 0
+    return {"workload": f"{size}x{size} deep zoom, 10000 iterations, perturbation", "views": rows}
 
 
 def progress(msg: str) -> None:
@@ -255,7 +264,8 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-serve", action="store_true", help="skip the serving (batched vs plain) legs")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving (batched vs plain), path-trace and deep-zoom legs")
     ap.add_argument("--serve-requests", type=int, default=256, help="requests of the uneven batch")
     ap.add_argument("--mix-requests", type=int, default=32, help="requests of the Poisson serving loop")
     ap.add_argument("--mix-rate", type=float, default=100.0, help="arrivals per second of the serving loop")
@@ -372,6 +382,9 @@ def main() -> int:
                                   "serving loop, batched server vs the plain one-launch-per-request form"}
         out["batched_vs_plain_rays"] = {"uneven_batch": sv["uneven_batch"]["speedup"],
                                         "serving_loop": lp["speedup"]}
+        # the other configurations' workloads, for the record (not part of `value`)
+        out["pathtrace"] = pathtrace_results(args.spp)
+        out["deep_zoom"] = planar_results(args.planar_size, args.planar_seeds)
     if not args.no_cpu:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.width, args.height)
