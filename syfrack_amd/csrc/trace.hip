@@ -7,10 +7,11 @@
 //                tetrahedral normal (common.wgsl:376-387, FP32) and the FP64 refine of the hit
 //                distance.  The hit work thus fills the SIMDs the march's tail leaves idle
 //                instead of running after it.
-//   k_hits_tail  the hits no wave took (those queued after the last consumer left).
+//   k_hits_tail  the hits no wave took (blocks claimed before they were complete, or queued
+//                after the last consumer left), then empties every queue entry the trace used.
 // The march reads nothing from HBM: the job's constants sit in the kernel argument segment and
 // each ray is derived from (pixel, sample, seed) by the PCG stream; HBM traffic is the output
-// planes and the hit queue (about 40 bytes per ray).
+// planes (32 bytes per pixel) and the hit queue (24 bytes per hit, written and read back).
 #include <algorithm>
 
 #include "device.hpp"
