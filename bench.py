@@ -12,7 +12,9 @@ timing on the kernel's own stream), the CPU baseline (the oracle's restatement o
 same trace on a bounded sample of rows, on the host cores), and the serving legs: the
 render server in batched mode against the plain one-launch-per-request form on an uneven
 batch of 256 requests and on a Poisson serving loop, with rays/s (all traced segments,
-primary + bounce + shadow) and p50/p99 submit-to-completion latency.
+primary + bounce + shadow) and p50/p99 submit-to-completion latency.  The same line records,
+outside `value`, the other workloads of this path: the Mandelbox 1080p path trace in its
+three kernel forms (`pathtrace`) and the 4096^2 perturbation deep zoom (`deep_zoom`).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--width 1920 --height 1080]
 """
