@@ -236,27 +236,83 @@ __global__ __launch_bounds__(256) void k_wf_generate(const JobP P, const WfP W, 
     }
 }
 
-// Intersect every path of queue `in`: the extension segment of its current bounce.
+// Intersect every path of queue `in`: the extension segment of its current bounce, as
+// trace_segment does it (the nearest emitter caps the march; a miss in front of an emitter is
+// a hit on it).  Lane-persistent: each wave owns a contiguous range of the queue, and a lane
+// whose march has ended takes the range's next path, so lanes do not wait for one another's
+// marches.  Every trip of the loop runs one field evaluation and march()'s step on each busy
+// lane (march_advance: the same arithmetic in the same order, so the segments equal
+// trace_segment's bit for bit).  Measured on the 1080p Mandelbox frame (wavefront form, Mrays/s):
+// a thread per path marching to the end 77.2, this 80.3; one Iterate body per trip 68.0, and
+// four evaluations per trip 79.6.
 template <uint32_t FAM>
 __global__ __launch_bounds__(256) void k_wf_intersect(const JobP P, const WfP W, Soa S, const uint32_t* __restrict__ queue,
                                                       uint32_t in, WfCtl* __restrict__ ctl,
                                                       unsigned long long* __restrict__ ctr) {
     const uint32_t count = ctl->count[in];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t per = (count + waves - 1u) / waves;
+    const uint32_t r1 = uint32_t(min(uint64_t(count), uint64_t(wave + 1u) * per));
+    uint32_t next = uint32_t(min(uint64_t(count), uint64_t(wave) * per));  // wave-uniform
+    const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+    const FractalP& F = P.f;
+    const float hit_cone = P.pixel_cone * P.pixel_tolerance;
     Work work;
     uint32_t segs = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const uint32_t id = queue[size_t(in) * W.capacity + i];
-        const F3 o = mk(S.F(WF_OX, id), S.F(WF_OY, id), S.F(WF_OZ, id));
-        const F3 d = mk(S.F(WF_DX, id), S.F(WF_DY, id), S.F(WF_DZ, id));
-        SegHit sh = trace_segment<FAM>(P, o, d, work);
-        ++segs;
-        S.F(WF_HT, id) = sh.t;
-        S.F(WF_HTRAP, id) = sh.trap0;
-        S.U(WF_HSTEPS, id) = sh.steps;
-        S.U(WF_HLIGHT, id) = sh.light;
-        uint32_t b = S.U(WF_BOUNCE, id) & ~(0x40000u | 0x80000u);
-        if ((b & 0xffffu) == 0u) S.F(WF_TPREV, id) = sh.hit && sh.light == NO_LIGHT ? sh.t_prev : -1.0f;
-        S.U(WF_BOUNCE, id) = b | (sh.hit ? 0x40000u : 0u) | (sh.exhausted ? 0x80000u : 0u);
+    bool busy = false, de_done = false;
+    uint32_t id = 0, light = NO_LIGHT;
+    MarchState m;
+    DeStepper<FAM> de;
+    for (;;) {
+        // idle lanes take the next paths of the wave's range
+        const uint64_t idle = __ballot(!busy);
+        if (idle != 0ull && next < r1) {
+            if (!busy) {
+                const uint32_t at = next + uint32_t(__popcll(idle & below));
+                if (at < r1) {
+                    id = queue[size_t(in) * W.capacity + at];
+                    const F3 o = mk(S.F(WF_OX, id), S.F(WF_OY, id), S.F(WF_OZ, id));
+                    const F3 d = mk(S.F(WF_DX, id), S.F(WF_DY, id), S.F(WF_DZ, id));
+                    const float nearest = nearest_emitter(P, o, d, P.t_end, light);
+                    // nearest > 0 (the ray extent is >= 200, emitter distances exceed 1e-6), so
+                    // march() evaluates its first position
+                    march_reset(m, o, d, nearest);
+                    de_done = de.start(F, add(o, scl(d, 0.0f)));
+                    busy = true;
+                    ++segs;
+                }
+            }
+            next = min(r1, next + uint32_t(__popcll(idle)));
+        }
+        if (__ballot(busy) == 0ull) break;  // the range is used up and every march has ended
+        if (busy) {
+            while (!de_done) de_done = de.step(F);
+            const De<float> s = de.finish(F);
+            work.de_evals++;
+            work.de_iters += s.runs;
+            work.steps++;
+            MarchEnd e;
+            if (!march_advance(m, s, F.safety, hit_cone, P.max_steps, e)) {
+                de_done = de.start(F, add(m.ro, scl(m.rd, m.t)));
+            } else {
+                SegHit sh{e.hit, e.exhausted, e.t, e.steps, NO_LIGHT, e.trap0, e.t_prev};
+                if (!e.hit && !e.exhausted && light != NO_LIGHT) {
+                    sh.hit = true;
+                    sh.t = m.t_end;
+                    sh.light = light;
+                }
+                S.F(WF_HT, id) = sh.t;
+                S.F(WF_HTRAP, id) = sh.trap0;
+                S.U(WF_HSTEPS, id) = sh.steps;
+                S.U(WF_HLIGHT, id) = sh.light;
+                uint32_t b = S.U(WF_BOUNCE, id) & ~(0x40000u | 0x80000u);
+                if ((b & 0xffffu) == 0u) S.F(WF_TPREV, id) = sh.hit && sh.light == NO_LIGHT ? sh.t_prev : -1.0f;
+                S.U(WF_BOUNCE, id) = b | (sh.hit ? 0x40000u : 0u) | (sh.exhausted ? 0x80000u : 0u);
+                busy = false;
+            }
+        }
     }
     flush_work(work, 0u, segs, ctr);
 }

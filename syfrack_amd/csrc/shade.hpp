@@ -419,4 +419,83 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
     st.bounce++;
 }
 
+// ---------------------------------------------------------------- march in flight
+// march() (device.hpp) cut at each field evaluation, for kernels that run many marches per
+// lane as state machines (the flattened path tracer, the wavefront intersect stage).
+
+// One march in flight: the state of march() between two evaluations.
+struct MarchState {
+    F3 ro, rd;
+    float t, omega, prev_radius, step_len, cand_t, cand_err, cand_trap0, t_last, t_end;
+    uint32_t i;
+};
+
+// How a march ended: as Hit<float> of march().
+struct MarchEnd {
+    bool hit, exhausted;
+    float t, t_prev, trap0;
+    uint32_t steps;
+};
+
+__device__ __forceinline__ void march_reset(MarchState& m, F3 ro, F3 rd, float t_end) {
+    m.ro = ro;
+    m.rd = rd;
+    m.t = 0.0f;
+    m.omega = 1.2f;
+    m.prev_radius = 0.0f;
+    m.step_len = 0.0f;
+    m.cand_t = 0.0f;
+    m.cand_err = 1e30f;
+    m.cand_trap0 = 1.0f;
+    m.t_last = 0.0f;
+    m.t_end = t_end;
+    m.i = 0;
+}
+
+// One evaluated march step (the loop body of march()).  Returns true when the march ended.
+__device__ __forceinline__ bool march_advance(MarchState& m, const De<float>& s, float safety, float hit_cone,
+                                              uint32_t max_steps, MarchEnd& e) {
+    const uint32_t steps = m.i + 1u;
+    const float radius = fabsf(s.d) * safety;
+    const bool sor_fail = m.omega > 1.0f && (radius + m.prev_radius) < m.step_len * 1.0001f;
+    if (sor_fail) {
+        m.step_len = m.step_len - m.omega * m.step_len;
+        m.omega = 1.0f;
+    } else {
+        m.step_len = radius * m.omega;
+    }
+    m.prev_radius = radius;
+    const float err = radius / fmaxf(m.t, 1e-6f);
+    if (!sor_fail && err < m.cand_err) {
+        m.cand_t = m.t;
+        m.cand_trap0 = s.trap0;
+        m.cand_err = err;
+    }
+    if (!sor_fail && err < hit_cone) {
+        e = MarchEnd{true, false, m.t, m.t_last, s.trap0, steps};
+        return true;
+    }
+    if (!sor_fail && m.t + m.step_len >= m.t_end) {
+        if (radius >= m.t_end - m.t) {
+            e = MarchEnd{false, false, m.t_end, 0.0f, 1.0f, steps};
+            return true;
+        }
+        m.omega = 1.0f;
+        m.step_len = radius;
+    }
+    m.t_last = m.t;
+    m.t = m.t + m.step_len;
+    ++m.i;
+    if (m.i >= max_steps) {
+        if (m.cand_err < hit_cone * 4.0f) e = MarchEnd{true, false, m.cand_t, m.cand_t, m.cand_trap0, steps};
+        else e = MarchEnd{false, true, m.t_end, 0.0f, 1.0f, steps};
+        return true;
+    }
+    if (m.t >= m.t_end) {
+        e = MarchEnd{false, false, m.t_end, 0.0f, 1.0f, steps};
+        return true;
+    }
+    return false;
+}
+
 }  // namespace syf
