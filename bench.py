@@ -363,12 +363,20 @@ def main() -> int:
                      "note": "FP32 VALU bound (no MFMA; HBM traffic a few tens of bytes per ray); algorithmic "
                              "FLOPs of the fused kernel alone from device counters (march: Iterate bodies, "
                              "evaluations, steps; its hits: normal taps in FP32, refine evaluations in FP64, which "
-                             "issue at the FP32 non-packed rate on MI355X), special ops (div/sqrt/log) not counted; "
+                             "issue at the FP32 non-packed rate on MI355X), special ops (div/sqrt/log) not counted "
+                             "in achieved but given apart (special_ops_per_launch, special_ops_per_s); "
                              "packed FP32 is disabled in the build, so the reachable ceiling is 78.6 TFLOP/s; "
                              "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a "
                              "plain copy kernel; valu_counters = VALU busy, lane utilisation and resident waves per "
                              "CU of the same kernel from a committed rocprofv3 --pmc run"},
     }
+    sp = [flops.special_ops(fam, p) for p in per]
+    if all(x is not None for x in sp):
+        # the correctly rounded division, square root and logarithm the FLOP count leaves out;
+        # each issues as a sequence of about ten or more VALU instructions
+        avg = {prec: {op: sum(x[prec][op] for x in sp) / len(sp) for op in sp[0][prec]} for prec in sp[0]}
+        out["roofline"]["special_ops_per_launch"] = avg
+        out["roofline"]["special_ops_per_s"] = sum(v for d in avg.values() for v in d.values()) / (fused_ms * 1e-3)
     counters = read_counters("k_primary", args.width, args.height)
     if counters:
         out["roofline"]["valu_counters"] = counters

@@ -69,3 +69,27 @@ def fused_flops(family: int, stats: dict) -> tuple:
     fp32 = march_flops(family, stats) + hit_fp32_flops(family, stats) - tail_fp32_flops(family, stats)
     fp64 = refine_flops(family, stats) - tail_fp64_flops(family, stats)
     return fp32, fp64
+
+
+def special_ops(family: int, stats: dict):
+    """Special operations (correctly rounded division, square root, logarithm) of the fused
+    march + hit-shading kernel alone, by type and precision, for the Mandelbulb families; None
+    for the others, whose counts depend on branches the device does not count (the Mandelbox
+    sphere fold divides only inside its shell).
+
+    Per Iterate body: rho and |z| (2 sqrt), z / r, rho / r, x / rho, y / rho (4 div).  Per
+    evaluation: the starting |z| (1 sqrt) and 0.5 * ln(r) * r / dr (1 log, 1 div).  Per march
+    step: radius / t (1 div).  Per normal: the normalisation (1 sqrt, 1 div).  The FP64 refine
+    evaluates the same field in FP64."""
+    if family not in (abi.FAMILY_MANDELBULB,):
+        return None
+    bodies32 = stats["de_iterations"] - stats["tail_hit_de_iterations"]
+    evals32 = stats["de_evals"] - stats["tail_hit_de_evals"]
+    normals = stats["hits"] - stats["tail_hits"]
+    bodies64 = stats["de64_iterations"] - stats["tail_de64_iterations"]
+    evals64 = stats["de64_evals"] - stats["tail_de64_evals"]
+    return {
+        "fp32": {"div": 4 * bodies32 + evals32 + stats["steps"] + normals, "sqrt": 2 * bodies32 + evals32 + normals,
+                 "log": evals32},
+        "fp64": {"div": 4 * bodies64 + evals64, "sqrt": 2 * bodies64 + evals64, "log": evals64},
+    }
