@@ -90,6 +90,18 @@ def spherical_power(points: np.ndarray, power: int):
     return out[:, :3], out[:, 3]
 
 
+def trig_power(points: np.ndarray, power: float, precision: int = 32):
+    """The Power node's trigonometric branch forced at every point: (z^n (n, 3), dr (n,))."""
+    lib = load()
+    pts = np.zeros((len(points), 4), np.float32)
+    pts[:, :3] = np.asarray(points, np.float32)[:, :3]
+    out = np.zeros((len(pts), 4), np.float64)
+    lib.orc_trig_power.restype = None
+    lib.orc_trig_power.argtypes = [C.c_void_p, C.c_size_t, C.c_float, C.c_int, C.c_void_p]
+    lib.orc_trig_power(_ptr(pts), len(pts), power, precision, _ptr(out))
+    return out[:, :3], out[:, 3]
+
+
 def camera_basis(desc) -> np.ndarray:
     out = np.zeros(16, np.float32)
     load().orc_camera_basis(C.byref(desc), _ptr(out))

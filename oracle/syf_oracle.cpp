@@ -23,7 +23,20 @@
 //   planar_sample_f32 / escape_f64 / escape_f128  -> planar.rs:275-345 known answers,
 //       c = 0 never escapes, |c| > 2 escapes at once, conjugate symmetry, agreement of
 //       the FP64 and quad-precision loops where both are exact.
-//   radiance (white furnace, Lambert under an area light)   -> closed forms.
+//   trig_power (non-integer Mandelbulb powers)    -> equals the integer spherical product
+//       at n = 2..12 in FP32 and FP64, |z^n| = r^n, and continuity in the power
+//       (tests/test_oracle_radiance.py).
+//   radiance (NEE, BSDF continuation, MIS, light selection, VNDF, environment, emitters)
+//       -> tests/test_oracle_radiance.py: the white and metallic furnaces and the rim
+//       (gpu_render.rs:456-475,572-608); a Lambert plane under a gradient environment against
+//       rho (H + 0.8 (Z - H)); a Lambert plane under a small emitter against rho/pi * E, with E
+//       by quadrature over the emitter, NEE alone and with MIS; two identical emitters against
+//       one of double intensity (gpu_render.rs:397-421); NEE-only against MIS + VNDF on a rough
+//       metal plane, seen along and off the normal; surface-variation endpoints
+//       (gpu_render.rs:282-306); cavity darkness monotone (gpu_render.rs:840-868).  Planted
+//       mistakes (a dropped cos_l, an unweighted emitter hit, a VNDF without the alpha stretch,
+//       a pdf without G1(v), a linear environment blend, a pdf without 1/pi, a missing
+//       selection probability, swapped sin/cos) each fail at least one of them.
 // A function without such a pin would be marked "parity unpinned" here; none is.
 
 #include <cmath>
@@ -129,6 +142,22 @@ template <class T> void spherical_power(V3<T> p, T r, uint32_t n, V3<T>* out, T*
 
 template <class T> T tpow(T a, T b) { return std::pow(a, b); }
 
+// The trigonometric branch of the Power node (eval.rs:462-475): polar angle from
+// atan2(|xy|, z) (acos of the clamped cosine at the origin), azimuth atan2(y, x), both
+// multiplied by the power; radius r^n and radial derivative n * r^(n-1).
+template <class T> void trig_power(V3<T>* p, T* dr, T power) {
+    T radius = length(*p);
+    T r = fmax_(radius, T(1e-12));
+    T theta = radius >= T(1e-12) ? std::atan2(std::sqrt(p->x * p->x + p->y * p->y), p->z)
+                                  : std::acos(clampv(p->z / r, T(-1), T(1)));
+    T phi = std::atan2(p->y, p->x);
+    *dr = *dr * (tpow(r, power - T(1)) * power);
+    T zr = tpow(r, power);
+    T st = std::sin(theta * power), ct = std::cos(theta * power);
+    T sp = std::sin(phi * power), cp = std::cos(phi * power);
+    *p = v3(st * cp, st * sp, ct) * zr;
+}
+
 // Power node (eval.rs:456-476): integer powers 1..16 take the exact spherical product,
 // any other power the trigonometric form.
 template <class T> void power_node(V3<T>* p, T* dr, T power) {
@@ -140,15 +169,7 @@ template <class T> void power_node(V3<T>* p, T* dr, T power) {
         *p = q;
         *dr = *dr * d;
     } else {
-        T r = fmax_(radius, T(1e-12));
-        T theta = radius >= T(1e-12) ? std::atan2(std::sqrt(p->x * p->x + p->y * p->y), p->z)
-                                      : std::acos(clampv(p->z / r, T(-1), T(1)));
-        T phi = std::atan2(p->y, p->x);
-        *dr = *dr * (tpow(r, power - T(1)) * power);
-        T zr = tpow(r, power);
-        T st = std::sin(theta * power), ct = std::cos(theta * power);
-        T sp = std::sin(phi * power), cp = std::cos(phi * power);
-        *p = v3(st * cp, st * sp, ct) * zr;
+        trig_power(p, dr, power);
     }
 }
 
@@ -1075,6 +1096,25 @@ void orc_spherical_power(const float* xyz4, size_t n, uint32_t power, float* out
         float d;
         spherical_power(p, length(p), power, &q, &d);
         out4[4 * i] = q.x; out4[4 * i + 1] = q.y; out4[4 * i + 2] = q.z; out4[4 * i + 3] = d;
+    }
+}
+
+// The trigonometric Power branch forced at n points, any power (integer ones included), in
+// precision 32 or 64: out4 = (z^n, n * r^(n-1)).  Pinned against the integer spherical
+// product at integer powers.
+void orc_trig_power(const float* xyz4, size_t n, float power, int precision, double* out4) {
+    for (size_t i = 0; i < n; ++i) {
+        if (precision == 64) {
+            V3<double> p = v3(double(xyz4[4 * i]), double(xyz4[4 * i + 1]), double(xyz4[4 * i + 2]));
+            double dr = 1.0;
+            trig_power(&p, &dr, double(power));
+            out4[4 * i] = p.x; out4[4 * i + 1] = p.y; out4[4 * i + 2] = p.z; out4[4 * i + 3] = dr;
+        } else {
+            V3<float> p = v3(xyz4[4 * i], xyz4[4 * i + 1], xyz4[4 * i + 2]);
+            float dr = 1.0f;
+            trig_power(&p, &dr, power);
+            out4[4 * i] = p.x; out4[4 * i + 1] = p.y; out4[4 * i + 2] = p.z; out4[4 * i + 3] = dr;
+        }
     }
 }
 
