@@ -8,6 +8,7 @@ hipcc cross-compiles without a GPU.  Flags:
 """
 from __future__ import annotations
 
+import concurrent.futures
 import glob
 import os
 import subprocess
@@ -31,17 +32,34 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
 
 
-def build(verbose: bool = False, extra=None) -> str:
-    cmd = [HIPCC] + FLAGS + (extra or []) + sources() + ["-o", OUT + ".tmp"]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError("hipcc failed: " + " ".join(cmd))
     err = [l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
     if verbose and err:
         sys.stderr.write("\n".join(err) + "\n")
+
+
+def build(verbose: bool = False, extra=None, jobs: int = 0) -> str:
+    """Compiles every source to an object in parallel (each file is its own code object; the
+    host launchers sit beside their kernels), then links the shared library."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(OBJ_DIR, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    compile_flags = [f for f in FLAGS if f != "-shared"] + (extra or [])
+    cmds = [[HIPCC] + compile_flags + ["-c", s, "-o", o] for s, o in zip(srcs, objs)]
+    workers = jobs or min(len(cmds), max(1, (os.cpu_count() or 4)))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-o", OUT + ".tmp"], verbose)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 
