@@ -214,6 +214,21 @@ int syf_render_tile(syf_job* job, uint32_t x, uint32_t y, uint32_t w, uint32_t h
 int syf_render_wavefront(syf_job* job, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
                          uint32_t sample_start, uint32_t sample_count);
 
+/* Allocation-free primary trace.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  The same trace as syf_trace_primary (the same kernels and planes, bit
+ * for bit) of a job described by `desc`, written into caller-owned device planes of
+ * width x height entries: depth (float), aux4 (float4, 16-byte aligned), depth64 (double) and
+ * steps (uint32).  Enqueued on `hip_stream` (a hipStream_t of ctx's device; NULL = its null
+ * stream) and asynchronous; it calls no hipMalloc, hipFree, memset or synchronisation.  Its
+ * hit queue and counters live in a fixed device array of the library (28.4 MB per device,
+ * held until the process ends), so width x height is limited to SYF_NOALLOC_MAX_PIXELS
+ * (SYF_E_INVALID above it, or on a device of more than 256 CUs).  All calls on one device share
+ * that array: calls that overlap on one device (two streams, two threads) race. */
+#define SYF_NOALLOC_MAX_PIXELS 1048576u
+int syf_trace_primary_noalloc(syf_ctx* ctx, const syf_job_desc* desc, uint32_t sample, float* depth, float* aux4,
+                              double* depth64, uint32_t* steps, void* hip_stream);
+
 /* Copy a device plane (SYF_PLANE_*) to a host buffer of exactly `bytes` bytes.
  * Replaces the readbacks of RenderSession::readback (session.rs:1217-1243). */
 int syf_readback(syf_job* job, uint32_t plane, void* host, size_t bytes);

@@ -226,6 +226,8 @@ class SyfServerCounters(C.Structure):
     ]
 
 
+NOALLOC_MAX_PIXELS = 1048576  # SYF_NOALLOC_MAX_PIXELS
+
 # Every symbol the header declares, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
@@ -240,6 +242,7 @@ SIGNATURES = {
     "syf_job_stats": (C.c_int, [_P, C.POINTER(SyfStats)]),
     "syf_de_probe": (C.c_int, [_P, _P, C.c_size_t, _P, _P]),
     "syf_trace_primary": (C.c_int, [_P, C.c_uint32]),
+    "syf_trace_primary_noalloc": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.c_uint32, _P, _P, _P, _P, _P]),
     "syf_render_tile": (C.c_int, [_P] + [C.c_uint32] * 6),
     "syf_render_wavefront": (C.c_int, [_P] + [C.c_uint32] * 6),
     "syf_readback": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),

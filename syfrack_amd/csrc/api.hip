@@ -252,8 +252,11 @@ int syf_ctx_create(int device, syf_ctx** out) {
     if (device < 0 || device >= n) return fail(SYF_E_NO_DEVICE, "device %d out of range (%d devices)", device, n);
     e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    void* scratch = nullptr;
+    if (int rc = noalloc_setup(device, &scratch); rc != SYF_OK) return rc;
     syf_ctx* c = new syf_ctx();
     c->device = device;
+    c->scratch = scratch;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->cu_count = prop.multiProcessorCount;

@@ -11,6 +11,7 @@ struct syf_ctx {
     int device = 0;
     int cu_count = 0;
     char arch[64] = {0};
+    void* scratch = nullptr;  // this device's address of the allocation-free trace's scratch array
 };
 
 namespace syf {
@@ -45,4 +46,5 @@ int lower_fractal(const syf_fractal& f, FractalP& F);
 int lower_job(const syf_job_desc& d, JobP& P);
 size_t plane_bytes(uint32_t plane, size_t pixels);
 void release_wavefront(syf_job* j);
+int noalloc_setup(int device, void** out);
 }  // namespace syf
