@@ -279,6 +279,9 @@ def main() -> int:
     ap.add_argument("--planar", action="store_true", help="run only the 2D deep-zoom leg")
     ap.add_argument("--planar-size", type=int, default=4096)
     ap.add_argument("--planar-seeds", type=int, default=2)
+    ap.add_argument("--march-math", choices=["fast", "exact"], default="fast",
+                    help="arithmetic of the primary trace's FP32 march (fast: hardware rcp/rsq/log + FMA; "
+                         "exact: IEEE operations as written, step for step with the oracle's FP32 twin)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
@@ -297,6 +300,7 @@ def main() -> int:
 
     progress("primary trace")
     scene, cfg = sc.mandelbulb_primary(args.width, args.height)
+    cfg.march_math = args.march_math
     r = Renderer(0)
     job = r.job(scene, cfg)
     fam = scene.fractal.family
@@ -344,7 +348,11 @@ def main() -> int:
         "data": "synthetic: procedural scene, rays from the PCG stream",
         "config": {"workload": f"Mandelbulb primary-ray sphere trace {args.width}x{args.height}, power 8, "
                                "10 iterations, bailout 2, 1 spp, <=512 steps, pixel tolerance 1, FP64 hit refine",
-                   "width": args.width, "height": args.height, "spp": 1, "max_steps": 512, "parallelism": "1 GPU"},
+                   "width": args.width, "height": args.height, "spp": 1, "max_steps": 512,
+                   "march_math": args.march_math,
+                   "hit_rule": "pixel cone |d| safety / t < pixel_cone * tolerance (the reference's rule); "
+                               "FP64 refine to 1e-9 t of its crossing",
+                   "parallelism": "1 GPU"},
         "de_evals_per_s": de_evals / elapsed,
         "de_evals_fp32_per_step": per[-1]["de_evals"],
         "de_evals_fp64_per_step": per[-1]["de64_evals"],

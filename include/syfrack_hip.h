@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SYF_ABI_VERSION 2
+#define SYF_ABI_VERSION 3
 
 /* ---------------------------------------------------------------- status codes
  * Replaces anyhow::Result at the seams: config errors (renderer.rs:81-138),
@@ -122,9 +122,16 @@ typedef struct syf_job_desc {
     uint32_t max_steps;            /* >= 1 */
     float pixel_tolerance;         /* 0.125..1.0 */
     float cavity_darkness;         /* Atmosphere.cavity_darkness, 0..1 */
-    uint32_t reserved;
+    uint32_t march_math;           /* SYF_MATH_FAST (0) or SYF_MATH_EXACT: arithmetic of the primary
+                                      trace's FP32 march (no counterpart in the reference) */
     uint64_t seed;                 /* Scene.seed; folded lo ^ hi as uniforms.rs:824 */
 } syf_job_desc;
+/* SYF_MATH_FAST: hardware reciprocal / reciprocal square root / logarithm and FMA contraction in
+ * the FP32 march; held to the FP64 oracle's tolerance (>= 99.9 % hit/miss, refined hit distance
+ * within 1e-4 relative).  SYF_MATH_EXACT: correctly rounded IEEE operations as written, so the
+ * march takes the steps of the CPU restatement of the reference arithmetic. */
+#define SYF_MATH_FAST 0u
+#define SYF_MATH_EXACT 1u
 
 /* ---------------------------------------------------------------- planes
  * Device planes of a job, read back with syf_readback.  Order follows the reference
@@ -221,7 +228,7 @@ int syf_render_wavefront(syf_job* job, uint32_t x, uint32_t y, uint32_t w, uint3
  * width x height entries: depth (float), aux4 (float4, 16-byte aligned), depth64 (double) and
  * steps (uint32).  Enqueued on `hip_stream` (a hipStream_t of ctx's device; NULL = its null
  * stream) and asynchronous; it calls no hipMalloc, hipFree, memset or synchronisation.  Its
- * hit queue and counters live in a fixed device array of the library (28.4 MB per device,
+ * hit queue and counters live in a fixed device array of the library (31.5 MB per device,
  * held until the process ends), so width x height is limited to SYF_NOALLOC_MAX_PIXELS
  * (SYF_E_INVALID above it, or on a device of more than 256 CUs).  All calls on one device share
  * that array: calls that overlap on one device (two streams, two threads) race. */

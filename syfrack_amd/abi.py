@@ -136,7 +136,7 @@ class SyfJobDesc(C.Structure):
         ("max_steps", C.c_uint32),
         ("pixel_tolerance", C.c_float),
         ("cavity_darkness", C.c_float),
-        ("reserved", C.c_uint32),
+        ("march_math", C.c_uint32),
         ("seed", C.c_uint64),
     ]
 
@@ -227,6 +227,8 @@ class SyfServerCounters(C.Structure):
 
 
 NOALLOC_MAX_PIXELS = 1048576  # SYF_NOALLOC_MAX_PIXELS
+MATH_FAST = 0  # SYF_MATH_FAST
+MATH_EXACT = 1  # SYF_MATH_EXACT
 
 # Every symbol the header declares, with its ctypes signature.
 _P = C.c_void_p

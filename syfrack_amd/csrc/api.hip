@@ -131,6 +131,8 @@ int lower_job(const syf_job_desc& d, JobP& P) {
         return fail(SYF_E_INVALID, "pixel_tolerance must be between 0.125 and 1.0, got %g", double(d.pixel_tolerance));
     if (!(d.cavity_darkness >= 0.0f && d.cavity_darkness <= 1.0f))
         return fail(SYF_E_INVALID, "cavity_darkness must be between 0 and 1, got %g", double(d.cavity_darkness));
+    if (d.march_math != SYF_MATH_FAST && d.march_math != SYF_MATH_EXACT)
+        return fail(SYF_E_INVALID, "march_math must be SYF_MATH_FAST (0) or SYF_MATH_EXACT (1), got %u", d.march_math);
     if (d.light_count > SYF_MAX_LIGHTS)
         return fail(SYF_E_INVALID, "light_count must be at most %d, got %u", SYF_MAX_LIGHTS, d.light_count);
     const syf_camera& c = d.camera;
@@ -218,6 +220,7 @@ int lower_job(const syf_job_desc& d, JobP& P) {
     float p0z = std::fmax(2.0f * d.fractal.bounding_radius_hint + len(origin), 200.0f);  // uniforms.rs:781
     P.t_end = std::fmax(200.0f, p0z);
     P.cavity_darkness = d.cavity_darkness;
+    P.march_math = d.march_math;
     return SYF_OK;
 }
 

@@ -325,6 +325,195 @@ template <uint32_t FAM> struct DeStepper {
     }
 };
 
+// Hardware reciprocal, reciprocal square root and base-2 logarithm (v_rcp_f32, v_rsq_f32,
+// v_log_f32: about 1 ulp, no range scaling) for the fast FP32 march.
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float ln_hw(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+// sqrt(x) for x >= 0 as x * rsq(x), 0 at 0
+__device__ __forceinline__ float sqrt_hw(float x) { return x > 0.0f ? x * rsq_hw(x) : 0.0f; }
+
+// The operations of the fast FP32 distance field: the hardware estimates above and FMA.
+template <class T> struct FastOps;
+template <> struct FastOps<float> {
+    static constexpr float tiny = 1.17549435e-38f;  // least normal
+    __device__ static __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+    __device__ static __forceinline__ float rcp(float x) { return rcp_hw(x); }
+    __device__ static __forceinline__ float rsq(float x) { return rsq_hw(x); }
+    __device__ static __forceinline__ float ln(float x) { return ln_hw(x); }
+};
+
+// The DeStepper interface with the fast arithmetic of the primary trace (SYF_MATH_FAST), for
+// the FP32 march and normal taps: the same formulas in the same order of
+// operations, but every quotient becomes a product with a reciprocal estimate, every square
+// root comes from a reciprocal square root estimate, and products and sums contract into
+// FMAs (FastOps).  The orbit keeps |z|^2 and tests it against bailout^2; the power body takes
+// 1/r and 1/rho from one reciprocal square root each (no division); minimum traps keep the
+// least |z|^2 and take its root once.  FP32 results differ from the exact stepper by a few ulps
+// per operation, which the orbit amplifies: the march is held to the FP64 oracle's tolerance,
+// not to the FP32 restatement step for step.
+template <class T, uint32_t FAM> struct DeStepperFast {
+    using O = FastOps<T>;
+    V3<T> c, z;
+    T dr, trap2, r2;  // r2 = |z|^2 of the current orbit point, trap2 = least r2 seen
+    uint32_t k;
+
+    __device__ __forceinline__ static T norm2(V3<T> v) { return O::fma(v.x, v.x, O::fma(v.y, v.y, v.z * v.z)); }
+    __device__ __forceinline__ static T root(T x) { return x > T(0) ? x * O::rsq(x) : T(0); }
+
+    __device__ __forceinline__ bool start(const FractalP& F, V3<T> p) {
+        c = p;
+        z = p;
+        dr = T(1);
+        trap2 = T(INFINITY);
+        k = 0;
+        r2 = norm2(p);
+        if constexpr (FAM == SYF_FAMILY_SPHERE) return true;
+        return F.iterations == 0u || r2 > T(F.bailout) * T(F.bailout);
+    }
+
+    // z^n + c for an integer power (integral_spherical_power, eval.rs:129-178), r >= 1e-12
+    __device__ __forceinline__ void power_add(uint32_t n, T inv_r, T r) {
+        const T rho2 = O::fma(z.x, z.x, z.y * z.y);
+        const T inv_rho = O::rsq(fmax(rho2, O::tiny));
+        const T rho = rho2 * inv_rho;
+        T tx = fmin(fmax(z.z * inv_r, T(-1)), T(1)), ty = rho * inv_r;
+        T fx = rho2 > T(0) ? z.x * inv_rho : T(1), fy = rho2 > T(0) ? z.y * inv_rho : T(0);
+        T ax, ay, bx, by, prev;
+        if (n == 8u || n == 6u) {
+            const T t2x = O::fma(tx, tx, -(ty * ty)), t2y = (tx + tx) * ty;
+            const T f2x = O::fma(fx, fx, -(fy * fy)), f2y = (fx + fx) * fy;
+            const T t4x = O::fma(t2x, t2x, -(t2y * t2y)), t4y = (t2x + t2x) * t2y;
+            const T f4x = O::fma(f2x, f2x, -(f2y * f2y)), f4y = (f2x + f2x) * f2y;
+            const T rr2 = r * r, r4 = rr2 * rr2;
+            if (n == 8u) {
+                ax = O::fma(t4x, t4x, -(t4y * t4y));
+                ay = (t4x + t4x) * t4y;
+                bx = O::fma(f4x, f4x, -(f4y * f4y));
+                by = (f4x + f4x) * f4y;
+                prev = r4 * rr2 * r;
+            } else {
+                ax = O::fma(t2x, t4x, -(t2y * t4y));
+                ay = O::fma(t2x, t4y, t2y * t4x);
+                bx = O::fma(f2x, f4x, -(f2y * f4y));
+                by = O::fma(f2x, f4y, f2y * f4x);
+                prev = r4 * r;
+            }
+        } else {
+            ax = T(1); ay = T(0); bx = T(1); by = T(0);
+            uint32_t e = n;
+            while (e != 0u) {
+                if (e & 1u) {
+                    T nx = O::fma(ax, tx, -(ay * ty)), ny = O::fma(ax, ty, ay * tx);
+                    ax = nx; ay = ny;
+                    nx = O::fma(bx, fx, -(by * fy));
+                    ny = O::fma(bx, fy, by * fx);
+                    bx = nx; by = ny;
+                }
+                e >>= 1;
+                if (e != 0u) {
+                    T nx = O::fma(tx, tx, -(ty * ty)), ny = (tx + tx) * ty;
+                    tx = nx; ty = ny;
+                    nx = O::fma(fx, fx, -(fy * fy));
+                    ny = (fx + fx) * fy;
+                    fx = nx; fy = ny;
+                }
+            }
+            prev = T(1);
+            for (uint32_t i = 1; i < n; ++i) prev = prev * r;
+        }
+        const T sc = prev * r;
+        const T ays = ay * sc;
+        z = mk(O::fma(ays, bx, c.x), O::fma(ays, by, c.y), O::fma(ax, sc, c.z));
+        dr = O::fma(dr, T(n) * prev, T(1));
+    }
+
+    // One Iterate body (the caller guarantees the orbit is not done).
+    __device__ __forceinline__ bool step(const FractalP& F) {
+        if constexpr (is_bulb(FAM)) {
+            trap2 = fmin(trap2, r2);
+            const T inv_r = O::rsq(r2), r = r2 * inv_r;
+            if constexpr (FAM != FAM_BULB_TRIG) {
+                const uint32_t n = FAM == FAM_BULB8 ? 8u : F.ipower;
+                if (r >= T(1e-12)) {
+                    power_add(n, inv_r, r);
+                } else {
+                    tiny_power(z, n, dr);
+                    z = add(z, c);
+                    dr = dr + T(1);
+                }
+            } else {
+                trig_power(z, r, T(F.power), dr);
+                z = add(z, c);
+                dr = dr + T(1);
+            }
+        } else if constexpr (FAM == SYF_FAMILY_MANDELBOX) {
+            const T L = T(F.fold_limit);
+            z = mk(O::fma(clampT(z.x, -L, L), T(2), -z.x), O::fma(clampT(z.y, -L, L), T(2), -z.y),
+                   O::fma(clampT(z.z, -L, L), T(2), -z.z));
+            const T q2 = norm2(z);
+            trap2 = fmin(trap2, q2);
+            const T kf = q2 < T(F.mr2) ? T(F.fr2_over_mr2) : (q2 < T(F.fr2) ? T(F.fr2) * O::rcp(q2) : T(1));
+            const T ks = kf * T(F.scale);
+            z = mk(O::fma(z.x, ks, c.x), O::fma(z.y, ks, c.y), O::fma(z.z, ks, c.z));
+            dr = O::fma(dr * kf, T(F.abs_scale), T(1));
+        } else if constexpr (FAM == SYF_FAMILY_MENGER) {
+            T wx = fabs(z.x), wy = fabs(z.y), wz = fabs(z.z), tmp;
+            if (wx < wy) { tmp = wx; wx = wy; wy = tmp; }
+            if (wx < wz) { tmp = wx; wx = wz; wz = tmp; }
+            if (wy < wz) { tmp = wy; wy = wz; wz = tmp; }
+            wx = O::fma(wx, T(F.scale), -T(F.off_s1[0]));
+            wy = O::fma(wy, T(F.scale), -T(F.off_s1[1]));
+            wz = O::fma(wz, T(F.scale), -T(F.off_s1[2]));
+            if (wz < T(F.menger_shift_threshold)) wz = wz + T(F.off_s1[2]);
+            z = mk(wx, wy, wz);
+            dr = dr * T(F.scale);
+        }
+        ++k;
+        r2 = norm2(z);
+        if constexpr (FAM == SYF_FAMILY_MENGER) trap2 = fmin(trap2, r2);
+        return k >= F.iterations || r2 > T(F.bailout) * T(F.bailout);
+    }
+
+    __device__ __forceinline__ De<T> finish(const FractalP& F) const {
+        T d;
+        T tr = trap2 < T(INFINITY) ? root(trap2) : T(INFINITY);
+        if constexpr (is_bulb(FAM)) {
+            const T rr = fmax(root(r2), T(1e-12));
+            d = T(0.5) * O::ln(rr) * rr * O::rcp(dr);
+        } else if constexpr (FAM == SYF_FAMILY_MANDELBOX) {
+            d = root(r2) * O::rcp(fabs(dr));
+        } else if constexpr (FAM == SYF_FAMILY_MENGER) {
+            const T qx = fabs(z.x) - T(F.half[0]), qy = fabs(z.y) - T(F.half[1]), qz = fabs(z.z) - T(F.half[2]);
+            const V3<T> qm = mk(fmax(qx, T(0)), fmax(qy, T(0)), fmax(qz, T(0)));
+            d = (root(norm2(qm)) + fmin(fmax(qx, fmax(qy, qz)), T(0))) * O::rcp(dr);
+        } else {
+            const T r = root(r2);
+            tr = r;
+            const T inv = O::rcp(dr);
+            const T a = (r - T(F.radius)) * inv;
+            const T b = (dot3(z, mk(T(F.plane_n[0]), T(F.plane_n[1]), T(F.plane_n[2]))) - T(F.plane_offset)) * inv;
+            d = fmin(a, b);
+        }
+        De<T> o;
+        o.d = d;
+        o.trap0 = isfinite(tr) ? fmin(tr / T(F.trap_scale), T(1)) : T(1);
+        o.iter = k < F.iteration_product ? k : F.iteration_product;
+        o.runs = k;
+        return o;
+    }
+};
+
+// One whole field evaluation with the fast arithmetic.
+template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval_fast(const FractalP& F, V3<T> p) {
+    DeStepperFast<T, FAM> s;
+    if (!s.start(F, p)) {
+        for (uint32_t k = 0; k < F.iterations; ++k)
+            if (s.step(F)) break;
+    }
+    return s.finish(F);
+}
+
 // Family chosen at run time (uniform per job or per tile): used where several families
 // share one launch.
 template <class T> __device__ __forceinline__ De<T> de_eval_any(const FractalP& F, V3<T> p) {
@@ -465,7 +654,7 @@ __device__ __forceinline__ Hit<T> march(const FractalP& F, V3<T> ro, V3<T> rd, T
 }
 
 // normal_at (common.wgsl:376-387).
-template <class T, uint32_t FAM>
+template <class T, uint32_t FAM, bool FAST = false>
 __device__ __forceinline__ V3<T> normal_at(const FractalP& F, V3<T> p, T h, Work& w) {
     V3<T> g = mk(T(0), T(0), T(0));
 #pragma unroll
@@ -476,11 +665,13 @@ __device__ __forceinline__ V3<T> normal_at(const FractalP& F, V3<T> p, T h, Work
         V3<T> k = mk(kx, ky, kz);
         De<T> s;
         if constexpr (FAM == 0xffffffffu) s = de_eval_any<T>(F, add(p, scl(k, h)));
+        else if constexpr (FAST) s = de_eval_fast<T, FAM>(F, add(p, scl(k, h)));
         else s = de_eval<T, FAM>(F, add(p, scl(k, h)));
         w.de_evals++;
         w.de_iters += s.runs;
         g = add(g, scl(k, s.d));
     }
+    if constexpr (FAST) return scl(g, FastOps<T>::rsq(dot3(g, g)));
     return nrm(g);
 }
 

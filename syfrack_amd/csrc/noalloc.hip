@@ -27,9 +27,10 @@ namespace syf {
 namespace {
 
 // Entries of the fixed hit queue: SYF_NOALLOC_MAX_PIXELS rounded to 64, plus one 64-entry
-// block for each wave of the widest primary grid (2 workgroups of 4 waves per CU, 256 CUs).
+// block for each wave of the widest primary grid (at most 4 workgroups of 4 waves per CU, 256
+// CUs).
 constexpr uint32_t NA_MAX_CU = 256;
-constexpr size_t NA_QUEUE_CAP = size_t(SYF_NOALLOC_MAX_PIXELS) + size_t(NA_MAX_CU) * 2u * 4u * 64u;
+constexpr size_t NA_QUEUE_CAP = size_t(SYF_NOALLOC_MAX_PIXELS) + size_t(NA_MAX_CU) * 4u * 4u * 64u;
 constexpr size_t NA_COUNTER_WORDS = 256;  // COUNTER_BYTES (824) rounded up to 1 KiB
 static_assert(COUNTER_BYTES <= NA_COUNTER_WORDS * sizeof(uint32_t), "counters do not fit");
 static_assert(NA_QUEUE_CAP % 64 == 0, "queue capacity must be whole blocks");
@@ -38,7 +39,7 @@ constexpr size_t NA_WORDS = NA_COUNTER_WORDS + HITQ_FIELDS * NA_QUEUE_CAP + NA_Q
 }  // namespace
 }  // namespace syf
 
-// 28.4 MB per device: counters, then six queue fields of NA_QUEUE_CAP words, then done flags.
+// 31.5 MB per device: counters, then six queue fields of NA_QUEUE_CAP words, then done flags.
 static __device__ __align__(256) uint32_t scratch[syf::NA_WORDS];
 
 namespace syf {

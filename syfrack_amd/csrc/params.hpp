@@ -86,6 +86,7 @@ struct JobP {
     float pixel_cone;         // tan_half * 2 / width (pathtrace.wgsl:809)
     float t_end;              // ray_extent (common.wgsl:206-208)
     float cavity_darkness;
+    uint32_t march_math;      // SYF_MATH_FAST or SYF_MATH_EXACT (the primary trace's FP32 march)
 };
 
 // Kernel specialisation of a family: the Mandelbulb with a non-integer power takes the

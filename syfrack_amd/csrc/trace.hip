@@ -2,7 +2,10 @@
 //
 // The primary trace runs in one persistent kernel and a short trailing pass:
 //   k_primary    FP32 march of every camera ray (march_segment, pathtrace.wgsl:78-145) as a
-//                lane-persistent state machine; hits go to a hit queue.  A wave whose lanes
+//                lane-persistent state machine; hits go to a hit queue.  The job's march_math
+//                picks the FP32 arithmetic: SYF_MATH_FAST (DeStepperFast: hardware reciprocal,
+//                reciprocal square root and logarithm, FMA; the default) or SYF_MATH_EXACT
+//                (k_primary_exact: the IEEE operations of the oracle's FP32 twin).  A wave whose lanes
 //                have no rays left lowers its issue priority and takes queued hits: the
 //                tetrahedral normal (common.wgsl:376-387, FP32) and the FP64 refine of the hit
 //                distance.  The hit work thus fills the SIMDs the march's tail leaves idle
@@ -71,7 +74,7 @@ __device__ __forceinline__ uint32_t ld_agent(uint32_t* p) {
 // (common.wgsl:376-387, h = max(t * cone / 2, 1e-4) * pixel_tolerance); the refine locates the
 // hit rule's crossing in FP64 between the march's last two positions, whose radii the march
 // queued with the hit.
-template <uint32_t FAM>
+template <uint32_t FAM, bool FAST>
 __device__ __forceinline__ void shade_hit(const JobP& P, uint32_t sample, uint32_t pix, float t, float t_prev,
                                           float r_prev, float r_hit, uint32_t steps, float4* __restrict__ aux,
                                           double* __restrict__ depth64, Work& w64, Work& wn) {
@@ -80,7 +83,8 @@ __device__ __forceinline__ void shade_hit(const JobP& P, uint32_t sample, uint32
     V3<float> ro, rd;
     camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
     const V3<float> p = add(ro, scl(rd, t));
-    const V3<float> nv = normal_at<float, FAM>(P.f, p, fmaxf(t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, wn);
+    const V3<float> nv =
+        normal_at<float, FAM, FAST>(P.f, p, fmaxf(t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, wn);
     aux[pix] = make_float4(nv.x, nv.y, nv.z, float(steps));
     const double cone64 = double(P.pixel_cone) * double(P.pixel_tolerance);
     depth64[pix] = refine64<FAM>(P.f, cvt<double>(ro), cvt<double>(rd), double(t_prev), double(t),
@@ -114,7 +118,7 @@ __device__ __forceinline__ void publish_hits(const uint32_t (*stage)[2 * HIT_BAT
 }
 
 // FP32 march of all primary rays, then hit shading.  March: every trip of the loop runs at
-// most one Iterate body on each lane; a lane whose field evaluation completes in the trip
+// most two Iterate bodies on each lane; a lane whose field evaluation completes in the trip
 // applies the fence and advances its march (the arithmetic of march(), in the same order),
 // queueing a hit or writing a miss when the ray ends.  Idle lanes take new rays from a
 // wave-owned chunk of the launch's ray counter.  Deep and shallow evaluations, long and short
@@ -125,12 +129,17 @@ __device__ __forceinline__ void publish_hits(const uint32_t (*stage)[2 * HIT_BAT
 // expected, so the compiler's note about it is not shown.)
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Wpass-failed"
-template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, uint32_t tiles_x, uint32_t n_slots,
-                                                 uint32_t* __restrict__ next, float* __restrict__ depth,
-                                                 float4* __restrict__ aux, double* __restrict__ depth64,
-                                                 uint32_t* __restrict__ steps_out, const HitQueue Q,
-                                                 unsigned long long* __restrict__ ctr) {
+// FAST selects the FP32 arithmetic (SYF_MATH_FAST: DeStepperFast for the march and the normal
+// taps, a hardware reciprocal in the hit rule's ratio; otherwise the exact arithmetic of the
+// oracle's FP32 twin).  The FP64 refine is the same in both: with reciprocal and reciprocal
+// square root estimates refined by Newton steps instead of the correctly rounded FP64 division
+// and square root it took the 1080p frame to 0.766 against 0.727 ms (measured).
+template <uint32_t FAM, bool FAST>
+__device__ __forceinline__ void primary_trace(const JobP& P, uint32_t sample, uint32_t tiles_x, uint32_t n_slots,
+                                              uint32_t* __restrict__ next, float* __restrict__ depth,
+                                              float4* __restrict__ aux, double* __restrict__ depth64,
+                                              uint32_t* __restrict__ steps_out, const HitQueue& Q,
+                                              unsigned long long* __restrict__ ctr) {
     enum : uint32_t { IDLE = 0, MARCH = 1, DONE = 2 };
     // per wave: staged hits' pixel, t, t before the hit, steps and the radii at both
     __shared__ uint32_t hit_stage[4][HITQ_FIELDS][2 * HIT_BATCH];
@@ -147,7 +156,7 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
     V3<float> ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
     float t = 0.0f, omega = 1.2f, prev_radius = 0.0f, step_len = 0.0f, cand_t = 0.0f, cand_err = 1e30f, t_last = 0.0f;
     uint32_t i = 0, pix = 0;
-    DeStepper<FAM> de;
+    std::conditional_t<FAST, DeStepperFast<float, FAM>, DeStepper<FAM>> de;
     bool de_done = false;
     Work w;
     uint32_t hits = 0, rays = 0;
@@ -202,11 +211,19 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
         }
         if (__ballot(mode != DONE) == 0ull) break;
         const bool active = mode == MARCH;
-        if (active && !de_done) de_done = de.step(F);
+        // up to two Iterate bodies per trip: the march step below then runs for half as many
+        // trips of a deep evaluation (1080p frame 0.693 against 0.710 ms with one)
+        if (active && !de_done) {
+            de_done = de.step(F);
+            if (!de_done) de_done = de.step(F);
+        }
         bool hit_now = false;
         uint32_t hit_steps = 0;
         float t_hit = 0.0f, tp_hit = 0.0f, r_hit = 0.0f, rp_hit = 0.0f;
         if (active && de_done) {
+            // the march step of march() in straight-line form: every decision is a select, so
+            // the lanes that finished an evaluation run one branch-free block (the reference's
+            // order of tests: back-step, best candidate, hit, t_end guard, step budget)
             const De<float> s = de.finish(F);
             w.de_evals++;
             w.de_iters += s.runs;
@@ -214,51 +231,36 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
             const uint32_t nsteps = i + 1u;  // march() reports i + 1 for the step just evaluated
             const float radius = fabsf(s.d) * safety;
             const bool sor_fail = omega > 1.0f && (radius + prev_radius) < step_len * 1.0001f;
-            if (sor_fail) {
-                step_len = step_len - omega * step_len;
-                omega = 1.0f;
-            } else {
-                step_len = radius * omega;
-            }
+            step_len = sor_fail ? step_len - omega * step_len : radius * omega;
+            omega = sor_fail ? 1.0f : omega;
             rp_hit = prev_radius;  // the radius at t_last
             r_hit = radius;
             prev_radius = radius;
-            const float err = radius / fmaxf(t, 1e-6f);
-            if (!sor_fail && err < cand_err) {
-                cand_t = t;
-                cand_err = err;
-            }
-            uint32_t outcome = 0;  // 0 continue, 1 hit, 2 miss
-            t_hit = t;
-            tp_hit = t_last;
-            if (!sor_fail && err < hit_cone) {
-                outcome = 1;
-            } else {
-                if (!sor_fail && t + step_len >= P.t_end) {
-                    if (radius >= P.t_end - t) outcome = 2;
-                    omega = 1.0f;
-                    step_len = radius;
-                }
-                if (outcome == 0u) {
-                    t_last = t;
-                    t = t + step_len;
-                    ++i;
-                    if (i >= P.max_steps) {
-                        if (cand_err < hit_cone * 4.0f) {
-                            outcome = 1;
-                            t_hit = cand_t;
-                            tp_hit = cand_t;  // best-candidate hit: no refine bracket
-                        } else {
-                            outcome = 2;
-                        }
-                    } else if (t >= P.t_end) {
-                        outcome = 2;
-                    }
-                }
-            }
-            if (outcome == 0u) {
-                de_done = de.start(F, add(ro, scl(rd, t)));
-            } else {
+            const float err = FAST ? radius * rcp_hw(fmaxf(t, 1e-6f)) : radius / fmaxf(t, 1e-6f);
+            const bool better = !sor_fail && err < cand_err;
+            cand_t = better ? t : cand_t;
+            cand_err = better ? err : cand_err;
+            const bool is_hit = !sor_fail && err < hit_cone;
+            const bool guard = !sor_fail && !is_hit && t + step_len >= P.t_end;
+            const bool guard_miss = guard && radius >= P.t_end - t;
+            omega = guard ? 1.0f : omega;
+            step_len = guard ? radius : step_len;
+            const bool advance = !is_hit && !guard_miss;
+            const float t_next = t + step_len;
+            const uint32_t i_next = i + 1u;
+            const bool out_of_steps = advance && i_next >= P.max_steps;
+            const bool cand_hit = out_of_steps && cand_err < hit_cone * 4.0f;
+            const bool past_end = advance && !out_of_steps && t_next >= P.t_end;
+            const uint32_t outcome =  // 0 continue, 1 hit, 2 miss
+                (is_hit || cand_hit) ? 1u : ((guard_miss || out_of_steps || past_end) ? 2u : 0u);
+            t_hit = cand_hit ? cand_t : t;
+            tp_hit = cand_hit ? cand_t : t_last;  // best-candidate hit: no refine bracket
+            t_last = advance ? t : t_last;
+            t = advance ? t_next : t;
+            i = advance ? i_next : i;
+            const bool next_done = de.start(F, add(ro, scl(rd, t)));  // unused once the ray has ended
+            de_done = outcome == 0u ? next_done : de_done;
+            if (outcome != 0u) {
                 steps_out[pix] = nsteps;
                 if (outcome == 1u) {
                     depth[pix] = t_hit;
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
             uint32_t rng = rng_init(px, py, sample, P.seed);
             V3<float> ro, rd;
             camera_ray(P.cam, P.width, P.height, px, py, rng, ro, rd);
-            const V3<float> nv = normal_at<float, FAM>(P.f, add(ro, scl(rd, h_t[j])),
+            const V3<float> nv = normal_at<float, FAM, FAST>(P.f, add(ro, scl(rd, h_t[j])),
                                                        fmaxf(h_t[j] * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance, wn);
             aux[e_pix] = make_float4(nv.x, nv.y, nv.z, float(e_st));
         }
@@ -406,6 +408,24 @@ __global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, 
     }
 }
 
+template <uint32_t FAM>
+__global__ __launch_bounds__(256) void k_primary(const JobP P, uint32_t sample, uint32_t tiles_x, uint32_t n_slots,
+                                                 uint32_t* __restrict__ next, float* __restrict__ depth,
+                                                 float4* __restrict__ aux, double* __restrict__ depth64,
+                                                 uint32_t* __restrict__ steps_out, const HitQueue Q,
+                                                 unsigned long long* __restrict__ ctr) {
+    primary_trace<FAM, true>(P, sample, tiles_x, n_slots, next, depth, aux, depth64, steps_out, Q, ctr);
+}
+
+template <uint32_t FAM>
+__global__ __launch_bounds__(256) void k_primary_exact(const JobP P, uint32_t sample, uint32_t tiles_x,
+                                                       uint32_t n_slots, uint32_t* __restrict__ next,
+                                                       float* __restrict__ depth, float4* __restrict__ aux,
+                                                       double* __restrict__ depth64, uint32_t* __restrict__ steps_out,
+                                                       const HitQueue Q, unsigned long long* __restrict__ ctr) {
+    primary_trace<FAM, false>(P, sample, tiles_x, n_slots, next, depth, aux, depth64, steps_out, Q, ctr);
+}
+
 #pragma clang diagnostic pop
 
 // The hits no wave of k_primary shaded: the entries of every reserved block without this
@@ -425,9 +445,16 @@ __global__ __launch_bounds__(256) void k_hits_tail(const JobP P, uint32_t sample
         const uint32_t pix = Q.pix[k];
         // not shaded by k_primary, and not a padding entry
         if (Q.done[k / HIT_BATCH] != Q.epoch && pix < P.width * P.height) {
-            shade_hit<FAM>(P, sample, pix, __uint_as_float(Q.t[k]), __uint_as_float(Q.t_prev[k]),
-                           __uint_as_float(Q.r_prev[k]), __uint_as_float(Q.r_hit[k]), Q.steps[k], aux, depth64, w,
-                           wn);
+            // the job's arithmetic, as k_primary's shading waves (the planes must not depend on
+            // which kernel shaded a hit)
+            if (P.march_math == SYF_MATH_EXACT)
+                shade_hit<FAM, false>(P, sample, pix, __uint_as_float(Q.t[k]), __uint_as_float(Q.t_prev[k]),
+                                      __uint_as_float(Q.r_prev[k]), __uint_as_float(Q.r_hit[k]), Q.steps[k], aux,
+                                      depth64, w, wn);
+            else
+                shade_hit<FAM, true>(P, sample, pix, __uint_as_float(Q.t[k]), __uint_as_float(Q.t_prev[k]),
+                                     __uint_as_float(Q.r_prev[k]), __uint_as_float(Q.r_hit[k]), Q.steps[k], aux,
+                                     depth64, w, wn);
             ++tail;
         }
         Q.pix[k] = HITQ_EMPTY;
@@ -478,9 +505,20 @@ __global__ __launch_bounds__(256) void k_de_probe(const FractalP F, const float4
 // waves share each SIMD's issue slots with them and stretch that critical path (measured at
 // 1080p, march alone: 2 per CU 0.80 ms, 1 per CU 1.00 ms, 3 per CU 0.88 ms, 8 per CU 1.65 ms).
 // At least one workgroup per ray lane, or that lane's tiles would never be traced.
+// Workgroups per CU of a primary-trace launch: 2, or SYF_PRIMARY_WG_PER_CU (1 to 4) from the
+// environment, read once (a tuning override; the planes do not depend on it).
+static uint32_t primary_wg_per_cu() {
+    static const uint32_t v = [] {
+        const char* e = std::getenv("SYF_PRIMARY_WG_PER_CU");
+        const long n = e ? std::strtol(e, nullptr, 10) : 0;
+        return uint32_t(n >= 1 && n <= 4 ? n : 2);
+    }();
+    return v;
+}
+
 static uint32_t primary_blocks(uint32_t width, uint32_t height, int cu_count) {
     const uint32_t n_slots = ((width + 7u) / 8u) * ((height + 7u) / 8u) * 64u;
-    const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * 2u;
+    const uint32_t fill = uint32_t(cu_count > 0 ? cu_count : 256) * primary_wg_per_cu();
     return std::max<uint32_t>(std::min<uint32_t>((n_slots + 255u) / 256u, fill), RAY_LANES);
 }
 
@@ -494,8 +532,13 @@ hipError_t launch_trace_primary(const JobP& P, uint32_t sample, const TraceBuffe
     const uint32_t tiles_x = (P.width + 7u) / 8u, tiles_y = (P.height + 7u) / 8u;
     const uint32_t n_slots = tiles_x * tiles_y * 64u;
     const dim3 grid(primary_blocks(P.width, P.height, cu_count)), block(256);
-    SYF_DISPATCH_FAMILY(kernel_family(P.f), k_primary, grid, block, 0, s, P, sample, tiles_x, n_slots, B.counters,
-                        B.depth, B.aux, B.depth64, B.steps, B.q, B.ctr);
+    if (P.march_math == SYF_MATH_EXACT) {
+        SYF_DISPATCH_FAMILY(kernel_family(P.f), k_primary_exact, grid, block, 0, s, P, sample, tiles_x, n_slots,
+                            B.counters, B.depth, B.aux, B.depth64, B.steps, B.q, B.ctr);
+    } else {
+        SYF_DISPATCH_FAMILY(kernel_family(P.f), k_primary, grid, block, 0, s, P, sample, tiles_x, n_slots, B.counters,
+                            B.depth, B.aux, B.depth64, B.steps, B.q, B.ctr);
+    }
     return hipGetLastError();
 }
 

@@ -227,6 +227,9 @@ class RenderConfig:
     max_steps: int = 256
     pixel_tolerance: float = 1.0
     tile_size: int = 256
+    # arithmetic of the primary trace's FP32 march: "fast" (hardware reciprocal, reciprocal
+    # square root and logarithm, FMA) or "exact" (IEEE operations as written); not a reference field
+    march_math: str = "fast"
 
 
 @dataclass
@@ -259,6 +262,9 @@ class Scene:
         d.max_steps = cfg.max_steps
         d.pixel_tolerance = cfg.pixel_tolerance
         d.cavity_darkness = self.cavity_darkness
+        if cfg.march_math not in ("fast", "exact"):
+            raise ValueError(f"march_math must be 'fast' or 'exact', got {cfg.march_math!r}")
+        d.march_math = abi.MATH_FAST if cfg.march_math == "fast" else abi.MATH_EXACT
         d.seed = self.seed & 0xFFFFFFFFFFFFFFFF
         return d
 

@@ -97,7 +97,7 @@ def _angle(a, b):
     return np.arccos(np.clip(c, -1.0, 1.0))
 
 
-def _check_normals(name, out, ref32, ref64, sl):
+def _check_normals(name, out, ref32, ref64, sl, exact=True):
     """AUX plane (normal, steps) against the oracle: where the device march took the FP32
     twin's steps the normals carry the same step count and agree to 1e-4 rad on >= 99.9 % of the
     pixels (at most 1e-3: a one-ulp libm difference in a tap's field value, divided by a tap
@@ -112,14 +112,17 @@ def _check_normals(name, out, ref32, ref64, sl):
     a64 = _angle(g[both64][:, :3], ref64["aux"][both64][:, :3])
     print(f"{name} normals: vs FP32 twin max {a32.max():.2e} rad over {both32.sum()}; vs FP64 twin "
           f"p99 {np.percentile(a64, 99):.2e} max {a64.max():.2e} rad over {both64.sum()}")
-    assert both32.sum() > 0.95 * both64.sum()
-    assert np.mean(a32 <= 1e-4) >= 0.999 and a32.max() <= 1e-3
+    if exact:  # the exact march takes the FP32 twin's steps; the fast one only nearly
+        assert both32.sum() > 0.95 * both64.sum()
+        assert np.mean(a32 <= 1e-4) >= 0.999 and a32.max() <= 1e-3
     assert np.mean(a64 <= 1e-3) >= 0.999
 
 
-def test_mandelbulb_primary_320x180_matches_fp64_oracle(renderer):
+@pytest.mark.parametrize("math", ["fast", "exact"])
+def test_mandelbulb_primary_320x180_matches_fp64_oracle(renderer, math):
     from syfrack_amd import scene as sc
     scene, cfg = sc.mandelbulb_primary(320, 180)
+    cfg.march_math = math
     out, ref32, ref64, sl = _trace_vs_oracle(renderer, scene, cfg)
     hit_g, hit_o = out.depth >= 0, ref64["depth"] >= 0
     agree = np.mean(hit_g == hit_o)
@@ -130,18 +133,21 @@ def test_mandelbulb_primary_320x180_matches_fp64_oracle(renderer):
           f"p99 {np.percentile(rel, 99):.3e}; march distance rel err max {relm.max():.3e}")
     assert agree >= 0.999
     assert np.mean(rel <= 1e-4) >= 0.999
-    # against the FP32 oracle the march itself agrees step for step
-    assert np.mean(out.steps == ref32["steps"]) >= 0.99
+    same = np.mean(out.steps == ref32["steps"])
+    print(f"{math} march: {same:.4f} of the pixels take the FP32 twin's step count")
+    if math == "exact":  # the exact march follows the FP32 restatement step for step
+        assert same >= 0.99
     hits = out.depth >= 0
     n = out.normals[hits][:, :3]
     assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-4)
     assert np.all(out.normals[~hits] == np.array([0, 0, 0, -1], np.float32))
     assert out.stats["primary_rays"] == 0 or out.stats["primary_rays"] == 320 * 180
-    _check_normals("mandelbulb 320x180", out, ref32, ref64, sl)
+    _check_normals(f"mandelbulb 320x180 {math}", out, ref32, ref64, sl, exact=(math == "exact"))
 
 
+@pytest.mark.parametrize("math", ["fast", "exact"])
 @pytest.mark.parametrize("name", ["mandelbox", "menger"])
-def test_other_families_primary_match_fp64_oracle(renderer, name):
+def test_other_families_primary_match_fp64_oracle(renderer, name, math):
     from syfrack_amd import scene as sc
     f = sc.Fractal.bundled(name)
     if name == "mandelbox":
@@ -149,7 +155,7 @@ def test_other_families_primary_match_fp64_oracle(renderer, name):
     else:
         cam = sc.Camera.orbit((0, 0, 0), 3.6, 40.0, 30.0, 40.0)
     scene = sc.Scene(f, cam, seed=5)
-    cfg = sc.RenderConfig(160, 90, 1, 0, 512)
+    cfg = sc.RenderConfig(160, 90, 1, 0, 512, march_math=math)
     out, ref32, ref64, sl = _trace_vs_oracle(renderer, scene, cfg)
     hit_g, hit_o = out.depth >= 0, ref64["depth"] >= 0
     agree = np.mean(hit_g == hit_o)
@@ -159,7 +165,7 @@ def test_other_families_primary_match_fp64_oracle(renderer, name):
     assert both.sum() > 100
     assert agree >= 0.999
     assert np.mean(rel <= 1e-4) >= 0.999
-    _check_normals(name, out, ref32, ref64, sl)
+    _check_normals(f"{name} {math}", out, ref32, ref64, sl, exact=(math == "exact"))
 
 
 def test_mandelbulb_1080p_properties_and_centre_rows(renderer):
@@ -179,7 +185,7 @@ def test_mandelbulb_1080p_properties_and_centre_rows(renderer):
     both = g & o
     rel = np.abs(out.depth64[sl][both] - ref64["depth64"][both]) / ref64["depth64"][both]
     assert np.mean(rel <= 1e-4) >= 0.999
-    _check_normals("mandelbulb 1080p centre rows", out, ref32, ref64, sl)
+    _check_normals("mandelbulb 1080p centre rows (fast)", out, ref32, ref64, sl, exact=False)
     st = out.stats
     assert st["hits"] == int(hits.sum())
     assert st["steps"] == int(out.steps.astype(np.int64).sum())
