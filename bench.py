@@ -36,12 +36,18 @@ HBM_PEAK_GBS = 8000.0
 
 
 def cpu_baseline(width: int, height: int, budget_s: float = 10.0) -> dict:
-    """The oracle (CPU restatement of the same trace: FP32 march + normals + FP64 refine)
-    on whole rows from the image centre, the sample growing to whole frames (repeated) until it
-    takes about budget_s."""
+    """The oracle (CPU restatement of the same trace: FP32 march + normals + FP64 refine), in
+    its baseline build (g++ -O3 -march=native -ffp-contract=off, OpenMP over the host's cores,
+    built on this host), on whole rows from the image centre, the sample growing to whole
+    frames (repeated) until it takes about budget_s."""
     from oracle import oracle  # the checker, used here only as the CPU baseline
     from syfrack_amd import scene as sc
 
+    try:
+        lib, build = oracle.load_native(), "g++ -O3 -march=native -ffp-contract=off"
+    except Exception as e:  # no compiler on this host: the test build, and say so
+        progress(f"CPU baseline: native build failed ({e}); timing the -O2 test build")
+        lib, build = oracle.load(), "g++ -O2 -ffp-contract=off"
     scene, cfg = sc.mandelbulb_primary(width, height)
     desc = scene.desc(cfg)
     rows, frames = 4, 1
@@ -49,7 +55,7 @@ def cpu_baseline(width: int, height: int, budget_s: float = 10.0) -> dict:
         y0 = max(0, height // 2 - rows // 2)
         t0 = time.perf_counter()
         for f in range(frames):
-            oracle.trace_primary(desc, f, 32, rect=(0, y0, width, rows))
+            oracle.trace_primary(desc, f, 32, rect=(0, y0, width, rows), lib=lib)
         elapsed = time.perf_counter() - t0
         if elapsed > budget_s / 4 or frames >= 64:
             break
@@ -62,7 +68,111 @@ def cpu_baseline(width: int, height: int, budget_s: float = 10.0) -> dict:
     what = f"{frames} whole frames" if rows == height else f"{rows} centre rows x {width} px"
     return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": f"{what} of the same trace ({rays} primary rays, {elapsed:.2f} s); oracle C++ restatement, "
-                      "OpenMP, g++ -O2 -ffp-contract=off"}
+                      f"OpenMP, {build}"}
+
+
+def _rel_stats(a, b, mask):
+    import numpy as np
+    a = np.asarray(a, np.float64)[mask]
+    b = np.asarray(b, np.float64)[mask]
+    if a.size == 0:
+        return {"n": 0}
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+    return {"n": int(a.size), "max": float(rel.max()), "p99": float(np.percentile(rel, 99)),
+            "frac_within_1e-4": float(np.mean(rel <= 1e-4))}
+
+
+def error_report(r, march_math: str) -> dict:
+    """The error of each configuration against the oracle (computed with the oracle, so here in
+    the CPU leg, outside the timed region): hit/miss agreement and the relative error of the
+    FP64-refined hit distance against the FP64 twin; path-traced radiance against the oracle's
+    FP32 radiance; deep-zoom counts against binary128 where FP32 deltas resolve the point;
+    served FP64 depth against the FP64 twin."""
+    import numpy as np
+    from oracle import oracle
+    from syfrack_amd import abi, scene as sc, serve
+    from syfrack_amd.planar import Planar, PlanarView, boundary_centre
+    out = {}
+
+    def primary(scene, cfg, rect=None):
+        g = r.trace_primary(scene, cfg)
+        ref = oracle.trace_primary(scene.desc(cfg), 0, 64, rect=rect)
+        sl = (slice(None), slice(None)) if rect is None else (slice(rect[1], rect[1] + rect[3]),
+                                                            slice(rect[0], rect[0] + rect[2]))
+        gd, gd64 = g.depth[sl], g.depth64[sl]
+        hg, ho = gd >= 0, ref["depth"] >= 0
+        both = hg & ho
+        return {"pixels": int(hg.size), "hit_miss_agreement": float(np.mean(hg == ho)),
+                "refined_distance_rel_err": _rel_stats(gd64, ref["depth64"], both),
+                "march_distance_rel_err": _rel_stats(gd, ref["depth"], both)}
+
+    progress("errors: primary trace")
+    s1, c1 = sc.mandelbulb_primary(320, 180)
+    c1.march_math = march_math
+    out["mandelbulb_primary_320x180"] = primary(s1, c1)
+    s1, c1 = sc.mandelbulb_primary(1920, 1080)
+    c1.march_math = march_math
+    out["mandelbulb_primary_1080p_centre_32_rows"] = primary(s1, c1, rect=(0, 524, 1920, 32))
+
+    progress("errors: path trace")
+    s2, c2 = sc.mandelbox_pathtrace(1920, 1080, 64)
+    rect = (896, 520, 128, 16)
+    job = r.job(s2, c2)
+    job.render_tile(*rect, 0, 64)
+    job.sync()
+    acc = job.readback(abi.PLANE_ACCUM)[rect[1]:rect[1] + rect[3], rect[0]:rect[0] + rect[2]]
+    d64 = job.readback(abi.PLANE_DEPTH64)[rect[1]:rect[1] + rect[3], rect[0]:rect[0] + rect[2]]
+    job.close()
+    oacc, _, _, _ = oracle.render(s2.desc(c2), rect, 0, 64)
+    twin = oracle.trace_primary(s2.desc(c2), 0, 64, rect=rect)["depth64"]
+    g, o = acc[..., :3], oacc[..., :3]
+    out["mandelbox_pathtrace_1080p_64spp_128x16_crop"] = {
+        "pixels": int(g.shape[0] * g.shape[1]),
+        "radiance_frac_within_1e-3": float(np.mean(np.all(np.abs(g - o) <= 1e-3 * np.abs(o) + 1e-4, axis=-1))),
+        "image_mean_rel_err": float(abs(g.mean() - o.mean()) / abs(o.mean())),
+        "first_hit_refined_distance_rel_err": _rel_stats(d64, twin, (d64 >= 0) & (twin >= 0))}
+
+    progress("errors: deep zoom")
+    views = []
+    rng = np.random.default_rng(0)
+    for fam, name in ((abi.PLANAR_MANDELBROT, "mandelbrot"), (abi.PLANAR_BURNING_SHIP, "burning_ship")):
+        c = boundary_centre(r.ctx, fam, seed=0, iterations=2000)
+        for span in (1e-6, 1e-12):
+            v = PlanarView(family=fam, mode=abi.PLANAR_PERTURB, width=4096, height=4096, iterations=10000,
+                           center=c, span=span, bailout=16.0)
+            p = Planar(r.ctx, v)
+            p.run()
+            it, _ = p.readback()
+            p.close()
+            xs = rng.integers(0, 4096, 192)
+            ys = rng.integers(0, 4096, 192)
+            q = oracle.escape_f128(v.c_struct(), xs, ys)[0].astype(np.int64)
+            stable = np.ones(len(xs), bool)
+            for nudge in (16.0, -16.0):
+                stable &= np.abs(oracle.escape_f128(v.c_struct(), xs, ys, nudge)[0].astype(np.int64) - q) <= 1
+            near = np.abs(it[ys, xs].astype(np.int64) - q) <= 1
+            views.append({"family": name, "span": span, "sampled": int(len(xs)), "resolvable": int(stable.sum()),
+                          "count_within_1_of_binary128_on_resolvable": float(near[stable].mean())
+                          if stable.any() else None})
+    out["deep_zoom_4096_10000_iterations"] = views
+
+    progress("errors: served requests")
+    reqs = [q for q in serve.uneven_requests(64, seed=3) if q.cfg.width <= 384][:4]
+    served = []
+    with serve.Server(r.ctx, "batched") as srv:
+        ids = [srv.submit(q.scene, q.cfg) for q in reqs]
+        for i in ids:
+            srv.wait(i, 300)
+        for q, i in zip(reqs, ids):
+            d64 = srv.plane(i, abi.PLANE_DEPTH64)
+            srv.release(i)
+            twin = oracle.trace_primary(q.scene.desc(q.cfg), 0, 64)["depth64"]
+            hg, ho = d64 >= 0, twin >= 0
+            served.append({"family": int(q.scene.fractal.family), "side": q.cfg.width,
+                           "hit_miss_agreement": float(np.mean(hg == ho)),
+                           "refined_distance_rel_err": _rel_stats(d64, twin, hg & ho)})
+    out["served_requests_first_hit"] = served
+    return out
 
 
 def read_traffic(kernel: str, width: int, height: int):
@@ -230,8 +340,91 @@ def serve_stream(ctx, reqs, mode: str, realtime: bool, budget: int = 200 * 10**9
             "all_p50_ms": _pct([i.latency_ms for i in infos], 50), "all_p99_ms": _pct([i.latency_ms for i in infos], 99)}
 
 
-def serving_bench(ctx, n_uneven: int, n_mix: int, rate: float) -> dict:
-    """Batched serving against the plain one-launch-per-request form on the same streams."""
+def serve_window(ctx, reqs, mode: str, window_s: float, budget: int = 200 * 10**9) -> dict:
+    """A serving loop over a fixed arrival window: requests are submitted at their arrival times
+    until ``window_s``; then every progressive render not finished is cancelled (its launched
+    work still completes) and the queued previews are left to finish.  Aggregate Mrays/s = rays
+    traced / wall time from the first submission to the drain; preview latency = submit to
+    completion."""
+    from syfrack_amd import serve
+    reqs = [q for q in reqs if q.arrival_s <= window_s]
+    with serve.Server(ctx, mode, hbm_budget_bytes=budget) as s:
+        t0 = time.perf_counter()
+        ids = []
+        next_note = 10.0
+        for q in reqs:
+            dt = q.arrival_s - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(s.submit(q.scene, q.cfg))
+            if time.perf_counter() - t0 > next_note:
+                progress(f"{mode} loop: {len(ids)}/{len(reqs)} submitted at {time.perf_counter() - t0:.0f} s")
+                next_note += 10.0
+        dt = window_s - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        t_end = time.perf_counter() - t0
+        renders_done_in_window = 0
+        cancelled = 0
+        for i, q in zip(ids, reqs):
+            if q.preview:
+                continue
+            if s.poll(i).state == "done":
+                renders_done_in_window += 1
+            else:
+                s.cancel(i)
+                cancelled += 1
+        s.wait_all(900)
+        wall = time.perf_counter() - t0
+        st = s.stats()
+        infos = [s.poll(i) for i in ids]
+        for i in ids:
+            s.release(i)
+    prev = [i.latency_ms for i, q in zip(infos, reqs) if q.preview and i.state == "done"]
+    prev_in_window = [i.latency_ms for i, q in zip(infos, reqs)
+                      if q.preview and i.state == "done" and i.done_ms <= t_end * 1e3 + infos[0].submit_ms]
+    # previews that finished before the progressive renders that arrived earlier and still had
+    # more than 300 ms of work left when the preview arrived (unfinished renders count as later)
+    rend = [(q.arrival_s, i) for i, q in zip(infos, reqs) if not q.preview]
+    over, pairs = 0, 0
+    for i, q in zip(infos, reqs):
+        if not q.preview or i.state != "done":
+            continue
+        for a, ri in rend:
+            if a < q.arrival_s and (ri.state != "done" or ri.done_ms > i.submit_ms + 300.0):
+                pairs += 1
+                over += (ri.state != "done") or (i.done_ms < ri.done_ms)
+    return {"mode": mode, "window_s": window_s, "requests": len(reqs), "previews": sum(q.preview for q in reqs),
+            "renders": sum(not q.preview for q in reqs), "renders_done_in_window": renders_done_in_window,
+            "renders_cancelled": cancelled, "drain_s": wall - t_end, "wall_s": wall, "rays": st["rays"],
+            "mrays_s": st["rays"] / wall / 1e6, "device_ms": st["device_ms"], "launches": st["launches"],
+            "hbm_peak_gb": st["hbm_peak"] / 1e9, "previews_done": len(prev),
+            "preview_p50_ms": _pct(prev, 50), "preview_p99_ms": _pct(prev, 99),
+            "previews_done_in_window": len(prev_in_window),
+            "preview_overtakes_earlier_renders": over / pairs if pairs else None}
+
+
+def serving_loop_bench(ctx, rates, plain_window_s: float) -> dict:
+    """The Poisson serving loop at each rate: batched over the window that brings 1,000 previews,
+    the plain form over the same window capped at plain_window_s."""
+    from syfrack_amd import serve
+    out = {"workload": "Poisson arrivals, 70% preview tiles 256^2 1 spp / 30% progressive renders 1024^2-2048^2 "
+                       "16-64 spp over the fractal mix (bounces 1-4, <=256 steps); each rate's window brings 1,000 "
+                       "previews; at the window's end unfinished renders are cancelled and queued previews "
+                       "finish; 200 GB admission budget", "rates": []}
+    for rate in rates:
+        reqs = serve.serving_window(rate, 1000, seed=5)
+        window = reqs[-1].arrival_s + 1e-3
+        progress(f"serving loop at {rate:g}/s: {len(reqs)} requests over {window:.1f} s")
+        b = serve_window(ctx, reqs, "batched", window)
+        pw = min(window, plain_window_s)
+        p = serve_window(ctx, reqs, "plain", pw)
+        out["rates"].append({"rate_per_s": rate, "batched": b, "plain": p, "speedup": b["mrays_s"] / p["mrays_s"]})
+    return out
+
+
+def serving_bench(ctx, n_uneven: int) -> dict:
+    """Batched serving against the plain one-launch-per-request form on the uneven batch."""
     from syfrack_amd import abi, serve
     out = {}
     uneven = serve.uneven_requests(n_uneven, seed=3)
@@ -245,15 +438,6 @@ def serving_bench(ctx, n_uneven: int, n_mix: int, rate: float) -> dict:
                     "Mandelbulb (power 2-12) / Mandelbox (scale -2.5..3) / Menger (4-7 levels), spp 1-16, "
                     "bounces 1-4, <=256 steps, sky light; batched: 30 ms launches while requests arrive, then "
                     "all remaining work in one persistent launch",
-        "batched": b, "plain": p, "speedup": b["mrays_s"] / p["mrays_s"], "same_rays": b["rays"] == p["rays"]}
-    mix = serve.serving_mix(n_mix, rate_per_s=rate, seed=5)
-    progress(f"serving loop of {n_mix} requests at {rate:g}/s")
-    b = serve_stream(ctx, mix, "batched", realtime=True)
-    p = serve_stream(ctx, mix, "plain", realtime=True)
-    out["serving_loop"] = {
-        "workload": f"{n_mix} requests, Poisson arrivals at {rate:g}/s: 70% preview tiles 256^2 1 spp, 30% "
-                    "progressive renders 1024^2-2048^2 16-64 spp, same fractal mix, bounces 1-4, "
-                    "200 GB admission budget",
         "batched": b, "plain": p, "speedup": b["mrays_s"] / p["mrays_s"], "same_rays": b["rays"] == p["rays"]}
     return out
 
@@ -269,8 +453,11 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving (batched vs plain), path-trace and deep-zoom legs")
     ap.add_argument("--serve-requests", type=int, default=256, help="requests of the uneven batch")
-    ap.add_argument("--mix-requests", type=int, default=32, help="requests of the Poisson serving loop")
-    ap.add_argument("--mix-rate", type=float, default=100.0, help="arrivals per second of the serving loop")
+    ap.add_argument("--loop-rates", default="20,200",
+                    help="arrival rates (per second) of the Poisson serving loop, comma-separated")
+    ap.add_argument("--loop-plain-window", type=float, default=20.0,
+                    help="seconds of each rate's stream the plain form serves (the batched form serves the "
+                         "whole window that brings 1,000 previews)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy (reference for the byte counters)")
     ap.add_argument("--pathtrace", action="store_true",
@@ -389,23 +576,29 @@ def main() -> int:
     if counters:
         out["roofline"]["valu_counters"] = counters
     if not args.no_serve:
-        sv = serving_bench(r.ctx, args.serve_requests, args.mix_requests, args.mix_rate)
+        sv = serving_bench(r.ctx, args.serve_requests)
+        sv["serving_loop"] = serving_loop_bench(r.ctx, [float(x) for x in args.loop_rates.split(",")],
+                                                args.loop_plain_window)
         out["serving"] = sv
-        lp = sv["serving_loop"]
-        out["latency"] = {"preview_p50_ms": lp["batched"]["preview_p50_ms"],
-                          "preview_p99_ms": lp["batched"]["preview_p99_ms"],
-                          "plain_preview_p50_ms": lp["plain"]["preview_p50_ms"],
-                          "plain_preview_p99_ms": lp["plain"]["preview_p99_ms"],
-                          "note": "submit-to-completion latency of the 256^2 preview requests in the Poisson "
-                                  "serving loop, batched server vs the plain one-launch-per-request form"}
+        rates = sv["serving_loop"]["rates"]
+        out["latency"] = {
+            "per_rate": [{"rate_per_s": x["rate_per_s"], "previews": x["batched"]["previews_done"],
+                          "preview_p50_ms": x["batched"]["preview_p50_ms"],
+                          "preview_p99_ms": x["batched"]["preview_p99_ms"],
+                          "plain_previews": x["plain"]["previews_done"],
+                          "plain_preview_p50_ms": x["plain"]["preview_p50_ms"],
+                          "plain_preview_p99_ms": x["plain"]["preview_p99_ms"]} for x in rates],
+            "note": "submit-to-completion latency of the 256^2 preview requests in the Poisson serving loop, "
+                    "batched server vs the plain one-launch-per-request form"}
         out["batched_vs_plain_rays"] = {"uneven_batch": sv["uneven_batch"]["speedup"],
-                                        "serving_loop": lp["speedup"]}
+                                        "serving_loop": {str(x["rate_per_s"]): x["speedup"] for x in rates}}
         # the other configurations' workloads, for the record (not part of `value`)
         out["pathtrace"] = pathtrace_results(args.spp)
         out["deep_zoom"] = planar_results(args.planar_size, args.planar_seeds)
     if not args.no_cpu:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.width, args.height)
+        out["errors"] = error_report(r, args.march_math)
     job.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -13,11 +13,20 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+NATIVE_PATH = os.path.join(HERE, "_build", "liboracle_native.so")
 _LIB = None
 
 
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load_native() -> C.CDLL:
+    """The CPU-baseline build (-O3 -march=native) of the same source, built on the calling host
+    (make native) when absent."""
+    if not os.path.exists(NATIVE_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "native"], check=True)
+    return _bind(C.CDLL(NATIVE_PATH))
 
 
 def load() -> C.CDLL:
@@ -26,7 +35,11 @@ def load() -> C.CDLL:
         return _LIB
     if not os.path.exists(LIB_PATH):
         build()
-    lib = C.CDLL(LIB_PATH)
+    _LIB = _bind(C.CDLL(LIB_PATH))
+    return _LIB
+
+
+def _bind(lib: C.CDLL) -> C.CDLL:
     P = C.c_void_p
     lib.orc_pcg.restype = C.c_uint32
     lib.orc_pcg.argtypes = [C.c_uint32]
@@ -58,7 +71,6 @@ def load() -> C.CDLL:
     if hasattr(lib, "orc_render"):
         lib.orc_render.restype = C.c_int
         lib.orc_render.argtypes = [P] + [C.c_uint32] * 6 + [P, P, P, P]
-    _LIB = lib
     return lib
 
 
@@ -116,9 +128,10 @@ def camera_rays(desc, sample: int):
     return ro, rd
 
 
-def trace_primary(desc, sample: int = 0, precision: int = 32, rect=None):
-    """Primary trace of a pixel rectangle (x0, y0, w, h); default the whole image."""
-    lib = load()
+def trace_primary(desc, sample: int = 0, precision: int = 32, rect=None, lib=None):
+    """Primary trace of a pixel rectangle (x0, y0, w, h); default the whole image.  ``lib``: another
+    build of the oracle (load_native()), else the test build."""
+    lib = lib or load()
     x0, y0, w, h = rect if rect is not None else (0, 0, desc.width, desc.height)
     n = w * h
     depth = np.zeros(n, np.float64)

@@ -103,7 +103,13 @@ constexpr int NBUF = 3;
 // and preview latency did not change (its requests arrive within 0.3 s, in latency mode).
 constexpr double QUIET_FACTOR = 32.0;
 constexpr uint32_t QUIET_QUANTUM = 16;
+// Requests count as still arriving for max(LATENCY_HOLD_MS, 10 x the mean gap between recent
+// submissions), at most LATENCY_HOLD_MAX_MS: a batch submitted at once goes quiet after 200 ms,
+// a Poisson stream of 20/s (mean gap 50 ms) only after 500 ms without a submission, which its
+// gaps exceed with probability e^-10 (with a fixed 200 ms, 2 % of its gaps switched the server
+// to quiet launches of about a second, and previews arriving then waited for them: p99 768 ms).
 constexpr double LATENCY_HOLD_MS = 200.0;
+constexpr double LATENCY_HOLD_MAX_MS = 2000.0;
 
 struct syf_server {
     syf_ctx* ctx = nullptr;
@@ -140,6 +146,7 @@ struct syf_server {
     hipEvent_t ev_pool[2 * NBUF] = {};
     uint32_t quantum = 0, quantum_long = 0, target_us = 0, quiet_us = 0, grid = 0;
     double last_submit_ms = -1e30;  // host clock of the latest submission
+    double gap_ms = 0.0;            // running mean of the gaps between submissions (below 5 s)
     uint32_t fill_items = 0;  // work items (one wave each) a launch needs to fill the device
     double ns_per_unit = 0.0;
     uint64_t hbm_in_use = 0, hbm_peak = 0, budget = 0;
@@ -258,7 +265,8 @@ bool plan_launch(syf_server* s) {
     // waits for those longer launches.  Measured on one 1080p Mandelbox request at 16 spp: 141
     // Mrays/s at (30 ms, 1 sample), 205 at (100 ms, 4 samples), 215 for the whole request in
     // one launch.
-    const bool latency = !s->queue.empty() || s->now_ms() - s->last_submit_ms < LATENCY_HOLD_MS;
+    const double hold_ms = std::min(std::max(LATENCY_HOLD_MS, 10.0 * s->gap_ms), LATENCY_HOLD_MAX_MS);
+    const bool latency = !s->queue.empty() || s->now_ms() - s->last_submit_ms < hold_ms;
     const double ns = std::max(s->ns_per_unit, 1e-6);
     const double budget_units = latency ? double(s->target_us) * 1000.0 / ns
                                 : s->quiet_us == SYF_QUIET_UNBOUNDED ? 1e300 : double(s->quiet_us) * 1000.0 / ns;
@@ -788,6 +796,8 @@ int syf_server_submit(syf_server* s, const syf_job_desc* desc, uint32_t spp, uin
     }
     r->id = s->next_id++;
     r->submit_ms = s->now_ms();
+    const double gap = r->submit_ms - s->last_submit_ms;
+    if (gap < 5000.0) s->gap_ms = 0.9 * s->gap_ms + 0.1 * gap;
     s->last_submit_ms = r->submit_ms;
     s->reqs[r->id] = r;
     s->queue.push_back(r);

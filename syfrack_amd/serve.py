@@ -198,6 +198,22 @@ def uneven_requests(n: int = 256, seed: int = 3, min_side: int = 128, max_side: 
     return out
 
 
+def serving_window(rate_per_s: float, min_previews: int = 1000, seed: int = 5, preview_fraction: float = 0.7,
+                   big_sides=(1024, 2048), big_spp=(16, 64)) -> List[ServeRequest]:
+    """The serving mix of ``serving_mix`` over a fixed arrival window: Poisson arrivals at
+    ``rate_per_s`` until ``min_previews`` preview tiles have arrived (the window is the last
+    arrival time)."""
+    out: List[ServeRequest] = []
+    n = 0
+    while sum(q.preview for q in out) < min_previews:
+        n = max(2 * n, int(min_previews / preview_fraction * 1.05) + 8)
+        out = serving_mix(n, rate_per_s, seed, preview_fraction, big_sides, big_spp)
+        if sum(q.preview for q in out) >= min_previews:
+            cut = [i for i, q in enumerate(out) if q.preview][min_previews - 1]
+            out = out[:cut + 1]
+    return out
+
+
 def serving_mix(n: int = 64, rate_per_s: float = 100.0, seed: int = 5, preview_fraction: float = 0.7,
                 big_sides=(1024, 2048), big_spp=(16, 64)) -> List[ServeRequest]:
     """Poisson arrivals at ``rate_per_s``: a ``preview_fraction`` of preview tiles (256^2, 1 spp)

@@ -206,3 +206,63 @@ def test_area_light_scene_served_equals_megakernel(ctx):
         s.release(rid)
     for a, b in zip(ref, got):
         assert np.array_equal(a, b)
+
+
+def test_poisson_serving_mix_scaled(ctx):
+    """The serving loop (Poisson arrivals, 70 % previews of 256^2 at 1 spp, 30 % progressive
+    renders), scaled to renders of 448^2-512^2 at 32-64 spp: every served result equals the same
+    request rendered alone bit for bit; previews' refined depth agrees with the FP64 oracle; and
+    a preview finishes before every progressive render that arrived earlier and still had work
+    for more than 300 ms when the preview arrived (a render in its last launches may end first:
+    up to three launches of about 30 ms are in flight)."""
+    from oracle import oracle
+    from syfrack_amd import abi, serve
+    reqs = serve.serving_mix(30, rate_per_s=100.0, seed=17, big_sides=(448, 512), big_spp=(32, 64))
+    assert 5 <= sum(not q.preview for q in reqs) <= 25
+    print("stream:", [("P" if q.preview else f"R{q.cfg.width}x{q.spp}", round(q.arrival_s, 3)) for q in reqs],
+          flush=True)
+    planes = (abi.PLANE_ACCUM, abi.PLANE_DEPTH, abi.PLANE_AUX, abi.PLANE_DEPTH64)
+    with serve.Server(ctx, "batched") as s:
+        t0 = time.perf_counter()
+        ids = []
+        for q in reqs:
+            dt = q.arrival_s - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(s.submit(q.scene, q.cfg))
+        s.wait_all(300)
+        infos = [s.poll(i) for i in ids]
+        served = [[s.plane(i, p) for p in planes] for i in ids]
+        for i in ids:
+            s.release(i)
+    assert all(i.state == "done" for i in infos)
+    with serve.Server(ctx, "plain") as s:
+        for k, q in enumerate(reqs):
+            rid = s.submit(q.scene, q.cfg)
+            s.wait(rid, 300)
+            alone = [s.plane(rid, p) for p in planes]
+            s.release(rid)
+            assert all(np.array_equal(a, b) for a, b in zip(served[k], alone)), k
+    # previews against the FP64 oracle
+    for k in [k for k, q in enumerate(reqs) if q.preview][:3]:
+        q = reqs[k]
+        twin = oracle.trace_primary(q.scene.desc(q.cfg), 0, 64)["depth64"]
+        d64 = served[k][3]
+        hg, ho = d64 >= 0, twin >= 0
+        assert np.mean(hg == ho) >= 0.999
+        both = hg & ho
+        if both.any():
+            rel = np.abs(d64[both] - twin[both]) / twin[both]
+            assert np.mean(rel <= 1e-4) >= 0.999
+    # ordering
+    pairs = ok = 0
+    for i, q in zip(infos, reqs):
+        if not q.preview:
+            continue
+        for j, r in zip(infos, reqs):
+            if r.preview or r.arrival_s >= q.arrival_s or j.done_ms <= i.submit_ms + 300.0:
+                continue
+            pairs += 1
+            ok += i.done_ms < j.done_ms
+    print(f"previews ahead of earlier renders with more than 300 ms left: {ok}/{pairs}")
+    assert pairs > 0 and ok == pairs
