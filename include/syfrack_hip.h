@@ -141,6 +141,13 @@ typedef struct syf_job_desc {
 #define SYF_PLANE_AUX 2u     /* float4 per pixel: first-hit normal, steps; (0,0,0,-1) on a miss */
 #define SYF_PLANE_DEPTH64 3u /* double per pixel: FP64-refined hit distance, -1 on a miss */
 #define SYF_PLANE_STEPS 4u   /* uint32 per pixel: march steps of the primary ray, hit or miss */
+/* The session's radiance layers (session.rs:1217-1243; accumulated per sample by
+ * pathtrace.wgsl:1243-1266).  This path has no participating medium, so the volume estimate of
+ * every sample is zero: VOLUME and MOMENTS read (0, 0, 0, samples) and SURFACE equals ACCUM;
+ * readback only (they follow from ACCUM). */
+#define SYF_PLANE_VOLUME 5u  /* float4 per pixel: volume radiance sum, samples */
+#define SYF_PLANE_MOMENTS 6u /* float4 per pixel: the volume's Welford second moments, samples */
+#define SYF_PLANE_SURFACE 7u /* float4 per pixel: surface radiance sum, samples */
 
 /* Work counters of the last launch on a job, and its device time. */
 typedef struct syf_stats {
@@ -239,6 +246,39 @@ int syf_trace_primary_noalloc(syf_ctx* ctx, const syf_job_desc* desc, uint32_t s
 /* Copy a device plane (SYF_PLANE_*) to a host buffer of exactly `bytes` bytes.
  * Replaces the readbacks of RenderSession::readback (session.rs:1217-1243). */
 int syf_readback(syf_job* job, uint32_t plane, void* host, size_t bytes);
+
+/* Copy a host buffer of exactly `bytes` bytes into a device plane (ACCUM, DEPTH, AUX, DEPTH64 or
+ * STEPS): the planes of a resumed session, as RenderSession::resume writes its saved buffers back
+ * (session.rs:1308-1330).  Synchronous. */
+int syf_upload(syf_job* job, uint32_t plane, const void* host, size_t bytes);
+
+/* ---------------------------------------------------------------- checkpoints and export
+ * The transport identity of a job: FNV-1a-64 over this library's transport version and the
+ * lowered job, the counterpart of the shader fingerprint a session stores in state.ron
+ * (shader_source.rs:117-132, session.rs:648-652).  A checkpoint resumes only into a job with the
+ * same identity. */
+int syf_job_fingerprint(syf_job* job, uint64_t* out);
+/* FNV-1a-64 of n bytes continuing from `hash` (start: 0xcbf29ce484222325), the checksum of the
+ * export payload (export.rs:1253-1257). */
+uint64_t syf_fnv1a64(const void* data, size_t n, uint64_t hash);
+/* RenderSession::save (session.rs:1199-1306): writes the planes of the job to `dir` as raw
+ * little-endian files (accum.bin, depth.bin, normals.bin, volume.bin, volume-moments.bin,
+ * surface.bin, a 48-byte volume-components.bin, and this path's depth64.bin and steps.bin) and
+ * the caller's `state_text` as state.ron, into a temporary directory renamed over `dir`; the old
+ * checkpoint is moved aside first and removed last, so one complete checkpoint always exists.
+ * A directory holding any other file is refused (SYF_E_INVALID naming it). */
+int syf_checkpoint_save(syf_job* job, const char* dir, const char* state_text);
+/* RenderSession::resume's plane half: reads the planes saved in `dir` (each must have the job's
+ * size; depth64.bin and steps.bin when present) into the job.  The caller has rebuilt the job
+ * from state.ron and compared its fingerprint. */
+int syf_checkpoint_load(syf_job* job, const char* dir);
+/* The export payload (export.rs:1226-1262): mean colour (sum / max(n, 1), alpha 1), depth and
+ * normals of the job, little-endian, to `path`; returns its byte count and FNV-1a-64 hash. */
+int syf_export_write(syf_job* job, const char* path, uint64_t* bytes, uint64_t* hash);
+/* export.rs:1263-1324: reads a payload of a width x height image, checking its byte count and
+ * hash ("invalid byte count", "checksum mismatch"); any output pointer may be NULL. */
+int syf_export_read(const char* path, uint64_t bytes, uint64_t hash, uint32_t width, uint32_t height, float* color4,
+                    float* depth, float* normals4);
 
 /* ---------------------------------------------------------------- 2D escape time
  * PlanarFractal::sample (planar.rs:77-194) / planar_sample (planar.wgsl:16-91) for the

@@ -38,6 +38,9 @@ PLANE_DEPTH = 1
 PLANE_AUX = 2
 PLANE_DEPTH64 = 3
 PLANE_STEPS = 4
+PLANE_VOLUME = 5
+PLANE_MOMENTS = 6
+PLANE_SURFACE = 7
 
 PLANAR_JULIA = 1
 PLANAR_MANDELBROT = 2
@@ -248,6 +251,13 @@ SIGNATURES = {
     "syf_render_tile": (C.c_int, [_P] + [C.c_uint32] * 6),
     "syf_render_wavefront": (C.c_int, [_P] + [C.c_uint32] * 6),
     "syf_readback": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    "syf_upload": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    "syf_job_fingerprint": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "syf_fnv1a64": (C.c_uint64, [_P, C.c_size_t, C.c_uint64]),
+    "syf_checkpoint_save": (C.c_int, [_P, C.c_char_p, C.c_char_p]),
+    "syf_checkpoint_load": (C.c_int, [_P, C.c_char_p]),
+    "syf_export_write": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "syf_export_read": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P]),
     "syf_planar_create": (C.c_int, [_P, C.POINTER(SyfPlanarDesc), C.POINTER(_P)]),
     "syf_planar_destroy": (C.c_int, [_P]),
     "syf_planar_run": (C.c_int, [_P]),

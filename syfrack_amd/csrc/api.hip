@@ -231,6 +231,9 @@ size_t plane_bytes(uint32_t plane, size_t pixels) {
     case SYF_PLANE_AUX: return pixels * 16;
     case SYF_PLANE_DEPTH64: return pixels * 8;
     case SYF_PLANE_STEPS: return pixels * 4;
+    case SYF_PLANE_VOLUME:
+    case SYF_PLANE_MOMENTS:
+    case SYF_PLANE_SURFACE: return pixels * 16;
     default: return 0;
     }
 }
@@ -469,13 +472,23 @@ int syf_trace_primary(syf_job* j, uint32_t sample) {
 int syf_readback(syf_job* j, uint32_t plane, void* host, size_t bytes) {
     clear_error();
     if (!j || !host) return fail(SYF_E_INVALID, "job and host must not be null");
-    if (plane > SYF_PLANE_STEPS) return fail(SYF_E_INVALID, "plane: unknown plane %u", plane);
+    if (plane > SYF_PLANE_SURFACE) return fail(SYF_E_INVALID, "plane: unknown plane %u", plane);
     size_t need = plane_bytes(plane, j->pixels);
     if (bytes != need) return fail(SYF_E_INVALID, "bytes must be %zu for plane %u, got %zu", need, plane, bytes);
     if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
-    hipError_t e = hipMemcpyAsync(host, j->plane[plane], bytes, hipMemcpyDeviceToHost, j->stream);
+    const uint32_t src = plane >= SYF_PLANE_VOLUME ? SYF_PLANE_ACCUM : plane;
+    hipError_t e = hipMemcpyAsync(host, j->plane[src], bytes, hipMemcpyDeviceToHost, j->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
-    return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_readback");
+    if (e != hipSuccess) return hip_fail(e, "syf_readback");
+    // The session planes of pathtrace.wgsl:1243-1266 for this path, which has no participating
+    // medium: every sample's volume estimate is zero, so VOLUME holds (0, 0, 0, n), its Welford
+    // moments max(0, delta * (v - mean)) stay (0, 0, 0, n), and SURFACE adds r - 0 = r where
+    // ACCUM adds r, in the same order: the same sums bit for bit.
+    if (plane == SYF_PLANE_VOLUME || plane == SYF_PLANE_MOMENTS) {
+        float* f = static_cast<float*>(host);
+        for (size_t i = 0; i < j->pixels; ++i) f[4 * i] = f[4 * i + 1] = f[4 * i + 2] = 0.0f;
+    }
+    return SYF_OK;
 }
 
 }  // extern "C"

@@ -113,11 +113,35 @@ class Job:
             abi.PLANE_AUX: ((h, w, 4), np.float32),
             abi.PLANE_DEPTH64: ((h, w), np.float64),
             abi.PLANE_STEPS: ((h, w), np.uint32),
+            abi.PLANE_VOLUME: ((h, w, 4), np.float32),
+            abi.PLANE_MOMENTS: ((h, w, 4), np.float32),
+            abi.PLANE_SURFACE: ((h, w, 4), np.float32),
         }
         shape, dt = shapes[plane]
         out = np.empty(shape, dt)
         abi.check(self.lib.syf_readback(self.handle, plane, _p(out), out.nbytes))
         return out
+
+    def upload(self, plane: int, data: np.ndarray) -> None:
+        data = np.ascontiguousarray(data)
+        abi.check(self.lib.syf_upload(self.handle, plane, _p(data), data.nbytes))
+
+    def fingerprint(self) -> int:
+        out = C.c_uint64()
+        abi.check(self.lib.syf_job_fingerprint(self.handle, C.byref(out)))
+        return out.value
+
+    def checkpoint_save(self, directory: str, state_text: str) -> None:
+        abi.check(self.lib.syf_checkpoint_save(self.handle, directory.encode(), state_text.encode()))
+
+    def checkpoint_load(self, directory: str) -> None:
+        abi.check(self.lib.syf_checkpoint_load(self.handle, directory.encode()))
+
+    def export_write(self, path: str):
+        """Writes the export payload; returns (bytes, FNV-1a-64 hash)."""
+        n, h = C.c_uint64(), C.c_uint64()
+        abi.check(self.lib.syf_export_write(self.handle, path.encode(), C.byref(n), C.byref(h)))
+        return n.value, h.value
 
     def probe(self, points: np.ndarray) -> DeSample:
         pts = np.zeros((len(points), 4), np.float32)
