@@ -577,7 +577,7 @@ def main() -> int:
         out["roofline"]["valu_counters"] = counters
     if not args.no_serve:
         sv = serving_bench(r.ctx, args.serve_requests)
-        sv["serving_loop"] = serving_loop_bench(r.ctx, [float(x) for x in args.loop_rates.split(",")],
+        sv["serving_loop"] = serving_loop_bench(r.ctx, [float(x) for x in args.loop_rates.split(",") if x.strip()],
                                                 args.loop_plain_window)
         out["serving"] = sv
         rates = sv["serving_loop"]["rates"]

@@ -337,10 +337,12 @@ typedef struct syf_server_config {
                                     requests arrive, 16 once they have stopped) */
     uint32_t batch_target_us;    /* target device time of one launch while requests arrive
                                     (0 = 30000 us) */
-    uint32_t grid_blocks;        /* workgroups of the persistent launch (0 = autotuned) */
+    uint32_t grid_blocks;        /* workgroups of the persistent launch (0 = autotuned per family) */
     uint32_t quiet_target_us;    /* target device time of one launch once no request has arrived
                                     for 200 ms (0 = 32 x batch_target_us; SYF_QUIET_UNBOUNDED =
                                     all remaining work in one launch) */
+    uint32_t path_steps;         /* field evaluations per lane per loop trip of the batched kernel:
+                                    4, 8 or 16 (0 = autotuned per family) */
 } syf_server_config;
 #define SYF_QUIET_UNBOUNDED 0xffffffffu
 
@@ -359,6 +361,10 @@ typedef struct syf_request_info {
     double done_ms;           /* completion of the launch holding its last sample */
 } syf_request_info;
 
+/* Kernel families of the batched kernel, the index of the per-family tuning arrays below:
+ * 0 Mandelbulb (integer power), 1 Mandelbox, 2 Menger, 3 sphere graph, 4 Mandelbulb (non-integer
+ * power), 5 Mandelbulb (power 8). */
+#define SYF_KERNEL_FAMILIES 6
 typedef struct syf_server_counters {
     uint64_t submitted, completed, cancelled, launches;
     uint64_t rays;            /* traced segments (primary + bounce + shadow) */
@@ -367,6 +373,10 @@ typedef struct syf_server_counters {
     double device_ms;         /* summed device time of the launches, HIP events */
     double wall_ms;           /* since the server was created */
     uint32_t quantum_samples, batch_target_us, grid_blocks, reserved;
+    /* the autotuner's choices (or the configuration's) per kernel family, and its time */
+    uint32_t family_grid_blocks[SYF_KERNEL_FAMILIES];
+    uint32_t family_path_steps[SYF_KERNEL_FAMILIES];
+    double autotune_ms;
 } syf_server_counters;
 
 int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** out);

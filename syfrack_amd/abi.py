@@ -191,6 +191,7 @@ class SyfServerConfig(C.Structure):
         ("batch_target_us", C.c_uint32),
         ("grid_blocks", C.c_uint32),
         ("quiet_target_us", C.c_uint32),
+        ("path_steps", C.c_uint32),
     ]
 
 
@@ -226,6 +227,9 @@ class SyfServerCounters(C.Structure):
         ("batch_target_us", C.c_uint32),
         ("grid_blocks", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("family_grid_blocks", C.c_uint32 * 6),
+        ("family_path_steps", C.c_uint32 * 6),
+        ("autotune_ms", C.c_double),
     ]
 
 

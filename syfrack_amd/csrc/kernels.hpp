@@ -74,9 +74,11 @@ struct BatchPlanes {
 // refine bracket in their DEPTH64 plane; launch_refine_first_hits over the same items replaces
 // it by the FP64 hit.
 constexpr uint32_t CLAIM_WORDS = 8 * 16;
+// `steps`: field evaluations per lane per trip of the kernel's loop (4, 8 or 16; other values
+// round up to the next of these).
 hipError_t launch_paths(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
                         uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t* exit_count,
-                        unsigned long long* host_out, uint32_t grid_blocks, hipStream_t s);
+                        unsigned long long* host_out, uint32_t grid_blocks, uint32_t steps, hipStream_t s);
 // Writes slot `slot` of the batched launches' job and plane tables (one small kernel: it runs on
 // the stream of the launch that first reads the slot).
 hipError_t launch_admit(JobP* jobs, BatchPlanes* planes, uint32_t slot, const JobP& P, const BatchPlanes& pl,

@@ -36,10 +36,11 @@ namespace {
 // 1024^2 16-spp requests (Mrays/s, Mandelbulb / Mandelbox / Menger): 1 281 / 257 / 1064,
 // 4 285 / 259 / 1047, 8 280 / 257 / 1036, 16 268 / 248 / 1000.
 constexpr uint32_t PATH_REFILL = 4;
-// Field evaluations (with their march steps) a lane runs per trip of the loop: the loop's own
-// bookkeeping is paid once per PATH_STEPS evaluations, at the cost of lanes whose march ends
-// early in the trip idling for the rest of it (measured: 8 beats 1 by 1.4x, 16 gains nothing).
-constexpr uint32_t PATH_STEPS = 8;
+// Field evaluations (with their march steps) a lane runs per trip of the loop, STEPS (4, 8 or
+// 16, chosen per kernel family by the server's autotuner): the loop's own bookkeeping is paid
+// once per STEPS evaluations, at the cost of lanes whose march ends early in the trip idling
+// for the rest of it (measured on single requests: 8 beats 1 by 1.4x; 4 and 16 are each best
+// on some families).  The planes do not depend on it.
 
 // The per-slot job table is read through the constant address space: its loads are then known
 // to be invariant, so the wave-uniform constants of the request being served are loaded with
@@ -258,7 +259,7 @@ template <uint32_t FAM> __device__ __forceinline__ void shade_surface(const JobP
 // unconstrained 2-wave allocation).
 constexpr int PATHS_WAVES = 3;
 
-template <uint32_t FAM>
+template <uint32_t FAM, uint32_t STEPS>
 __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restrict__ jobs, const BatchPlanes* __restrict__ planes,
                                                const BatchItem* __restrict__ items, uint32_t n_items,
                                                uint32_t* __restrict__ claim, unsigned long long* __restrict__ ctr,
@@ -336,13 +337,13 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
         }
         if (__ballot(L.mode != L_IDLE) == 0ull && pool_next >= 64u && !have_pending && exhausted) break;
         if (L.mode == L_IDLE) continue;
-        // ---- evaluation: up to PATH_STEPS field evaluations per evaluating lane, each
+        // ---- evaluation: up to STEPS field evaluations per evaluating lane, each
         // followed by its march step or normal tap; a lane stops early when its march or
         // normal ends
         if (L.mode <= L_NORMAL) {
             const JobP& P = job_of(jobs, slot);
             const FractalP& F = P.f;
-            for (uint32_t k = 0; k < PATH_STEPS && L.mode <= L_NORMAL; ++k) {
+            for (uint32_t k = 0; k < STEPS && L.mode <= L_NORMAL; ++k) {
             while (!L.de_done) L.de_done = L.de.step(F);
             {
                 const De<float> s = L.de.finish(F);
@@ -541,21 +542,34 @@ hipError_t launch_admit(JobP* jobs, BatchPlanes* planes, uint32_t slot, const Jo
     return hipGetLastError();
 }
 
+template <uint32_t STEPS>
+static void launch_paths_steps(uint32_t fam, dim3 grid, const JobP* jobs, const BatchPlanes* planes,
+                               const BatchItem* items, uint32_t n_items, uint32_t* claim, unsigned long long* ctr,
+                               uint32_t* exit_count, unsigned long long* host_out, hipStream_t s) {
+    const dim3 block(256);
+    switch (fam) {
+    case SYF_FAMILY_MANDELBULB: hipLaunchKernelGGL((k_paths<SYF_FAMILY_MANDELBULB, STEPS>), grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    case SYF_FAMILY_MANDELBOX: hipLaunchKernelGGL((k_paths<SYF_FAMILY_MANDELBOX, STEPS>), grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    case SYF_FAMILY_MENGER: hipLaunchKernelGGL((k_paths<SYF_FAMILY_MENGER, STEPS>), grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    case FAM_BULB_TRIG: hipLaunchKernelGGL((k_paths<FAM_BULB_TRIG, STEPS>), grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    case FAM_BULB8: hipLaunchKernelGGL((k_paths<FAM_BULB8, STEPS>), grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    default: hipLaunchKernelGGL((k_paths<SYF_FAMILY_SPHERE, STEPS>), grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
+    }
+}
+
 hipError_t launch_paths(uint32_t fam, const JobP* jobs, const BatchPlanes* planes, const BatchItem* items,
                         uint32_t n_items, uint32_t* claim, unsigned long long* ctr, uint32_t* exit_count,
-                        unsigned long long* host_out, uint32_t grid_blocks, hipStream_t s) {
+                        unsigned long long* host_out, uint32_t grid_blocks, uint32_t steps, hipStream_t s) {
     if (n_items == 0) return hipSuccess;
     if (fam >= FAM_COUNT) return hipErrorInvalidValue;
     const uint32_t waves = (n_items + 7u) / 8u * 8u;
-    const dim3 grid(std::max<uint32_t>(8u, std::min<uint32_t>(grid_blocks, (waves + 3u) / 4u))), block(256);
-    switch (fam) {
-    case SYF_FAMILY_MANDELBULB: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MANDELBULB>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
-    case SYF_FAMILY_MANDELBOX: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MANDELBOX>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
-    case SYF_FAMILY_MENGER: hipLaunchKernelGGL(k_paths<SYF_FAMILY_MENGER>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
-    case FAM_BULB_TRIG: hipLaunchKernelGGL(k_paths<FAM_BULB_TRIG>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
-    case FAM_BULB8: hipLaunchKernelGGL(k_paths<FAM_BULB8>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
-    default: hipLaunchKernelGGL(k_paths<SYF_FAMILY_SPHERE>, grid, block, 0, s, jobs, planes, items, n_items, claim, ctr, exit_count, host_out); break;
-    }
+    const dim3 grid(std::max<uint32_t>(8u, std::min<uint32_t>(grid_blocks, (waves + 3u) / 4u)));
+    if (steps <= 4u)
+        launch_paths_steps<4>(fam, grid, jobs, planes, items, n_items, claim, ctr, exit_count, host_out, s);
+    else if (steps <= 8u)
+        launch_paths_steps<8>(fam, grid, jobs, planes, items, n_items, claim, ctr, exit_count, host_out, s);
+    else
+        launch_paths_steps<16>(fam, grid, jobs, planes, items, n_items, claim, ctr, exit_count, host_out, s);
     return hipGetLastError();
 }
 

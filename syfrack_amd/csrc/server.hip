@@ -145,6 +145,10 @@ struct syf_server {
     unsigned long long* d_pctr = nullptr;  // counters of plain-mode launches
     hipEvent_t ev_pool[2 * NBUF] = {};
     uint32_t quantum = 0, quantum_long = 0, target_us = 0, quiet_us = 0, grid = 0;
+    // per kernel family: persistent workgroups and field evaluations per loop trip of k_paths
+    // (the autotuner's choices, or the configuration's)
+    uint32_t fam_grid[FAM_COUNT] = {0}, fam_steps[FAM_COUNT] = {0};
+    double autotune_ms = 0.0;
     double last_submit_ms = -1e30;  // host clock of the latest submission
     double gap_ms = 0.0;            // running mean of the gaps between submissions (below 5 s)
     uint32_t fill_items = 0;  // work items (one wave each) a launch needs to fill the device
@@ -358,7 +362,8 @@ bool plan_launch(syf_server* s) {
         if (count[f]) {
             e = launch_paths(f, s->d_jobs, s->d_planes, s->h_items[buf] + first[f], count[f],
                              s->d_claim[buf] + f * CLAIM_WORDS, s->d_ctr[buf] + f * CTR_COUNT, s->d_exit[buf] + f * 16u,
-                             s->h_out[buf] + f * CTR_COUNT, s->grid, ls);
+                             s->h_out[buf] + f * CTR_COUNT, latency ? s->fam_grid[f] : s->grid, s->fam_steps[f],
+                             ls);
             if (e == hipSuccess && first_quantum[f])
                 e = launch_refine_first_hits(f, s->d_jobs, s->d_planes, s->h_items[buf] + first[f], count[f],
                                              s->d_rctr, ls);
@@ -532,26 +537,59 @@ void worker_main(syf_server* s) {
     }
 }
 
-// Autotuner: times a fixed calibration workload (a 768x768 Mandelbulb job, 4 samples, 2
-// bounces, in 8x8-tile items: enough to fill the device) for each candidate size of the
-// persistent grid (1, 2 or 3 workgroups per CU), keeps the fastest and seeds the cost model
-// (ns per pixel-sample-segment) with its measured rate.  The sample quantum is not tuned here
-// (a fixed default, measured on both families of streams; requests that fit in a launch go in
-// one piece whatever it is).
-void autotune(syf_server* s) {
+// The calibration scene of a kernel family: 512x512, 4 samples, 2 bounces under a white sky, the
+// family's bundled graph (Mandelbulb powers 5, 8 and 6.5 for the three Mandelbulb kernels) seen
+// as the serving mix's random scenes see it.
+syf_job_desc calibration_desc(uint32_t kfam) {
     syf_job_desc d;
     std::memset(&d, 0, sizeof d);
-    d.fractal.family = SYF_FAMILY_MANDELBULB;
-    d.fractal.iterations = 10;
-    d.fractal.bailout = 2.0f;
-    d.fractal.power = 8.0f;
-    d.fractal.trap_scale = 4.0f;
-    d.fractal.safety = 0.7f;
-    d.fractal.bounding_radius_hint = 2.5f;
-    d.fractal.plane_normal[1] = 1.0f;
-    d.camera.mode = SYF_CAMERA_FREE;
-    d.camera.position[2] = -2.5f;
-    d.camera.focal_mm = 30.0f;
+    syf_fractal& f = d.fractal;
+    f.plane_normal[1] = 1.0f;
+    float dist = 3.2f;
+    if (kfam == SYF_FAMILY_MANDELBOX) {
+        f.family = SYF_FAMILY_MANDELBOX;
+        f.iterations = 12;
+        f.bailout = 1000.0f;
+        f.scale = 2.0f;
+        f.min_radius = 0.5f;
+        f.fixed_radius = 1.0f;
+        f.fold_limit = 1.0f;
+        f.trap_scale = 2.0f;
+        f.safety = 0.6f;
+        f.bounding_radius_hint = 10.392304f;
+        dist = 11.0f;
+    } else if (kfam == SYF_FAMILY_MENGER) {
+        f.family = SYF_FAMILY_MENGER;
+        f.iterations = 5;
+        f.bailout = 1e6f;
+        f.scale = 3.0f;
+        for (int k = 0; k < 3; ++k) f.offset[k] = f.half[k] = 1.0f;
+        f.trap_scale = 4.0f;
+        f.safety = 0.9f;
+        f.bounding_radius_hint = 4.464102f;
+        dist = 4.2f;
+    } else if (kfam == SYF_FAMILY_SPHERE) {
+        f.family = SYF_FAMILY_SPHERE;
+        f.radius = 1.0f;
+        f.plane_offset = -1.0f;
+        f.trap_scale = 1.0f;
+        f.safety = 1.0f;
+        f.bounding_radius_hint = 100.0f;
+        dist = 3.0f;
+    } else {
+        f.family = SYF_FAMILY_MANDELBULB;
+        f.iterations = 10;
+        f.bailout = 4.0f;
+        f.power = kfam == FAM_BULB8 ? 8.0f : kfam == FAM_BULB_TRIG ? 6.5f : 5.0f;
+        f.trap_scale = 4.0f;
+        f.safety = 0.7f;
+        f.bounding_radius_hint = 2.5f;
+    }
+    d.camera.mode = SYF_CAMERA_ORBIT;
+    d.camera.distance = dist;
+    d.camera.azimuth = 30.0f;
+    d.camera.elevation = 20.0f;
+    d.camera.focal_mm = 35.0f;
     d.material.albedo[0] = d.material.albedo[1] = d.material.albedo[2] = 0.8f;
     d.material.roughness = 0.4f;
     d.material.specular = 0.04f;
@@ -559,14 +597,33 @@ void autotune(syf_server* s) {
     d.lights[0].kind = SYF_LIGHT_ENV;
     for (int i = 0; i < 3; ++i) d.lights[0].zenith[i] = d.lights[0].horizon[i] = d.lights[0].ground[i] = 1.0f;
     d.lights[0].intensity = 1.0f;
-    d.width = d.height = 768;
+    d.width = d.height = 512;
     d.max_bounces = 2;
     d.max_steps = 256;
     d.pixel_tolerance = 1.0f;
     d.seed = 1;
-    JobP P;
-    if (lower_job(d, P) != SYF_OK) return;
-    const size_t pixels = size_t(d.width) * d.height;
+    return d;
+}
+
+// Autotuner.  For each kernel family it times its calibration scene (8x8-tile items holding all
+// 4 samples: 4096 items, the least a launch holds on a 256-CU device, in one persistent launch,
+// the better of two) at 4, 8 and 16 field evaluations per loop trip on the default grid of 3
+// workgroups per CU, then on 2 workgroups per CU with the best step count.  The grid it picks
+// serves latency-mode launches, which are of that size; quiet-mode launches, many times longer,
+// keep 3 workgroups per CU (with the tuned grid they took the uneven batch from 535 to 495
+// Mrays/s, while the 200/s serving loop gained 350 -> 363).  It leaves the
+// default (3 per CU, 8 evaluations) only for a choice at least 3 % faster: a first version on a
+// 384^2, 2-sample scene (fewer items than the grid has waves) picked 1 workgroup per CU for
+// some families and took the uneven 256-request batch from 535 to 392 Mrays/s.  A choice the
+// configuration fixes (grid_blocks, path_steps) is not tuned.  The cost model (ns per
+// pixel-sample-segment) is seeded with the power-8 Mandelbulb's best rate.  None of the
+// choices changes a served image: they only schedule the same per-pixel arithmetic.
+void autotune(syf_server* s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t cu = uint32_t(s->ctx->cu_count > 0 ? s->ctx->cu_count : 256);
+    const uint32_t W = 512, H = 512, SPP = 4;
+    const size_t pixels = size_t(W) * H;
+    const uint32_t tiles_x = W / TILE, tiles = tiles_x * (H / TILE);
     float4 *accum = nullptr, *aux = nullptr;
     float* depth = nullptr;
     double* depth64 = nullptr;
@@ -577,52 +634,59 @@ void autotune(syf_server* s) {
     uint32_t* exit_count = nullptr;
     unsigned long long* ctr = nullptr;
     unsigned long long* host_out = nullptr;
-    const uint32_t tiles_x = d.width / TILE, tiles = tiles_x * (d.height / TILE), spp = 4;
-    hipEvent_t a, b;
+    hipEvent_t a = nullptr, b = nullptr;
     bool ok = hipMalloc(&accum, pixels * 16) == hipSuccess && hipMalloc(&aux, pixels * 16) == hipSuccess &&
               hipMalloc(&depth, pixels * 4) == hipSuccess && hipMalloc(&depth64, pixels * 8) == hipSuccess &&
-              hipMalloc(&djob, sizeof(JobP)) == hipSuccess &&
-              hipMalloc(&dpl, sizeof(BatchPlanes)) == hipSuccess &&
-              hipMalloc(&ditems, size_t(tiles) * spp * sizeof(BatchItem)) == hipSuccess &&
+              hipMalloc(&djob, sizeof(JobP)) == hipSuccess && hipMalloc(&dpl, sizeof(BatchPlanes)) == hipSuccess &&
+              hipMalloc(&ditems, size_t(tiles) * sizeof(BatchItem)) == hipSuccess &&
               hipMalloc(&claim, CLAIM_WORDS * 4) == hipSuccess && hipMalloc(&ctr, CTR_COUNT * 8) == hipSuccess &&
               hipMalloc(&exit_count, 16 * 4) == hipSuccess &&
               hipHostMalloc(&host_out, CTR_COUNT * 8, hipHostMallocCoherent) == hipSuccess &&
               hipMemset(claim, 0, CLAIM_WORDS * 4) == hipSuccess && hipMemset(ctr, 0, CTR_COUNT * 8) == hipSuccess &&
-              hipMemset(exit_count, 0, 16 * 4) == hipSuccess &&
-              hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess;
+              hipMemset(exit_count, 0, 16 * 4) == hipSuccess && hipEventCreate(&a) == hipSuccess &&
+              hipEventCreate(&b) == hipSuccess;
     BatchPlanes pl{accum, depth, aux, depth64};
-    ok = ok && hipMemcpy(djob, &P, sizeof P, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(dpl, &pl, sizeof pl, hipMemcpyHostToDevice) == hipSuccess;
-    const uint32_t cu = uint32_t(s->ctx->cu_count > 0 ? s->ctx->cu_count : 256);
-    // every candidate grid is resident at once (k_paths holds 3 waves per SIMD: 3 workgroups
-    // per CU); a larger grid leaves workgroups waiting for a CU until the launch's end, which
-    // keeps the next launch off the CUs its tail frees
-    const uint32_t g_cands[3] = {cu * 1, cu * 2, cu * 3};
-    double best = 1e30;
-    uint32_t bg = cu * 3;
-    double best_units_ns = 0.0;
-    // every tile with all its samples in one item: one launch per candidate and repetition
+    ok = ok && hipMemcpy(dpl, &pl, sizeof pl, hipMemcpyHostToDevice) == hipSuccess;
     std::vector<BatchItem> items;
-    for (uint32_t t = 0; t < tiles; ++t) items.push_back(BatchItem{0, (t % tiles_x) * TILE, (t / tiles_x) * TILE, 0, spp});
+    for (uint32_t t = 0; t < tiles; ++t) items.push_back(BatchItem{0, (t % tiles_x) * TILE, (t / tiles_x) * TILE, 0, SPP});
     ok = ok && hipMemcpy(ditems, items.data(), size_t(tiles) * sizeof(BatchItem), hipMemcpyHostToDevice) == hipSuccess;
-    for (uint32_t gi = 0; ok && gi < 3; ++gi) {
-        float ms = 0.0f;
-        for (int rep = 0; rep < 2 && ok; ++rep) {  // the first repetition warms up
-            // the claim and work counters and the exit count are left zero by each launch
+    // device time of one launch of family f's calibration (better of two; the claim and work
+    // counters and the exit count are left zero by each launch)
+    auto time_launch = [&](uint32_t f, uint32_t grid, uint32_t steps) -> double {
+        float best = 1e30f;
+        for (int rep = 0; rep < 2 && ok; ++rep) {
+            float ms = 0.0f;
             ok = hipEventRecord(a, s->stream) == hipSuccess &&
-                 launch_paths(kernel_family(P.f), djob, dpl, ditems, tiles, claim, ctr, exit_count, host_out,
-                              g_cands[gi], s->stream) == hipSuccess &&
+                 launch_paths(f, djob, dpl, ditems, tiles, claim, ctr, exit_count, host_out, grid, steps, s->stream) ==
+                     hipSuccess &&
                  hipEventRecord(b, s->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
                  hipEventElapsedTime(&ms, a, b) == hipSuccess;
+            best = std::min(best, ms);
         }
-        if (ok && ms < best) {
-            best = ms;
-            bg = g_cands[gi];
-            best_units_ns = double(ms) * 1e6 / (double(pixels) * spp * (d.max_bounces + 1));
+        return ok ? double(best) : 1e30;
+    };
+    constexpr double MARGIN = 0.97;  // a challenger must be 3 % faster than the incumbent
+    for (uint32_t f = 0; ok && f < FAM_COUNT; ++f) {
+        JobP P;
+        const syf_job_desc d = calibration_desc(f);
+        if (lower_job(d, P) != SYF_OK || kernel_family(P.f) != f) continue;
+        ok = hipMemcpy(djob, &P, sizeof P, hipMemcpyHostToDevice) == hipSuccess;
+        uint32_t steps = s->fam_steps[f], grid = s->fam_grid[f];
+        double best = time_launch(f, grid, steps);  // the incumbent: the defaults or the fixed values
+        if (s->cfg.path_steps == 0)
+            for (uint32_t k : {4u, 16u}) {
+                const double ms = time_launch(f, grid, k);
+                if (ms < best * MARGIN) best = ms, steps = k;
+            }
+        if (s->cfg.grid_blocks == 0) {
+            const double ms = time_launch(f, cu * 2, steps);
+            if (ms < best * MARGIN) best = ms, grid = cu * 2;
         }
+        s->fam_steps[f] = steps;
+        s->fam_grid[f] = grid;
+        if (f == FAM_BULB8 && best < 1e29)
+            s->ns_per_unit = best * 1e6 / (double(pixels) * SPP * (d.max_bounces + 1));
     }
-    if (s->cfg.grid_blocks == 0) s->grid = bg;
-    if (best_units_ns > 0) s->ns_per_unit = best_units_ns;
     hipFree(accum);
     hipFree(aux);
     hipFree(depth);
@@ -634,8 +698,9 @@ void autotune(syf_server* s) {
     hipFree(ctr);
     hipFree(exit_count);
     hipHostFree(host_out);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    s->autotune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 Request* find(syf_server* s, uint64_t id) {
@@ -652,6 +717,8 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     if (!ctx || !cfg || !out) return fail(SYF_E_INVALID, "ctx, cfg and out must not be null");
     if (cfg->mode > SYF_SERVE_PLAIN) return fail(SYF_E_INVALID, "mode: unknown serving mode %u", cfg->mode);
     if (cfg->quantum_samples > 4096) return fail(SYF_E_INVALID, "quantum_samples must be at most 4096");
+    if (cfg->path_steps != 0 && cfg->path_steps != 4 && cfg->path_steps != 8 && cfg->path_steps != 16)
+        return fail(SYF_E_INVALID, "path_steps must be 0 (autotuned), 4, 8 or 16, got %u", cfg->path_steps);
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     syf_server* s = new syf_server();
@@ -672,6 +739,10 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     // a quantum the caller chose holds in both modes
     s->quantum_long = cfg->quantum_samples ? cfg->quantum_samples : QUIET_QUANTUM;
     s->grid = cfg->grid_blocks ? cfg->grid_blocks : cu * 3u;
+    for (uint32_t f = 0; f < FAM_COUNT; ++f) {
+        s->fam_grid[f] = s->grid;
+        s->fam_steps[f] = cfg->path_steps ? cfg->path_steps : 8u;
+    }
     // each launch ends with a tail as long as its slowest pixel's quantum: launches of a few
     // tens of ms keep that below a few per cent, and a preview waits at most two of them
     s->target_us = cfg->batch_target_us ? cfg->batch_target_us : 30000u;
@@ -716,7 +787,7 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
             syf_server_destroy(s);
             return hip_fail(e, "server streams");
         }
-    if (s->cfg.mode == SYF_SERVE_BATCHED && cfg->grid_blocks == 0) autotune(s);
+    if (s->cfg.mode == SYF_SERVE_BATCHED && (cfg->grid_blocks == 0 || cfg->path_steps == 0)) autotune(s);
     s->worker = std::thread(worker_main, s);
     *out = s;
     return SYF_OK;
@@ -936,7 +1007,12 @@ int syf_server_stats(syf_server* s, syf_server_counters* out) {
     out->wall_ms = s->now_ms();
     out->quantum_samples = s->quantum;
     out->batch_target_us = s->target_us;
-    out->grid_blocks = s->grid;
+    out->grid_blocks = s->fam_grid[FAM_BULB8];
+    for (uint32_t f = 0; f < FAM_COUNT && f < SYF_KERNEL_FAMILIES; ++f) {
+        out->family_grid_blocks[f] = s->fam_grid[f];
+        out->family_path_steps[f] = s->fam_steps[f];
+    }
+    out->autotune_ms = s->autotune_ms;
     return SYF_OK;
 }
 

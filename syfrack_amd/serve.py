@@ -57,7 +57,7 @@ class Server:
 
     def __init__(self, ctx: Context, mode: str = "batched", hbm_budget_bytes: int = 0, max_active: int = 0,
                  quantum_samples: int = 0, batch_target_us: int = 0, grid_blocks: int = 0,
-                 quiet_target_us: int = 0):
+                 quiet_target_us: int = 0, path_steps: int = 0):
         if mode not in ("batched", "plain"):
             raise ValueError(f"mode must be 'batched' or 'plain', got {mode!r}")
         self.ctx = ctx
@@ -70,6 +70,7 @@ class Server:
         cfg.batch_target_us = batch_target_us
         cfg.grid_blocks = grid_blocks
         cfg.quiet_target_us = quiet_target_us
+        cfg.path_steps = path_steps
         h = C.c_void_p()
         abi.check(self.lib.syf_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
         self.handle = h
@@ -139,7 +140,10 @@ class Server:
     def stats(self) -> dict:
         c = abi.SyfServerCounters()
         abi.check(self.lib.syf_server_stats(self.handle, C.byref(c)))
-        return {name: getattr(c, name) for name, _ in c._fields_ if name != "reserved"}
+        out = {name: getattr(c, name) for name, _ in c._fields_ if name != "reserved"}
+        out["family_grid_blocks"] = list(c.family_grid_blocks)
+        out["family_path_steps"] = list(c.family_path_steps)
+        return out
 
 
 # ---------------------------------------------------------------- request streams

@@ -266,3 +266,37 @@ def test_poisson_serving_mix_scaled(ctx):
             ok += i.done_ms < j.done_ms
     print(f"previews ahead of earlier renders with more than 300 ms left: {ok}/{pairs}")
     assert pairs > 0 and ok == pairs
+
+
+def test_autotuner_choices_leave_images_unchanged(ctx):
+    """The autotuner picks, per kernel family, the persistent grid (1-3 workgroups per CU) and
+    the field evaluations per loop trip (4, 8, 16) of the batched kernel; every choice only
+    schedules the same per-pixel arithmetic, so forced choices give the same planes bit for bit."""
+    from syfrack_amd import abi, serve
+    reqs = [q for q in serve.uneven_requests(40, seed=13, min_side=64, max_side=160)]
+    seen, pick = set(), []
+    for q in reqs:
+        fam = (q.scene.fractal.family, q.scene.fractal.power == 8.0)
+        if fam not in seen:
+            seen.add(fam)
+            pick.append(q)
+    planes = (abi.PLANE_ACCUM, abi.PLANE_DEPTH, abi.PLANE_AUX, abi.PLANE_DEPTH64)
+
+    def serve_all(**kw):
+        with serve.Server(ctx, "batched", **kw) as s:
+            ids = [s.submit(q.scene, q.cfg) for q in pick]
+            s.wait_all(300)
+            out = [[s.plane(i, p) for p in planes] for i in ids]
+            st = s.stats()
+        return out, st
+
+    tuned, st = serve_all()
+    print("autotuner:", st["family_grid_blocks"], st["family_path_steps"], f"{st['autotune_ms']:.0f} ms")
+    assert st["autotune_ms"] > 0
+    assert all(k in (4, 8, 16) for k in st["family_path_steps"])
+    assert all(g > 0 and g % 256 == 0 and g // 256 in (2, 3) for g in st["family_grid_blocks"])
+    for kw in (dict(path_steps=4, grid_blocks=256), dict(path_steps=16, grid_blocks=768)):
+        forced, st2 = serve_all(**kw)
+        assert st2["family_path_steps"] == [kw["path_steps"]] * 6 and st2["autotune_ms"] == 0
+        for a, b in zip(tuned, forced):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))
