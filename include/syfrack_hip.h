@@ -45,6 +45,7 @@ extern "C" {
 #define SYF_FAMILY_MANDELBOX 1u  /* Iterate{BoxFold, Trap, SphereFold, Scale, AddOrigin} -> NormFence */
 #define SYF_FAMILY_MENGER 2u     /* Iterate{MengerFold, Trap} -> Box */
 #define SYF_FAMILY_SPHERE 3u     /* Trap -> Sphere  U  Plane */
+#define SYF_FAMILY_GRAPH 4u      /* any node graph, given to syf_job_create_graph */
 
 typedef struct syf_fractal {
     uint32_t family;          /* SYF_FAMILY_* */
@@ -108,6 +109,98 @@ typedef struct syf_material {
     float surface_roughness[2];    /* roughness at t = 0 and t = 1 */
     float surface_metallic[2];     /* metallic at t = 0 and t = 1 */
 } syf_material;
+
+/* ---------------------------------------------------------------- node graphs
+ * A distance field as a node graph (scene/graph/mod.rs:103-257, 427-451): nodes, links between
+ * their ports, the Iterate bodies, the graph's safety and bounding-radius hint.  Node kinds are
+ * numbered in the order of NodeKind; a node's parameters sit in `v` (floats) and `u`
+ * (integers) as listed per kind.  The library validates the graph (a kind this path cannot
+ * evaluate, a cycle or a dangling body id is refused with SYF_E_INVALID naming the node),
+ * orders and compiles it into a register program for the device, the counterpart of
+ * graph_wgsl::compile (graph_wgsl.rs:683-949), and derives its step scale (eval.rs:828-881),
+ * trap scales and iteration product. */
+#define SYF_NODE_INPUT 0u
+#define SYF_NODE_OUTPUT 1u
+#define SYF_NODE_TRANSLATE 2u          /* v[0..3] offset */
+#define SYF_NODE_ROTATE 3u             /* v[0..3] angles (degrees, XYZ Euler) */
+#define SYF_NODE_SCALE 4u              /* v[0] factor */
+#define SYF_NODE_SCALE_AXES 5u         /* v[0..3] factors */
+#define SYF_NODE_MIRROR 6u             /* v[0..3] normal, v[3] offset */
+#define SYF_NODE_BOX_FOLD 7u           /* v[0] limit */
+#define SYF_NODE_BOX_FOLD_AXES 8u      /* v[0..3] limit; flags bits 0-2: x, y, z */
+#define SYF_NODE_SPHERE_INVERSION 9u   /* v[0] radius, v[1] min_radius */
+#define SYF_NODE_SPHERE_FOLD 10u       /* v[0] min_radius, v[1] fixed_radius */
+#define SYF_NODE_ABS 11u               /* flags bits 0-2: x, y, z */
+#define SYF_NODE_MENGER_FOLD 12u       /* v[0] scale, v[1..4] offset */
+#define SYF_NODE_KALEIDO 13u           /* u[0] planes, v[0] phase (degrees) */
+#define SYF_NODE_POWER 14u             /* v[0] power */
+#define SYF_NODE_ADD_ORIGIN 15u
+#define SYF_NODE_ADD_SEED 16u          /* v[0..3] multiplier */
+#define SYF_NODE_ADD_CONSTANT 17u      /* v[0..3] multiplier, v[3..6] julia */
+#define SYF_NODE_TWIST 18u             /* v[0] rate */
+#define SYF_NODE_BEND 19u              /* v[0] rate */
+#define SYF_NODE_TILE 20u              /* v[0..3] period */
+#define SYF_NODE_REPEAT_FINITE 21u     /* v[0..3] period, u[0..3] count */
+#define SYF_NODE_ITERATE 22u           /* u[0] iterations, v[0] bailout, body_first/body_count */
+#define SYF_NODE_TRAP 23u              /* u[0] slot, u[1] measure (SYF_MEASURE_*), u[2] mode, v[0] scale */
+#define SYF_NODE_SPHERE 24u            /* v[0] radius */
+#define SYF_NODE_TORUS 25u             /* v[0] radius, v[1] tube */
+#define SYF_NODE_CYLINDER 26u          /* v[0] radius, v[1] half_height */
+#define SYF_NODE_BOX 27u               /* v[0..3] half */
+#define SYF_NODE_ROUND_BOX 28u         /* v[0..3] half, v[3] radius */
+#define SYF_NODE_ROCK 29u              /* v[0..3] half, v[3] erosion, u[0] seed */
+#define SYF_NODE_PLANE 30u             /* v[0..3] normal, v[3] offset */
+#define SYF_NODE_NORM_FENCE 31u        /* v[0] inset */
+#define SYF_NODE_BULB_FENCE 32u
+#define SYF_NODE_MATERIAL_TAG 33u      /* u[0] index */
+#define SYF_NODE_BOUNDED 34u           /* v[0..3] half, v[3] margin */
+#define SYF_NODE_INSTANCES 35u         /* not evaluated on this path: refused */
+#define SYF_NODE_UNION 36u
+#define SYF_NODE_INTERSECTION 37u
+#define SYF_NODE_SUBTRACTION 38u
+#define SYF_NODE_SMOOTH_UNION 39u      /* v[0] k */
+#define SYF_NODE_SMOOTH_INTERSECTION 40u /* v[0] k */
+#define SYF_NODE_KINDS 41u
+/* Trap measures and modes (Measure, TrapMode: scene/graph/mod.rs:81-100). */
+#define SYF_MEASURE_NORM 0u
+#define SYF_MEASURE_AXIS_X 1u
+#define SYF_MEASURE_AXIS_Y 2u
+#define SYF_MEASURE_AXIS_Z 3u
+#define SYF_MEASURE_PLANE_Y 4u
+#define SYF_MEASURE_ANGLE_XY 5u
+#define SYF_MEASURE_ANGLE_XZ 6u
+#define SYF_MEASURE_ANGLE_YZ 7u
+#define SYF_MEASURE_LOG_NORM 8u
+#define SYF_TRAP_MINIMUM 0u
+#define SYF_TRAP_MAXIMUM 1u
+#define SYF_TRAP_LAST 2u
+
+typedef struct syf_graph_node {
+    uint32_t id;              /* NodeId, unique in the graph */
+    uint32_t kind;            /* SYF_NODE_* */
+    uint32_t enabled;         /* Node.enabled: 0 = bypassed (scene/graph/mod.rs:432) */
+    uint32_t flags;           /* Abs, BoxFoldAxes: bits 0, 1, 2 = x, y, z */
+    float v[6];               /* float parameters (per kind, above) */
+    uint32_t u[3];            /* integer parameters (per kind, above) */
+    uint32_t body_first;      /* Iterate: body ids are bodies[body_first .. body_first + body_count) */
+    uint32_t body_count;
+} syf_graph_node;
+
+typedef struct syf_graph_link {  /* Link (scene/graph/mod.rs:436-440): output port -> input port */
+    uint32_t from_node, from_port, to_node, to_port;
+} syf_graph_link;
+
+typedef struct syf_graph {
+    const syf_graph_node* nodes;
+    uint32_t node_count;
+    uint32_t link_count;
+    const syf_graph_link* links;
+    const uint32_t* bodies;   /* the Iterate bodies, concatenated */
+    uint32_t body_len;
+    float safety;             /* Graph.safety */
+    float bounding_radius_hint;
+    uint32_t reserved;
+} syf_graph;
 
 /* One render job: the scene plus RenderConfig (renderer.rs:17-43) fields. */
 typedef struct syf_job_desc {
@@ -192,6 +285,17 @@ int syf_ctx_destroy(syf_ctx* ctx);
  * RenderSession::build (session.rs:187-219). */
 int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out);
 int syf_job_destroy(syf_job* job);
+/* A render job whose distance field is a node graph: desc->fractal.family must be
+ * SYF_FAMILY_GRAPH (its other fractal fields are unused; the graph carries safety and bounding
+ * hint).  Every job entry point then evaluates the graph (probe, primary trace, megakernel and
+ * wavefront path tracing); the batched server takes classic families only.  Replaces the
+ * per-topology pipeline of the reference (graph_wgsl::compile, pipeline_cache.rs) for this
+ * path. */
+int syf_job_create_graph(syf_ctx* ctx, const syf_job_desc* desc, const syf_graph* graph, syf_job** out);
+/* Derived properties of a graph without a job: the step scale (eval.rs:879-881), the iteration
+ * product (validate.rs:112-137) and the trap normalisation scales (eval.rs:853-877).  Validates
+ * like syf_job_create_graph. */
+int syf_graph_info(const syf_graph* graph, float* step_scale, uint32_t* iteration_product, float* trap_scales4);
 /* Zero the accumulation and first-hit planes. */
 int syf_job_clear(syf_job* job);
 /* Wait for every launch queued on the job's stream. */

@@ -222,3 +222,64 @@ def planar_f32(desc, rect=None):
     lib.orc_planar_f32.argtypes = [C.c_void_p] + [C.c_uint32] * 4 + [C.c_void_p, C.c_void_p]
     lib.orc_planar_f32(C.byref(desc), x0, y0, w, h, _ptr(iters), _ptr(smooth))
     return iters.reshape(h, w), smooth.reshape(h, w)
+
+
+# ---------------------------------------------------------------- node graphs
+
+def graph_use(graph) -> None:
+    """Loads a graph (syfrack_amd.graph.Graph) for graph_probe and for jobs of the graph family."""
+    lib = load()
+    lib.orc_graph_use.restype = C.c_int
+    lib.orc_graph_use.argtypes = [C.c_void_p]
+    cg = graph.c_struct()
+    if lib.orc_graph_use(C.addressof(cg.s)) != 0:
+        raise ValueError("oracle: the graph has an unknown node id or a cycle")
+
+
+def graph_props(radius: float = None):
+    """(step scale, iteration product, trap scales, Lipschitz factor at radius, plan) of the loaded
+    graph; radius defaults to nothing meaningful (pass the bounding hint for the step scale's own)."""
+    lib = load()
+    lib.orc_graph_props.restype = C.c_int
+    lib.orc_graph_props.argtypes = [C.c_float] + [C.c_void_p] * 5 + [C.c_uint32, C.c_void_p]
+    step, lip = C.c_float(), C.c_float()
+    prod, plen = C.c_uint32(), C.c_uint32()
+    scales = (C.c_float * 4)()
+    plan = (C.c_uint32 * 4096)()
+    if lib.orc_graph_props(0.0 if radius is None else radius, C.byref(step), C.byref(prod), scales, C.byref(lip),
+                           plan, 4096, C.byref(plen)) != 0:
+        raise ValueError("oracle: no graph loaded")
+    return step.value, prod.value, tuple(scales), lip.value, list(plan[:plen.value])
+
+
+def graph_probe(points: np.ndarray, precision: int = 32):
+    """The loaded graph at points (n, 3): distance, traps (n, 4), iterations, materials, body runs."""
+    lib = load()
+    lib.orc_graph_probe.restype = C.c_int
+    lib.orc_graph_probe.argtypes = [C.c_int, C.c_void_p, C.c_size_t] + [C.c_void_p] * 5
+    p = np.ascontiguousarray(points, np.float32)
+    n = p.shape[0]
+    xyz4 = np.zeros((n, 4), np.float32)
+    xyz4[:, :3] = p
+    dist = np.empty(n, np.float64)
+    traps = np.empty((n, 4), np.float64)
+    iters = np.empty(n, np.uint32)
+    mats = np.empty(n, np.uint32)
+    runs = np.empty(n, np.uint32)
+    if lib.orc_graph_probe(precision, _ptr(xyz4), n, _ptr(dist), _ptr(traps), _ptr(iters), _ptr(mats), _ptr(runs)) != 0:
+        raise ValueError("oracle: no graph loaded or bad precision")
+    return dist, traps, iters, mats, runs
+
+
+def graph_point(node_id: int, points: np.ndarray, precision: int = 32):
+    """One point node of the loaded graph applied to (p, dr 1) at points (n, 3): (p' (n, 3), dr (n,))."""
+    lib = load()
+    lib.orc_graph_point.restype = C.c_int
+    lib.orc_graph_point.argtypes = [C.c_uint32, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p]
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    xyz4 = np.zeros((len(p), 4), np.float32)
+    xyz4[:, :3] = p
+    out = np.empty((len(p), 4), np.float64)
+    if lib.orc_graph_point(node_id, precision, _ptr(xyz4), len(p), _ptr(out)) != 0:
+        raise ValueError(f"oracle: no node {node_id} in the loaded graph")
+    return out[:, :3], out[:, 3]

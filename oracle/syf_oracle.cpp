@@ -45,6 +45,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <limits>
+#include <vector>
 #include <quadmath.h>
 
 #include "../include/syfrack_hip.h"
@@ -173,7 +174,13 @@ template <class T> void power_node(V3<T>* p, T* dr, T power) {
     }
 }
 
+#include "graph_eval.hpp"
+
+// The node graph that a job of family SYF_FAMILY_GRAPH evaluates (set by orc_graph_use).
+OGraph g_graph;
+
 template <class T> struct Fractal {
+    const OGraph* graph;  // non-null for SYF_FAMILY_GRAPH
     uint32_t family, iterations;
     T bailout, power, scale, min_radius, fixed_radius, fold_limit;
     V3<T> offset, half, plane_n;
@@ -183,6 +190,7 @@ template <class T> struct Fractal {
 
 template <class T> Fractal<T> lower_fractal(const syf_fractal& f) {
     Fractal<T> r;
+    r.graph = nullptr;
     r.family = f.family;
     r.iterations = f.iterations;
     r.bailout = T(f.bailout);
@@ -205,12 +213,25 @@ template <class T> Fractal<T> lower_fractal(const syf_fractal& f) {
     r.bound_hint = T(f.bounding_radius_hint);
     // iteration_product (validate.rs:112-137): the Iterate count; 1 without an Iterate.
     r.iteration_product = f.family == SYF_FAMILY_SPHERE ? 1u : f.iterations;
+    if (f.family == SYF_FAMILY_GRAPH) {
+        // GpuFractal of a graph: step scale, bounding hint and iteration product of the graph
+        r.graph = &g_graph;
+        r.safety = T(step_scale(g_graph));
+        r.bound_hint = T(g_graph.hint);
+        r.iteration_product = g_graph.iteration_product;
+    }
     return r;
+}
+
+// Graph.bounding_radius_hint of the job's field (uniforms.rs:781-782)
+inline float bound_hint_of(const syf_fractal& f) {
+    return f.family == SYF_FAMILY_GRAPH ? g_graph.hint : f.bounding_radius_hint;
 }
 
 // Graph::evaluate for the classic graphs (eval.rs:98-114 normalisation, :346-556 point
 // ops in the bundled body order, :558-599 fences, :601-621 union).
 template <class T> DeOut<T> de_eval(const Fractal<T>& F, V3<T> p) {
+    if (F.graph) return graph_evaluate(*F.graph, p);
     const T INF = std::numeric_limits<T>::infinity();
     DeOut<T> o;
     T trap0 = INF;
@@ -583,7 +604,7 @@ SceneL lower_scene(const syf_job_desc& d) {
     S.pixel_tol = d.pixel_tolerance;
     S.pixel_cone = S.cam.tan_half * 2.0f / float(d.width);
     S.cavity = d.cavity_darkness;
-    float p0z = std::fmax(2.0f * d.fractal.bounding_radius_hint + length(S.cam.origin), 200.0f);
+    float p0z = std::fmax(2.0f * bound_hint_of(d.fractal) + length(S.cam.origin), 200.0f);
     S.t_end = std::fmax(200.0f, p0z);
     S.light_count = d.light_count;
     for (uint32_t i = 0; i < d.light_count; ++i) {
@@ -980,7 +1001,7 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
 
 float ray_extent(const syf_job_desc& d, const Basis& b) {
     // uniforms.rs:781-782 and common.wgsl:206-208
-    float p0z = std::fmax(2.0f * d.fractal.bounding_radius_hint + length(b.origin), 200.0f);
+    float p0z = std::fmax(2.0f * bound_hint_of(d.fractal) + length(b.origin), 200.0f);
     return std::fmax(200.0f, p0z);
 }
 
@@ -1084,6 +1105,73 @@ int orc_de_probe(const syf_fractal* f, int precision, const float* xyz4, size_t 
         }
     } else {
         return -1;
+    }
+    return 0;
+}
+
+// Loads the node graph that jobs and probes of family SYF_FAMILY_GRAPH evaluate from then on.
+// Returns 0, or -1 when an id does not resolve or the graph has a cycle.
+int orc_graph_use(const syf_graph* g) { return load_graph(*g, &g_graph) ? 0 : -1; }
+
+// Derived properties of the loaded graph: step scale (eval.rs:879-881), iteration product
+// (validate.rs:112-137), trap scales (eval.rs:853-877), Lipschitz factor at `radius`
+// (eval.rs:828-849), and the plan (node ids in evaluation order, at most plan_cap).
+int orc_graph_props(float radius, float* step, uint32_t* product, float* scales4, float* lipschitz_at,
+                    uint32_t* plan, uint32_t plan_cap, uint32_t* plan_len) {
+    if (!g_graph.ok) return -1;
+    *step = step_scale(g_graph);
+    *product = g_graph.iteration_product;
+    for (int i = 0; i < 4; ++i) scales4[i] = g_graph.trap_scales[i];
+    *lipschitz_at = lipschitz(g_graph, radius);
+    *plan_len = uint32_t(g_graph.plan.size());
+    for (uint32_t i = 0; i < g_graph.plan.size() && i < plan_cap; ++i) plan[i] = g_graph.plan[i];
+    return 0;
+}
+
+// One point node of the loaded graph applied to the start state at n points, as the
+// reference's unit tests do (Graph::apply, eval.rs:341-344): out4 = (p', dr).
+int orc_graph_point(uint32_t node_id, int precision, const float* xyz4, size_t n, double* out4) {
+    if (!g_graph.node(node_id)) return -1;
+    for (size_t i = 0; i < n; ++i) {
+        if (precision == 64) {
+            V3<double> p = v3(double(xyz4[4 * i]), double(xyz4[4 * i + 1]), double(xyz4[4 * i + 2]));
+            PState<double> s{p, 1.0, 0u, {INFINITY, INFINITY, INFINITY, INFINITY}};
+            s = apply_seeded(g_graph, node_id, s, p, p, 1.0);
+            out4[4 * i] = s.p.x; out4[4 * i + 1] = s.p.y; out4[4 * i + 2] = s.p.z; out4[4 * i + 3] = s.dr;
+        } else {
+            V3<float> p = v3(xyz4[4 * i], xyz4[4 * i + 1], xyz4[4 * i + 2]);
+            PState<float> s{p, 1.0f, 0u, {INFINITY, INFINITY, INFINITY, INFINITY}};
+            s = apply_seeded(g_graph, node_id, s, p, p, 1.0f);
+            out4[4 * i] = s.p.x; out4[4 * i + 1] = s.p.y; out4[4 * i + 2] = s.p.z; out4[4 * i + 3] = s.dr;
+        }
+    }
+    return 0;
+}
+
+// The loaded graph at n points (xyz4), precision 32 or 64: distance, traps[4], iterations,
+// material index, and the Iterate-body executions.
+int orc_graph_probe(int precision, const float* xyz4, size_t n, double* dist, double* traps4, uint32_t* iters,
+                    uint32_t* materials, uint32_t* body_runs) {
+    if (!g_graph.ok) return -1;
+    for (size_t i = 0; i < n; ++i) {
+        DeOut<double> o;
+        if (precision == 32) {
+            DeOut<float> f = graph_evaluate(g_graph, v3(xyz4[4 * i], xyz4[4 * i + 1], xyz4[4 * i + 2]));
+            o.d = f.d;
+            for (int k = 0; k < 4; ++k) o.traps[k] = f.traps[k];
+            o.iter = f.iter;
+            o.material = f.material;
+            o.body_runs = f.body_runs;
+        } else if (precision == 64) {
+            o = graph_evaluate(g_graph, v3(double(xyz4[4 * i]), double(xyz4[4 * i + 1]), double(xyz4[4 * i + 2])));
+        } else {
+            return -1;
+        }
+        dist[i] = o.d;
+        for (int k = 0; k < 4; ++k) traps4[4 * i + k] = o.traps[k];
+        iters[i] = o.iter;
+        if (materials) materials[i] = o.material;
+        if (body_runs) body_runs[i] = o.body_runs;
     }
     return 0;
 }

@@ -25,6 +25,7 @@ FAMILY_MANDELBULB = 0
 FAMILY_MANDELBOX = 1
 FAMILY_MENGER = 2
 FAMILY_SPHERE = 3
+FAMILY_GRAPH = 4
 
 CAMERA_FREE = 0
 CAMERA_ORBIT = 1
@@ -233,6 +234,37 @@ class SyfServerCounters(C.Structure):
     ]
 
 
+class SyfGraphNode(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32),
+        ("kind", C.c_uint32),
+        ("enabled", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("v", C.c_float * 6),
+        ("u", C.c_uint32 * 3),
+        ("body_first", C.c_uint32),
+        ("body_count", C.c_uint32),
+    ]
+
+
+class SyfGraphLink(C.Structure):
+    _fields_ = [("from_node", C.c_uint32), ("from_port", C.c_uint32), ("to_node", C.c_uint32), ("to_port", C.c_uint32)]
+
+
+class SyfGraph(C.Structure):
+    _fields_ = [
+        ("nodes", C.c_void_p),
+        ("node_count", C.c_uint32),
+        ("link_count", C.c_uint32),
+        ("links", C.c_void_p),
+        ("bodies", C.c_void_p),
+        ("body_len", C.c_uint32),
+        ("safety", C.c_float),
+        ("bounding_radius_hint", C.c_float),
+        ("reserved", C.c_uint32),
+    ]
+
+
 NOALLOC_MAX_PIXELS = 1048576  # SYF_NOALLOC_MAX_PIXELS
 MATH_FAST = 0  # SYF_MATH_FAST
 MATH_EXACT = 1  # SYF_MATH_EXACT
@@ -246,6 +278,8 @@ SIGNATURES = {
     "syf_ctx_destroy": (C.c_int, [_P]),
     "syf_job_create": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.POINTER(_P)]),
     "syf_job_destroy": (C.c_int, [_P]),
+    "syf_job_create_graph": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.POINTER(SyfGraph), C.POINTER(_P)]),
+    "syf_graph_info": (C.c_int, [C.POINTER(SyfGraph), C.POINTER(C.c_float), C.POINTER(C.c_uint32), _P]),
     "syf_job_clear": (C.c_int, [_P]),
     "syf_job_sync": (C.c_int, [_P]),
     "syf_job_stats": (C.c_int, [_P, C.POINTER(SyfStats)]),

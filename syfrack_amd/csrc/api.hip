@@ -117,9 +117,13 @@ int lower_fractal(const syf_fractal& f, FractalP& F) {
     return SYF_OK;
 }
 
-int lower_job(const syf_job_desc& d, JobP& P) {
+int lower_job(const syf_job_desc& d, JobP& P, const FractalP* graph_fractal) {
     std::memset(&P, 0, sizeof P);
-    int rc = lower_fractal(d.fractal, P.f);
+    if (d.fractal.family == SYF_FAMILY_GRAPH && !graph_fractal)
+        return fail(SYF_E_INVALID, "fractal.family: a node graph (SYF_FAMILY_GRAPH) is given to syf_job_create_graph, "
+                                   "this entry point takes the classic families");
+    int rc = graph_fractal ? SYF_OK : lower_fractal(d.fractal, P.f);
+    if (graph_fractal) P.f = *graph_fractal;
     if (rc != SYF_OK) return rc;
     if (d.width < 1) return fail(SYF_E_INVALID, "width must be at least 1, got %u", d.width);
     if (d.height < 1) return fail(SYF_E_INVALID, "height must be at least 1, got %u", d.height);
@@ -205,6 +209,12 @@ int lower_job(const syf_job_desc& d, JobP& P) {
     if (m.surface_enabled && m.surface_slot > 3)
         return fail(SYF_E_INVALID, "material.surface_slot must be a trap slot 0..3, got %u", m.surface_slot);
     P.mat.surface_slot = m.surface_slot;
+    if (graph_fractal) {
+        // a graph's field evaluation returns the trap of the shading's slot as its first trap
+        P.f.gshade_slot = m.surface_enabled ? m.surface_slot : 0u;
+        P.f.trap_scale = P.f.gtrap_scale[P.f.gshade_slot];
+        P.mat.surface_slot = 0u;
+    }
     P.mat.surface[0] = m.surface_roughness[0];
     P.mat.surface[1] = m.surface_roughness[1];
     P.mat.surface[2] = m.surface_metallic[0];
@@ -284,8 +294,16 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
     JobP P;
     int rc = lower_job(*desc, P);
     if (rc != SYF_OK) return rc;
+    return job_alloc(ctx, *desc, P, out);
+}
+
+}  // extern "C"
+
+int syf::job_alloc(syf_ctx* ctx, const syf_job_desc& dref, const JobP& P, syf_job** out) {
+    const syf_job_desc* desc = &dref;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    int rc = SYF_OK;
     syf_job* j = new syf_job();
     j->ctx = ctx;
     j->desc = *desc;
@@ -330,6 +348,8 @@ int syf_job_create(syf_ctx* ctx, const syf_job_desc* desc, syf_job** out) {
     return SYF_OK;
 }
 
+extern "C" {
+
 int syf_job_destroy(syf_job* j) {
     clear_error();
     if (!j) return SYF_OK;
@@ -344,6 +364,7 @@ int syf_job_destroy(syf_job* j) {
     if (j->ev1) (void)hipEventDestroy(j->ev1);
     if (j->ev2) (void)hipEventDestroy(j->ev2);
     release_wavefront(j);
+    if (j->gprog) (void)hipFree(j->gprog);
     if (j->stream) (void)hipStreamDestroy(j->stream);
     delete j;
     return SYF_OK;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 #include "params.hpp"
 
@@ -36,6 +37,8 @@ struct syf_job {
     bool traced = false;  // a primary trace ran: its hit-queue error flag is checked by syf_job_stats
     double extra_ms = 0.0;
     syf::WavefrontArena* wf = nullptr;
+    syf::GOp* gprog = nullptr;       // a graph job's compiled program (device memory)
+    std::vector<uint8_t> gprog_host;  // its bytes, for the job fingerprint
 };
 
 namespace syf {
@@ -43,7 +46,9 @@ int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 void clear_error();
 int lower_fractal(const syf_fractal& f, FractalP& F);
-int lower_job(const syf_job_desc& d, JobP& P);
+int lower_job(const syf_job_desc& d, JobP& P, const FractalP* graph_fractal = nullptr);
+// Allocates a job's planes, counters and hit queue for a lowered description (syf_job_create).
+int job_alloc(syf_ctx* ctx, const syf_job_desc& desc, const JobP& P, syf_job** out);
 size_t plane_bytes(uint32_t plane, size_t pixels);
 void release_wavefront(syf_job* j);
 int noalloc_setup(int device, void** out);

@@ -108,7 +108,11 @@ int syf_job_fingerprint(syf_job* j, uint64_t* out) {
     clear_error();
     if (!j || !out) return fail(SYF_E_INVALID, "job and out must not be null");
     uint64_t h = fnv1a(TRANSPORT_TAG, sizeof TRANSPORT_TAG - 1, 0xcbf29ce484222325ull);
-    *out = fnv1a(&j->P, sizeof j->P, h);
+    // the job parameters without the device address of a graph program, then the program itself
+    JobP P = j->P;
+    P.f.gprog = nullptr;
+    h = fnv1a(&P, sizeof P, h);
+    *out = j->gprog_host.empty() ? h : fnv1a(j->gprog_host.data(), j->gprog_host.size(), h);
     return SYF_OK;
 }
 

@@ -139,8 +139,30 @@ template <class T> __device__ __forceinline__ void tiny_power(V3<T>& p, uint32_t
     p = mk(T(0), T(0), T(0));
 }
 
+}  // namespace syf
+
+#include "graph_device.hpp"
+
+namespace syf {
+
+// A node graph's field evaluation as the march and the shading use it: the distance, the
+// normalised trap of the shading's slot, the iteration count clamped to the iteration product
+// (eval.rs:98-114), and the Iterate-body runs.
+template <class T> __device__ __forceinline__ De<T> graph_de(const FractalP& F, V3<T> c) {
+    const GRaw<T> g = graph_run<T>(F.gprog, c);
+    const uint32_t slot = F.gshade_slot;
+    const T t = slot == 0u ? g.r.t0 : slot == 1u ? g.r.t1 : slot == 2u ? g.r.t2 : g.r.t3;
+    De<T> o;
+    o.d = g.r.d;
+    o.trap0 = isfinite(t) ? fmin(t / T(F.trap_scale), T(1)) : T(1);
+    o.iter = g.r.iter < F.iteration_product ? g.r.iter : F.iteration_product;
+    o.runs = g.runs;
+    return o;
+}
+
 // Graph::evaluate for one classic family (eval.rs:98-114, :346-556, :558-599, :601-621).
 template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval(const FractalP& F, V3<T> c) {
+    if constexpr (FAM == FAM_GRAPH) return graph_de<T>(F, c);
     V3<T> z = c;
     T dr = T(1);
     T trap = T(INFINITY);
@@ -504,6 +526,27 @@ template <class T, uint32_t FAM> struct DeStepperFast {
     }
 };
 
+// A node graph is evaluated whole by start() (the interpreter runs its loops itself); the
+// fast arithmetic does not apply to it.
+template <> struct DeStepper<FAM_GRAPH> {
+    De<float> res;
+    __device__ __forceinline__ bool start(const FractalP& F, V3<float> p) {
+        res = graph_de<float>(F, p);
+        return true;
+    }
+    __device__ __forceinline__ bool step(const FractalP&) { return true; }
+    __device__ __forceinline__ De<float> finish(const FractalP&) const { return res; }
+};
+template <class T> struct DeStepperFast<T, FAM_GRAPH> {
+    De<T> res;
+    __device__ __forceinline__ bool start(const FractalP& F, V3<T> p) {
+        res = graph_de<T>(F, p);
+        return true;
+    }
+    __device__ __forceinline__ bool step(const FractalP&) { return true; }
+    __device__ __forceinline__ De<T> finish(const FractalP&) const { return res; }
+};
+
 // One whole field evaluation with the fast arithmetic.
 template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval_fast(const FractalP& F, V3<T> p) {
     DeStepperFast<T, FAM> s;
@@ -523,6 +566,7 @@ template <class T> __device__ __forceinline__ De<T> de_eval_any(const FractalP& 
         return F.ipower != 0u ? de_eval<T, SYF_FAMILY_MANDELBULB>(F, p) : de_eval<T, FAM_BULB_TRIG>(F, p);
     case SYF_FAMILY_MANDELBOX: return de_eval<T, SYF_FAMILY_MANDELBOX>(F, p);
     case SYF_FAMILY_MENGER: return de_eval<T, SYF_FAMILY_MENGER>(F, p);
+    case SYF_FAMILY_GRAPH: return graph_de<T>(F, p);
     default: return de_eval<T, SYF_FAMILY_SPHERE>(F, p);
     }
 }

@@ -11,6 +11,8 @@
 
 namespace syf {
 
+struct GOp;  // one instruction of a compiled node graph (graph_device.hpp)
+
 // Classic-graph parameters.  Products the reference recomputes per iteration from node
 // parameters alone (fr2/mr2, offset*(scale-1), -0.5*offset.z*(scale-1)) are folded once
 // on the host with the same f32 operations, so the values are identical.
@@ -33,8 +35,13 @@ struct FractalP {
     float radius;
     float plane_n[3];
     float plane_offset;
-    float trap_scale;
+    float trap_scale;         // normalisation of the trap the shading reads (a graph: of its shade slot)
     float safety;             // step scale
+    // SYF_FAMILY_GRAPH: the compiled program (device memory), the four trap normalisation
+    // scales and the trap slot the shading reads
+    const GOp* gprog;
+    float gtrap_scale[4];
+    uint32_t gshade_slot;
 };
 
 struct CameraP {
@@ -94,11 +101,13 @@ struct JobP {
 // separately so the common paths stay lean.
 constexpr uint32_t FAM_BULB_TRIG = 4u;
 constexpr uint32_t FAM_BULB8 = 5u;
-constexpr uint32_t FAM_COUNT = 6u;  // kernel families: the four graph families, trig and power-8 Mandelbulb
+constexpr uint32_t FAM_COUNT = 6u;  // kernel families of the classic graphs: the four, trig and power-8 Mandelbulb
+constexpr uint32_t FAM_GRAPH = 6u;  // any node graph, run by the graph interpreter (not batched by the server)
 constexpr bool is_bulb(uint32_t fam) {
     return fam == SYF_FAMILY_MANDELBULB || fam == FAM_BULB_TRIG || fam == FAM_BULB8;
 }
 __host__ __device__ inline uint32_t kernel_family(const FractalP& F) {
+    if (F.family == SYF_FAMILY_GRAPH) return FAM_GRAPH;
     if (F.family != SYF_FAMILY_MANDELBULB) return F.family;
     return F.ipower == 0u ? FAM_BULB_TRIG : (F.ipower == 8u ? FAM_BULB8 : SYF_FAMILY_MANDELBULB);
 }

@@ -433,6 +433,7 @@ __global__ void k_wf_end_sample(WfCtl* ctl) {
     case SYF_FAMILY_MENGER: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_MENGER>, __VA_ARGS__); break;         \
     case FAM_BULB_TRIG: hipLaunchKernelGGL(KERNEL<FAM_BULB_TRIG>, __VA_ARGS__); break;                 \
     case FAM_BULB8: hipLaunchKernelGGL(KERNEL<FAM_BULB8>, __VA_ARGS__); break;                         \
+    case FAM_GRAPH: hipLaunchKernelGGL(KERNEL<FAM_GRAPH>, __VA_ARGS__); break;                         \
     default: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_SPHERE>, __VA_ARGS__); break;                        \
     }
 

@@ -485,9 +485,19 @@ __global__ __launch_bounds__(256) void k_de_probe(const FractalP F, const float4
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 q = pts[i];
-    De<float> s = de_eval<float, FAM>(F, mk(q.x, q.y, q.z));
-    out[i] = make_float4(s.d, s.trap0, float(s.iter), 0.0f);
-    traps[i] = make_float4(s.trap0, 1.0f, 1.0f, 1.0f);
+    if constexpr (FAM == FAM_GRAPH) {
+        // every trap slot and the material index (PreparedGraph::evaluate, eval.rs:98-114)
+        const GRaw<float> g = graph_run<float>(F.gprog, mk(q.x, q.y, q.z));
+        auto norm = [&](float t, int k) { return isfinite(t) ? fminf(t / F.gtrap_scale[k], 1.0f) : 1.0f; };
+        const float t0 = norm(g.r.t0, 0);
+        const uint32_t it = g.r.iter < F.iteration_product ? g.r.iter : F.iteration_product;
+        out[i] = make_float4(g.r.d, t0, float(it), float(g.r.mat));
+        traps[i] = make_float4(t0, norm(g.r.t1, 1), norm(g.r.t2, 2), norm(g.r.t3, 3));
+    } else {
+        De<float> s = de_eval<float, FAM>(F, mk(q.x, q.y, q.z));
+        out[i] = make_float4(s.d, s.trap0, float(s.iter), 0.0f);
+        traps[i] = make_float4(s.trap0, 1.0f, 1.0f, 1.0f);
+    }
 }
 
 #define SYF_DISPATCH_FAMILY(fam, KERNEL, ...)                                                      \
@@ -497,6 +507,7 @@ __global__ __launch_bounds__(256) void k_de_probe(const FractalP F, const float4
     case SYF_FAMILY_MENGER: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_MENGER>, __VA_ARGS__); break;         \
     case FAM_BULB_TRIG: hipLaunchKernelGGL(KERNEL<FAM_BULB_TRIG>, __VA_ARGS__); break;                 \
     case FAM_BULB8: hipLaunchKernelGGL(KERNEL<FAM_BULB8>, __VA_ARGS__); break;                         \
+    case FAM_GRAPH: hipLaunchKernelGGL(KERNEL<FAM_GRAPH>, __VA_ARGS__); break;                         \
     default: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_SPHERE>, __VA_ARGS__); break;                        \
     }
 

@@ -4,7 +4,9 @@
 ``Renderer.render`` mirrors Renderer::render (renderer.rs:668-693) returning a
 ``RenderOutput`` with accumulation, depth and normals planes (renderer.rs:151-165), and
 ``Renderer.trace_primary`` is its first-hit half (max_bounces = 0) with the FP64-refined hit
-distance.  Every call runs the HIP kernels of libsyfrack_hip.so; there is no other path.
+distance.  A scene whose fractal is a node graph (syfrack_amd.graph.Graph) becomes a graph job
+(syf_job_create_graph).  Every call runs the HIP kernels of libsyfrack_hip.so; there is no
+other path.
 """
 from __future__ import annotations
 
@@ -15,6 +17,7 @@ from typing import Optional
 import numpy as np
 
 from . import abi
+from .graph import Graph
 from .scene import RenderConfig, Scene
 
 
@@ -64,12 +67,16 @@ class Context:
 class Job:
     """A render job (syf_job): the lowered scene plus its device planes."""
 
-    def __init__(self, ctx: Context, desc: abi.SyfJobDesc):
+    def __init__(self, ctx: Context, desc: abi.SyfJobDesc, graph: Optional[Graph] = None):
         self.ctx = ctx
         self.lib = ctx.lib
         self.desc = desc
         h = C.c_void_p()
-        abi.check(self.lib.syf_job_create(ctx.handle, C.byref(desc), C.byref(h)))
+        if graph is not None:
+            cg = graph.c_struct()
+            abi.check(self.lib.syf_job_create_graph(ctx.handle, C.byref(desc), cg.ref(), C.byref(h)))
+        else:
+            abi.check(self.lib.syf_job_create(ctx.handle, C.byref(desc), C.byref(h)))
         self.handle = h
         self.width = desc.width
         self.height = desc.height
@@ -159,13 +166,54 @@ class Renderer:
         self.ctx = Context(device)
 
     def job(self, scene: Scene, cfg: RenderConfig) -> Job:
-        return Job(self.ctx, scene.desc(cfg))
+        graph = scene.fractal if isinstance(scene.fractal, Graph) else None
+        return Job(self.ctx, scene.desc(cfg), graph)
 
     def probe_distances(self, scene: Scene, points: np.ndarray) -> DeSample:
         cfg = RenderConfig(1, 1, 1, 0, 1)
         j = self.job(scene, cfg)
         try:
             return j.probe(points)
+        finally:
+            j.close()
+
+    def estimate_bounding_radius(self, graph: Graph) -> float:
+        """Graph::estimate_bounding_radius (eval.rs:883-922) with the field evaluated on the
+        device: from 100 units out along the 26 directions of the unit cube's neighbours, walk
+        inward while the field stays positive and above 5 % of the radius; directions whose far
+        point already lies within that margin (a ground plane) are skipped.  The walks of all
+        directions advance together, one probe per step."""
+        from .scene import Camera
+        dirs = []
+        for x in (-1, 0, 1):
+            for y in (-1, 0, 1):
+                for z in (-1, 0, 1):
+                    if (x, y, z) != (0, 0, 0):
+                        v = np.array([x, y, z], np.float32)
+                        dirs.append(v * (np.float32(1.0) / np.float32(np.sqrt(np.float32(np.dot(v, v))))))
+        dirs = np.array(dirs, np.float32)
+        j = self.job(Scene(graph, Camera()), RenderConfig(1, 1, 1, 0, 1))
+        try:
+            f32 = np.float32
+            r = np.full(len(dirs), f32(100.0), np.float32)
+            far = j.probe(dirs * r[:, None]).distance
+            walking = np.isfinite(far) & (far > f32(0.05) * r)
+            if not walking.any():
+                return 100.0
+            active = walking.copy()
+            while active.any():
+                idx = np.nonzero(active)[0]
+                idx = idx[r[idx] > f32(0.5)]
+                active[:] = False
+                if len(idx) == 0:
+                    break
+                dist = j.probe(dirs[idx] * r[idx, None]).distance
+                go = np.isfinite(dist) & (dist > f32(0.05) * r[idx])
+                step = np.maximum(np.minimum(dist[go], r[idx][go] * f32(0.5)), f32(0.01)).astype(np.float32)
+                r[idx[go]] = (r[idx[go]] - step).astype(np.float32)
+                active[idx[go]] = True
+            radius = max(f32(0.5), r[walking].max())
+            return float(min(max(radius, f32(0.5)), f32(100.0)))
         finally:
             j.close()
 

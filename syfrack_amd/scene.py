@@ -234,7 +234,7 @@ class RenderConfig:
 
 @dataclass
 class Scene:
-    fractal: Fractal
+    fractal: Fractal  # or a syfrack_amd.graph.Graph
     camera: Camera
     lights: List[Light] = field(default_factory=list)
     material: Material = field(default_factory=Material)
@@ -248,7 +248,12 @@ class Scene:
 
     def desc(self, cfg: RenderConfig) -> abi.SyfJobDesc:
         d = abi.SyfJobDesc()
-        d.fractal = self.fractal.c_struct()
+        if hasattr(self.fractal, "nodes"):  # a node graph: the job is created with syf_job_create_graph
+            d.fractal.family = abi.FAMILY_GRAPH
+            d.fractal.safety = self.fractal.safety
+            d.fractal.bounding_radius_hint = self.fractal.bounding_radius_hint
+        else:
+            d.fractal = self.fractal.c_struct()
         d.camera = self.camera.c_struct()
         d.material = self.material.c_struct()
         if len(self.lights) > abi.MAX_LIGHTS:

@@ -14,147 +14,18 @@ from __future__ import annotations
 
 import dataclasses
 import os
-import re
 import time
 from typing import Any
 
 import numpy as np
 
 from . import abi
+from .ron import from_ron, to_ron  # noqa: F401  (state.ron)
+from .graph import Graph
 from .render import Renderer, RenderOutput
 from .scene import Camera, Fractal, Light, Material, RenderConfig, Scene
 
 STATE_FILE = "state.ron"
-
-
-# ---------------------------------------------------------------- RON (the subset state.ron uses)
-
-def to_ron(value: Any, indent: int = 0) -> str:
-    """Named structs for dataclasses, anonymous structs for dicts, lists, tuples, Option (None /
-    Some), strings, booleans and numbers (floats with repr, so they read back exactly)."""
-    pad, inner = "    " * indent, "    " * (indent + 1)
-    if dataclasses.is_dataclass(value):
-        fields = [(f.name, getattr(value, f.name)) for f in dataclasses.fields(value)]
-        body = ",\n".join(f"{inner}{k}: {to_ron(v, indent + 1)}" for k, v in fields)
-        return f"{type(value).__name__}(\n{body},\n{pad})"
-    if isinstance(value, dict):
-        body = ",\n".join(f"{inner}{k}: {to_ron(v, indent + 1)}" for k, v in value.items())
-        return f"(\n{body},\n{pad})"
-    if isinstance(value, bool):
-        return "true" if value else "false"
-    if value is None:
-        return "None"
-    if isinstance(value, int):
-        return str(value)
-    if isinstance(value, float):
-        return "NaN" if value != value else repr(float(value))
-    if isinstance(value, str):
-        return '"' + value.replace("\\", "\\\\").replace('"', '\\"') + '"'
-    if isinstance(value, tuple):
-        return "(" + ", ".join(to_ron(v, indent + 1) for v in value) + ")"
-    if isinstance(value, list):
-        return "[" + ", ".join(to_ron(v, indent + 1) for v in value) + "]"
-    raise TypeError(f"cannot write {type(value).__name__} as RON")
-
-
-_TOKEN = re.compile(r'\s*(?:(//[^\n]*)|("(?:[^"\\]|\\.)*")|([-+]?(?:\d+\.?\d*(?:[eE][-+]?\d+)?|inf(?![A-Za-z_0-9])|NaN(?![A-Za-z_0-9])))'
-                    r'|([A-Za-z_][A-Za-z_0-9]*)|(.))', re.S)
-
-
-def from_ron(text: str) -> Any:
-    """Parses the subset to_ron writes: named or anonymous structs become dicts (the name under
-    "__struct__"), tuples tuples, lists lists, Some(x) x, None None."""
-    toks = []
-    pos = 0
-    while pos < len(text):
-        m = _TOKEN.match(text, pos)
-        if not m or m.end() == pos:
-            break
-        pos = m.end()
-        comment, string, number, ident, punct = m.groups()
-        if comment is not None:
-            continue
-        if string is not None:
-            toks.append(("str", bytes(string[1:-1], "utf-8").decode("unicode_escape")))
-        elif number is not None:
-            toks.append(("num", float(number) if any(c in number for c in ".eEiN") else int(number)))
-        elif ident is not None:
-            toks.append(("id", ident))
-        elif punct is not None and not punct.isspace():
-            toks.append(("p", punct))
-    i = 0
-
-    def peek():
-        return toks[i] if i < len(toks) else ("eof", None)
-
-    def take(kind=None, val=None):
-        nonlocal i
-        t = peek()
-        if (kind and t[0] != kind) or (val is not None and t[1] != val):
-            raise ValueError(f"state.ron: expected {val or kind}, found {t[1]!r}")
-        i += 1
-        return t
-
-    def value():
-        t = peek()
-        if t[0] in ("str", "num"):
-            take()
-            return t[1]
-        if t[0] == "id":
-            take()
-            name = t[1]
-            if name in ("true", "false"):
-                return name == "true"
-            if name == "None":
-                return None
-            if name == "Some":
-                take("p", "(")
-                v = value()
-                take("p", ")")
-                return v
-            if peek() == ("p", "("):
-                d = group()
-                if isinstance(d, dict):
-                    d["__struct__"] = name
-                return d
-            return name
-        if t == ("p", "["):
-            take()
-            out = []
-            while peek() != ("p", "]"):
-                out.append(value())
-                if peek() == ("p", ","):
-                    take()
-            take("p", "]")
-            return out
-        if t == ("p", "("):
-            return group()
-        raise ValueError(f"state.ron: unexpected {t[1]!r}")
-
-    def group():
-        take("p", "(")
-        if peek()[0] == "id" and i + 1 < len(toks) and toks[i + 1] == ("p", ":"):
-            out = {}
-            while peek() != ("p", ")"):
-                key = take("id")[1]
-                take("p", ":")
-                out[key] = value()
-                if peek() == ("p", ","):
-                    take()
-            take("p", ")")
-            return out
-        items = []
-        while peek() != ("p", ")"):
-            items.append(value())
-            if peek() == ("p", ","):
-                take()
-        take("p", ")")
-        return tuple(items)
-
-    v = value()
-    if peek()[0] != "eof":
-        raise ValueError(f"state.ron: trailing {peek()[1]!r}")
-    return v
 
 
 def _build(cls, d: dict):
@@ -175,7 +46,9 @@ def scene_from_state(d: dict) -> Scene:
     surface = s["material"].get("surface")
     m = _build(Material, {k: v for k, v in s["material"].items() if k != "surface"})
     m.surface = None if surface is None else (int(surface[0]), tuple(surface[1]), tuple(surface[2]))
-    return Scene(_build(Fractal, s["fractal"]), _build(Camera, s["camera"]), lights, m, seed=int(s["seed"]),
+    fr = s["fractal"]
+    fractal = Graph.from_dict(fr) if fr.get("__struct__") == "Graph" else _build(Fractal, fr)
+    return Scene(fractal, _build(Camera, s["camera"]), lights, m, seed=int(s["seed"]),
                  cavity_darkness=float(s["cavity_darkness"]))
 
 

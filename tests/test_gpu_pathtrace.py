@@ -107,7 +107,7 @@ def family_scenes():
     return out
 
 
-def _against_oracle(name, out, acc, dep, aux):
+def _against_oracle(name, out, acc, dep, aux, min_frac=0.98):
     """Per-pixel radiance within 1e-3 relative on >= 98 % of the pixels and the image mean
     within 1e-3: device and oracle draw the same random numbers, but a one-ulp difference of the
     device's sin/cos/pow in a BSDF sample moves that path's next bounce, and over 4 bounces a
@@ -118,7 +118,7 @@ def _against_oracle(name, out, acc, dep, aux):
     close = np.abs(g - o) <= 1e-3 * np.abs(o) + 1e-4
     frac = np.mean(np.all(close, axis=-1))
     print(f"{name}: {frac:.4f} of pixels within 1e-3; image means {g.mean():.6f} {o.mean():.6f}")
-    assert frac >= 0.98
+    assert frac >= min_frac
     assert abs(g.mean() - o.mean()) <= 1e-3 * abs(o.mean()) + 1e-5
     hits = dep >= 0
     assert np.mean((out.depth >= 0) == hits) >= 0.999
