@@ -268,6 +268,8 @@ class SyfGraph(C.Structure):
 NOALLOC_MAX_PIXELS = 1048576  # SYF_NOALLOC_MAX_PIXELS
 MATH_FAST = 0  # SYF_MATH_FAST
 MATH_EXACT = 1  # SYF_MATH_EXACT
+GRAPH_SPECIALISED = 0  # SYF_GRAPH_SPECIALISED
+GRAPH_INTERPRETED = 1  # SYF_GRAPH_INTERPRETED
 
 # Every symbol the header declares, with its ctypes signature.
 _P = C.c_void_p
@@ -278,8 +280,9 @@ SIGNATURES = {
     "syf_ctx_destroy": (C.c_int, [_P]),
     "syf_job_create": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.POINTER(_P)]),
     "syf_job_destroy": (C.c_int, [_P]),
-    "syf_job_create_graph": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.POINTER(SyfGraph), C.POINTER(_P)]),
+    "syf_job_create_graph": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.POINTER(SyfGraph), C.c_uint32, C.POINTER(_P)]),
     "syf_graph_info": (C.c_int, [C.POINTER(SyfGraph), C.POINTER(C.c_float), C.POINTER(C.c_uint32), _P]),
+    "syf_graph_source": (C.c_int, [C.POINTER(SyfGraph), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "syf_job_clear": (C.c_int, [_P]),
     "syf_job_sync": (C.c_int, [_P]),
     "syf_job_stats": (C.c_int, [_P, C.POINTER(SyfStats)]),

@@ -59,7 +59,9 @@ def build(verbose: bool = False, extra=None, jobs: int = 0) -> str:
     with concurrent.futures.ThreadPoolExecutor(workers) as pool:
         for f in [pool.submit(_run, c, verbose) for c in cmds]:
             f.result()
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-o", OUT + ".tmp"], verbose)
+    # hiprtc compiles the kernels of specialised graph topologies at run time (graph.hip)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-L/opt/rocm/lib", "-lhiprtc", "-o", OUT + ".tmp"],
+         verbose)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

@@ -14,7 +14,15 @@
 
 #include "params.hpp"
 
+#ifndef INFINITY  // run-time compiled units have no <math.h>
+#define INFINITY __builtin_inff()
+#endif
+
 namespace syf {
+
+template <bool B, class X, class Y> struct cond_type { using type = X; };
+template <class X, class Y> struct cond_type<false, X, Y> { using type = Y; };
+template <bool B, class X, class Y> using cond_t = typename cond_type<B, X, Y>::type;
 
 template <class T> struct V3 {
     T x, y, z;
@@ -148,8 +156,10 @@ namespace syf {
 // A node graph's field evaluation as the march and the shading use it: the distance, the
 // normalised trap of the shading's slot, the iteration count clamped to the iteration product
 // (eval.rs:98-114), and the Iterate-body runs.
-template <class T> __device__ __forceinline__ De<T> graph_de(const FractalP& F, V3<T> c) {
-    const GRaw<T> g = graph_run<T>(F.gprog, c);
+template <class T, bool JIT> __device__ __forceinline__ De<T> graph_de(const FractalP& F, V3<T> c) {
+    GRaw<T> g;
+    if constexpr (JIT) g = graph_jit<T>(F.gprog, c);
+    else g = graph_run<T>(F.gprog, c);
     const uint32_t slot = F.gshade_slot;
     const T t = slot == 0u ? g.r.t0 : slot == 1u ? g.r.t1 : slot == 2u ? g.r.t2 : g.r.t3;
     De<T> o;
@@ -162,7 +172,7 @@ template <class T> __device__ __forceinline__ De<T> graph_de(const FractalP& F, 
 
 // Graph::evaluate for one classic family (eval.rs:98-114, :346-556, :558-599, :601-621).
 template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval(const FractalP& F, V3<T> c) {
-    if constexpr (FAM == FAM_GRAPH) return graph_de<T>(F, c);
+    if constexpr (FAM == FAM_GRAPH || FAM == FAM_JIT) return graph_de<T, FAM == FAM_JIT>(F, c);
     V3<T> z = c;
     T dr = T(1);
     T trap = T(INFINITY);
@@ -528,24 +538,19 @@ template <class T, uint32_t FAM> struct DeStepperFast {
 
 // A node graph is evaluated whole by start() (the interpreter runs its loops itself); the
 // fast arithmetic does not apply to it.
-template <> struct DeStepper<FAM_GRAPH> {
-    De<float> res;
-    __device__ __forceinline__ bool start(const FractalP& F, V3<float> p) {
-        res = graph_de<float>(F, p);
-        return true;
-    }
-    __device__ __forceinline__ bool step(const FractalP&) { return true; }
-    __device__ __forceinline__ De<float> finish(const FractalP&) const { return res; }
-};
-template <class T> struct DeStepperFast<T, FAM_GRAPH> {
+template <class T, bool JIT> struct GraphStepper {
     De<T> res;
     __device__ __forceinline__ bool start(const FractalP& F, V3<T> p) {
-        res = graph_de<T>(F, p);
+        res = graph_de<T, JIT>(F, p);
         return true;
     }
     __device__ __forceinline__ bool step(const FractalP&) { return true; }
     __device__ __forceinline__ De<T> finish(const FractalP&) const { return res; }
 };
+template <> struct DeStepper<FAM_GRAPH> : GraphStepper<float, false> {};
+template <> struct DeStepper<FAM_JIT> : GraphStepper<float, true> {};
+template <class T> struct DeStepperFast<T, FAM_GRAPH> : GraphStepper<T, false> {};
+template <class T> struct DeStepperFast<T, FAM_JIT> : GraphStepper<T, true> {};
 
 // One whole field evaluation with the fast arithmetic.
 template <class T, uint32_t FAM> __device__ __forceinline__ De<T> de_eval_fast(const FractalP& F, V3<T> p) {
@@ -566,7 +571,7 @@ template <class T> __device__ __forceinline__ De<T> de_eval_any(const FractalP& 
         return F.ipower != 0u ? de_eval<T, SYF_FAMILY_MANDELBULB>(F, p) : de_eval<T, FAM_BULB_TRIG>(F, p);
     case SYF_FAMILY_MANDELBOX: return de_eval<T, SYF_FAMILY_MANDELBOX>(F, p);
     case SYF_FAMILY_MENGER: return de_eval<T, SYF_FAMILY_MENGER>(F, p);
-    case SYF_FAMILY_GRAPH: return graph_de<T>(F, p);
+    case SYF_FAMILY_GRAPH: return graph_de<T, false>(F, p);
     default: return de_eval<T, SYF_FAMILY_SPHERE>(F, p);
     }
 }

@@ -20,11 +20,20 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
 #include <string>
 #include <vector>
+
+#include <dlfcn.h>
+#include <hip/hiprtc.h>
+
+#include <filesystem>
+#include <mutex>
+#include <thread>
 
 #include "api_internal.hpp"
 #include "device.hpp"
@@ -902,7 +911,222 @@ int compile(const syf_graph& cg, Compiled& out) {
     return SYF_OK;
 }
 
+// ---------------------------------------------------------------- specialised topologies
+
+// The evaluator of one compiled program as straight-line device code: the program's structure
+// (instruction sequence, loop nesting, slots, fence and combination kinds, trap slots,
+// measures and modes, integer powers, axis masks, AddSeed levels) is written into the code;
+// every value the structure does not fix (the node parameters, loop counts and bailouts,
+// material indices) is read from the program in device memory at run time, so graphs of one
+// topology share their kernels whatever their parameters, as the reference's pipeline cache
+// shares a shader per topology key (NodeKind::topology_key, mod.rs:371-398).  The operations
+// are graph_device.hpp's, the interpreter's own: a specialised graph computes what the
+// interpreter computes.
+const char* const GOP_NAME[GOP_COUNT] = {
+    "GOP_END", "GOP_P_START", "GOP_P_SAVE", "GOP_P_LOAD", "GOP_ITER_BEGIN", "GOP_ITER_END", "GOP_TRANSLATE",
+    "GOP_ROTATE", "GOP_SCALE", "GOP_SCALE_AXES", "GOP_MIRROR", "GOP_BOX_FOLD", "GOP_BOX_FOLD_AXES",
+    "GOP_SPHERE_INVERSION", "GOP_SPHERE_FOLD", "GOP_ABS", "GOP_MENGER_FOLD", "GOP_KALEIDO", "GOP_POWER",
+    "GOP_ADD_ORIGIN", "GOP_ADD_SEED", "GOP_ADD_CONSTANT", "GOP_TWIST", "GOP_BEND", "GOP_TILE", "GOP_REPEAT_FINITE",
+    "GOP_TRAP", "GOP_FENCE", "GOP_D_SAVE", "GOP_D_LOAD", "GOP_MATTAG", "GOP_BOUNDED", "GOP_COMBINE"};
+static_assert(GOP_COMBINE + 1 == GOP_COUNT, "a name per opcode");
+
+std::string jit_source(const std::vector<GOp>& prog) {
+    std::string o;
+    char buf[512];
+    auto line = [&](const char* fmt, auto... a) {
+        std::snprintf(buf, sizeof buf, fmt, a...);
+        o += buf;
+        o += '\n';
+    };
+    uint32_t np = 0, nd = 0;
+    for (const GOp& g : prog) {
+        if (g.op == GOP_P_SAVE || g.op == GOP_P_LOAD) np = std::max(np, g.a + 1);
+        if (g.op == GOP_D_SAVE || g.op == GOP_D_LOAD) nd = std::max(nd, g.a + 1);
+        if (g.op == GOP_COMBINE) {
+            if (g.b != GRAPH_ACC) nd = std::max(nd, g.b + 1);
+            if (g.c != GRAPH_ACC) nd = std::max(nd, g.c + 1);
+        }
+    }
+    line("#include \"trace_kernels.hpp\"");
+    line("#include \"pathtrace_kernels.hpp\"");
+    line("namespace syf {");
+    line("template <class T> __device__ __forceinline__ GRaw<T> graph_jit(const GOp* prog_in, V3<T> c) {");
+    line("const GOp* __restrict__ prog = g_uniform(prog_in);");
+    line("const T INF = g_inf<T>();");
+    line("GPoint<T> cur{c, T(1), 0u, INF, INF, INF, INF};");
+    line("GDist<T> dc{INF, 0u, 0u, INF, INF, INF, INF};");
+    line("uint32_t runs = 0u;");
+    for (uint32_t k = 0; k < np; ++k) line("GPoint<T> ps%u;", k);
+    for (uint32_t k = 0; k < nd; ++k) line("GDist<T> ds%u;", k);
+    for (uint32_t i = 0; i < prog.size(); ++i) {
+        const GOp& g = prog[i];
+        switch (g.op) {
+        case GOP_END: line("return GRaw<T>{dc, runs};"); break;
+        case GOP_P_START: line("cur = GPoint<T>{c, T(1), 0u, INF, INF, INF, INF};"); break;
+        case GOP_P_SAVE: line("ps%u = cur;", g.a); break;
+        case GOP_P_LOAD: line("cur = ps%u;", g.a); break;
+        case GOP_D_SAVE: line("ds%u = dc;", g.a); break;
+        case GOP_D_LOAD: line("dc = ds%u;", g.a); break;
+        case GOP_ITER_BEGIN:
+            line("{ const V3<T> sp%u = cur.p; const T sd%u = cur.dr;", g.a, g.a);
+            line("const uint32_t n%u = prog[%u].b; const T bail%u = T(prog[%u].v[0]);", i, i, i, i);
+            line("for (uint32_t k%u = 0u; k%u < n%u; ++k%u) { if (len3(cur.p) > bail%u) break;", i, i, i, i, i);
+            break;
+        case GOP_ITER_END: line("cur.iter += 1u; runs += 1u; } }"); break;
+        case GOP_ADD_SEED:
+            if (g.a == 0u) line("g_point<T>(%s, 0u, 0u, 0u, prog[%u], cur, c, cur.p, cur.dr);", GOP_NAME[g.op], i);
+            else line("g_point<T>(%s, 0u, 0u, 0u, prog[%u], cur, c, sp%u, sd%u);", GOP_NAME[g.op], i, g.a - 1, g.a - 1);
+            break;
+        case GOP_FENCE: line("dc = g_fence_state<T>(%uu, prog[%u], cur);", g.a, i); break;
+        case GOP_MATTAG: line("dc.mat = prog[%u].a;", i); break;
+        case GOP_BOUNDED: line("g_bounded<T>(prog[%u], cur, dc);", i); break;
+        case GOP_COMBINE: {
+            char a[16], b[16];
+            std::snprintf(a, sizeof a, g.b == GRAPH_ACC ? "dc" : "ds%u", g.b);
+            std::snprintf(b, sizeof b, g.c == GRAPH_ACC ? "dc" : "ds%u", g.c);
+            line("dc = g_combine<T>(%uu, T(prog[%u].v[0]), GDist<T>(%s), GDist<T>(%s));", g.a, i, a, b);
+            break;
+        }
+        case GOP_REPEAT_FINITE:  // the counts are parameters: read from the program
+            line("g_point<T>(%s, 0u, 0u, 0u, prog[%u], cur, c, cur.p, cur.dr);", GOP_NAME[g.op], i);
+            break;
+        default:  // point operations: a, b, c (trap slot / measure / mode, power, masks) are structure
+            line("g_point<T>(%s, %uu, %uu, %uu, prog[%u], cur, c, cur.p, cur.dr);", GOP_NAME[g.op], g.a, g.b, g.c, i);
+            break;
+        }
+    }
+    line("return GRaw<T>{dc, runs};");
+    line("}");
+    line("}  // namespace syf");
+    return o;
+}
+
+const char* const JIT_NAMES[JIT_KERNELS] = {
+    "syf::k_primary<7u>",   "syf::k_primary_exact<7u>", "syf::k_hits_tail<7u>", "syf::k_de_probe<7u>",
+    "syf::k_render_tile<7u>", "syf::k_wf_intersect<7u>", "syf::k_wf_shade<7u>",  "syf::k_wf_commit<7u>"};
+static_assert(FAM_JIT == 7u, "the run-time kernel names carry the family");
+
+struct JitModule {
+    hipModule_t module[JIT_KERNELS] = {};
+    hipFunction_t fn[JIT_KERNELS] = {};
+    double compile_ms = 0.0;
+};
+
+std::mutex g_jit_mutex;
+std::map<std::pair<int, std::string>, JitModule*> g_jit_cache;  // (device, source): loaded for the process
+
+// The directory of the library's device sources (csrc/ beside libsyfrack_hip.so), or
+// SYF_CSRC_DIR.
+std::string csrc_dir() {
+    if (const char* e = std::getenv("SYF_CSRC_DIR")) return e;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&syf_graph_info), &info) && info.dli_fname)
+        return (std::filesystem::path(info.dli_fname).parent_path() / "csrc").string();
+    return "csrc";
+}
+std::string rocm_dir() {
+    if (const char* e = std::getenv("ROCM_PATH")) return e;
+    return "/opt/rocm";
+}
+// The compiler's own headers (stddef.h, stdint.h), which run-time compilation does not add by itself.
+std::string clang_include_dir(const std::string& rocm) {
+    std::error_code ec;
+    const std::filesystem::path base = std::filesystem::path(rocm) / "lib/llvm/lib/clang";
+    std::string best;
+    for (const auto& d : std::filesystem::directory_iterator(base, ec))
+        if (std::filesystem::exists(d.path() / "include/stddef.h")) best = (d.path() / "include").string();
+    return best;
+}
+
+// One kernel of a topology compiled by hiprtc: its code object and lowered name, or the error.
+struct JitPiece {
+    std::vector<char> code;
+    std::string lowered, error;
+};
+void jit_compile_one(const std::string& src, const char* name, const std::vector<std::string>& options, JitPiece* out) {
+    std::vector<const char*> opts;
+    for (const std::string& o : options) opts.push_back(o.c_str());
+    hiprtcProgram rp = nullptr;
+    if (hiprtcCreateProgram(&rp, src.c_str(), "graph_topology.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        out->error = "hiprtcCreateProgram failed";
+        return;
+    }
+    hiprtcAddNameExpression(rp, name);
+    const hiprtcResult cr = hiprtcCompileProgram(rp, int(opts.size()), opts.data());
+    if (cr != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(rp, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(rp, &log[0]);
+        out->error = std::string(hiprtcGetErrorString(cr)) + ": " + log.substr(0, 600);
+        hiprtcDestroyProgram(&rp);
+        return;
+    }
+    size_t code_n = 0;
+    hiprtcGetCodeSize(rp, &code_n);
+    out->code.resize(code_n);
+    hiprtcGetCode(rp, out->code.data());
+    const char* lowered = nullptr;
+    if (hiprtcGetLoweredName(rp, name, &lowered) == HIPRTC_SUCCESS && lowered) out->lowered = lowered;
+    else out->error = std::string("no compiled kernel ") + name;
+    hiprtcDestroyProgram(&rp);
+}
+
+// The kernels of a program's topology on the current device: from the process cache, or
+// compiled (hiprtc, one program per kernel, all at once on host threads) and loaded.
+int jit_get(int device, const std::vector<GOp>& prog, const JitModule** out) {
+    const std::string src = jit_source(prog);
+    std::lock_guard<std::mutex> lock(g_jit_mutex);
+    auto it = g_jit_cache.find({device, src});
+    if (it != g_jit_cache.end()) {
+        *out = it->second;
+        return SYF_OK;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::string rocm = rocm_dir(), clang_inc = clang_include_dir(rocm);
+    if (clang_inc.empty())
+        return fail(SYF_E_DEVICE, "graph: the compiler headers (stddef.h) were not found under %s/lib/llvm/lib/clang",
+                    rocm.c_str());
+    const std::vector<std::string> options = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                                              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-w",
+                                              "-I" + csrc_dir(), "-I" + rocm + "/include", "-I" + clang_inc};
+    JitPiece pieces[JIT_KERNELS];
+    std::vector<std::thread> threads;
+    for (uint32_t k = 0; k < JIT_KERNELS; ++k)
+        threads.emplace_back(jit_compile_one, std::cref(src), JIT_NAMES[k], std::cref(options), &pieces[k]);
+    for (std::thread& t : threads) t.join();
+    for (uint32_t k = 0; k < JIT_KERNELS; ++k)
+        if (!pieces[k].error.empty())
+            return fail(SYF_E_DEVICE, "graph: run-time compilation of %s for the topology failed: %s", JIT_NAMES[k],
+                        pieces[k].error.c_str());
+    JitModule* m = new JitModule();
+    hipError_t e = hipSuccess;
+    for (uint32_t k = 0; k < JIT_KERNELS && e == hipSuccess; ++k) {
+        e = hipModuleLoadData(&m->module[k], pieces[k].code.data());
+        if (e == hipSuccess) e = hipModuleGetFunction(&m->fn[k], m->module[k], pieces[k].lowered.c_str());
+    }
+    if (e != hipSuccess) {
+        for (hipModule_t mod : m->module)
+            if (mod) (void)hipModuleUnload(mod);
+        delete m;
+        return hip_fail(e, "graph: loading the topology's kernels");
+    }
+    m->compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    g_jit_cache[{device, src}] = m;
+    *out = m;
+    return SYF_OK;
+}
+
 }  // namespace
+
+hipError_t jit_launch_args(const void* jit, uint32_t which, dim3 grid, dim3 block, size_t shmem, hipStream_t s,
+                           void** args) {
+    const JitModule* m = static_cast<const JitModule*>(jit);
+    if (!m || which >= JIT_KERNELS || !m->fn[which]) return hipErrorInvalidDeviceFunction;
+    return hipModuleLaunchKernel(m->fn[which], grid.x, grid.y, grid.z, block.x, block.y, block.z, uint32_t(shmem), s,
+                                 args, nullptr);
+}
+
 }  // namespace syf
 
 using namespace syf;
@@ -921,9 +1145,27 @@ int syf_graph_info(const syf_graph* graph, float* step_scale, uint32_t* iteratio
     return SYF_OK;
 }
 
-int syf_job_create_graph(syf_ctx* ctx, const syf_job_desc* desc, const syf_graph* graph, syf_job** out) {
+int syf_graph_source(const syf_graph* graph, char* buf, size_t cap, size_t* len) {
+    clear_error();
+    if (!graph || !len) return fail(SYF_E_INVALID, "graph and len must not be null");
+    Compiled c;
+    if (int rc = compile(*graph, c); rc != SYF_OK) return rc;
+    const std::string src = jit_source(c.prog);
+    *len = src.size() + 1;
+    if (buf && cap > 0) {
+        const size_t n = std::min(cap - 1, src.size());
+        std::memcpy(buf, src.data(), n);
+        buf[n] = '\0';
+    }
+    return SYF_OK;
+}
+
+int syf_job_create_graph(syf_ctx* ctx, const syf_job_desc* desc, const syf_graph* graph, uint32_t mode,
+                         syf_job** out) {
     clear_error();
     if (!ctx || !desc || !graph || !out) return fail(SYF_E_INVALID, "ctx, desc, graph and out must not be null");
+    if (mode != SYF_GRAPH_SPECIALISED && mode != SYF_GRAPH_INTERPRETED)
+        return fail(SYF_E_INVALID, "mode must be SYF_GRAPH_SPECIALISED (0) or SYF_GRAPH_INTERPRETED (1), got %u", mode);
     if (desc->fractal.family != SYF_FAMILY_GRAPH)
         return fail(SYF_E_INVALID, "fractal.family must be SYF_FAMILY_GRAPH (%u) for a graph job, got %u", SYF_FAMILY_GRAPH,
                     desc->fractal.family);
@@ -950,6 +1192,14 @@ int syf_job_create_graph(syf_ctx* ctx, const syf_job_desc* desc, const syf_graph
         return hip_fail(e, "graph program upload");
     }
     P.f.gprog = dprog;
+    if (mode == SYF_GRAPH_SPECIALISED) {
+        const JitModule* m = nullptr;
+        if (int rc = jit_get(ctx->device, c.prog, &m); rc != SYF_OK) {
+            (void)hipFree(dprog);
+            return rc;
+        }
+        P.f.gjit = m;
+    }
     syf_job* j = nullptr;
     if (int rc = job_alloc(ctx, d, P, &j); rc != SYF_OK) {
         (void)hipFree(dprog);

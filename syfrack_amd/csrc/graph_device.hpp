@@ -172,10 +172,10 @@ template <class T> __device__ __forceinline__ T g_box(V3<T> p, T hx, T hy, T hz)
     return len3(mk(fmax(qx, T(0)), fmax(qy, T(0)), fmax(qz, T(0)))) + fmin(fmax(qx, fmax(qy, qz)), T(0));
 }
 
-// fence (eval.rs:558-599); kind SYF_NODE_KINDS is a disabled fence (+inf)
-template <class T> __device__ __forceinline__ T g_fence(const GOp& o, const GPoint<T>& s) {
+// fence (eval.rs:558-599) of kind `kind` (SYF_NODE_KINDS: a disabled fence, +inf)
+template <class T> __device__ __forceinline__ T g_fence(uint32_t kind, const GOp& o, const GPoint<T>& s) {
     const V3<T> p = s.p;
-    switch (o.a) {
+    switch (kind) {
     case SYF_NODE_SPHERE: return (len3(p) - T(o.v[0])) / s.dr;
     case SYF_NODE_TORUS: return (len2d(len2d(p.x, p.z) - T(o.v[0]), p.y) - T(o.v[1])) / s.dr;
     case SYF_NODE_CYLINDER: {
@@ -252,6 +252,197 @@ template <class T> __device__ __forceinline__ GDist<T> g_combine(uint32_t kind, 
     }
 }
 
+// One point operation (eval.rs:346-556) on the current point.  `op` and the integer fields
+// a, b, cc are passed separately from the instruction so that specialised code passes them
+// as constants (the branches below then fold away); sp, sdr are the AddSeed source.
+template <class T>
+__device__ __forceinline__ void g_point(uint32_t op, uint32_t a, uint32_t b, uint32_t cc, const GOp& o,
+                                        GPoint<T>& cur, V3<T> c, V3<T> sp, T sdr) {
+    switch (op) {
+    case GOP_TRANSLATE: cur.p = add(cur.p, mk(T(o.v[0]), T(o.v[1]), T(o.v[2]))); break;
+    case GOP_ROTATE: {
+        const V3<T> bq = mk(T(o.v[0]), T(o.v[1]), T(o.v[2]));
+        const T w = T(o.v[3]);
+        const V3<T> v = cur.p;
+        cur.p = add(add(scl(v, w * w - dot3(bq, bq)), scl(bq, dot3(v, bq) * T(2))), scl(cross3(bq, v), w * T(2)));
+        break;
+    }
+    case GOP_SCALE:
+        cur.p = scl(cur.p, T(o.v[0]));
+        cur.dr = cur.dr * T(o.v[1]);
+        break;
+    case GOP_SCALE_AXES:
+        cur.p = mk(cur.p.x * T(o.v[0]), cur.p.y * T(o.v[1]), cur.p.z * T(o.v[2]));
+        cur.dr = cur.dr * T(o.v[3]);
+        break;
+    case GOP_MIRROR: {
+        const V3<T> n = mk(T(o.v[0]), T(o.v[1]), T(o.v[2]));
+        const T h = dot3(cur.p, n) - T(o.v[3]);
+        if (h < T(0)) cur.p = sub(cur.p, scl(n, T(2) * h));
+        break;
+    }
+    case GOP_BOX_FOLD: {
+        const T l = T(o.v[0]);
+        cur.p = mk(clampT(cur.p.x, -l, l) * T(2) - cur.p.x, clampT(cur.p.y, -l, l) * T(2) - cur.p.y,
+                   clampT(cur.p.z, -l, l) * T(2) - cur.p.z);
+        break;
+    }
+    case GOP_BOX_FOLD_AXES: {
+        const T lx = T(o.v[0]), ly = T(o.v[1]), lz = T(o.v[2]);
+        if (a & 1u) cur.p.x = clampT(cur.p.x, -lx, lx) * T(2) - cur.p.x;
+        if (a & 2u) cur.p.y = clampT(cur.p.y, -ly, ly) * T(2) - cur.p.y;
+        if (a & 4u) cur.p.z = clampT(cur.p.z, -lz, lz) * T(2) - cur.p.z;
+        break;
+    }
+    case GOP_SPHERE_INVERSION: {
+        T r2, m2;
+        if constexpr (sizeof(T) == 4) {
+            r2 = o.v[2];
+            m2 = o.v[3];
+        } else {
+            r2 = T(o.v[0]) * T(o.v[0]);
+            m2 = T(o.v[1]) * T(o.v[1]);
+        }
+        const T k = r2 / fmax(dot3(cur.p, cur.p), m2);
+        cur.p = scl(cur.p, k);
+        cur.dr = cur.dr * k;
+        break;
+    }
+    case GOP_SPHERE_FOLD: {
+        T mr2, fr2, q;
+        if constexpr (sizeof(T) == 4) {
+            mr2 = o.v[2];
+            fr2 = o.v[3];
+            q = o.v[4];
+        } else {
+            mr2 = T(o.v[0]) * T(o.v[0]);
+            fr2 = T(o.v[1]) * T(o.v[1]);
+            q = fr2 / mr2;
+        }
+        const T r2 = dot3(cur.p, cur.p);
+        const T k = r2 < mr2 ? q : (r2 < fr2 ? fr2 / r2 : T(1));
+        cur.p = scl(cur.p, k);
+        cur.dr = cur.dr * k;
+        break;
+    }
+    case GOP_ABS:
+        if (a & 1u) cur.p.x = fabs(cur.p.x);
+        if (a & 2u) cur.p.y = fabs(cur.p.y);
+        if (a & 4u) cur.p.z = fabs(cur.p.z);
+        break;
+    case GOP_MENGER_FOLD: {
+        const T s = T(o.v[0]);
+        T ox, oy, oz, thr;
+        if constexpr (sizeof(T) == 4) {
+            ox = o.v[4];
+            oy = o.v[5];
+            oz = o.v[6];
+            thr = o.v[7];
+        } else {
+            const T s1 = s - T(1);
+            ox = T(o.v[1]) * s1;
+            oy = T(o.v[2]) * s1;
+            oz = T(o.v[3]) * s1;
+            thr = T(-0.5) * T(o.v[3]) * s1;
+        }
+        T wx = fabs(cur.p.x), wy = fabs(cur.p.y), wz = fabs(cur.p.z), tmp;
+        if (wx < wy) { tmp = wx; wx = wy; wy = tmp; }
+        if (wx < wz) { tmp = wx; wx = wz; wz = tmp; }
+        if (wy < wz) { tmp = wy; wy = wz; wz = tmp; }
+        wx = wx * s - ox;
+        wy = wy * s - oy;
+        wz = wz * s - oz;
+        if (wz < thr) wz = wz + oz;
+        cur.p = mk(wx, wy, wz);
+        cur.dr = cur.dr * s;
+        break;
+    }
+    case GOP_KALEIDO: {
+        const T sector = T(o.v[0]);
+        T ang = atan2(cur.p.z, cur.p.x) + T(o.v[1]);
+        ang = fabs(rem_euclid(ang + sector * T(0.5), sector) - sector * T(0.5));
+        const T r = len2d(cur.p.x, cur.p.z);
+        cur.p.x = r * cos(ang);
+        cur.p.z = r * sin(ang);
+        break;
+    }
+    case GOP_POWER: {
+        const T radius = len3(cur.p);
+        if (a != 0u && radius >= T(1e-12)) {
+            T deriv;
+            spherical_power(cur.p, radius, a, deriv);
+            cur.dr = cur.dr * deriv;
+        } else {
+            trig_power(cur.p, radius, T(o.v[0]), cur.dr);
+        }
+        break;
+    }
+    case GOP_ADD_ORIGIN:
+        cur.p = add(cur.p, c);
+        cur.dr = cur.dr + T(1);
+        break;
+    case GOP_ADD_SEED:
+        cur.p = add(cur.p, mk(sp.x * T(o.v[0]), sp.y * T(o.v[1]), sp.z * T(o.v[2])));
+        cur.dr = cur.dr + sdr * T(o.v[3]);
+        break;
+    case GOP_ADD_CONSTANT:
+        cur.p = add(cur.p, mk(c.x * T(o.v[0]) + T(o.v[3]), c.y * T(o.v[1]) + T(o.v[4]), c.z * T(o.v[2]) + T(o.v[5])));
+        cur.dr = cur.dr + T(o.v[6]);
+        break;
+    case GOP_TWIST: {
+        const T ang = T(o.v[0]) * cur.p.y;
+        const T sn = sin(ang), cs = cos(ang);
+        const T x = cur.p.x, z = cur.p.z;
+        cur.p.x = cs * x - sn * z;
+        cur.p.z = sn * x + cs * z;
+        break;
+    }
+    case GOP_BEND: {
+        const T ang = T(o.v[0]) * cur.p.x;
+        const T sn = sin(ang), cs = cos(ang);
+        const T x = cur.p.x, y = cur.p.y;
+        cur.p.x = cs * x - sn * y;
+        cur.p.y = sn * x + cs * y;
+        break;
+    }
+    case GOP_TILE: {
+        const T px = T(o.v[0]), py = T(o.v[1]), pz = T(o.v[2]);
+        const T hx = px * T(0.5), hy = py * T(0.5), hz = pz * T(0.5);
+        cur.p = mk(rem_euclid(cur.p.x + hx, px) - hx, rem_euclid(cur.p.y + hy, py) - hy, rem_euclid(cur.p.z + hz, pz) - hz);
+        break;
+    }
+    case GOP_REPEAT_FINITE: {
+        const T px = T(o.v[0]), py = T(o.v[1]), pz = T(o.v[2]);
+        const T lx = T(float(o.a)) - T(1), ly = T(float(o.b)) - T(1), lz = T(float(o.c)) - T(1);
+        const T hx = lx * T(0.5), hy = ly * T(0.5), hz = lz * T(0.5);
+        const T cx = clampT(floor(cur.p.x / px + hx + T(0.5)), T(0), lx) - hx;
+        const T cy = clampT(floor(cur.p.y / py + hy + T(0.5)), T(0), ly) - hy;
+        const T cz = clampT(floor(cur.p.z / pz + hz + T(0.5)), T(0), lz) - hz;
+        cur.p = sub(cur.p, mk(cx * px, cy * py, cz * pz));
+        break;
+    }
+    case GOP_TRAP: {
+        const T m = g_measure(b, cur.p);
+        if (a == 0u) cur.t0 = g_trap_update(cur.t0, m, cc);
+        else if (a == 1u) cur.t1 = g_trap_update(cur.t1, m, cc);
+        else if (a == 2u) cur.t2 = g_trap_update(cur.t2, m, cc);
+        else cur.t3 = g_trap_update(cur.t3, m, cc);
+        break;
+    }
+    default: break;
+    }
+}
+
+// The distance state of a fence of kind `kind` at the current point.
+template <class T> __device__ __forceinline__ GDist<T> g_fence_state(uint32_t kind, const GOp& o, const GPoint<T>& s) {
+    return GDist<T>{g_fence(kind, o, s), 0u, s.iter, s.t0, s.t1, s.t2, s.t3};
+}
+
+// Bounded: the branch distance clipped to the box of the point (eval.rs:722-731).
+template <class T> __device__ __forceinline__ void g_bounded(const GOp& o, const GPoint<T>& s, GDist<T>& d) {
+    d.d = fmax(d.d, g_box(s.p, T(o.v[0]), T(o.v[1]), T(o.v[2])) / s.dr);
+}
+
 // A 64-bit pointer made wave-uniform (its instructions are then read with scalar loads).
 __device__ __forceinline__ const GOp* g_uniform(const GOp* p) {
     const uint64_t v = reinterpret_cast<uint64_t>(p);
@@ -259,7 +450,7 @@ __device__ __forceinline__ const GOp* g_uniform(const GOp* p) {
     return reinterpret_cast<const GOp*>((uint64_t(hi) << 32) | lo);
 }
 
-// Slot access by a wave-uniform index: one branch per slot, so the slots stay in registers.
+// Slot access by a wave-uniform index: one branch per slot.
 template <class S, uint32_t N> __device__ __forceinline__ void g_store(S (&slots)[N], uint32_t i, const S& v) {
 #pragma unroll
     for (uint32_t k = 0; k < N; ++k)
@@ -273,9 +464,14 @@ template <class S, uint32_t N> __device__ __forceinline__ S g_load(const S (&slo
     return v;
 }
 
-// Runs a graph program at c.  The program is valid by construction (graph.hip): every
-// ITER_BEGIN/ITER_END pair is properly nested within GRAPH_LEVELS, every slot index is below
-// the slot count, and the program ends with GOP_END within GRAPH_MAX_OPS instructions.
+#define SYF_GRAPH_POINT_OPS(X) \
+    X(GOP_TRANSLATE) X(GOP_ROTATE) X(GOP_SCALE) X(GOP_SCALE_AXES) X(GOP_MIRROR) X(GOP_BOX_FOLD) X(GOP_BOX_FOLD_AXES) \
+    X(GOP_SPHERE_INVERSION) X(GOP_SPHERE_FOLD) X(GOP_ABS) X(GOP_MENGER_FOLD) X(GOP_KALEIDO) X(GOP_POWER)          \
+    X(GOP_ADD_ORIGIN) X(GOP_ADD_CONSTANT) X(GOP_TWIST) X(GOP_BEND) X(GOP_TILE) X(GOP_REPEAT_FINITE) X(GOP_TRAP)
+
+// Interprets a graph program at c.  The program is valid by construction and checked on the
+// host (graph.hip): every ITER_BEGIN/ITER_END pair is properly nested within GRAPH_LEVELS,
+// every slot index is below the slot count, and the program ends with GOP_END.
 template <class T> __device__ __noinline__ GRaw<T> graph_run(const GOp* prog_in, V3<T> c) {
     const GOp* prog = g_uniform(prog_in);
     const T INF = g_inf<T>();
@@ -335,215 +531,22 @@ template <class T> __device__ __noinline__ GRaw<T> graph_run(const GOp* prog_in,
             dead &= ~(1u << lv);
             break;
         }
-        case GOP_TRANSLATE:
-            if (live) cur.p = add(cur.p, mk(T(o.v[0]), T(o.v[1]), T(o.v[2])));
-            break;
-        case GOP_ROTATE:
-            if (live) {
-                const V3<T> b = mk(T(o.v[0]), T(o.v[1]), T(o.v[2]));
-                const T w = T(o.v[3]);
-                const V3<T> v = cur.p;
-                cur.p = add(add(scl(v, w * w - dot3(b, b)), scl(b, dot3(v, b) * T(2))), scl(cross3(b, v), w * T(2)));
-            }
-            break;
-        case GOP_SCALE:
-            if (live) {
-                cur.p = scl(cur.p, T(o.v[0]));
-                cur.dr = cur.dr * T(o.v[1]);
-            }
-            break;
-        case GOP_SCALE_AXES:
-            if (live) {
-                cur.p = mk(cur.p.x * T(o.v[0]), cur.p.y * T(o.v[1]), cur.p.z * T(o.v[2]));
-                cur.dr = cur.dr * T(o.v[3]);
-            }
-            break;
-        case GOP_MIRROR:
-            if (live) {
-                const V3<T> n = mk(T(o.v[0]), T(o.v[1]), T(o.v[2]));
-                const T h = dot3(cur.p, n) - T(o.v[3]);
-                if (h < T(0)) cur.p = sub(cur.p, scl(n, T(2) * h));
-            }
-            break;
-        case GOP_BOX_FOLD:
-            if (live) {
-                const T l = T(o.v[0]);
-                cur.p = mk(clampT(cur.p.x, -l, l) * T(2) - cur.p.x, clampT(cur.p.y, -l, l) * T(2) - cur.p.y,
-                           clampT(cur.p.z, -l, l) * T(2) - cur.p.z);
-            }
-            break;
-        case GOP_BOX_FOLD_AXES:
-            if (live) {
-                const T lx = T(o.v[0]), ly = T(o.v[1]), lz = T(o.v[2]);
-                if (o.a & 1u) cur.p.x = clampT(cur.p.x, -lx, lx) * T(2) - cur.p.x;
-                if (o.a & 2u) cur.p.y = clampT(cur.p.y, -ly, ly) * T(2) - cur.p.y;
-                if (o.a & 4u) cur.p.z = clampT(cur.p.z, -lz, lz) * T(2) - cur.p.z;
-            }
-            break;
-        case GOP_SPHERE_INVERSION:
-            if (live) {
-                T r2, m2;
-                if constexpr (sizeof(T) == 4) {
-                    r2 = o.v[2];
-                    m2 = o.v[3];
-                } else {
-                    r2 = T(o.v[0]) * T(o.v[0]);
-                    m2 = T(o.v[1]) * T(o.v[1]);
-                }
-                const T k = r2 / fmax(dot3(cur.p, cur.p), m2);
-                cur.p = scl(cur.p, k);
-                cur.dr = cur.dr * k;
-            }
-            break;
-        case GOP_SPHERE_FOLD:
-            if (live) {
-                T mr2, fr2, q;
-                if constexpr (sizeof(T) == 4) {
-                    mr2 = o.v[2];
-                    fr2 = o.v[3];
-                    q = o.v[4];
-                } else {
-                    mr2 = T(o.v[0]) * T(o.v[0]);
-                    fr2 = T(o.v[1]) * T(o.v[1]);
-                    q = fr2 / mr2;
-                }
-                const T r2 = dot3(cur.p, cur.p);
-                const T k = r2 < mr2 ? q : (r2 < fr2 ? fr2 / r2 : T(1));
-                cur.p = scl(cur.p, k);
-                cur.dr = cur.dr * k;
-            }
-            break;
-        case GOP_ABS:
-            if (live) {
-                if (o.a & 1u) cur.p.x = fabs(cur.p.x);
-                if (o.a & 2u) cur.p.y = fabs(cur.p.y);
-                if (o.a & 4u) cur.p.z = fabs(cur.p.z);
-            }
-            break;
-        case GOP_MENGER_FOLD:
-            if (live) {
-                const T s = T(o.v[0]);
-                T ox, oy, oz, thr;
-                if constexpr (sizeof(T) == 4) {
-                    ox = o.v[4];
-                    oy = o.v[5];
-                    oz = o.v[6];
-                    thr = o.v[7];
-                } else {
-                    const T s1 = s - T(1);
-                    ox = T(o.v[1]) * s1;
-                    oy = T(o.v[2]) * s1;
-                    oz = T(o.v[3]) * s1;
-                    thr = T(-0.5) * T(o.v[3]) * s1;
-                }
-                T wx = fabs(cur.p.x), wy = fabs(cur.p.y), wz = fabs(cur.p.z), tmp;
-                if (wx < wy) { tmp = wx; wx = wy; wy = tmp; }
-                if (wx < wz) { tmp = wx; wx = wz; wz = tmp; }
-                if (wy < wz) { tmp = wy; wy = wz; wz = tmp; }
-                wx = wx * s - ox;
-                wy = wy * s - oy;
-                wz = wz * s - oz;
-                if (wz < thr) wz = wz + oz;
-                cur.p = mk(wx, wy, wz);
-                cur.dr = cur.dr * s;
-            }
-            break;
-        case GOP_KALEIDO:
-            if (live) {
-                const T sector = T(o.v[0]);
-                T a = atan2(cur.p.z, cur.p.x) + T(o.v[1]);
-                a = fabs(rem_euclid(a + sector * T(0.5), sector) - sector * T(0.5));
-                const T r = len2d(cur.p.x, cur.p.z);
-                cur.p.x = r * cos(a);
-                cur.p.z = r * sin(a);
-            }
-            break;
-        case GOP_POWER:
-            if (live) {
-                const T radius = len3(cur.p);
-                if (o.a != 0u && radius >= T(1e-12)) {
-                    T deriv;
-                    spherical_power(cur.p, radius, o.a, deriv);
-                    cur.dr = cur.dr * deriv;
-                } else {
-                    trig_power(cur.p, radius, T(o.v[0]), cur.dr);
-                }
-            }
-            break;
-        case GOP_ADD_ORIGIN:
-            if (live) {
-                cur.p = add(cur.p, c);
-                cur.dr = cur.dr + T(1);
-            }
-            break;
+#define SYF_G_CASE(OP) \
+        case OP: if (live) g_point<T>(OP, o.a, o.b, o.c, o, cur, c, cur.p, cur.dr); break;
+        SYF_GRAPH_POINT_OPS(SYF_G_CASE)
+#undef SYF_G_CASE
         case GOP_ADD_SEED:
             if (live) {
                 const V3<T> sp = o.a == 0u ? cur.p : g_load(seed_p, o.a - 1u);
                 const T sdr = o.a == 0u ? cur.dr : g_load(seed_dr, o.a - 1u);
-                cur.p = add(cur.p, mk(sp.x * T(o.v[0]), sp.y * T(o.v[1]), sp.z * T(o.v[2])));
-                cur.dr = cur.dr + sdr * T(o.v[3]);
+                g_point<T>(GOP_ADD_SEED, o.a, o.b, o.c, o, cur, c, sp, sdr);
             }
             break;
-        case GOP_ADD_CONSTANT:
-            if (live) {
-                cur.p = add(cur.p, mk(c.x * T(o.v[0]) + T(o.v[3]), c.y * T(o.v[1]) + T(o.v[4]),
-                                      c.z * T(o.v[2]) + T(o.v[5])));
-                cur.dr = cur.dr + T(o.v[6]);
-            }
-            break;
-        case GOP_TWIST:
-            if (live) {
-                const T a = T(o.v[0]) * cur.p.y;
-                const T sn = sin(a), cs = cos(a);
-                const T x = cur.p.x, z = cur.p.z;
-                cur.p.x = cs * x - sn * z;
-                cur.p.z = sn * x + cs * z;
-            }
-            break;
-        case GOP_BEND:
-            if (live) {
-                const T a = T(o.v[0]) * cur.p.x;
-                const T sn = sin(a), cs = cos(a);
-                const T x = cur.p.x, y = cur.p.y;
-                cur.p.x = cs * x - sn * y;
-                cur.p.y = sn * x + cs * y;
-            }
-            break;
-        case GOP_TILE:
-            if (live) {
-                const T px = T(o.v[0]), py = T(o.v[1]), pz = T(o.v[2]);
-                const T hx = px * T(0.5), hy = py * T(0.5), hz = pz * T(0.5);
-                cur.p = mk(rem_euclid(cur.p.x + hx, px) - hx, rem_euclid(cur.p.y + hy, py) - hy,
-                           rem_euclid(cur.p.z + hz, pz) - hz);
-            }
-            break;
-        case GOP_REPEAT_FINITE:
-            if (live) {
-                const T px = T(o.v[0]), py = T(o.v[1]), pz = T(o.v[2]);
-                const T lx = T(float(o.a)) - T(1), ly = T(float(o.b)) - T(1), lz = T(float(o.c)) - T(1);
-                const T hx = lx * T(0.5), hy = ly * T(0.5), hz = lz * T(0.5);
-                const T cx = clampT(floor(cur.p.x / px + hx + T(0.5)), T(0), lx) - hx;
-                const T cy = clampT(floor(cur.p.y / py + hy + T(0.5)), T(0), ly) - hy;
-                const T cz = clampT(floor(cur.p.z / pz + hz + T(0.5)), T(0), lz) - hz;
-                cur.p = sub(cur.p, mk(cx * px, cy * py, cz * pz));
-            }
-            break;
-        case GOP_TRAP:
-            if (live) {
-                const T m = g_measure(o.b, cur.p);
-                if (o.a == 0u) cur.t0 = g_trap_update(cur.t0, m, o.c);
-                else if (o.a == 1u) cur.t1 = g_trap_update(cur.t1, m, o.c);
-                else if (o.a == 2u) cur.t2 = g_trap_update(cur.t2, m, o.c);
-                else cur.t3 = g_trap_update(cur.t3, m, o.c);
-            }
-            break;
-        case GOP_FENCE:
-            dc = GDist<T>{g_fence(o, cur), 0u, cur.iter, cur.t0, cur.t1, cur.t2, cur.t3};
-            break;
+        case GOP_FENCE: dc = g_fence_state(o.a, o, cur); break;
         case GOP_D_SAVE: g_store(ds, o.a, dc); break;
         case GOP_D_LOAD: dc = g_load(ds, o.a); break;
         case GOP_MATTAG: dc.mat = o.a; break;
-        case GOP_BOUNDED: dc.d = fmax(dc.d, g_box(cur.p, T(o.v[0]), T(o.v[1]), T(o.v[2])) / cur.dr); break;
+        case GOP_BOUNDED: g_bounded(o, cur, dc); break;
         case GOP_COMBINE: {
             const GDist<T> A = o.b == GRAPH_ACC ? dc : g_load(ds, o.b);
             const GDist<T> B = o.c == GRAPH_ACC ? dc : g_load(ds, o.c);
@@ -556,5 +559,10 @@ template <class T> __device__ __noinline__ GRaw<T> graph_run(const GOp* prog_in,
     }
     return GRaw<T>{dc, runs};
 }
+
+// The specialised evaluator of one graph topology: generated from the compiled program on the
+// host (graph.hip) and compiled at run time into the kernels of that topology (kernel family
+// FAM_JIT); the library's own build never instantiates it.
+template <class T> __device__ GRaw<T> graph_jit(const GOp* prog, V3<T> c);
 
 }  // namespace syf

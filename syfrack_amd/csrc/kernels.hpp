@@ -1,9 +1,16 @@
-// Host-side launchers of the library's kernels (each defined next to its kernels).
+// Host-side launchers of the library's kernels (each defined next to its kernels), the device
+// structures they share with the kernels, and the family dispatch of a launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "params.hpp"
+
+#ifndef __HIPCC_RTC__
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#endif
 
 namespace syf {
 
@@ -49,6 +56,7 @@ struct TraceBuffers {
     unsigned long long* ctr;   // work counters (CTR_*)
 };
 
+#ifndef __HIPCC_RTC__
 // Entries of the hit queue a width x height primary trace needs on a device of cu_count CUs.
 size_t hitq_capacity(uint32_t width, uint32_t height, int cu_count);
 // The fused march + hit-shading kernel, then the trailing pass over the hits it left.
@@ -92,5 +100,53 @@ hipError_t launch_render_tile(const JobP& P, uint32_t x0, uint32_t y0, uint32_t 
 
 hipError_t launch_de_probe(const FractalP& F, const float4* pts, uint32_t n, float4* out, float4* traps,
                            hipStream_t s);
+
+// Kernels of a specialised graph topology (FAM_JIT), compiled at run time by graph.hip: one
+// entry per family-templated kernel, in this order.
+enum : uint32_t {
+    JIT_k_primary,
+    JIT_k_primary_exact,
+    JIT_k_hits_tail,
+    JIT_k_de_probe,
+    JIT_k_render_tile,
+    JIT_k_wf_intersect,
+    JIT_k_wf_shade,
+    JIT_k_wf_commit,
+    JIT_KERNELS
+};
+// Launches kernel `which` of the compiled topology `jit` (FractalP::gjit) with its arguments'
+// addresses, like hipModuleLaunchKernel.
+hipError_t jit_launch_args(const void* jit, uint32_t which, dim3 grid, dim3 block, size_t shmem, hipStream_t s,
+                           void** args);
+// The arguments converted to the parameter types of `kernel` (the library's instantiation of the
+// same template, whose signature the run-time one shares), then launched.
+template <class... Params, class... Args>
+hipError_t jit_launch_as(void (*)(Params...), const void* jit, uint32_t which, dim3 grid, dim3 block, size_t shmem,
+                         hipStream_t s, Args&&... a) {
+    static_assert(sizeof...(Params) == sizeof...(Args), "argument count of a run-time kernel launch");
+    std::tuple<std::decay_t<Params>...> v(std::forward<Args>(a)...);
+    return std::apply(
+        [&](auto&... x) {
+            void* args[] = {static_cast<void*>(&x)...};
+            return jit_launch_args(jit, which, grid, block, shmem, s, args);
+        },
+        v);
+}
+
+// Launches KERNEL<family of F> with hipLaunchKernelGGL's arguments (grid, block, shared memory,
+// stream, kernel arguments); a specialised graph topology launches its run-time compiled
+// instance.  Sets `err` to the launch status.
+#define SYF_DISPATCH(err, F, KERNEL, grid, block, shmem, stream, ...)                                       \
+    switch (kernel_family(F)) {                                                                           \
+    case SYF_FAMILY_MANDELBULB: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_MANDELBULB>, grid, block, shmem, stream, __VA_ARGS__); err = hipGetLastError(); break; \
+    case SYF_FAMILY_MANDELBOX: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_MANDELBOX>, grid, block, shmem, stream, __VA_ARGS__); err = hipGetLastError(); break;   \
+    case SYF_FAMILY_MENGER: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_MENGER>, grid, block, shmem, stream, __VA_ARGS__); err = hipGetLastError(); break;         \
+    case FAM_BULB_TRIG: hipLaunchKernelGGL(KERNEL<FAM_BULB_TRIG>, grid, block, shmem, stream, __VA_ARGS__); err = hipGetLastError(); break;                 \
+    case FAM_BULB8: hipLaunchKernelGGL(KERNEL<FAM_BULB8>, grid, block, shmem, stream, __VA_ARGS__); err = hipGetLastError(); break;                         \
+    case FAM_GRAPH: hipLaunchKernelGGL(KERNEL<FAM_GRAPH>, grid, block, shmem, stream, __VA_ARGS__); err = hipGetLastError(); break;                         \
+    case FAM_JIT: err = jit_launch_as(&KERNEL<FAM_GRAPH>, (F).gjit, JIT_##KERNEL, grid, block, shmem, stream, __VA_ARGS__); break;                         \
+    default: hipLaunchKernelGGL(KERNEL<SYF_FAMILY_SPHERE>, grid, block, shmem, stream, __VA_ARGS__); err = hipGetLastError(); break;                        \
+    }
+#endif  // __HIPCC_RTC__
 
 }  // namespace syf

@@ -42,6 +42,9 @@ struct FractalP {
     const GOp* gprog;
     float gtrap_scale[4];
     uint32_t gshade_slot;
+    // the run-time compiled kernels of the graph's topology (host memory, never read by a
+    // kernel), or null: the graph runs on the interpreter
+    const void* gjit;
 };
 
 struct CameraP {
@@ -103,11 +106,12 @@ constexpr uint32_t FAM_BULB_TRIG = 4u;
 constexpr uint32_t FAM_BULB8 = 5u;
 constexpr uint32_t FAM_COUNT = 6u;  // kernel families of the classic graphs: the four, trig and power-8 Mandelbulb
 constexpr uint32_t FAM_GRAPH = 6u;  // any node graph, run by the graph interpreter (not batched by the server)
+constexpr uint32_t FAM_JIT = 7u;    // a node graph whose topology has run-time compiled kernels
 constexpr bool is_bulb(uint32_t fam) {
     return fam == SYF_FAMILY_MANDELBULB || fam == FAM_BULB_TRIG || fam == FAM_BULB8;
 }
 __host__ __device__ inline uint32_t kernel_family(const FractalP& F) {
-    if (F.family == SYF_FAMILY_GRAPH) return FAM_GRAPH;
+    if (F.family == SYF_FAMILY_GRAPH) return F.gjit ? FAM_JIT : FAM_GRAPH;
     if (F.family != SYF_FAMILY_MANDELBULB) return F.family;
     return F.ipower == 0u ? FAM_BULB_TRIG : (F.ipower == 8u ? FAM_BULB8 : SYF_FAMILY_MANDELBULB);
 }

@@ -270,6 +270,18 @@ class Graph:
             raise ValueError(lib.syf_last_error().decode())
         return step.value, prod.value, tuple(scales)
 
+    def source(self) -> str:
+        """The device source of this graph's specialised topology (syf_graph_source), the
+        counterpart of the WGSL of graph_wgsl::compile."""
+        lib = abi.load()
+        cg = self.c_struct()
+        n = ctypes.c_size_t()
+        if lib.syf_graph_source(ctypes.byref(cg.s), None, 0, ctypes.byref(n)) != abi.SYF_OK:
+            raise ValueError(lib.syf_last_error().decode())
+        buf = ctypes.create_string_buffer(n.value)
+        abi.check(lib.syf_graph_source(ctypes.byref(cg.s), buf, n.value, ctypes.byref(n)))
+        return buf.value.decode()
+
     def validate(self) -> None:
         self.info()
 

@@ -67,14 +67,18 @@ class Context:
 class Job:
     """A render job (syf_job): the lowered scene plus its device planes."""
 
-    def __init__(self, ctx: Context, desc: abi.SyfJobDesc, graph: Optional[Graph] = None):
+    def __init__(self, ctx: Context, desc: abi.SyfJobDesc, graph: Optional[Graph] = None,
+                 graph_mode: str = "specialised"):
         self.ctx = ctx
         self.lib = ctx.lib
         self.desc = desc
         h = C.c_void_p()
         if graph is not None:
+            if graph_mode not in ("specialised", "interpreted"):
+                raise ValueError(f"graph_mode must be 'specialised' or 'interpreted', got {graph_mode!r}")
+            mode = abi.GRAPH_SPECIALISED if graph_mode == "specialised" else abi.GRAPH_INTERPRETED
             cg = graph.c_struct()
-            abi.check(self.lib.syf_job_create_graph(ctx.handle, C.byref(desc), cg.ref(), C.byref(h)))
+            abi.check(self.lib.syf_job_create_graph(ctx.handle, C.byref(desc), cg.ref(), mode, C.byref(h)))
         else:
             abi.check(self.lib.syf_job_create(ctx.handle, C.byref(desc), C.byref(h)))
         self.handle = h
@@ -167,7 +171,7 @@ class Renderer:
 
     def job(self, scene: Scene, cfg: RenderConfig) -> Job:
         graph = scene.fractal if isinstance(scene.fractal, Graph) else None
-        return Job(self.ctx, scene.desc(cfg), graph)
+        return Job(self.ctx, scene.desc(cfg), graph, cfg.graph_mode)
 
     def probe_distances(self, scene: Scene, points: np.ndarray) -> DeSample:
         cfg = RenderConfig(1, 1, 1, 0, 1)

@@ -230,6 +230,9 @@ class RenderConfig:
     # arithmetic of the primary trace's FP32 march: "fast" (hardware reciprocal, reciprocal
     # square root and logarithm, FMA) or "exact" (IEEE operations as written); not a reference field
     march_math: str = "fast"
+    # a node graph's kernels: "specialised" (compiled for its topology at run time) or
+    # "interpreted" (the library's graph interpreter); not a reference field
+    graph_mode: str = "specialised"
 
 
 @dataclass

@@ -203,3 +203,41 @@ def test_graph_job_errors(renderer):
     bad.node(9).params["radius"] = 1000.0
     with pytest.raises(abi.SyfError, match=r"node 9: parameter `fractal.nodes\[9\].radius`"):
         renderer.job(sc.Scene(bad, scene.camera), cfg)
+
+
+def _job_probe(renderer, scene, points, mode):
+    from syfrack_amd import scene as sc
+    j = renderer.job(scene, sc.RenderConfig(1, 1, 1, 0, 1, graph_mode=mode))
+    try:
+        return j.probe(points)
+    finally:
+        j.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_specialised_kernels_equal_the_interpreter(renderer, name):
+    """A topology's run-time compiled kernels compute what the interpreter computes, bit for
+    bit: the probe, the primary trace in both arithmetics, the megakernel's radiance."""
+    from syfrack_amd import scene as sc
+    g = Graph.bundled(name)
+    p = lcg_points(2048, min(oracle_radius(g), 8.0) * 1.25, 7)
+    a = _job_probe(renderer, sc.Scene(g, sc.Camera()), p, "specialised")
+    b = _job_probe(renderer, sc.Scene(g, sc.Camera()), p, "interpreted")
+    for f in ("distance", "traps", "iterations", "material"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    scene, cfg = framed(name, g, 96, 54)
+    for math in ("fast", "exact"):
+        cs = sc.RenderConfig(96, 54, 1, 0, 512, march_math=math, graph_mode="specialised")
+        ci = sc.RenderConfig(96, 54, 1, 0, 512, march_math=math, graph_mode="interpreted")
+        os_, oi = renderer.trace_primary(scene, cs), renderer.trace_primary(scene, ci)
+        for plane in ("depth", "normals", "depth64", "steps"):
+            assert np.array_equal(getattr(os_, plane), getattr(oi, plane)), (math, plane)
+    sky = sc.Light.environment((0.85, 0.9, 1.0), (0.55, 0.55, 0.6), (0.15, 0.13, 0.12), 1.0)
+    lit = sc.Scene(g, scene.camera, [sky], sc.Material(albedo=(0.8, 0.75, 0.7), roughness=0.35), seed=4)
+    rs = renderer.render(lit, sc.RenderConfig(48, 27, 4, 2, 256, graph_mode="specialised"))
+    ri = renderer.render(lit, sc.RenderConfig(48, 27, 4, 2, 256, graph_mode="interpreted"))
+    assert np.array_equal(rs.accum, ri.accum) and np.array_equal(rs.depth64, ri.depth64)
+    if name in ("cathedrale", "evidement"):
+        ws = renderer.render(lit, sc.RenderConfig(48, 27, 4, 2, 256, graph_mode="specialised"), wavefront=True)
+        wi = renderer.render(lit, sc.RenderConfig(48, 27, 4, 2, 256, graph_mode="interpreted"), wavefront=True)
+        assert np.array_equal(ws.accum, wi.accum)

@@ -213,3 +213,50 @@ def test_set_param_follows_the_bounds():
     assert g.node(8).params["offset"] == (0.0, 0.25, 0.0)
     g.set_param("fractal.safety", 0.4)
     assert g.safety == 0.4
+
+
+def _hiprtc():
+    import ctypes
+    path = "/opt/rocm/lib/libhiprtc.so"
+    if not os.path.exists(path):
+        pytest.skip("hiprtc is not installed")
+    return ctypes.CDLL(path)
+
+
+@pytest.mark.parametrize("name", GRAPHS)
+def test_specialised_source_compiles(name):
+    """The generated device source of every bundled topology compiles (its probe kernel, with the
+    options the library passes to hiprtc at run time); the other kernels share its header code."""
+    import ctypes
+    import glob
+    rtc = _hiprtc()
+    src = Graph.bundled(name).source()
+    assert "graph_jit" in src and "GOP_" in src
+    csrc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "syfrack_amd", "csrc")
+    clang = sorted(glob.glob("/opt/rocm/lib/llvm/lib/clang/*/include"))[-1]
+    opts = [b"--offload-arch=gfx950", b"-O1", b"-std=c++17", b"-ffp-contract=off", b"-w",
+            ("-I" + csrc).encode(), b"-I/opt/rocm/include", ("-I" + clang).encode()]
+    prog = ctypes.c_void_p()
+    assert rtc.hiprtcCreateProgram(ctypes.byref(prog), src.encode(), b"t.hip", 0, None, None) == 0
+    rtc.hiprtcAddNameExpression(prog, b"syf::k_de_probe<7u>")
+    arr = (ctypes.c_char_p * len(opts))(*opts)
+    rc = rtc.hiprtcCompileProgram(prog, len(opts), arr)
+    n = ctypes.c_size_t()
+    rtc.hiprtcGetProgramLogSize(prog, ctypes.byref(n))
+    log = ctypes.create_string_buffer(n.value + 1)
+    rtc.hiprtcGetProgramLog(prog, log)
+    rtc.hiprtcDestroyProgram(ctypes.byref(prog))
+    assert rc == 0, log.value.decode(errors="replace")[:2000]
+
+
+def test_graphs_of_one_topology_share_their_source():
+    """The specialised source depends on the structure only: parameters are read at run time."""
+    a = Graph.bundled("mandelbox")
+    b = a.copy()
+    b.set_param("fractal.nodes[1].iterations", 9)
+    b.set_param("fractal.nodes[5].factor", -1.5)
+    b.set_param("fractal.nodes[4].min_radius", 0.25)
+    assert a.source() == b.source()
+    c = a.copy()
+    c.node(3).params["slot"] = 2  # a trap slot is structure
+    assert a.source() != c.source()
