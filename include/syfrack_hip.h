@@ -290,17 +290,23 @@ int syf_job_destroy(syf_job* job);
  * hint).  Every job entry point then evaluates the graph (probe, primary trace, megakernel and
  * wavefront path tracing); the batched server takes classic families only.  Replaces the
  * per-topology pipeline of the reference (graph_wgsl::compile, pipeline_cache.rs) for this
- * path.  `mode`: SYF_GRAPH_SPECIALISED compiles the kernels of the graph's topology at run time
- * (hiprtc, a few seconds the first time a topology is seen in the process; graphs of one
- * topology share them whatever their parameters), SYF_GRAPH_INTERPRETED runs the library's
- * graph interpreter (no compilation, slower).  Both compute the same values. */
+ * path.  `mode`: SYF_GRAPH_SPECIALISED runs a graph of a classic family's topology (see
+ * syf_graph_info) on that family's kernels, as syf_job_create with the family's parameters, and
+ * compiles the kernels of any other topology at run time (hiprtc, a few seconds the first time a
+ * topology is seen in the process; graphs of one topology share them whatever their
+ * parameters); SYF_GRAPH_INTERPRETED runs the library's graph interpreter (no compilation,
+ * slower).  They compute the same values (with SYF_MATH_EXACT; the fast march arithmetic
+ * applies to the classic families). */
 #define SYF_GRAPH_SPECIALISED 0u
 #define SYF_GRAPH_INTERPRETED 1u
 int syf_job_create_graph(syf_ctx* ctx, const syf_job_desc* desc, const syf_graph* graph, uint32_t mode, syf_job** out);
 /* Derived properties of a graph without a job: the step scale (eval.rs:879-881), the iteration
- * product (validate.rs:112-137) and the trap normalisation scales (eval.rs:853-877).  Validates
- * like syf_job_create_graph. */
-int syf_graph_info(const syf_graph* graph, float* step_scale, uint32_t* iteration_product, float* trap_scales4);
+ * product (validate.rs:112-137), the trap normalisation scales (eval.rs:853-877), and the kernel
+ * family a specialised job of it runs on: a classic family (SYF_FAMILY_MANDELBULB ...
+ * SYF_FAMILY_SPHERE) for the bundled classic graphs' topologies with those graphs' trap, or
+ * SYF_FAMILY_GRAPH.  Validates like syf_job_create_graph.  Any output may be null. */
+int syf_graph_info(const syf_graph* graph, float* step_scale, uint32_t* iteration_product, float* trap_scales4,
+                   uint32_t* kernel_family);
 /* The device source of the graph's specialised topology (what SYF_GRAPH_SPECIALISED compiles),
  * the counterpart of the WGSL text of graph_wgsl::compile (graph_wgsl.rs:188-192, 949): *len is
  * set to the size with its terminating NUL; up to cap bytes are written to buf (may be null). */

@@ -281,7 +281,8 @@ SIGNATURES = {
     "syf_job_create": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.POINTER(_P)]),
     "syf_job_destroy": (C.c_int, [_P]),
     "syf_job_create_graph": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.POINTER(SyfGraph), C.c_uint32, C.POINTER(_P)]),
-    "syf_graph_info": (C.c_int, [C.POINTER(SyfGraph), C.POINTER(C.c_float), C.POINTER(C.c_uint32), _P]),
+    "syf_graph_info": (C.c_int, [C.POINTER(SyfGraph), C.POINTER(C.c_float), C.POINTER(C.c_uint32), _P,
+                                 C.POINTER(C.c_uint32)]),
     "syf_graph_source": (C.c_int, [C.POINTER(SyfGraph), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "syf_job_clear": (C.c_int, [_P]),
     "syf_job_sync": (C.c_int, [_P]),

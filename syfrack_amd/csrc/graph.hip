@@ -157,6 +157,9 @@ struct Node {
 // The checked graph, its plan and its derived properties.
 struct Compiled {
     std::vector<GOp> prog;
+    std::vector<uint32_t> node_of;  // the node of each instruction
+    syf_fractal classic{};          // the classic family the graph is, when classic_family != SYF_FAMILY_GRAPH
+    uint32_t classic_family = SYF_FAMILY_GRAPH;
     float step_scale = 1.0f;
     uint32_t iteration_product = 1;
     float trap_scales[4] = {1, 1, 1, 1};
@@ -483,6 +486,8 @@ struct Compiler {
     bool pfree[GRAPH_PSLOTS], dfree[GRAPH_DSLOTS];
     uint32_t cur = VAL_NONE, dc = VAL_NONE;
     int err = SYF_OK;
+    std::vector<uint32_t> node_of;  // per instruction: the node it comes from (VAL_NONE: none)
+    uint32_t node = VAL_NONE;       // the node being emitted
 
     explicit Compiler(const Graph& g) : G(g) {
         for (bool& f : pfree) f = true;
@@ -547,6 +552,7 @@ struct Compiler {
         o.b = b;
         o.c = c;
         prog.push_back(o);
+        node_of.push_back(node);
         return prog.back();
     }
     int slot_full(uint32_t id, const char* what, uint32_t n) {
@@ -592,6 +598,7 @@ struct Compiler {
     // point of the enclosing Iterate at level L - 1).
     void point_op(const Node& n, uint32_t level, uint32_t seed) {
         const float* v = n.v;
+        node = n.id;
         switch (n.kind) {
         case SYF_NODE_TRANSLATE: {
             GOp& o = emit(GOP_TRANSLATE);
@@ -834,6 +841,7 @@ struct Compiler {
         dc = v;
     }
     void fence_op(const Node& n) {
+        node = n.id;
         GOp& o = emit(GOP_FENCE, n.enabled ? n.kind : SYF_NODE_KINDS, n.kind == SYF_NODE_ROCK ? n.u[0] : 0u);
         for (int k = 0; k < 6; ++k) o.v[k] = n.v[k];
         if (n.kind == SYF_NODE_PLANE) normal_of(n.v, o.v);
@@ -877,6 +885,68 @@ int verify(const std::vector<GOp>& prog) {
     return SYF_OK;
 }
 
+// The classic family a compiled graph is, if any: the program of the bundled Mandelbulb,
+// Mandelbox, Menger or sphere graph (graphs/*.ron) whatever its parameters, every node enabled,
+// the trap in slot 0 measuring the norm at its minimum, the Mandelbox's norm surface without
+// inset.  Those graphs run on the classic families' kernels, which compute the same values (the
+// same formulas in the same order; bit-identical in exact arithmetic) and also have the fast FP32
+// march.  Fills out.classic from the nodes' own parameters.
+void classic_of(const Graph& G, Compiled& out) {
+    const std::vector<GOp>& p = out.prog;
+    auto node = [&](size_t i) -> const Node& { return *G.node(out.node_of[i]); };
+    auto ops = [&](std::initializer_list<uint32_t> want) {
+        if (p.size() != want.size()) return false;
+        size_t i = 0;
+        for (uint32_t w : want)
+            if (p[i++].op != w) return false;
+        return true;
+    };
+    auto trap_ok = [&](size_t i) { return p[i].a == 0u && p[i].b == SYF_MEASURE_NORM && p[i].c == SYF_TRAP_MINIMUM; };
+    syf_fractal f;
+    std::memset(&f, 0, sizeof f);
+    f.safety = G.safety;
+    f.bounding_radius_hint = G.hint;
+    f.trap_scale = out.trap_scales[0];
+    f.plane_normal[1] = 1.0f;
+    uint32_t fam = SYF_FAMILY_GRAPH;
+    if (ops({GOP_P_START, GOP_ITER_BEGIN, GOP_TRAP, GOP_POWER, GOP_ADD_ORIGIN, GOP_ITER_END, GOP_FENCE, GOP_END}) &&
+        trap_ok(2) && p[6].a == SYF_NODE_BULB_FENCE) {
+        fam = SYF_FAMILY_MANDELBULB;
+        f.power = node(3).v[0];
+    } else if (ops({GOP_P_START, GOP_ITER_BEGIN, GOP_BOX_FOLD, GOP_TRAP, GOP_SPHERE_FOLD, GOP_SCALE, GOP_ADD_ORIGIN,
+                    GOP_ITER_END, GOP_FENCE, GOP_END}) &&
+               trap_ok(3) && p[8].a == SYF_NODE_NORM_FENCE && node(8).v[0] == 0.0f) {
+        fam = SYF_FAMILY_MANDELBOX;
+        f.fold_limit = node(2).v[0];
+        f.min_radius = node(4).v[0];
+        f.fixed_radius = node(4).v[1];
+        f.scale = node(5).v[0];
+    } else if (ops({GOP_P_START, GOP_ITER_BEGIN, GOP_MENGER_FOLD, GOP_TRAP, GOP_ITER_END, GOP_FENCE, GOP_END}) &&
+               trap_ok(3) && p[5].a == SYF_NODE_BOX) {
+        fam = SYF_FAMILY_MENGER;
+        f.scale = node(2).v[0];
+        for (int k = 0; k < 3; ++k) {
+            f.offset[k] = node(2).v[1 + k];
+            f.half[k] = node(5).v[k];
+        }
+    } else if (ops({GOP_P_START, GOP_TRAP, GOP_FENCE, GOP_D_SAVE, GOP_FENCE, GOP_COMBINE, GOP_END}) && trap_ok(1) &&
+               p[2].a == SYF_NODE_SPHERE && p[4].a == SYF_NODE_PLANE && p[5].a == SYF_NODE_UNION &&
+               p[3].a == p[5].b && p[5].c == GRAPH_ACC) {
+        fam = SYF_FAMILY_SPHERE;
+        f.radius = node(2).v[0];
+        for (int k = 0; k < 3; ++k) f.plane_normal[k] = node(4).v[k];
+        f.plane_offset = node(4).v[3];
+    }
+    if (fam == SYF_FAMILY_GRAPH) return;
+    if (fam != SYF_FAMILY_SPHERE) {
+        f.iterations = p[1].b;
+        f.bailout = p[1].v[0];
+    }
+    f.family = fam;
+    out.classic = f;
+    out.classic_family = fam;
+}
+
 // Validates, derives and compiles a graph.
 int compile(const syf_graph& cg, Compiled& out) {
     Graph G;
@@ -908,6 +978,8 @@ int compile(const syf_graph& cg, Compiled& out) {
         return graph_err("the compiled graph has %zu instructions, above this library's %u", C.prog.size(), GRAPH_MAX_OPS);
     if (int rc = verify(C.prog); rc != SYF_OK) return rc;
     out.prog = std::move(C.prog);
+    out.node_of = std::move(C.node_of);
+    classic_of(G, out);
     return SYF_OK;
 }
 
@@ -1133,11 +1205,13 @@ using namespace syf;
 
 extern "C" {
 
-int syf_graph_info(const syf_graph* graph, float* step_scale, uint32_t* iteration_product, float* trap_scales4) {
+int syf_graph_info(const syf_graph* graph, float* step_scale, uint32_t* iteration_product, float* trap_scales4,
+                   uint32_t* kernel_family) {
     clear_error();
     if (!graph) return fail(SYF_E_INVALID, "graph must not be null");
     Compiled c;
     if (int rc = compile(*graph, c); rc != SYF_OK) return rc;
+    if (kernel_family) *kernel_family = c.classic_family;
     if (step_scale) *step_scale = c.step_scale;
     if (iteration_product) *iteration_product = c.iteration_product;
     if (trap_scales4)
@@ -1171,6 +1245,12 @@ int syf_job_create_graph(syf_ctx* ctx, const syf_job_desc* desc, const syf_graph
                     desc->fractal.family);
     Compiled c;
     if (int rc = compile(*graph, c); rc != SYF_OK) return rc;
+    if (mode == SYF_GRAPH_SPECIALISED && c.classic_family != SYF_FAMILY_GRAPH) {
+        // a classic graph: the classic family's kernels
+        syf_job_desc d = *desc;
+        d.fractal = c.classic;
+        return syf_job_create(ctx, &d, out);
+    }
     FractalP F;
     std::memset(&F, 0, sizeof F);
     F.family = SYF_FAMILY_GRAPH;

@@ -265,10 +265,18 @@ class Graph:
         step = ctypes.c_float()
         prod = ctypes.c_uint32()
         scales = (ctypes.c_float * 4)()
-        rc = lib.syf_graph_info(ctypes.byref(cg.s), ctypes.byref(step), ctypes.byref(prod), scales)
+        fam = ctypes.c_uint32()
+        rc = lib.syf_graph_info(ctypes.byref(cg.s), ctypes.byref(step), ctypes.byref(prod), scales, ctypes.byref(fam))
         if rc != abi.SYF_OK:
             raise ValueError(lib.syf_last_error().decode())
+        self._kernel_family = fam.value
         return step.value, prod.value, tuple(scales)
+
+    def kernel_family(self) -> int:
+        """The classic family (abi.FAMILY_*) a specialised job of this graph runs on, or
+        abi.FAMILY_GRAPH (syf_graph_info)."""
+        self.info()
+        return self._kernel_family
 
     def source(self) -> str:
         """The device source of this graph's specialised topology (syf_graph_source), the

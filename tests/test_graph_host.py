@@ -260,3 +260,22 @@ def test_graphs_of_one_topology_share_their_source():
     c = a.copy()
     c.node(3).params["slot"] = 2  # a trap slot is structure
     assert a.source() != c.source()
+
+
+def test_classic_topologies_run_on_the_classic_kernels():
+    from syfrack_amd import abi
+    fam = {"mandelbulb": abi.FAMILY_MANDELBULB, "mandelbox": abi.FAMILY_MANDELBOX, "menger": abi.FAMILY_MENGER,
+           "sphere": abi.FAMILY_SPHERE}
+    for name in GRAPHS:
+        assert Graph.bundled(name).kernel_family() == fam.get(name, abi.FAMILY_GRAPH), name
+    g = Graph.bundled("mandelbulb")
+    g.set_param("fractal.nodes[1].iterations", 7)
+    g.set_param("fractal.nodes[3].power", 5.5)
+    assert g.kernel_family() == abi.FAMILY_MANDELBULB  # parameters do not matter
+    for change in [lambda g: setattr(g.node(2), "enabled", False),            # a disabled box fold
+                   lambda g: g.node(7).params.__setitem__("inset", 0.5),      # an inset surface
+                   lambda g: g.node(3).params.__setitem__("slot", 1),         # another trap slot
+                   lambda g: g.node(3).params.__setitem__("mode", "Last")]:   # another trap mode
+        g = Graph.bundled("mandelbox")
+        change(g)
+        assert g.kernel_family() == abi.FAMILY_GRAPH
