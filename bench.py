@@ -137,23 +137,32 @@ def error_report(r, march_math: str) -> dict:
     rng = np.random.default_rng(0)
     for fam, name in ((abi.PLANAR_MANDELBROT, "mandelbrot"), (abi.PLANAR_BURNING_SHIP, "burning_ship")):
         c = boundary_centre(r.ctx, fam, seed=0, iterations=2000)
-        for span in (1e-6, 1e-12):
-            v = PlanarView(family=fam, mode=abi.PLANAR_PERTURB, width=4096, height=4096, iterations=10000,
+        for span, mode in ((1e-6, abi.PLANAR_PERTURB), (1e-12, abi.PLANAR_PERTURB), (1e-12, abi.PLANAR_PERTURB64)):
+            v = PlanarView(family=fam, mode=mode, width=4096, height=4096, iterations=10000,
                            center=c, span=span, bailout=16.0)
             p = Planar(r.ctx, v)
             p.run()
             it, _ = p.readback()
+            st = p.stats()
             p.close()
             xs = rng.integers(0, 4096, 192)
             ys = rng.integers(0, 4096, 192)
-            q = oracle.escape_f128(v.c_struct(), xs, ys)[0].astype(np.int64)
+            # PERTURB follows c = reference + f32 offset: binary128 there, resolvable under 16 f32
+            # ulps of the offset; PERTURB64 follows the exact FP64 offset: binary128 there,
+            # resolvable under 64 FP64 ulps
+            truth, nudge = ((oracle.escape_f128, 16.0) if mode == abi.PLANAR_PERTURB
+                            else (oracle.escape_f128_exact, 64.0))
+            q = truth(v.c_struct(), xs, ys)[0].astype(np.int64)
             stable = np.ones(len(xs), bool)
-            for nudge in (16.0, -16.0):
-                stable &= np.abs(oracle.escape_f128(v.c_struct(), xs, ys, nudge)[0].astype(np.int64) - q) <= 1
+            for n in (nudge, -nudge):
+                stable &= np.abs(truth(v.c_struct(), xs, ys, n)[0].astype(np.int64) - q) <= 1
             near = np.abs(it[ys, xs].astype(np.int64) - q) <= 1
-            views.append({"family": name, "span": span, "sampled": int(len(xs)), "resolvable": int(stable.sum()),
+            views.append({"family": name, "span": span,
+                          "mode": "perturb" if mode == abi.PLANAR_PERTURB else "perturb64",
+                          "sampled": int(len(xs)), "resolvable": int(stable.sum()),
                           "count_within_1_of_binary128_on_resolvable": float(near[stable].mean())
-                          if stable.any() else None})
+                          if stable.any() else None,
+                          "fallback_pixels": st["fallback_pixels"], "references": st["references"]})
     out["deep_zoom_4096_10000_iterations"] = views
 
     progress("errors: served requests")
@@ -269,8 +278,11 @@ def planar_leg(size: int, seeds: int) -> int:
 
 def planar_results(size: int, seeds: int) -> dict:
     """2D deep zoom: size x size Mandelbrot and Burning Ship images at centres drawn in the
-    boundary band (seeded), spans 1e-6 and 1e-12, 10000 iterations, FP64 reference orbit + FP32
-    perturbation with glitch re-reference.  Mpixel/s and iterations/s per view."""
+    boundary band (seeded), spans 1e-6 and 1e-12, 10000 iterations: PERTURB (double-double
+    reference orbits + FP32 deltas with glitch re-reference, the glitches left after 8
+    references taking FP64 deltas with rebasing) at both spans, and PERTURB64 (FP64 deltas
+    with rebasing on every pixel) at 1e-12.  Mpixel/s and iterations/s per view, with the
+    references used and the pixels left to the rebasing fallback."""
     from syfrack_amd import abi
     from syfrack_amd.planar import Planar, PlanarView, boundary_centre
     from syfrack_amd.render import Context
@@ -279,9 +291,10 @@ def planar_results(size: int, seeds: int) -> dict:
     for fam, name in ((abi.PLANAR_MANDELBROT, "mandelbrot"), (abi.PLANAR_BURNING_SHIP, "burning_ship")):
         for seed in range(seeds):
             c = boundary_centre(ctx, fam, seed=seed, iterations=2000)
-            for span in (1e-6, 1e-12):
-                progress(f"planar: {name} seed {seed} span {span:g}")
-                v = PlanarView(family=fam, mode=abi.PLANAR_PERTURB, width=size, height=size, iterations=10000,
+            for span, mode, mname in ((1e-6, abi.PLANAR_PERTURB, "perturb"), (1e-12, abi.PLANAR_PERTURB, "perturb"),
+                                      (1e-12, abi.PLANAR_PERTURB64, "perturb64")):
+                progress(f"planar: {name} seed {seed} span {span:g} {mname}")
+                v = PlanarView(family=fam, mode=mode, width=size, height=size, iterations=10000,
                                center=c, span=span, bailout=16.0)
                 p = Planar(ctx, v)
                 t0 = time.perf_counter()
@@ -290,10 +303,11 @@ def planar_results(size: int, seeds: int) -> dict:
                 el = time.perf_counter() - t0
                 st = p.stats()
                 p.close()
-                rows.append({"family": name, "seed": seed, "centre": list(c), "span": span, "s": el,
+                rows.append({"family": name, "seed": seed, "mode": mname, "centre": list(c), "span": span, "s": el,
                              "mpix_s": size * size / el / 1e6, "giter_s": st["iterations"] / el / 1e9,
                              "iterations": st["iterations"], "glitch_restarts": st["rebases"],
-                             "kernel_ms": st["kernel_ms"]})
+                             "references": st["references"], "fallback_pixels": st["fallback_pixels"],
+                             "rebasings": st["rebasings"], "kernel_ms": st["kernel_ms"]})
     return {"workload": f"{size}x{size} deep zoom, 10000 iterations, perturbation", "views": rows}
 
 

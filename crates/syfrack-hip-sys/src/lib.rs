@@ -113,6 +113,7 @@ pub const SYF_PLANAR_BURNING_SHIP: u32 = 3;
 pub const SYF_PLANAR_FP32: u32 = 0;
 pub const SYF_PLANAR_FP64: u32 = 1;
 pub const SYF_PLANAR_PERTURB: u32 = 2;
+pub const SYF_PLANAR_PERTURB64: u32 = 3;
 pub const SYF_PLANAR_SMOOTH: u32 = 0;
 pub const SYF_PLANAR_ITER: u32 = 1;
 
@@ -289,7 +290,7 @@ pub struct syf_planar_desc {
     pub width: u32,
     pub height: u32,
     pub iterations: u32,
-    pub reserved: u32,
+    pub max_references: u32,
     pub center: [f64; 2],
     pub span: f64,
     pub julia: [f64; 2],
@@ -402,6 +403,8 @@ extern "C" {
     pub fn syf_planar_readback(p: *mut syf_planar, plane: u32, host: *mut c_void, bytes: usize) -> c_int;
     pub fn syf_planar_stats(p: *mut syf_planar, iterations: *mut u64, rebases: *mut u64, kernel_ms: *mut f64)
         -> c_int;
+    pub fn syf_planar_glitches(p: *mut syf_planar, references: *mut u32, fallback_pixels: *mut u32,
+        rebasings: *mut u64) -> c_int;
     pub fn syf_server_create(ctx: *mut syf_ctx, cfg: *const syf_server_config, out: *mut *mut syf_server) -> c_int;
     pub fn syf_server_destroy(s: *mut syf_server) -> c_int;
     pub fn syf_server_submit(s: *mut syf_server, desc: *const syf_job_desc, spp: u32, id: *mut u64) -> c_int;

@@ -408,14 +408,19 @@ int syf_export_read(const char* path, uint64_t bytes, uint64_t hash, uint32_t wi
 #define SYF_PLANAR_JULIA 1u
 #define SYF_PLANAR_FP32 0u        /* direct FP32 iteration (the reference's precision) */
 #define SYF_PLANAR_FP64 1u        /* direct FP64 iteration: counts bit-exact */
-#define SYF_PLANAR_PERTURB 2u     /* FP64 reference orbit + FP32 deltas, rebasing on glitch */
+#define SYF_PLANAR_PERTURB 2u     /* FP64 reference orbits + FP32 deltas, re-referencing glitches;
+                                   * the glitches left after max_references take PERTURB64 */
+#define SYF_PLANAR_PERTURB64 3u   /* the view centre's FP64 orbit + FP64 deltas with rebasing
+                                   * (Zhuoran 2021): every pixel's c keeps its offset's full
+                                   * FP64 precision, whatever the zoom */
 typedef struct syf_planar_desc {
     uint32_t family;          /* SYF_PLANAR_* family */
     uint32_t mode;            /* SYF_PLANAR_FP32 / FP64 / PERTURB */
     uint32_t width;
     uint32_t height;
     uint32_t iterations;      /* maximum iterations (budget) */
-    uint32_t reserved;
+    uint32_t max_references;  /* PERTURB: FP32-delta reference orbits before the FP64 rebasing
+                               * fallback, 1..64 (0 selects 8); ignored by the other modes */
     double center[2];         /* centre of the view in the complex plane */
     double span;              /* width of the view in the complex plane */
     double julia[2];          /* Julia constant */
@@ -434,6 +439,10 @@ int syf_planar_sync(syf_planar* p);
 int syf_planar_readback(syf_planar* p, uint32_t plane, void* host, size_t bytes);
 /* Iterations evaluated, glitch re-references, and device time of the last run. */
 int syf_planar_stats(syf_planar* p, uint64_t* iterations, uint64_t* rebases, double* kernel_ms);
+/* Glitch handling of the last PERTURB / PERTURB64 run: reference orbits used by the FP32-delta
+ * passes, pixels left to the FP64 rebasing loop after them, and rebasing events of that loop.
+ * No reference counterpart (the reference has no deep-zoom mode; planar.rs:77-194 is FP32). */
+int syf_planar_glitches(syf_planar* p, uint32_t* references, uint32_t* fallback_pixels, uint64_t* rebasings);
 
 /* ---------------------------------------------------------------- serving
  * A render server on one device: requests of any family, size, spp and bounce count are

@@ -187,6 +187,43 @@ def escape_f128(desc, px: np.ndarray, py: np.ndarray, nudge: float = 0.0):
     return iters, smooth
 
 
+def escape_f128_exact(desc, px: np.ndarray, py: np.ndarray, nudge: float = 0.0):
+    """binary128 counts at c = centre + the pixel's FP64 offset (+ nudge FP64 ulps of it)."""
+    px = np.ascontiguousarray(px, np.uint32)
+    py = np.ascontiguousarray(py, np.uint32)
+    iters = np.zeros(len(px), np.uint32)
+    smooth = np.zeros(len(px), np.float32)
+    lib = load()
+    lib.orc_escape_f128_exact.restype = None
+    lib.orc_escape_f128_exact(C.byref(desc), _ptr(px), _ptr(py), C.c_size_t(len(px)), C.c_double(nudge),
+                              _ptr(iters), _ptr(smooth))
+    return iters, smooth
+
+
+def reference_orbit(desc, ox: float = 0.0, oy: float = 0.0, cap: int = 100000, quad: bool = False):
+    """The deep-zoom reference orbit of C = centre + (ox, oy): double-double iteration rounded
+    to FP64 (quad=False, the product's orbit), or binary128 iteration rounded to FP64."""
+    zx = np.zeros(cap, np.float64)
+    zy = np.zeros(cap, np.float64)
+    lib = load()
+    fn = lib.orc_reference_orbit_f128 if quad else lib.orc_reference_orbit
+    fn.restype = C.c_uint32
+    last = fn(C.byref(desc), C.c_double(ox), C.c_double(oy), _ptr(zx), _ptr(zy), C.c_uint32(cap))
+    return zx[:last + 1], zy[:last + 1]
+
+
+def perturb64(desc, px: np.ndarray, py: np.ndarray):
+    """The FP64 rebasing perturbation loop (PERTURB64) at listed pixels: counts, smooth."""
+    px = np.ascontiguousarray(px, np.uint32)
+    py = np.ascontiguousarray(py, np.uint32)
+    iters = np.zeros(len(px), np.uint32)
+    smooth = np.zeros(len(px), np.float32)
+    lib = load()
+    lib.orc_perturb64.restype = None
+    lib.orc_perturb64(C.byref(desc), _ptr(px), _ptr(py), C.c_size_t(len(px)), _ptr(iters), _ptr(smooth))
+    return iters, smooth
+
+
 def escape_f64_points(desc, px: np.ndarray, py: np.ndarray):
     """FP64 iteration counts at the points escape_f128 evaluates (centre + f32 offset)."""
     px = np.ascontiguousarray(px, np.uint32)

@@ -23,6 +23,12 @@
 //   planar_sample_f32 / escape_f64 / escape_f128  -> planar.rs:275-345 known answers,
 //       c = 0 never escapes, |c| > 2 escapes at once, conjugate symmetry, agreement of
 //       the FP64 and quad-precision loops where both are exact.
+//   reference_orbit_dd / perturb64 (deep zoom; no reference counterpart)
+//       -> the double-double orbit equals the binary128 orbit rounded to FP64 where a plain
+//       FP64 orbit drifts within two steps; PERTURB64 within 1 of binary128 at the exact
+//       pixel points of 1e-12 views (Burning Ship's chaotic bands included) on every point
+//       whose count survives a 64-ulp move of the offset, where the FP64 direct loop fails;
+//       within 1 of FP64 direct on a whole-set view (tests/test_planar.py).
 //   trig_power (non-integer Mandelbulb powers)    -> equals the integer spherical product
 //       at n = 2..12 in FP32 and FP64, |z^n| = r^n, and continuity in the power
 //       (tests/test_oracle_radiance.py).
@@ -1585,6 +1591,203 @@ void orc_escape_f64_points(const syf_planar_desc* d, const uint32_t* px, const u
         double cx = d->center[0] + double(float(dx));
         double cy = d->center[1] + double(float(dy));
         iters[k] = escape_loop<double>(d->family, cx, cy, 0.0, 0.0, d->iterations, b2).count;
+    }
+    delete[] fx;
+    delete[] fy;
+}
+
+// Binary128 direct evaluation at c = centre + the pixel's FP64 offset itself (moved by `nudge`
+// FP64 ulps of the offset, a conditioning probe): the point PERTURB64 follows, and the one a
+// plain FP64 loop cannot represent once the pixel spacing nears an ulp of the centre.
+void orc_escape_f128_exact(const syf_planar_desc* d, const uint32_t* px, const uint32_t* py, size_t n, double nudge,
+                           uint32_t* iters, float* smooth) {
+    double* fx = new double[d->width];
+    double* fy = new double[d->height];
+    orc_planar_offsets(d, fx, fy);
+    double rot = double(d->rotation) * (3.14159265358979323846 / 180.0);
+    double cr = std::cos(rot), sr = std::sin(rot);
+    __float128 b2 = (__float128)d->bailout * (__float128)d->bailout;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long k = 0; k < long(n); ++k) {
+        double dx = cr * fx[px[k]] - sr * fy[py[k]];
+        double dy = sr * fx[px[k]] + cr * fy[py[k]];
+        const double ux = std::nextafter(std::fabs(dx), INFINITY) - std::fabs(dx);
+        const double uy = std::nextafter(std::fabs(dy), INFINITY) - std::fabs(dy);
+        __float128 cx = (__float128)d->center[0] + (__float128)dx + (__float128)(nudge * ux);
+        __float128 cy = (__float128)d->center[1] + (__float128)dy + (__float128)(nudge * uy);
+        Escape<__float128> e = escape_loop<__float128>(d->family, cx, cy, (__float128)0, (__float128)0, d->iterations, b2);
+        iters[k] = e.count;
+        double r = std::fmin(std::sqrt((double)e.radius2), 1e20);
+        smooth[k] = e.escaped ? float(double(e.count) + 1.0 - std::log(std::log(std::fmax(r, 1.0001))) / std::log(2.0))
+                              : float(d->iterations);
+    }
+    delete[] fx;
+    delete[] fy;
+}
+
+// |c + d| - |c| without cancellation (the perturbation of |x|).
+static double diffabs_f64(double c, double d) {
+    const double cd = c + d;
+    if (c >= 0.0) return cd >= 0.0 ? d : -(2.0 * c + d);
+    return cd > 0.0 ? 2.0 * c + d : -d;
+}
+
+// Double-double arithmetic (Dekker 1971; Knuth's TwoSum): a value is hi + lo with |lo| <=
+// ulp(hi)/2, about 106 bits.  Every step is an IEEE operation rounded as written (fma gives
+// the exact product error), so any restatement of these steps gets the same bits.
+struct DD {
+    double hi, lo;
+};
+static DD dd_two_sum(double a, double b) {
+    const double s = a + b;
+    const double bb = s - a;
+    const double e = (a - (s - bb)) + (b - bb);
+    return DD{s, e};
+}
+static DD dd_norm(double s, double e) {  // |s| >= |e|
+    const double h = s + e;
+    return DD{h, e - (h - s)};
+}
+static DD dd_add(DD a, DD b) {
+    DD s = dd_two_sum(a.hi, b.hi);
+    return dd_norm(s.hi, s.lo + (a.lo + b.lo));
+}
+static DD dd_mul(DD a, DD b) {
+    const double p = a.hi * b.hi;
+    const double e = std::fma(a.hi, b.hi, -p);
+    return dd_norm(p, e + (a.hi * b.lo + a.lo * b.hi));
+}
+static DD dd_neg(DD a) { return DD{-a.hi, -a.lo}; }
+static DD dd_abs(DD a) { return a.hi < 0.0 ? dd_neg(a) : a; }
+
+// The reference orbit of the deep-zoom modes: C = centre + (ox, oy) held exactly as a
+// double-double, Z_0 = 0, Z_{n+1} = f(Z_n) + C iterated in double-double (Burning Ship on
+// |Re|, |Im|), each entry stored rounded to FP64, up to the first entry with
+// |Z|^2 > bailout^2 (FP64 test on the rounded entry) or the iteration budget.
+void orc_reference_orbit_dd(const syf_planar_desc* d, double ox, double oy, std::vector<double>& ZX,
+                            std::vector<double>& ZY) {
+    const double b2 = double(d->bailout) * double(d->bailout);
+    const bool ship = d->family == SYF_PLANAR_BURNING_SHIP;
+    const DD cx = dd_two_sum(d->center[0], ox), cy = dd_two_sum(d->center[1], oy);
+    DD zx{0.0, 0.0}, zy{0.0, 0.0};
+    ZX.assign(1, 0.0);
+    ZY.assign(1, 0.0);
+    for (uint32_t n = 0; n < d->iterations; ++n) {
+        const DD x = ship ? dd_abs(zx) : zx, y = ship ? dd_abs(zy) : zy;
+        const DD xx = dd_mul(x, x), yy = dd_mul(y, y), xy = dd_mul(x, y);
+        zx = dd_add(dd_add(xx, dd_neg(yy)), cx);
+        zy = dd_add(DD{2.0 * xy.hi, 2.0 * xy.lo}, cy);
+        ZX.push_back(zx.hi);
+        ZY.push_back(zy.hi);
+        if (zx.hi * zx.hi + zy.hi * zy.hi > b2) break;
+    }
+}
+
+// C-ABI views of the reference orbit for tests: the double-double orbit's FP64 entries, and
+// the same recurrence in binary128 rounded to FP64 (its pin).  Both return the index of the
+// last entry written (at most cap - 1).
+uint32_t orc_reference_orbit(const syf_planar_desc* d, double ox, double oy, double* zx, double* zy, uint32_t cap) {
+    std::vector<double> ZX, ZY;
+    orc_reference_orbit_dd(d, ox, oy, ZX, ZY);
+    const uint32_t n = uint32_t(std::min<size_t>(ZX.size(), cap));
+    for (uint32_t i = 0; i < n; ++i) {
+        zx[i] = ZX[i];
+        zy[i] = ZY[i];
+    }
+    return n - 1;
+}
+uint32_t orc_reference_orbit_f128(const syf_planar_desc* d, double ox, double oy, double* zx, double* zy,
+                                  uint32_t cap) {
+    const __float128 b2 = (__float128)d->bailout * (__float128)d->bailout;
+    const __float128 cx = (__float128)d->center[0] + (__float128)ox, cy = (__float128)d->center[1] + (__float128)oy;
+    __float128 x = 0, y = 0;
+    uint32_t n = 0;
+    zx[0] = zy[0] = 0.0;
+    while (n + 1 < cap && n < d->iterations) {
+        __float128 a = x, b = y;
+        if (d->family == SYF_PLANAR_BURNING_SHIP) {
+            a = a < 0 ? -a : a;
+            b = b < 0 ? -b : b;
+        }
+        const __float128 nx = a * a - b * b + cx, ny = 2 * a * b + cy;
+        x = nx;
+        y = ny;
+        ++n;
+        zx[n] = (double)x;
+        zy[n] = (double)y;
+        if (x * x + y * y > b2) break;
+    }
+    return n;
+}
+
+// The PERTURB64 loop at listed pixels: the view centre's reference orbit Z
+// (orc_reference_orbit_dd), FP64 deltas delta <- (2 Z + delta) delta + delta_c with
+// delta_c = the pixel's FP64 offset (Burning Ship: the |.| products through diffabs), and
+// rebasing (Zhuoran 2021): when |Z_m + delta| < |delta| or Z has escaped, delta <- Z_m + delta
+// and m <- 0.  Each operation rounds as written.  The deep-zoom mode has no counterpart in the
+// reference (planar.rs:77-194 iterates f32 directly); this loop is pinned by binary128 at the
+// exact pixel points (orc_escape_f128_exact) and by the FP64 direct loop where that resolves.
+void orc_perturb64(const syf_planar_desc* d, const uint32_t* px, const uint32_t* py, size_t n, uint32_t* iters,
+                   float* smooth) {
+    double* fx = new double[d->width];
+    double* fy = new double[d->height];
+    orc_planar_offsets(d, fx, fy);
+    double rot = double(d->rotation) * (3.14159265358979323846 / 180.0);
+    double cr = std::cos(rot), sr = std::sin(rot);
+    const double b2 = double(d->bailout) * double(d->bailout);
+    const bool ship = d->family == SYF_PLANAR_BURNING_SHIP;
+    std::vector<double> ZX, ZY;
+    orc_reference_orbit_dd(d, 0.0, 0.0, ZX, ZY);
+    const uint32_t last = uint32_t(ZX.size() - 1);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long k = 0; k < long(n); ++k) {
+        const double dcx = cr * fx[px[k]] - sr * fy[py[k]];
+        const double dcy = sr * fx[px[k]] + cr * fy[py[k]];
+        double ex = 0.0, ey = 0.0, zx = 0.0, zy = 0.0;
+        uint32_t m = 0, count = 0;
+        bool escaped = false;
+        for (uint32_t i = 0; i < d->iterations; ++i) {
+            double X = ZX[m], Y = ZY[m];
+            zx = X + ex;
+            zy = Y + ey;
+            const double r2 = zx * zx + zy * zy;
+            if (r2 > b2) {
+                escaped = true;
+                break;
+            }
+            if (m == last || r2 < ex * ex + ey * ey) {
+                ex = zx;
+                ey = zy;
+                m = 0;
+                X = 0.0;
+                Y = 0.0;
+            }
+            double nx, ny;
+            if (ship) {
+                nx = (2.0 * X + ex) * ex - (2.0 * Y + ey) * ey + dcx;
+                const double XY = X * Y;
+                const double dXY = (X + ex) * ey + ex * Y;
+                ny = 2.0 * diffabs_f64(XY, dXY) + dcy;
+            } else {
+                const double tx = 2.0 * X + ex, ty = 2.0 * Y + ey;
+                nx = tx * ex - ty * ey + dcx;
+                ny = tx * ey + ty * ex + dcy;
+            }
+            ex = nx;
+            ey = ny;
+            ++m;
+            ++count;
+        }
+        if (!escaped) {
+            zx = ZX[m] + ex;
+            zy = ZY[m] + ey;
+        }
+        const double r2 = zx * zx + zy * zy;
+        escaped = escaped || r2 > b2;
+        iters[k] = count;
+        smooth[k] = escaped ? float(double(count) + 1.0 -
+                                    std::log(std::log(std::fmax(std::fmin(std::sqrt(r2), 1e20), 1.0001))) / std::log(2.0))
+                            : float(d->iterations);
     }
     delete[] fx;
     delete[] fy;

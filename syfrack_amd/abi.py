@@ -49,6 +49,7 @@ PLANAR_BURNING_SHIP = 3
 PLANAR_FP32 = 0
 PLANAR_FP64 = 1
 PLANAR_PERTURB = 2
+PLANAR_PERTURB64 = 3
 PLANAR_SMOOTH = 0
 PLANAR_ITER = 1
 
@@ -174,7 +175,7 @@ class SyfPlanarDesc(C.Structure):
         ("width", C.c_uint32),
         ("height", C.c_uint32),
         ("iterations", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("max_references", C.c_uint32),
         ("center", C.c_double * 2),
         ("span", C.c_double),
         ("julia", C.c_double * 2),
@@ -306,6 +307,7 @@ SIGNATURES = {
     "syf_planar_sync": (C.c_int, [_P]),
     "syf_planar_readback": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
     "syf_planar_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "syf_planar_glitches": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "syf_server_create": (C.c_int, [_P, C.POINTER(SyfServerConfig), C.POINTER(_P)]),
     "syf_server_destroy": (C.c_int, [_P]),
     "syf_server_submit": (C.c_int, [_P, C.POINTER(SyfJobDesc), C.c_uint32, C.POINTER(C.c_uint64)]),

@@ -1,15 +1,25 @@
 // 2D escape-time kernels: PlanarFractal::sample (planar.rs:77-194, planar.wgsl:16-91) for
-// the parameter-plane families, in three precisions:
+// the parameter-plane families, in four modes:
 //   FP32     direct iteration in f32, the reference's arithmetic (radius = |z| by sqrt);
 //   FP64     direct iteration in f64 with the squared escape test |z|^2 > bailout^2, so
 //            every operation is an exactly rounded IEEE add or multiply and the iteration
 //            counts are bit-exact against any IEEE FP64 evaluation of the same loop;
-//   PERTURB  deep zoom: an FP64 reference orbit Z_n (host) and per-pixel FP32 deltas
+//   PERTURB  deep zoom: a reference orbit Z_n (host, double-double, stored rounded to f32)
+//            and per-pixel FP32 deltas
 //            delta <- (2 Z + delta) delta + delta_c (Burning Ship: with diffabs).  A pixel
 //            whose orbit loses precision against the reference (Pauldelbrot's criterion
 //            |Z + delta| < 1e-3 |Z|) or outlives it is a glitch: it is queued (wave64 ballot,
 //            one atomic per wave) and re-run against a new reference, the glitched pixel that
-//            got furthest; after 8 references the few left take the FP64 direct loop.
+//            got furthest; after the reference budget (8 by default) the few left take the
+//            PERTURB64 loop below, so no pixel is ever iterated from a rounded c;
+//   PERTURB64 FP64 deltas against the view centre's orbit (double-double, stored rounded to
+//            FP64) with rebasing (Zhuoran 2021):
+//            when |Z_m + delta| < |delta|, or the reference has escaped, the delta takes the
+//            whole z and the orbit index restarts at Z_0 = 0.  One reference serves every
+//            pixel and c = centre + offset keeps the offset's full FP64 precision (about
+//            1e-28 at span 1e-12), where the direct FP64 loop's c = fl(centre + offset) is
+//            off by up to half an ulp of the centre: half the pixel spacing of a 4096-pixel
+//            view at span 1e-12, and several pixels at smaller spans.
 // Each wave owns an 8x8 pixel tile (neighbours escape at similar counts).  The pixel
 // offsets are host-computed FP64 tables (one per column, one per row), so the device and
 // any checker derive the same c from the same doubles.
@@ -292,42 +302,83 @@ __global__ __launch_bounds__(256) void k_planar_perturb(const PlanarP P, const d
     flush_iters(done_iters, glitched_total, ctr);
 }
 
-// FP64 direct loop over a pixel list: the last resort for glitches left after the
-// reference budget.
+// |c + d| - |c| without cancellation, in FP64.
+__device__ __forceinline__ double diffabs64(double c, double d) {
+    const double cd = c + d;
+    if (c >= 0.0) return cd >= 0.0 ? d : -(2.0 * c + d);
+    return cd > 0.0 ? 2.0 * c + d : -d;
+}
+
+// FP64 perturbation with rebasing over a pixel list (all pixels when list is null), against
+// one FP64 orbit of C = centre + (ref_dx, ref_dy) whose last entry is orbit[orbit_len].
+// A lane's orbit index m is its own (rebasing restarts it), so the orbit is read by per-lane
+// gathers; it is 16 bytes an entry and stays in L2.  Every operation is an IEEE add or
+// multiply rounded as written, so the counts are bit-exact against any FP64 restatement of
+// this loop.
 template <uint32_t FAMILY>
-__global__ __launch_bounds__(256) void k_planar_fp64_list(const PlanarP P, const double* __restrict__ fx,
-                                                          const double* __restrict__ fy,
-                                                          const uint32_t* __restrict__ list, uint32_t n_items,
-                                                          float* __restrict__ smooth, uint32_t* __restrict__ iters,
-                                                          unsigned long long* ctr) {
-    uint32_t total = 0;
+__global__ __launch_bounds__(256) void k_planar_rebase(const PlanarP P, const double* __restrict__ fx,
+                                                       const double* __restrict__ fy,
+                                                       const double2* __restrict__ orbit, uint32_t orbit_len,
+                                                       double ref_dx, double ref_dy,
+                                                       const uint32_t* __restrict__ list, uint32_t n_items,
+                                                       float* __restrict__ smooth, uint32_t* __restrict__ iters,
+                                                       unsigned long long* ctr) {
+    uint32_t total = 0, rebased = 0;
     for (uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
-        const uint32_t pix = list[item];
+        const uint32_t pix = list ? list[item] : item;
         const uint32_t px = pix % P.width, py = pix / P.width;
-        double dx, dy;
-        offset_of(P, fx, fy, px, py, dx, dy);
-        const double cx = P.cx + dx, cy = P.cy + dy;
-        double zx = 0.0, zy = 0.0;
-        uint32_t count = 0;
+        double ddx, ddy;
+        offset_of(P, fx, fy, px, py, ddx, ddy);
+        const double dcx = ddx - ref_dx, dcy = ddy - ref_dy;
+        double ex = 0.0, ey = 0.0, zx = 0.0, zy = 0.0;
+        uint32_t m = 0, count = 0;
         bool escaped = false;
+        // Z = orbit[m] always; the next two entries are loaded two steps ahead, so a gather's
+        // latency overlaps the arithmetic of the steps before it (entries past the last are
+        // never used: the step at m == orbit_len rebases first)
+        double2 Z = make_double2(0.0, 0.0);
+        double2 Z1 = orbit[min(1u, orbit_len)], Z2 = orbit[min(2u, orbit_len)];
         for (uint32_t i = 0; i < P.iterations; ++i) {
-            double r2 = zx * zx + zy * zy;
+            zx = Z.x + ex;
+            zy = Z.y + ey;
+            const double r2 = zx * zx + zy * zy;
             if (r2 > P.b2) {
                 escaped = true;
                 break;
             }
-            double x = zx, y = zy;
-            if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
-                x = x < 0.0 ? -x : x;
-                y = y < 0.0 ? -y : y;
+            if (m == orbit_len || r2 < ex * ex + ey * ey) {  // rebase onto Z_0 = 0
+                ex = zx;
+                ey = zy;
+                m = 0;
+                Z = make_double2(0.0, 0.0);
+                Z1 = orbit[min(1u, orbit_len)];
+                Z2 = orbit[min(2u, orbit_len)];
+                ++rebased;
             }
-            double nx = x * x - y * y + cx;
-            double ny = 2.0 * x * y + cy;
-            zx = nx;
-            zy = ny;
+            double nx, ny;
+            if (FAMILY == SYF_PLANAR_BURNING_SHIP) {
+                nx = (2.0 * Z.x + ex) * ex - (2.0 * Z.y + ey) * ey + dcx;
+                const double XY = Z.x * Z.y;
+                const double dXY = (Z.x + ex) * ey + ex * Z.y;
+                ny = 2.0 * diffabs64(XY, dXY) + dcy;
+            } else {
+                const double tx = 2.0 * Z.x + ex, ty = 2.0 * Z.y + ey;
+                nx = tx * ex - ty * ey + dcx;
+                ny = tx * ey + ty * ex + dcy;
+            }
+            ex = nx;
+            ey = ny;
+            ++m;
             ++count;
+            Z = Z1;
+            Z1 = Z2;
+            Z2 = orbit[min(m + 2u, orbit_len)];
         }
-        double r2 = zx * zx + zy * zy;
+        if (!escaped) {
+            zx = Z.x + ex;
+            zy = Z.y + ey;
+        }
+        const double r2 = zx * zx + zy * zy;
         escaped = escaped || r2 > P.b2;
         float sm = float(P.iterations);
         if (escaped) sm = float(double(count) + 1.0 - log(log(fmax(fmin(sqrt(r2), 1e20), 1.0001))) / log(2.0));
@@ -335,8 +386,11 @@ __global__ __launch_bounds__(256) void k_planar_fp64_list(const PlanarP P, const
         iters[pix] = count;
         total += count;
     }
-    uint32_t a = wave_sum(total);
-    if ((threadIdx.x & 63u) == 0u) atomicAdd(&ctr[0], (unsigned long long)a);
+    const uint32_t a = wave_sum(total), b = wave_sum(rebased);
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(&ctr[0], (unsigned long long)a);
+        atomicAdd(&ctr[2], (unsigned long long)b);
+    }
 }
 
 }  // namespace syf
@@ -353,12 +407,14 @@ struct syf_planar {
     float* smooth = nullptr;
     uint32_t* iters = nullptr;
     float2* orbit = nullptr;
+    double2* orbit64 = nullptr;             // PERTURB64 / fallback: the centre's orbit in FP64
     size_t orbit_cap = 0;
     uint32_t* list[2] = {nullptr, nullptr};  // PERTURB glitch queues (ping-pong)
     PerturbCtl* pc = nullptr;
     std::vector<double> hfx, hfy;           // host copies of the offset tables
+    uint32_t max_references = 8;            // PERTURB: FP32-delta references before the fallback
     uint32_t references = 0;                // references used by the last PERTURB run
-    uint32_t fallback = 0;                  // pixels left to the FP64 loop by the last run
+    uint32_t fallback = 0;                  // pixels left to the FP64 rebasing loop by the last run
     unsigned long long* ctr = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -375,6 +431,7 @@ void planar_free(syf_planar* p) {
     (void)hipFree(p->smooth);
     (void)hipFree(p->iters);
     (void)hipFree(p->orbit);
+    (void)hipFree(p->orbit64);
     (void)hipFree(p->list[0]);
     (void)hipFree(p->list[1]);
     (void)hipFree(p->pc);
@@ -385,37 +442,89 @@ void planar_free(syf_planar* p) {
     delete p;
 }
 
-// FP64 reference orbit of C = centre + (ox, oy): Z_0 = 0, Z_{n+1} = f(Z_n) + C, up to the
-// first entry with |Z|^2 > bailout^2 or the iteration budget; returns the index of its last
-// entry.
-uint32_t reference_orbit(const syf_planar_desc& d, double ox, double oy, std::vector<float2>& out) {
+// Double-double arithmetic (Dekker 1971; Knuth's TwoSum): hi + lo with |lo| <= ulp(hi)/2,
+// about 106 bits, each step an IEEE operation rounded as written (fma gives the exact product
+// error), so the oracle's restatement of these steps produces the same bits.
+struct DD {
+    double hi, lo;
+};
+DD dd_two_sum(double a, double b) {
+    const double s = a + b;
+    const double bb = s - a;
+    return DD{s, (a - (s - bb)) + (b - bb)};
+}
+DD dd_norm(double s, double e) {  // |s| >= |e|
+    const double h = s + e;
+    return DD{h, e - (h - s)};
+}
+DD dd_add(DD a, DD b) {
+    const DD s = dd_two_sum(a.hi, b.hi);
+    return dd_norm(s.hi, s.lo + (a.lo + b.lo));
+}
+DD dd_mul(DD a, DD b) {
+    const double p = a.hi * b.hi;
+    return dd_norm(p, std::fma(a.hi, b.hi, -p) + (a.hi * b.lo + a.lo * b.hi));
+}
+DD dd_neg(DD a) { return DD{-a.hi, -a.lo}; }
+DD dd_abs(DD a) { return a.hi < 0.0 ? dd_neg(a) : a; }
+
+// Reference orbit of C = centre + (ox, oy), held exactly as a double-double: Z_0 = 0,
+// Z_{n+1} = f(Z_n) + C iterated in double-double (Burning Ship on |Re|, |Im|) and stored
+// rounded to FP64 (and, when f32 is not null, to f32), up to the first entry with
+// |Z|^2 > bailout^2 or the iteration budget; returns the index of its last entry.  An orbit
+// iterated in FP64 itself drifts from C's true orbit in chaotic views (its rounding errors
+// grow with the dynamics), and every pixel perturbed against it inherits the drift; this
+// one is the true orbit to about 1e-32 per step.
+uint32_t reference_orbit(const syf_planar_desc& d, double ox, double oy, std::vector<double2>& out,
+                         std::vector<float2>* f32) {
     const double b2 = double(d.bailout) * double(d.bailout);
-    const double cx = d.center[0] + ox, cy = d.center[1] + oy;
-    double zx = 0.0, zy = 0.0;
+    const bool ship = d.family == SYF_PLANAR_BURNING_SHIP;
+    const DD cx = dd_two_sum(d.center[0], ox), cy = dd_two_sum(d.center[1], oy);
+    DD zx{0.0, 0.0}, zy{0.0, 0.0};
     out.clear();
     out.reserve(size_t(d.iterations) + 1);
-    out.push_back(make_float2(0.0f, 0.0f));
+    out.push_back(make_double2(0.0, 0.0));
     for (uint32_t n = 0; n < d.iterations; ++n) {
-        double x = zx, y = zy;
-        if (d.family == SYF_PLANAR_BURNING_SHIP) {
-            x = std::fabs(x);
-            y = std::fabs(y);
-        }
-        double nx = x * x - y * y + cx;
-        double ny = 2.0 * x * y + cy;
-        zx = nx;
-        zy = ny;
-        out.push_back(make_float2(float(zx), float(zy)));
-        if (zx * zx + zy * zy > b2) break;
+        const DD x = ship ? dd_abs(zx) : zx, y = ship ? dd_abs(zy) : zy;
+        const DD xx = dd_mul(x, x), yy = dd_mul(y, y), xy = dd_mul(x, y);
+        zx = dd_add(dd_add(xx, dd_neg(yy)), cx);
+        zy = dd_add(DD{2.0 * xy.hi, 2.0 * xy.lo}, cy);
+        out.push_back(make_double2(zx.hi, zy.hi));
+        if (zx.hi * zx.hi + zy.hi * zy.hi > b2) break;
+    }
+    if (f32) {
+        f32->resize(out.size());
+        for (size_t i = 0; i < out.size(); ++i) (*f32)[i] = make_float2(float(out[i].x), float(out[i].y));
     }
     return uint32_t(out.size() - 1);
 }
 
-constexpr uint32_t MAX_REFERENCES = 8;
+// PERTURB64 over `list` (all pixels when null): the view centre's FP64 orbit, FP64 deltas
+// with rebasing.
+hipError_t run_rebase(syf_planar* p, const uint32_t* list, uint32_t n_items) {
+    const syf_planar_desc& d = p->desc;
+    std::vector<double2> orbit;
+    const uint32_t orbit_len = reference_orbit(d, 0.0, 0.0, orbit, nullptr);
+    hipError_t e = hipMemcpyAsync(p->orbit64, orbit.data(), orbit.size() * sizeof(double2), hipMemcpyHostToDevice,
+                                  p->stream);
+    // the copy reads the host vector: it must finish before the vector is freed
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return e;
+    const int cus = p->ctx->cu_count > 0 ? p->ctx->cu_count : 256;
+    const dim3 block(256), grid(std::min<uint32_t>((n_items + 255u) / 256u, uint32_t(cus) * 8u));
+    if (d.family == SYF_PLANAR_MANDELBROT)
+        hipLaunchKernelGGL(k_planar_rebase<SYF_PLANAR_MANDELBROT>, grid, block, 0, p->stream, p->P, p->fx, p->fy,
+                           p->orbit64, orbit_len, 0.0, 0.0, list, n_items, p->smooth, p->iters, p->ctr);
+    else
+        hipLaunchKernelGGL(k_planar_rebase<SYF_PLANAR_BURNING_SHIP>, grid, block, 0, p->stream, p->P, p->fx, p->fy,
+                           p->orbit64, orbit_len, 0.0, 0.0, list, n_items, p->smooth, p->iters, p->ctr);
+    return hipGetLastError();
+}
 
 // Perturbation passes: reference 0 is the view centre; each later reference is the glitched
 // pixel of the previous pass that got furthest (ties: lowest pixel index), and only that
-// pass's glitches are re-run.  Glitches left after MAX_REFERENCES take the FP64 loop.
+// pass's glitches are re-run.  Glitches left after max_references take the FP64 rebasing
+// loop (PERTURB64) against the centre.
 hipError_t run_perturbation(syf_planar* p) {
     const syf_planar_desc& d = p->desc;
     const PlanarP& P = p->P;
@@ -425,12 +534,13 @@ hipError_t run_perturbation(syf_planar* p) {
     double ref_dx = 0.0, ref_dy = 0.0;
     const uint32_t* in = nullptr;  // all pixels
     uint32_t n_in = n, which = 0;
+    std::vector<double2> orbit64;
     std::vector<float2> orbit;
     hipError_t e = hipSuccess;
     p->references = 0;
     p->fallback = 0;
-    for (uint32_t pass = 0; pass < MAX_REFERENCES && n_in > 0; ++pass) {
-        const uint32_t orbit_len = reference_orbit(d, ref_dx, ref_dy, orbit);
+    for (uint32_t pass = 0; pass < p->max_references && n_in > 0; ++pass) {
+        const uint32_t orbit_len = reference_orbit(d, ref_dx, ref_dy, orbit64, &orbit);
         e = hipMemcpyAsync(p->orbit, orbit.data(), orbit.size() * sizeof(float2), hipMemcpyHostToDevice, p->stream);
         if (e == hipSuccess) e = hipMemsetAsync(p->pc, 0, sizeof(PerturbCtl), p->stream);
         if (e != hipSuccess) return e;
@@ -459,14 +569,7 @@ hipError_t run_perturbation(syf_planar* p) {
     }
     if (n_in > 0) {
         p->fallback = n_in;
-        const dim3 grid(std::min<uint32_t>((n_in + 255u) / 256u, uint32_t(cus) * 8u));
-        if (d.family == SYF_PLANAR_MANDELBROT)
-            hipLaunchKernelGGL(k_planar_fp64_list<SYF_PLANAR_MANDELBROT>, grid, block, 0, p->stream, P, p->fx, p->fy,
-                               in, n_in, p->smooth, p->iters, p->ctr);
-        else
-            hipLaunchKernelGGL(k_planar_fp64_list<SYF_PLANAR_BURNING_SHIP>, grid, block, 0, p->stream, P, p->fx, p->fy,
-                               in, n_in, p->smooth, p->iters, p->ctr);
-        e = hipGetLastError();
+        e = run_rebase(p, in, n_in);
     }
     return e;
 }
@@ -481,9 +584,12 @@ int syf_planar_create(syf_ctx* ctx, const syf_planar_desc* d, syf_planar** out) 
     if (d->family != SYF_PLANAR_MANDELBROT && d->family != SYF_PLANAR_BURNING_SHIP && d->family != SYF_PLANAR_JULIA)
         return fail(SYF_E_INVALID, "family: unsupported planar family %u (Julia 1, Mandelbrot 2, Burning Ship 3)",
                     d->family);
-    if (d->mode > SYF_PLANAR_PERTURB) return fail(SYF_E_INVALID, "mode: unknown mode %u", d->mode);
-    if (d->mode == SYF_PLANAR_PERTURB && d->family == SYF_PLANAR_JULIA)
+    if (d->mode > SYF_PLANAR_PERTURB64) return fail(SYF_E_INVALID, "mode: unknown mode %u", d->mode);
+    const bool perturb = d->mode == SYF_PLANAR_PERTURB || d->mode == SYF_PLANAR_PERTURB64;
+    if (perturb && d->family == SYF_PLANAR_JULIA)
         return fail(SYF_E_INVALID, "mode: perturbation is for the parameter-plane families");
+    if (d->max_references > 64)
+        return fail(SYF_E_INVALID, "max_references must be at most 64 (0 selects 8), got %u", d->max_references);
     if (d->width < 1 || d->height < 1 || d->width > 32768 || d->height > 32768)
         return fail(SYF_E_INVALID, "width x height (%ux%u) must be within 1..32768 per side", d->width, d->height);
     if (d->iterations < 1 || d->iterations > (1u << 24))
@@ -498,6 +604,7 @@ int syf_planar_create(syf_ctx* ctx, const syf_planar_desc* d, syf_planar** out) 
     syf_planar* p = new syf_planar();
     p->ctx = ctx;
     p->desc = *d;
+    p->max_references = d->max_references ? d->max_references : 8u;
     PlanarP& P = p->P;
     P.family = d->family;
     P.power = 2;
@@ -524,13 +631,19 @@ int syf_planar_create(syf_ctx* ctx, const syf_planar_desc* d, syf_planar** out) 
     if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc(&p->fx, fx.size() * 8)) != hipSuccess || (e = hipMalloc(&p->fy, fy.size() * 8)) != hipSuccess ||
         (e = hipMalloc(&p->smooth, n * 4)) != hipSuccess || (e = hipMalloc(&p->iters, n * 4)) != hipSuccess ||
-        (e = hipMalloc(&p->ctr, 16)) != hipSuccess || (e = hipEventCreate(&p->ev0)) != hipSuccess ||
+        (e = hipMalloc(&p->ctr, 3 * sizeof(unsigned long long))) != hipSuccess || (e = hipEventCreate(&p->ev0)) != hipSuccess ||
         (e = hipEventCreate(&p->ev1)) != hipSuccess) {
         planar_free(p);
         return hip_fail(e, "syf_planar_create allocation");
     }
-    if (d->mode == SYF_PLANAR_PERTURB) {
+    if (perturb) {
         p->orbit_cap = size_t(d->iterations) + 1;
+        if ((e = hipMalloc(&p->orbit64, p->orbit_cap * sizeof(double2))) != hipSuccess) {
+            planar_free(p);
+            return hip_fail(e, "syf_planar_create perturbation buffers");
+        }
+    }
+    if (d->mode == SYF_PLANAR_PERTURB) {
         if ((e = hipMalloc(&p->orbit, p->orbit_cap * sizeof(float2))) != hipSuccess ||
             (e = hipMalloc(&p->list[0], n * 4)) != hipSuccess || (e = hipMalloc(&p->list[1], n * 4)) != hipSuccess ||
             (e = hipMalloc(&p->pc, sizeof(PerturbCtl))) != hipSuccess) {
@@ -561,7 +674,7 @@ int syf_planar_run(syf_planar* p) {
     if (!p) return fail(SYF_E_INVALID, "planar must not be null");
     if (hipError_t de = hipSetDevice(p->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
     const syf_planar_desc& d = p->desc;
-    hipError_t e = hipMemsetAsync(p->ctr, 0, 16, p->stream);
+    hipError_t e = hipMemsetAsync(p->ctr, 0, 3 * sizeof(unsigned long long), p->stream);
     if (e == hipSuccess) e = hipEventRecord(p->ev0, p->stream);
     const dim3 grid((p->P.tiles + 3u) / 4u), block(256);
     const PlanarP& P = p->P;
@@ -583,8 +696,13 @@ int syf_planar_run(syf_planar* p) {
             else
                 hipLaunchKernelGGL(k_planar_fp64<SYF_PLANAR_JULIA>, grid, block, 0, p->stream, P, p->fx, p->fy, p->smooth, p->iters, p->ctr);
             break;
-        default:
+        case SYF_PLANAR_PERTURB:
             e = run_perturbation(p);
+            break;
+        default:
+            p->references = 1;
+            p->fallback = 0;
+            e = run_rebase(p, nullptr, d.width * d.height);
             break;
         }
         if (e == hipSuccess) e = hipGetLastError();
@@ -617,8 +735,8 @@ int syf_planar_readback(syf_planar* p, uint32_t plane, void* host, size_t bytes)
 int syf_planar_stats(syf_planar* p, uint64_t* iterations, uint64_t* rebases, double* kernel_ms) {
     clear_error();
     if (!p) return fail(SYF_E_INVALID, "planar must not be null");
-    unsigned long long c[2] = {0, 0};
-    hipError_t e = hipMemcpyAsync(c, p->ctr, 16, hipMemcpyDeviceToHost, p->stream);
+    unsigned long long c[3] = {0, 0, 0};
+    hipError_t e = hipMemcpyAsync(c, p->ctr, sizeof c, hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     if (e != hipSuccess) return hip_fail(e, "syf_planar_stats");
     float ms = 0.0f;
@@ -626,6 +744,19 @@ int syf_planar_stats(syf_planar* p, uint64_t* iterations, uint64_t* rebases, dou
     if (iterations) *iterations = c[0];
     if (rebases) *rebases = c[1];
     if (kernel_ms) *kernel_ms = ms;
+    return SYF_OK;
+}
+
+int syf_planar_glitches(syf_planar* p, uint32_t* references, uint32_t* fallback_pixels, uint64_t* rebasings) {
+    clear_error();
+    if (!p) return fail(SYF_E_INVALID, "planar must not be null");
+    unsigned long long c[3] = {0, 0, 0};
+    hipError_t e = hipMemcpyAsync(c, p->ctr, sizeof c, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return hip_fail(e, "syf_planar_glitches");
+    if (references) *references = p->references;
+    if (fallback_pixels) *fallback_pixels = p->fallback;
+    if (rebasings) *rebasings = c[2];
     return SYF_OK;
 }
 

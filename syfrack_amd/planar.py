@@ -1,6 +1,7 @@
 """2D escape time over the C ABI: PlanarFractal::sample (planar.rs:77-194) for the
 parameter-plane families in FP32 (reference precision), FP64 (bit-exact counts) and deep-zoom
-perturbation (FP64 reference orbit + FP32 deltas)."""
+perturbation (FP64 reference orbits + FP32 deltas, the glitches left after the reference budget
+taking FP64 deltas with rebasing; or FP64 rebasing for every pixel)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -23,6 +24,7 @@ class PlanarView:
     julia: tuple = (-0.74543, 0.11301)
     bailout: float = 16.0
     rotation: float = 0.0
+    max_references: int = 0  # PERTURB: FP32-delta references before the FP64 rebasing fallback (0: 8)
 
     def c_struct(self) -> abi.SyfPlanarDesc:
         d = abi.SyfPlanarDesc()
@@ -36,6 +38,7 @@ class PlanarView:
         d.julia[:] = list(self.julia)
         d.bailout = self.bailout
         d.rotation = self.rotation
+        d.max_references = self.max_references
         return d
 
 
@@ -79,7 +82,10 @@ class Planar:
     def stats(self):
         a, b, ms = C.c_uint64(), C.c_uint64(), C.c_double()
         abi.check(self.lib.syf_planar_stats(self.handle, C.byref(a), C.byref(b), C.byref(ms)))
-        return {"iterations": a.value, "rebases": b.value, "kernel_ms": ms.value}
+        refs, fallback, rebasings = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        abi.check(self.lib.syf_planar_glitches(self.handle, C.byref(refs), C.byref(fallback), C.byref(rebasings)))
+        return {"iterations": a.value, "rebases": b.value, "kernel_ms": ms.value, "references": refs.value,
+                "fallback_pixels": fallback.value, "rebasings": rebasings.value}
 
 
 def boundary_centre(ctx, family: int, seed: int, iterations: int, grid: int = 256):
