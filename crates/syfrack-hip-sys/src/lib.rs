@@ -106,6 +106,7 @@ pub const SYF_PLANE_MOMENTS: u32 = 6;
 pub const SYF_PLANE_SURFACE: u32 = 7;
 
 pub const SYF_NOALLOC_MAX_PIXELS: u32 = 1_048_576;
+pub const SYF_CONTROL_QUANTUM: u32 = 64;
 
 pub const SYF_PLANAR_JULIA: u32 = 1;
 pub const SYF_PLANAR_MANDELBROT: u32 = 2;
@@ -362,6 +363,10 @@ pub struct syf_planar {
 pub struct syf_server {
     _p: [u8; 0],
 }
+#[repr(C)]
+pub struct syf_control {
+    _p: [u8; 0],
+}
 
 extern "C" {
     pub fn syf_last_error() -> *const c_char;
@@ -396,6 +401,19 @@ extern "C" {
     pub fn syf_export_write(job: *mut syf_job, path: *const c_char, bytes: *mut u64, hash: *mut u64) -> c_int;
     pub fn syf_export_read(path: *const c_char, bytes: u64, hash: u64, width: u32, height: u32, color4: *mut f32,
                            depth: *mut f32, normals4: *mut f32) -> c_int;
+    pub fn syf_control_create(ctx: *mut syf_ctx, desc: *const syf_job_desc, graph: *const syf_graph, quantum: u32,
+                              out: *mut *mut syf_control) -> c_int;
+    pub fn syf_control_destroy(c: *mut syf_control) -> c_int;
+    pub fn syf_control_set_scene(c: *mut syf_control, desc: *const syf_job_desc, graph: *const syf_graph) -> c_int;
+    pub fn syf_control_tick(c: *mut syf_control) -> c_int;
+    pub fn syf_control_wait(c: *mut syf_control, timeout_s: f64) -> c_int;
+    pub fn syf_control_progress(c: *mut syf_control, completed_pixels: *mut u32, dispatches: *mut u32,
+                                complete: *mut u32, width: *mut u32, height: *mut u32) -> c_int;
+    pub fn syf_control_readback(c: *mut syf_control, plane: u32, host: *mut c_void, bytes: usize) -> c_int;
+    pub fn syf_control_pick(c: *mut syf_control, u: f32, v: f32, hit: *mut u32, depth: *mut f32, aux4: *mut f32)
+        -> c_int;
+    pub fn syf_control_probe(ctx: *mut syf_ctx, graph: *const syf_graph, xyz4: *const f32, n: usize, out4: *mut f32,
+                             traps4: *mut f32) -> c_int;
     pub fn syf_planar_create(ctx: *mut syf_ctx, desc: *const syf_planar_desc, out: *mut *mut syf_planar) -> c_int;
     pub fn syf_planar_destroy(p: *mut syf_planar) -> c_int;
     pub fn syf_planar_run(p: *mut syf_planar) -> c_int;

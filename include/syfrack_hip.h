@@ -399,6 +399,55 @@ int syf_export_write(syf_job* job, const char* path, uint64_t* bytes, uint64_t* 
 int syf_export_read(const char* path, uint64_t bytes, uint64_t hash, uint32_t width, uint32_t height, float* color4,
                     float* depth, float* normals4);
 
+/* ---------------------------------------------------------------- control machine
+ * Interactive geometry without per-edit compilation: ControlSession (control/mod.rs:145-515,
+ * shaders/control.wgsl).  A session holds an image of the scene's node graph, evaluated by the
+ * library's graph interpreter (never a compiled topology), and advances it in bounded quanta:
+ * each pixel is a resumable machine (pixel-centre ray from the camera; the over-relaxed march
+ * of march_segment with pixel tolerance 1 and no step limit: a ray continues until it hits or
+ * provably leaves the scene; then the four tetrahedral normal taps and a headlight shade,
+ * control.wgsl:54-136,267-303).  The image is ACCUM (colour, 1; background (0.025, 0.03, 0.04)
+ * until a pixel completes, (0.025, 0.035, 0.05) for a miss), DEPTH (hit distance, -1) and AUX
+ * (normal, march steps; (0, 0, 0, -1)).  Scenes without phenomena (the reference's nature
+ * field, media and curved light are outside this path). */
+typedef struct syf_control syf_control;
+#define SYF_CONTROL_QUANTUM 64u /* default field evaluations per wave per dispatch */
+/* ControlSession::new (control/mod.rs:172-335): desc gives the camera and the image size
+ * (its fractal, lights, material and sampling fields are unused), graph the field; `quantum`
+ * bounds one dispatch to that many field evaluations per wave (0 selects SYF_CONTROL_QUANTUM,
+ * at most 4096).  The whole requested size is kept (up to 2^26 pixels): the machine's state is
+ * 36 bytes per pixel, where the reference shrinks the image to fit a 64 MiB register file. */
+int syf_control_create(syf_ctx* ctx, const syf_job_desc* desc, const syf_graph* graph, uint32_t quantum,
+                       syf_control** out);
+int syf_control_destroy(syf_control* c);
+/* ControlSession::set_scene (control/mod.rs:354-401): validates and compiles the new graph
+ * (host only), and only then replaces the session's program, parameters and camera and restarts
+ * the image; a pending quantum of the old image is discarded.  An invalid graph leaves the
+ * session as it was.  Waits for the quantum in flight (a few milliseconds at most). */
+int syf_control_set_scene(syf_control* c, const syf_job_desc* desc, const syf_graph* graph);
+/* ControlSession::tick (control/mod.rs:404-465): polls the quantum in flight and, once it has
+ * completed, submits the next one; never waits.  Returns the session's failure, with the node,
+ * after a non-finite distance (or -inf: +inf is empty space) until set_scene replaces the
+ * scene. */
+int syf_control_tick(syf_control* c);
+/* ControlSession::wait_complete (control/mod.rs:502-515): ticks until every pixel is complete
+ * (SYF_E_TIMEOUT after timeout_s seconds). */
+int syf_control_wait(syf_control* c, double timeout_s);
+/* complete / completed_pixels / dispatches / size (control/mod.rs:467-484); any output may be
+ * null. */
+int syf_control_progress(syf_control* c, uint32_t* completed_pixels, uint32_t* dispatches, uint32_t* complete,
+                         uint32_t* width, uint32_t* height);
+/* The image planes SYF_PLANE_ACCUM, SYF_PLANE_DEPTH or SYF_PLANE_AUX (accumulation_buffer,
+ * control/mod.rs:473); waits for the quantum in flight. */
+int syf_control_readback(syf_control* c, uint32_t plane, void* host, size_t bytes);
+/* ControlSession::pick (control/mod.rs:488-500): the pixel at (u, v) in [0, 1); *hit = 0 where
+ * the depth is not positive, else the depth and AUX. */
+int syf_control_pick(syf_control* c, float u, float v, uint32_t* hit, float* depth, float* aux4);
+/* Renderer::control_distances (control/mod.rs:530-574): the field of the graph evaluated by the
+ * interpreter at n points, in syf_de_probe's layout.  Synchronous. */
+int syf_control_probe(syf_ctx* ctx, const syf_graph* graph, const float* xyz4, size_t n, float* out4,
+                      float* traps4);
+
 /* ---------------------------------------------------------------- 2D escape time
  * PlanarFractal::sample (planar.rs:77-194) / planar_sample (planar.wgsl:16-91) for the
  * parameter-plane families, plus FP64 and deep-zoom perturbation modes that have no

@@ -224,6 +224,24 @@ def perturb64(desc, px: np.ndarray, py: np.ndarray):
     return iters, smooth
 
 
+def control_trace(desc, rect=None, max_evals: int = 1 << 20):
+    """The control machine's image of the job's field (graph loaded with graph_use): accum
+    (h, w, 4), depth (h, w), aux (h, w, 4), status (h, w; 0 done, 1 undefined distance,
+    2 over max_evals evaluations)."""
+    x0, y0, w, h = rect if rect is not None else (0, 0, desc.width, desc.height)
+    acc = np.zeros((h, w, 4), np.float32)
+    dep = np.zeros((h, w), np.float32)
+    aux = np.zeros((h, w, 4), np.float32)
+    st = np.zeros((h, w), np.uint32)
+    lib = load()
+    lib.orc_control_trace.restype = C.c_int
+    rc = lib.orc_control_trace(C.byref(desc), C.c_uint32(x0), C.c_uint32(y0), C.c_uint32(w), C.c_uint32(h),
+                               C.c_uint32(max_evals), _ptr(acc), _ptr(dep), _ptr(aux), _ptr(st))
+    if rc != 0:
+        raise RuntimeError("orc_control_trace failed")
+    return acc, dep, aux, st
+
+
 def escape_f64_points(desc, px: np.ndarray, py: np.ndarray):
     """FP64 iteration counts at the points escape_f128 evaluates (centre + f32 offset)."""
     px = np.ascontiguousarray(px, np.uint32)

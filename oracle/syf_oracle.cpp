@@ -1309,6 +1309,118 @@ int orc_trace_primary(const syf_job_desc* d, uint32_t sample, int precision, uin
     return 0;
 }
 
+// The control machine's image (ControlSession, control.wgsl:54-136,267-291) of a pixel
+// rectangle for the job's field and camera: the pixel-centre ray (control.wgsl:271-274), the
+// over-relaxed march of march_segment with pixel tolerance 1 and no step limit
+// (control.wgsl:113-136; here at most max_evals evaluations, status 2 beyond), on a hit the
+// four tetrahedral taps at h = max(t * cone / 2, 1e-4) summed as sum(k_i d_i) and
+// safe-normalised, and the headlight shade (0.68, 0.70, 0.73) * (0.2 + 0.8 max(n.l, 0)) with
+// l = normalize(0.4, 0.7, 0.6) (control.wgsl:79-97).  A miss shades (0.025, 0.035, 0.05).
+// A NaN or -inf distance stops the pixel with status 1 (+inf is empty space).  Outputs are
+// rectangle-local: accum4 (colour, 1), depth (-1 on a miss), aux4 (normal, steps; (0,0,0,-1)),
+// status (0 done).
+int orc_control_trace(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t max_evals,
+                      float* accum4, float* depth, float* aux4, uint32_t* status) {
+    const Basis b = camera_basis(d->camera, d->width, d->height);
+    const float cone = b.tan_half * 2.0f / float(d->width);
+    const float t_end = ray_extent(*d, b);
+    const Fractal<float> F = lower_fractal<float>(d->fractal);
+    const V3<float> light = [] {
+        const V3<float> l = v3(0.4f, 0.7f, 0.6f);
+        return l * (1.0f / length(l));
+    }();
+    const V3<float> taps[4] = {v3(1.0f, -1.0f, -1.0f), v3(-1.0f, -1.0f, 1.0f), v3(-1.0f, 1.0f, -1.0f),
+                               v3(1.0f, 1.0f, 1.0f)};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long yy = 0; yy < long(h); ++yy)
+        for (uint32_t xx = 0; xx < w; ++xx) {
+            const uint32_t px = x0 + xx, py = y0 + uint32_t(yy);
+            const size_t i = size_t(yy) * w + xx;
+            const float ndcx = ((float(px) + 0.5f) / float(d->width)) * 2.0f - 1.0f;
+            const float ndcy = ((float(py) + 0.5f) / float(d->height)) * 2.0f - 1.0f;
+            V3<float> rd = b.forward + b.right * (ndcx * b.tan_half) + b.up * (-ndcy * b.tan_half / b.aspect);
+            rd = rd * (1.0f / length(rd));
+            const V3<float> ro = b.origin;
+            float* acc = accum4 + 4 * i;
+            float* ax = aux4 + 4 * i;
+            acc[0] = 0.025f, acc[1] = 0.03f, acc[2] = 0.04f, acc[3] = 1.0f;
+            depth[i] = -1.0f;
+            ax[0] = ax[1] = ax[2] = 0.0f, ax[3] = -1.0f;
+            status[i] = 2u;
+            float t = 0.0f, omega = 1.2f, prevr = 0.0f, step = 0.0f;
+            uint32_t steps = 1u;
+            bool hit = false, undefined = false;
+            for (uint32_t e = 0; e < max_evals; ++e) {
+                const float dist = de_eval(F, ro + rd * t).d;
+                if (dist != dist || dist == -INFINITY) {
+                    undefined = true;
+                    break;
+                }
+                const float radius = std::fabs(dist) * F.safety;
+                const bool failed = omega > 1.0f && radius + prevr < step * 1.0001f;
+                if (failed) {
+                    step = step - omega * step;
+                    omega = 1.0f;
+                } else {
+                    step = radius * omega;
+                }
+                prevr = radius;
+                const float err = radius / std::fmax(t, 1e-6f);
+                if (!failed && err < cone) {
+                    hit = true;
+                    break;
+                }
+                bool miss = false;
+                if (!failed && t + step >= t_end) {
+                    if (radius >= t_end - t) {
+                        miss = true;
+                    } else {
+                        omega = 1.0f;
+                        step = radius;
+                    }
+                }
+                if (!miss) {
+                    t = t + step;
+                    miss = t >= t_end;
+                }
+                if (miss) {
+                    acc[0] = 0.025f, acc[1] = 0.035f, acc[2] = 0.05f;
+                    status[i] = 0u;
+                    break;
+                }
+                steps = std::min(steps, 0xfffffffeu) + 1u;
+            }
+            if (undefined) {
+                status[i] = 1u;
+                continue;
+            }
+            if (!hit) continue;
+            const V3<float> hp = ro + rd * t;
+            const float hh = std::fmax(t * cone * 0.5f, 1e-4f);
+            V3<float> nacc = v3(0.0f, 0.0f, 0.0f);
+            for (int k = 0; k < 4; ++k) {
+                const float dist = de_eval(F, hp + taps[k] * hh).d;
+                if (dist != dist || dist == -INFINITY) {
+                    undefined = true;
+                    break;
+                }
+                nacc = nacc + taps[k] * dist;
+            }
+            if (undefined) {
+                status[i] = 1u;
+                continue;
+            }
+            const float l = length(nacc), rcp = 1.0f / l;
+            const V3<float> n = (l > 0.0f && rcp > 0.0f && rcp <= 3.4028235e38f) ? nacc * rcp : v3(0.0f, 1.0f, 0.0f);
+            const float lit = 0.20f + 0.80f * std::fmax(dot(n, light), 0.0f);
+            acc[0] = 0.68f * lit, acc[1] = 0.70f * lit, acc[2] = 0.73f * lit;
+            depth[i] = t;
+            ax[0] = n.x, ax[1] = n.y, ax[2] = n.z, ax[3] = float(steps);
+            status[i] = 0u;
+        }
+    return 0;
+}
+
 // Megakernel accumulation (pathtrace.wgsl:1203-1276) of a pixel rectangle: samples
 // sample_start .. sample_start+sample_count-1 in ascending order, accum += (r if finite, 1);
 // depth/aux from the first sample of the call (written by the caller when sample_start == 0),

@@ -267,6 +267,7 @@ class SyfGraph(C.Structure):
 
 
 NOALLOC_MAX_PIXELS = 1048576  # SYF_NOALLOC_MAX_PIXELS
+CONTROL_QUANTUM = 64  # SYF_CONTROL_QUANTUM
 MATH_FAST = 0  # SYF_MATH_FAST
 MATH_EXACT = 1  # SYF_MATH_EXACT
 GRAPH_SPECIALISED = 0  # SYF_GRAPH_SPECIALISED
@@ -301,6 +302,15 @@ SIGNATURES = {
     "syf_checkpoint_load": (C.c_int, [_P, C.c_char_p]),
     "syf_export_write": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "syf_export_read": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P]),
+    "syf_control_create": (C.c_int, [_P, C.POINTER(SyfJobDesc), _P, C.c_uint32, C.POINTER(_P)]),
+    "syf_control_destroy": (C.c_int, [_P]),
+    "syf_control_set_scene": (C.c_int, [_P, C.POINTER(SyfJobDesc), _P]),
+    "syf_control_tick": (C.c_int, [_P]),
+    "syf_control_wait": (C.c_int, [_P, C.c_double]),
+    "syf_control_progress": (C.c_int, [_P] + [C.POINTER(C.c_uint32)] * 5),
+    "syf_control_readback": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    "syf_control_pick": (C.c_int, [_P, C.c_float, C.c_float, C.POINTER(C.c_uint32), C.POINTER(C.c_float), _P]),
+    "syf_control_probe": (C.c_int, [_P, _P, _P, C.c_size_t, _P, _P]),
     "syf_planar_create": (C.c_int, [_P, C.POINTER(SyfPlanarDesc), C.POINTER(_P)]),
     "syf_planar_destroy": (C.c_int, [_P]),
     "syf_planar_run": (C.c_int, [_P]),

@@ -52,4 +52,9 @@ int job_alloc(syf_ctx* ctx, const syf_job_desc& desc, const JobP& P, syf_job** o
 size_t plane_bytes(uint32_t plane, size_t pixels);
 void release_wavefront(syf_job* j);
 int noalloc_setup(int device, void** out);
+// A graph in the interpreter's form, for callers that keep their own program buffer (the
+// control machine): the job lowered from desc with the graph's fractal parameters, the compiled
+// program and the node of each instruction.  Validates like syf_job_create_graph.
+int graph_lower_interpreted(const syf_job_desc& desc, const syf_graph& graph, JobP& P, std::vector<GOp>& prog,
+                            std::vector<uint32_t>& node_of);
 }  // namespace syf

@@ -1191,6 +1191,26 @@ int jit_get(int device, const std::vector<GOp>& prog, const JitModule** out) {
 
 }  // namespace
 
+int graph_lower_interpreted(const syf_job_desc& desc, const syf_graph& graph, JobP& P, std::vector<GOp>& prog,
+                            std::vector<uint32_t>& node_of) {
+    Compiled c;
+    if (int rc = compile(graph, c); rc != SYF_OK) return rc;
+    FractalP F;
+    std::memset(&F, 0, sizeof F);
+    F.family = SYF_FAMILY_GRAPH;
+    F.iteration_product = c.iteration_product;
+    F.safety = c.step_scale;
+    for (int i = 0; i < 4; ++i) F.gtrap_scale[i] = c.trap_scales[i];
+    F.trap_scale = c.trap_scales[0];
+    syf_job_desc d = desc;
+    d.fractal.family = SYF_FAMILY_GRAPH;
+    d.fractal.bounding_radius_hint = c.hint;
+    if (int rc = lower_job(d, P, &F); rc != SYF_OK) return rc;
+    prog = std::move(c.prog);
+    node_of = std::move(c.node_of);
+    return SYF_OK;
+}
+
 hipError_t jit_launch_args(const void* jit, uint32_t which, dim3 grid, dim3 block, size_t shmem, hipStream_t s,
                            void** args) {
     const JitModule* m = static_cast<const JitModule*>(jit);

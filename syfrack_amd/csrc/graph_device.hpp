@@ -469,10 +469,31 @@ template <class S, uint32_t N> __device__ __forceinline__ S g_load(const S (&slo
     X(GOP_SPHERE_INVERSION) X(GOP_SPHERE_FOLD) X(GOP_ABS) X(GOP_MENGER_FOLD) X(GOP_KALEIDO) X(GOP_POWER)          \
     X(GOP_ADD_ORIGIN) X(GOP_ADD_CONSTANT) X(GOP_TWIST) X(GOP_BEND) X(GOP_TILE) X(GOP_REPEAT_FINITE) X(GOP_TRAP)
 
+// Whether a combination's result carries operand A's payload (material, iterations, and the
+// fence it came from): the choices g_combine makes.
+template <class T> __device__ __forceinline__ bool g_combine_takes_a(uint32_t kind, T k, const GDist<T>& A,
+                                                                     const GDist<T>& B) {
+    switch (kind) {
+    case SYF_NODE_UNION: return A.d <= B.d;
+    case SYF_NODE_INTERSECTION: return A.d >= B.d;
+    case SYF_NODE_SUBTRACTION: return true;
+    default: {
+        if (A.d >= T(1e30) || B.d >= T(1e30)) return !(A.d >= T(1e30));
+        const bool uni = kind == SYF_NODE_SMOOTH_UNION;
+        const T h = uni ? clamp01(T(0.5) + T(0.5) * (B.d - A.d) / k) : clamp01(T(0.5) - T(0.5) * (B.d - A.d) / k);
+        return h >= T(0.5);
+    }
+    }
+}
+
 // Interprets a graph program at c.  The program is valid by construction and checked on the
 // host (graph.hip): every ITER_BEGIN/ITER_END pair is properly nested within GRAPH_LEVELS,
-// every slot index is below the slot count, and the program ends with GOP_END.
-template <class T> __device__ __noinline__ GRaw<T> graph_run(const GOp* prog_in, V3<T> c) {
+// every slot index is below the slot count, and the program ends with GOP_END.  With TRACK,
+// it also follows which fence produced the result's distance (node_ids: the node of each
+// instruction) and writes that node to *node_out: the diagnosis of an undefined distance.
+template <class T, bool TRACK = false>
+__device__ __noinline__ GRaw<T> graph_run(const GOp* prog_in, V3<T> c, const uint32_t* node_ids = nullptr,
+                                          uint32_t* node_out = nullptr) {
     const GOp* prog = g_uniform(prog_in);
     const T INF = g_inf<T>();
     GPoint<T> cur{c, T(1), 0u, INF, INF, INF, INF};
@@ -491,12 +512,16 @@ template <class T> __device__ __noinline__ GRaw<T> graph_run(const GOp* prog_in,
     uint32_t dead = 0u;  // bit L: this lane has left the loop at level L
     uint32_t runs = 0u;
     uint32_t pc = 0u;
+    uint32_t dnode = 0u;  // TRACK: the fence of dc, and of each distance slot
+    uint32_t dsnode[GRAPH_DSLOTS] = {};
     for (uint32_t guard = 0; guard < GRAPH_MAX_STEPS; ++guard) {
         const GOp& o = prog[pc];
         const uint32_t op = __builtin_amdgcn_readfirstlane(o.op);
         const bool live = dead == 0u;
         switch (op) {
-        case GOP_END: return GRaw<T>{dc, runs};
+        case GOP_END:
+            if constexpr (TRACK) *node_out = dnode;
+            return GRaw<T>{dc, runs};
         case GOP_P_START: cur = GPoint<T>{c, T(1), 0u, INF, INF, INF, INF}; break;
         case GOP_P_SAVE: g_store(ps, o.a, cur); break;
         case GOP_P_LOAD: cur = g_load(ps, o.a); break;
@@ -542,21 +567,38 @@ template <class T> __device__ __noinline__ GRaw<T> graph_run(const GOp* prog_in,
                 g_point<T>(GOP_ADD_SEED, o.a, o.b, o.c, o, cur, c, sp, sdr);
             }
             break;
-        case GOP_FENCE: dc = g_fence_state(o.a, o, cur); break;
-        case GOP_D_SAVE: g_store(ds, o.a, dc); break;
-        case GOP_D_LOAD: dc = g_load(ds, o.a); break;
+        case GOP_FENCE:
+            dc = g_fence_state(o.a, o, cur);
+            if constexpr (TRACK) dnode = node_ids[pc];
+            break;
+        case GOP_D_SAVE:
+            g_store(ds, o.a, dc);
+            if constexpr (TRACK) g_store(dsnode, o.a, dnode);
+            break;
+        case GOP_D_LOAD:
+            dc = g_load(ds, o.a);
+            if constexpr (TRACK) dnode = g_load(dsnode, o.a);
+            break;
         case GOP_MATTAG: dc.mat = o.a; break;
         case GOP_BOUNDED: g_bounded(o, cur, dc); break;
         case GOP_COMBINE: {
             const GDist<T> A = o.b == GRAPH_ACC ? dc : g_load(ds, o.b);
             const GDist<T> B = o.c == GRAPH_ACC ? dc : g_load(ds, o.c);
+            if constexpr (TRACK) {
+                const uint32_t na = o.b == GRAPH_ACC ? dnode : g_load(dsnode, o.b);
+                const uint32_t nb = o.c == GRAPH_ACC ? dnode : g_load(dsnode, o.c);
+                dnode = g_combine_takes_a(o.a, T(o.v[0]), A, B) ? na : nb;
+            }
             dc = g_combine(o.a, T(o.v[0]), A, B);
             break;
         }
-        default: return GRaw<T>{dc, runs};
+        default:
+            if constexpr (TRACK) *node_out = dnode;
+            return GRaw<T>{dc, runs};
         }
         ++pc;
     }
+    if constexpr (TRACK) *node_out = dnode;
     return GRaw<T>{dc, runs};
 }
 

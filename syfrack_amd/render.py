@@ -181,6 +181,17 @@ class Renderer:
         finally:
             j.close()
 
+    def control_distances(self, scene: Scene, points: np.ndarray) -> DeSample:
+        """Renderer::control_distances (control/mod.rs:530-574): the scene's graph evaluated by
+        the control machine's interpreter, without compiling its topology."""
+        from .control import control_distances
+        return control_distances(self.ctx, scene, points)
+
+    def control_session(self, scene: Scene, size, quantum: int = 0):
+        """ControlSession::new(renderer, scene, size) (control/mod.rs:172)."""
+        from .control import ControlSession
+        return ControlSession(self.ctx, scene, size, quantum)
+
     def estimate_bounding_radius(self, graph: Graph) -> float:
         """Graph::estimate_bounding_radius (eval.rs:883-922) with the field evaluated on the
         device: from 100 units out along the 26 directions of the unit cube's neighbours, walk
