@@ -270,6 +270,39 @@ def pathtrace_results(spp: int) -> dict:
     return out
 
 
+def control_results() -> dict:
+    """The control machine (interactive preview of a node graph, no compilation): for the bundled
+    Mandelbulb, Mandelbox and Menger graphs and the "cathedrale" graph at 1280x720 under an orbit
+    camera, the wall time from set_scene to a complete image as an interactive client sees it
+    (tick polled every 0.1 ms, never waiting) and the quanta it took (each bounded to 64 field
+    evaluations per wave).  Scenes are set on a warm session."""
+    from syfrack_amd.graph import Graph
+    from syfrack_amd.render import Renderer
+    from syfrack_amd.scene import Camera, Scene
+    r = Renderer(0)
+    size = (1280, 720)
+    radii = {"mandelbulb": 1.25, "mandelbox": 4.0, "menger": 1.6, "cathedrale": 1.6}
+    rows = []
+    s = r.control_session(Scene(Graph.new_minimal(), Camera.orbit((0, 0, 0), 2.5, 35.0, 25.0, 35.0)), size)
+    s.wait_complete()
+    for name, radius in radii.items():
+        progress(f"control: {name}")
+        scene = Scene(Graph.bundled(name), Camera.orbit((0, 0, 0), 2.5 * radius, 35.0, 25.0, 35.0))
+        t0 = time.perf_counter()
+        s.set_scene(scene)
+        while True:
+            s.tick()
+            if s.complete():
+                break
+            time.sleep(1e-4)
+        el = time.perf_counter() - t0
+        rows.append({"graph": name, "width": size[0], "height": size[1], "s": el, "quanta": s.dispatches(),
+                     "mpix_s": size[0] * size[1] / el / 1e6})
+    s.close()
+    return {"workload": "control machine preview, 1280x720, bundled graphs, quantum 64 evaluations per wave",
+            "views": rows}
+
+
 def planar_leg(size: int, seeds: int) -> int:
     """The deep-zoom leg alone: prints its JSON line."""
     print(json.dumps(planar_results(size, seeds)), flush=True)
@@ -478,6 +511,7 @@ def main() -> int:
                     help="run only the Mandelbox 1080p path-trace leg (three kernel forms)")
     ap.add_argument("--spp", type=int, default=64, help="samples per pixel of the path-trace leg")
     ap.add_argument("--planar", action="store_true", help="run only the 2D deep-zoom leg")
+    ap.add_argument("--control", action="store_true", help="run only the control-machine leg")
     ap.add_argument("--planar-size", type=int, default=4096)
     ap.add_argument("--planar-seeds", type=int, default=2)
     ap.add_argument("--march-math", choices=["fast", "exact"], default="fast",
@@ -486,6 +520,9 @@ def main() -> int:
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.control:
+        print(json.dumps(control_results()), flush=True)
+        return 0
     if args.pathtrace:
         return pathtrace_leg(args.spp)
     if args.planar:
@@ -609,6 +646,7 @@ def main() -> int:
         # the other configurations' workloads, for the record (not part of `value`)
         out["pathtrace"] = pathtrace_results(args.spp)
         out["deep_zoom"] = planar_results(args.planar_size, args.planar_seeds)
+        out["control"] = control_results()
     if not args.no_cpu:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.width, args.height)
