@@ -10,7 +10,7 @@
 
 use std::os::raw::{c_char, c_int, c_void};
 
-pub const SYF_ABI_VERSION: c_int = 3;
+pub const SYF_ABI_VERSION: c_int = 4;
 
 pub const SYF_OK: c_int = 0;
 pub const SYF_E_INVALID: c_int = -1;
@@ -30,9 +30,17 @@ pub const SYF_FAMILY_GRAPH: u32 = 4;
 pub const SYF_CAMERA_FREE: u32 = 0;
 pub const SYF_CAMERA_ORBIT: u32 = 1;
 
+pub const SYF_LIGHT_SUN: u32 = 0;
 pub const SYF_LIGHT_AREA: u32 = 1;
 pub const SYF_LIGHT_ENV: u32 = 2;
 pub const SYF_MAX_LIGHTS: usize = 4;
+
+pub const SYF_MAX_FINISHES: usize = 8;
+pub const SYF_MAX_STOPS: usize = 16;
+pub const SYF_COLOR_ITER: u32 = 4;
+pub const SYF_COLOR_NORMAL: u32 = 5;
+pub const SYF_COLOR_ORBIT: u32 = 6;
+pub const SYF_COLOR_NONE: u32 = 0xffff;
 
 pub const SYF_NODE_INPUT: u32 = 0;
 pub const SYF_NODE_OUTPUT: u32 = 1;
@@ -193,6 +201,43 @@ pub struct syf_material {
 
 #[repr(C)]
 #[derive(Clone, Copy, Default, Debug)]
+pub struct syf_finish {
+    pub albedo: [f32; 3],
+    pub roughness: f32,
+    pub metallic: f32,
+    pub specular: f32,
+    pub emission: [f32; 3],
+    pub oren_nayar: u32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct syf_coloring {
+    pub slot: u32,
+    pub stop_count: u32,
+    pub cyclic: u32,
+    pub smooth: u32,
+    pub positioned: u32,
+    pub frequency: f32,
+    pub phase: f32,
+    pub gamma: f32,
+    pub stops: [[f32; 4]; SYF_MAX_STOPS],
+    pub weights: [f32; 4],
+    pub iteration_weight: f32,
+    pub normal_weight: f32,
+    pub logarithmic: f32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct syf_ground {
+    pub enabled: u32,
+    pub height: f32,
+    pub finish: u32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
 pub struct syf_graph_node {
     pub id: u32,
     pub kind: u32,
@@ -259,6 +304,10 @@ pub struct syf_job_desc {
     pub cavity_darkness: f32,
     pub march_math: u32,
     pub seed: u64,
+    pub ground: syf_ground,
+    pub coloring: syf_coloring,
+    pub finish_count: u32,
+    pub finishes: [syf_finish; SYF_MAX_FINISHES],
 }
 
 #[repr(C)]

@@ -2,18 +2,23 @@
 //! RAII handles for a device context and a render job.
 //!
 //! [`job_desc_from_scene`] restates `Uniforms::from_scene` (crates/syfrack-render/src/uniforms.rs:748-853)
-//! for this path: camera mode and lens, lights, base material and surface variation, cavity darkness,
-//! seed and the render parameters; the library derives the camera basis, light frames, step scale and
-//! ray extent itself.  [`GraphBuf::from_graph`] lowers the form graph node for node
+//! for this path: camera mode and lens, sun, area and environment lights, the base material with its
+//! colouring ramp, surface variation and named finishes, the flat ground, cavity darkness, seed and the
+//! render parameters; the library derives the camera basis, light frames, step scale and ray extent
+//! itself.  [`GraphBuf::from_graph`] lowers the form graph node for node
 //! (crates/syfrack-core/src/scene/graph/mod.rs:103-257) to `syf_graph`; the library validates it with
 //! the reference's rules and messages.  What this path does not render is refused with an error naming
-//! the scene element: a 2D scene, lights other than area and environment, participating media and the
-//! atmosphere's extra surfaces, colourings, textures, named finishes, optics and phenomena.
+//! the scene element: a 2D scene, projector, tube and HDRI lights, participating media, the seascape,
+//! portal, nebula and dunes, textures, finish detail patterns that change the shading, optics and
+//! phenomena.
 
 use std::ffi::{CStr, CString};
 use std::os::raw::c_int;
 
-use syfrack_core::{Camera, CameraMode, Color, Dimension, Graph, Light, Measure, NodeKind, Scene, TrapMode};
+use syfrack_core::{
+    Camera, CameraMode, Color, Coloring, Dimension, Finish, FinishPattern, Graph, Light, Measure, NodeKind,
+    PaletteInterpolation, Scene, TrapMode,
+};
 
 use crate::*;
 
@@ -111,17 +116,88 @@ fn light(i: usize, l: &Light) -> Result<syf_light, Error> {
             intensity: *intensity,
             ..Default::default()
         }),
+        // GpuLight::from_light (uniforms.rs:656-668): the library normalises the direction and
+        // takes the cosine of the angular radius
+        Light::Sun { direction, color, intensity, angular_size } => Ok(syf_light {
+            kind: SYF_LIGHT_SUN,
+            normal: v3(*direction),
+            size: [*angular_size, 0.0],
+            color: rgb(color),
+            intensity: *intensity,
+            ..Default::default()
+        }),
         other => {
             let kind = match other {
-                Light::Sun { .. } => "Sun",
                 Light::Projector { .. } => "Projector",
                 Light::Tube { .. } => "Tube",
                 Light::Hdri { .. } => "Hdri",
                 _ => "this kind of light",
             };
-            Err(refuse(format!("lights[{i}]: {kind} is not rendered on this path (area and environment lights are)")))
+            Err(refuse(format!("lights[{i}]: {kind} is not rendered on this path (sun, area and environment lights are)")))
         }
     }
+}
+
+/// Coloring with its Ramp as GpuColoring / GpuRamp (uniforms.rs:31-55, 194-214).
+fn coloring(c: &Coloring) -> Result<syf_coloring, Error> {
+    let mut out = syf_coloring { slot: SYF_COLOR_NONE, frequency: 1.0, gamma: 1.0, ..Default::default() };
+    let (slot, ramp, orbit) = match c {
+        Coloring::None => return Ok(out),
+        Coloring::Trap { slot, ramp } => (*slot, ramp, None),
+        Coloring::Orbit { channels, ramp } => (SYF_COLOR_ORBIT, ramp, Some(channels)),
+    };
+    if ramp.stops.len() > SYF_MAX_STOPS {
+        return Err(refuse(format!("material.coloring: at most {SYF_MAX_STOPS} stops, got {}", ramp.stops.len())));
+    }
+    if !ramp.positions.is_empty() && ramp.positions.len() != ramp.stops.len() {
+        return Err(refuse("material.coloring: one position per stop, or none"));
+    }
+    out.slot = slot;
+    out.stop_count = ramp.stops.len() as u32;
+    out.cyclic = ramp.cyclic as u32;
+    out.smooth = (ramp.interpolation == PaletteInterpolation::Smooth) as u32;
+    out.positioned = (!ramp.positions.is_empty()) as u32;
+    out.frequency = ramp.frequency;
+    out.phase = ramp.phase;
+    out.gamma = ramp.gamma;
+    for (i, s) in ramp.stops.iter().enumerate() {
+        out.stops[i] = [s[0], s[1], s[2], ramp.position(i)];
+    }
+    if let Some(o) = orbit {
+        out.weights = o.traps;
+        out.iteration_weight = o.iterations;
+        out.normal_weight = o.normal;
+        out.logarithmic = o.logarithmic;
+    }
+    Ok(out)
+}
+
+/// Finish as GpuFinish (uniforms.rs:787-817).  A detail pattern is lowered only where it leaves the
+/// shading unchanged (strength, bump and roughness_delta zero: make_surface, pathtrace.wgsl:307-345);
+/// the concrete patterns then keep their Oren-Nayar diffuse lobe (pathtrace.wgsl:381-384).
+fn finish(i: usize, f: &Finish) -> Result<syf_finish, Error> {
+    if f.optics.is_some() {
+        return Err(refuse(format!("material.finishes[{i}].optics: not rendered on this path")));
+    }
+    let mut oren_nayar = 0;
+    if let Some(d) = &f.detail {
+        if d.strength != 0.0 || d.bump != 0.0 || d.roughness_delta != 0.0 {
+            return Err(refuse(format!(
+                "material.finishes[{i}].detail: finish patterns are not rendered on this path (a detail with zero \
+                 strength, bump and roughness_delta is)"
+            )));
+        }
+        use FinishPattern::*;
+        oren_nayar = matches!(d.pattern, Concrete | WeatheredConcrete | WetConcrete) as u32;
+    }
+    Ok(syf_finish {
+        albedo: f.albedo,
+        roughness: f.roughness,
+        metallic: f.metallic,
+        specular: f.specular,
+        emission: f.emission,
+        oren_nayar,
+    })
 }
 
 /// The job description of a 3D scene (everything but the form graph, which goes to
@@ -137,18 +213,18 @@ pub fn job_desc_from_scene(scene: &Scene, p: &RenderParams) -> Result<syf_job_de
     if a.fog_density > 0.0 {
         return Err(refuse("atmosphere.fog_density: participating media are not rendered on this path"));
     }
-    if a.seascape.is_some() || a.ground.is_some() || a.portal.is_some() || a.nebula.is_some() {
-        return Err(refuse("atmosphere: seascape, ground, portal and nebula are not rendered on this path"));
+    if a.seascape.is_some() || a.portal.is_some() || a.nebula.is_some() {
+        return Err(refuse("atmosphere: seascape, portal and nebula are not rendered on this path"));
+    }
+    if a.ground.as_ref().is_some_and(|g| g.dunes.is_some()) {
+        return Err(refuse("atmosphere.ground.dunes: not rendered on this path (a flat ground is)"));
     }
     let m = &scene.material;
-    if !matches!(m.coloring, syfrack_core::Coloring::None) {
-        return Err(refuse("material.coloring: colourings are not rendered on this path (surface variation is)"));
-    }
     if m.texture.is_some() {
         return Err(refuse("material.texture: not rendered on this path"));
     }
-    if !m.finishes.is_empty() {
-        return Err(refuse("material.finishes: named finishes are not rendered on this path"));
+    if m.finishes.len() > SYF_MAX_FINISHES {
+        return Err(refuse(format!("material.finishes: at most {SYF_MAX_FINISHES}, got {}", m.finishes.len())));
     }
     if m.optics.is_some() {
         return Err(refuse("material.optics: not rendered on this path"));
@@ -179,6 +255,14 @@ pub fn job_desc_from_scene(scene: &Scene, p: &RenderParams) -> Result<syf_job_de
         d.lights[i] = light(i, l)?;
     }
     d.light_count = scene.lights.len() as u32;
+    d.coloring = coloring(&m.coloring)?;
+    for (i, f) in m.finishes.iter().enumerate() {
+        d.finishes[i] = finish(i, f)?;
+    }
+    d.finish_count = m.finishes.len() as u32;
+    if let Some(g) = &a.ground {
+        d.ground = syf_ground { enabled: 1, height: g.height, finish: g.finish };
+    }
     d.width = p.width;
     d.height = p.height;
     d.max_bounces = p.max_bounces;

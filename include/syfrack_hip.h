@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SYF_ABI_VERSION 3
+#define SYF_ABI_VERSION 4
 
 /* ---------------------------------------------------------------- status codes
  * Replaces anyhow::Result at the seams: config errors (renderer.rs:81-138),
@@ -83,16 +83,19 @@ typedef struct syf_camera {
     float focus_distance;
 } syf_camera;
 
+#define SYF_LIGHT_SUN 0u  /* GpuLight KIND_SUN (uniforms.rs:656-668): a distant disk */
 #define SYF_LIGHT_AREA 1u /* GpuLight KIND_AREA (uniforms.rs:676-716) */
 #define SYF_LIGHT_ENV 2u  /* GpuLight KIND_ENV (uniforms.rs:732-744) */
 #define SYF_MAX_LIGHTS 4
 typedef struct syf_light {
-    uint32_t kind;            /* SYF_LIGHT_AREA or SYF_LIGHT_ENV */
+    uint32_t kind;            /* SYF_LIGHT_SUN, SYF_LIGHT_AREA or SYF_LIGHT_ENV */
     float position[3];        /* area: centre */
-    float normal[3];          /* area: emitting side */
-    float size[2];            /* area: full width and height */
-    float color[3];           /* area: linear colour */
-    float intensity;          /* both: multiplies the colour(s) */
+    float normal[3];          /* area: emitting side; sun: direction towards the sun */
+    float size[2];            /* area: full width and height; sun: size[0] = angular diameter,
+                                 degrees, 0.01..90 */
+    float color[3];           /* area, sun: linear colour */
+    float intensity;          /* all: multiplies the colour(s); a sun's colour x intensity is
+                                 its irradiance */
     float zenith[3];          /* env */
     float horizon[3];         /* env */
     float ground[3];          /* env */
@@ -105,10 +108,56 @@ typedef struct syf_material {
     float metallic;
     float specular;
     uint32_t surface_enabled;      /* SurfaceVariation present */
-    uint32_t surface_slot;         /* trap slot 0..3 that drives the variation */
+    uint32_t surface_slot;         /* what drives the variation: trap slot 0..3, SYF_COLOR_ITER or
+                                      SYF_COLOR_NORMAL (surface_value, pathtrace.wgsl:265-269) */
     float surface_roughness[2];    /* roughness at t = 0 and t = 1 */
     float surface_metallic[2];     /* metallic at t = 0 and t = 1 */
 } syf_material;
+
+/* The shading of a path-traced job beyond the base material (pathtrace.wgsl:271-420):
+ * named finishes that MaterialTag nodes (and the ground) select, the colouring ramp, and a
+ * flat ground plane.  All zero: no finishes, no colouring, no ground. */
+#define SYF_MAX_FINISHES 8   /* GpuFinish slots (uniforms.rs:787) */
+#define SYF_MAX_STOPS 16     /* MAX_PALETTE_STOPS (palette.rs:6) */
+/* Finish (material.rs:161-174) as GpuFinish (uniforms.rs:787-817).  A finish's detail pattern
+ * is accepted only where it leaves the shading unchanged (strength, bump and roughness_delta
+ * zero); then the Concrete, WeatheredConcrete and WetConcrete patterns keep their Oren-Nayar
+ * diffuse lobe (pathtrace.wgsl:381-384, 407-418), selected by oren_nayar = 1. */
+typedef struct syf_finish {
+    float albedo[3];
+    float roughness;
+    float metallic;
+    float specular;
+    float emission[3];        /* linear emitted radiance */
+    uint32_t oren_nayar;      /* 1: rough diffuse with sigma = 0.45 x roughness */
+} syf_finish;
+#define SYF_COLOR_ITER 4u     /* COLOR_SLOT_ITER (common.wgsl:13): iterations / iteration product */
+#define SYF_COLOR_NORMAL 5u   /* COLOR_SLOT_NORMAL: n.y / 2 + 1/2 */
+#define SYF_COLOR_ORBIT 6u    /* COLOR_SLOT_ORBIT: the weighted orbit mix */
+#define SYF_COLOR_NONE 0xffffu
+/* Coloring (material.rs:12-37, 113-119) with its Ramp (palette.rs:74-115) as GpuColoring
+ * (uniforms.rs:194-214) and GpuRamp (uniforms.rs:31-55). */
+typedef struct syf_coloring {
+    uint32_t slot;            /* trap slot 0..3, SYF_COLOR_ITER, _NORMAL, _ORBIT or _NONE */
+    uint32_t stop_count;      /* 0..SYF_MAX_STOPS (0: the ramp is white) */
+    uint32_t cyclic;
+    uint32_t smooth;          /* smooth-step interpolation between stops */
+    uint32_t positioned;      /* the stop positions were given (else evenly spaced) */
+    float frequency;
+    float phase;
+    float gamma;
+    float stops[SYF_MAX_STOPS][4]; /* linear rgb and position (Ramp::position) */
+    float weights[4];         /* orbit: trap weights */
+    float iteration_weight;   /* orbit */
+    float normal_weight;      /* orbit */
+    float logarithmic;        /* orbit: signed log gain, 0 = linear */
+} syf_coloring;
+/* Atmosphere.ground (GpuAtmosphere.ground, uniforms.rs:227-230): a flat plane (no dunes). */
+typedef struct syf_ground {
+    uint32_t enabled;
+    float height;
+    uint32_t finish;          /* its finish: 1..finish_count, 0 = the base material */
+} syf_ground;
 
 /* ---------------------------------------------------------------- node graphs
  * A distance field as a node graph (scene/graph/mod.rs:103-257, 427-451): nodes, links between
@@ -218,6 +267,10 @@ typedef struct syf_job_desc {
     uint32_t march_math;           /* SYF_MATH_FAST (0) or SYF_MATH_EXACT: arithmetic of the primary
                                       trace's FP32 march (no counterpart in the reference) */
     uint64_t seed;                 /* Scene.seed; folded lo ^ hi as uniforms.rs:824 */
+    syf_ground ground;             /* Atmosphere.ground */
+    syf_coloring coloring;         /* Material.coloring */
+    uint32_t finish_count;         /* Material.finishes, 0..SYF_MAX_FINISHES */
+    syf_finish finishes[SYF_MAX_FINISHES];
 } syf_job_desc;
 /* SYF_MATH_FAST: hardware reciprocal / reciprocal square root / logarithm and FMA contraction in
  * the FP32 march; held to the FP64 oracle's tolerance (>= 99.9 % hit/miss, refined hit distance

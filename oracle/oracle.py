@@ -68,6 +68,8 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_escape_f64.argtypes = [P] + [C.c_uint32] * 4 + [P, P]
     lib.orc_escape_f128.restype = None
     lib.orc_escape_f128.argtypes = [P, P, P, C.c_size_t, C.c_double, P, P]
+    lib.orc_ramp_sample.restype = None
+    lib.orc_ramp_sample.argtypes = [P, P, C.c_size_t, P]
     if hasattr(lib, "orc_render"):
         lib.orc_render.restype = C.c_int
         lib.orc_render.argtypes = [P] + [C.c_uint32] * 6 + [P, P, P, P]
@@ -251,6 +253,14 @@ def escape_f64_points(desc, px: np.ndarray, py: np.ndarray):
     lib.orc_escape_f64_points.restype = None
     lib.orc_escape_f64_points(C.byref(desc), _ptr(px), _ptr(py), C.c_size_t(len(px)), _ptr(iters))
     return iters
+
+
+def ramp_sample(coloring, values) -> np.ndarray:
+    """palette_sample of a syf_coloring's ramp at each value: (n, 3) linear rgb."""
+    v = np.ascontiguousarray(np.asarray(values, np.float32).ravel())
+    out = np.zeros((len(v), 3), np.float32)
+    load().orc_ramp_sample(C.byref(coloring), _ptr(v), len(v), _ptr(out))
+    return out
 
 
 def render(desc, rect, sample_start: int, sample_count: int):

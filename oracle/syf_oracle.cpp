@@ -567,6 +567,7 @@ struct LightL {
     uint32_t kind;
     F3 a, b, c, color, horizon, ground, n;
     float area;
+    float cos_max;  // sun: cosine of the disk's angular radius
 };
 
 struct SceneL {
@@ -577,6 +578,11 @@ struct SceneL {
     LightL lights[SYF_MAX_LIGHTS];
     uint32_t light_count;
     syf_material mat;
+    syf_coloring coloring;  // stop positions resolved (Ramp::position)
+    uint32_t color_slot;    // the coloring's slot, SYF_COLOR_NONE without stops
+    uint32_t finish_count;
+    syf_finish finishes[SYF_MAX_FINISHES];
+    syf_ground ground;
 };
 
 inline F3 f3(const float* p) { return v3(p[0], p[1], p[2]); }
@@ -631,6 +637,12 @@ SceneL lower_scene(const syf_job_desc& d) {
             L.n = safe_normalize(cross(L.b, L.c));  // area_normal (lights.wgsl:164-166)
             L.area = length(cross(L.b, L.c));       // emitter_area (lights.wgsl:69-74)
             L.color = f3(l.color) * l.intensity;
+        } else if (l.kind == SYF_LIGHT_SUN) {  // Light::Sun (uniforms.rs:656-668)
+            F3 n0 = f3(l.normal);
+            float rcp = 1.0f / length(n0);
+            L.a = (std::isfinite(rcp) && rcp > 0.0f) ? n0 * rcp : v3(0.0f, 1.0f, 0.0f);
+            L.cos_max = std::cos((l.size[0] * 0.5f) * (3.14159265358979323846f / 180.0f));
+            L.color = f3(l.color) * l.intensity;
         } else {
             L.color = f3(l.zenith) * l.intensity;  // env: a = zenith, b = horizon, c = ground
             L.horizon = f3(l.horizon) * l.intensity;
@@ -638,12 +650,24 @@ SceneL lower_scene(const syf_job_desc& d) {
         }
     }
     S.mat = d.material;
+    // GpuColoring / GpuRamp (uniforms.rs:31-55, 194-214): positions from Ramp::position
+    S.coloring = d.coloring;
+    const uint32_t ns = d.coloring.stop_count;
+    for (uint32_t i = 0; i < ns; ++i)
+        if (!d.coloring.positioned)
+            S.coloring.stops[i][3] = float(i) / float(d.coloring.cyclic ? ns : (ns > 1u ? ns - 1u : 1u));
+    S.color_slot = ns > 0u ? d.coloring.slot : SYF_COLOR_NONE;
+    S.finish_count = d.finish_count;
+    for (uint32_t i = 0; i < SYF_MAX_FINISHES; ++i) S.finishes[i] = d.finishes[i];
+    S.ground = d.ground;
     return S;
 }
 
 struct Surf {
     F3 n, diffuse, f0;
     float alpha;
+    F3 emission;
+    float sigma;  // diffuse_sigma: Oren-Nayar roughness, 0 = Lambert
 };
 
 // Branchless orthonormal basis, columns t, bt, n (pathtrace.wgsl:194-201).
@@ -693,14 +717,81 @@ F3 sample_ggx_h(F3 n, F3 v, float alpha, float r1, float r2) {
     F3 nh = t1 * x + t2 * y + vh * std::sqrt(std::fmax(0.0f, 1.0f - x * x - y * y));
     return frame_mul(t, bt, n, safe_normalize(v3(alpha * nh.x, alpha * nh.y, std::fmax(nh.z, 0.0f))));
 }
-// make_surface subset (pathtrace.wgsl:271-386): base material, SurfaceVariation.
-Surf make_surface(const SceneL& S, F3 n, float trap0, uint32_t iterations) {
+// palette_mix / palette_sample (common.wgsl:331-362)
+F3 palette_mix(const syf_coloring& r, uint32_t a, uint32_t b, float amount) {
+    float f = clampv(amount, 0.0f, 1.0f);
+    if (r.smooth) f = f * f * (3.0f - 2.0f * f);
+    return mix3(f3(r.stops[a]), f3(r.stops[b]), f);
+}
+F3 ramp_sample(const syf_coloring& r, float value) {
+    const uint32_t n = r.stop_count;
+    if (n == 0u) return v3(1.0f, 1.0f, 1.0f);
+    if (n == 1u) return f3(r.stops[0]);
+    float t = value * r.frequency + r.phase;
+    if (r.cyclic) t = t - std::floor(t);
+    else t = clampv(t, 0.0f, 1.0f);
+    if (r.gamma != 1.0f) t = std::pow(t, r.gamma);
+    if (!r.positioned && !r.cyclic) {
+        const float x = t * float(n - 1u);
+        const uint32_t i = std::min(uint32_t(std::floor(x)), n - 2u);
+        return palette_mix(r, i, i + 1u, x - float(i));
+    }
+    const float first = r.stops[0][3], last = r.stops[n - 1u][3];
+    if (r.cyclic && (t < first || t >= last)) {
+        const float x = t < first ? t + 1.0f : t;
+        return palette_mix(r, n - 1u, 0u, (x - last) / std::fmax(1.0f + first - last, 1e-8f));
+    }
+    if (t <= first) return f3(r.stops[0]);
+    if (t >= last) return f3(r.stops[n - 1u]);
+    for (uint32_t i = 0; i < 15u; ++i)
+        if (i + 1u < n && t < r.stops[i + 1u][3])
+            return palette_mix(r, i, i + 1u, (t - r.stops[i][3]) / (r.stops[i + 1u][3] - r.stops[i][3]));
+    return f3(r.stops[n - 1u]);
+}
+// surface_value (pathtrace.wgsl:265-269)
+float surface_value(const SceneL& S, uint32_t slot, F3 n, const DeOut<float>& smp) {
+    if (slot < 4u) return smp.traps[slot];
+    if (slot == SYF_COLOR_NORMAL) return n.y * 0.5f + 0.5f;
+    return float(smp.iter) / std::fmax(float(S.F.iteration_product), 1.0f);
+}
+// make_surface (pathtrace.wgsl:271-386) for the subset: colouring ramp, named finishes
+// (emission; Oren-Nayar for the concrete patterns, sigma = 0.45 roughness), SurfaceVariation.
+Surf make_surface(const SceneL& S, F3 n, const DeOut<float>& smp) {
     const syf_material& m = S.mat;
     F3 albedo = f3(m.albedo);
-    float rough = m.roughness, metallic = m.metallic, spec = m.specular;
-    if (m.surface_enabled) {
-        float value = m.surface_slot == 0 ? trap0 : 1.0f;  // classic graphs write slot 0 only
-        float t = clampv(value, 0.0f, 1.0f);
+    const uint32_t slot = S.color_slot;
+    const bool assigned = smp.material > 0u && smp.material <= S.finish_count;
+    const float iters = float(smp.iter) / std::fmax(float(S.F.iteration_product), 1.0f);
+    if (!assigned && slot == SYF_COLOR_ORBIT) {
+        const syf_coloring& c = S.coloring;
+        float value = c.weights[0] * smp.traps[0] + c.weights[1] * smp.traps[1] + c.weights[2] * smp.traps[2] +
+                      c.weights[3] * smp.traps[3] + c.iteration_weight * iters + c.normal_weight * (n.y * 0.5f + 0.5f);
+        const float gain = c.logarithmic;
+        if (gain > 0.0f) {
+            const float sg = value > 0.0f ? 1.0f : (value < 0.0f ? -1.0f : 0.0f);
+            value = sg * std::log(1.0f + gain * std::fabs(value)) / std::log(1.0f + gain);
+        }
+        albedo = mul3(albedo, ramp_sample(c, value));
+    } else if (!assigned && slot < 4u) {
+        albedo = mul3(albedo, ramp_sample(S.coloring, smp.traps[slot]));
+    } else if (!assigned && slot == SYF_COLOR_NORMAL) {
+        albedo = mul3(albedo, ramp_sample(S.coloring, n.y * 0.5f + 0.5f));
+    } else if (!assigned && slot == SYF_COLOR_ITER) {
+        albedo = mul3(albedo, ramp_sample(S.coloring, iters));
+    }
+    float rough = m.roughness, metallic = m.metallic, spec = m.specular, sigma_scale = 0.0f;
+    F3 emission = v3(0.0f, 0.0f, 0.0f);
+    if (assigned) {
+        const syf_finish& f = S.finishes[smp.material - 1u];
+        albedo = f3(f.albedo);
+        rough = f.roughness;
+        metallic = f.metallic;
+        spec = f.specular;
+        emission = f3(f.emission);
+        sigma_scale = f.oren_nayar ? 0.45f : 0.0f;
+    }
+    if (!assigned && m.surface_enabled) {
+        const float t = clampv(surface_value(S, m.surface_slot, n, smp), 0.0f, 1.0f);
         rough = m.surface_roughness[0] + (m.surface_roughness[1] - m.surface_roughness[0]) * t;
         metallic = m.surface_metallic[0] + (m.surface_metallic[1] - m.surface_metallic[0]) * t;
     }
@@ -710,10 +801,11 @@ Surf make_surface(const SceneL& S, F3 n, float trap0, uint32_t iterations) {
     s.diffuse = albedo * (1.0f - metallic);
     s.f0 = mix3(v3(spec, spec, spec), albedo, metallic);
     s.alpha = rough * rough;
-    (void)iterations;
+    s.emission = emission;
+    s.sigma = rough * sigma_scale;
     return s;
 }
-// brdf_eval (pathtrace.wgsl:394-420), Lambert + GGX, no Oren-Nayar term.
+// brdf_eval (pathtrace.wgsl:394-420): Lambert or Oren-Nayar diffuse + GGX.
 F3 brdf_eval(const Surf& s, F3 v, F3 l) {
     float nl = dot(s.n, l), nv = dot(s.n, v);
     if (nl <= 0.0f || nv <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
@@ -725,8 +817,18 @@ F3 brdf_eval(const Surf& s, F3 v, F3 l) {
     float k = ggx_d(nh, a2) * smith_g1(nv, a2) * smith_g1(nl, a2);
     float den = std::fmax(4.0f * nv * nl, 1e-4f);
     F3 spec = v3(k * f.x / den, k * f.y / den, k * f.z / den);
+    float df = 1.0f;
+    if (s.sigma > 0.0f) {
+        const float sigma2 = s.sigma * s.sigma;
+        const float a = 1.0f - sigma2 / (2.0f * (sigma2 + 0.33f));
+        const float b = 0.45f * sigma2 / (sigma2 + 0.09f);
+        const float sin_l = std::sqrt(std::fmax(0.0f, 1.0f - nl * nl));
+        const float sin_v = std::sqrt(std::fmax(0.0f, 1.0f - nv * nv));
+        const float cos_phi = std::fmax(0.0f, dot(l - s.n * nl, v - s.n * nv) / std::fmax(sin_l * sin_v, 1e-8f));
+        df = a + b * cos_phi * sin_l * sin_v / std::fmax(std::fmax(nl, nv), 1e-5f);
+    }
     F3 one_f = v3(1.0f - f.x, 1.0f - f.y, 1.0f - f.z);
-    F3 dif = mul3(s.diffuse, one_f) * 1.0f;
+    F3 dif = mul3(s.diffuse, one_f) * df;
     return v3(dif.x / PI_F + spec.x, dif.y / PI_F + spec.y, dif.z / PI_F + spec.z);
 }
 inline float mis_power(float a, float b) {
@@ -790,10 +892,26 @@ struct HitL {
     Hit<float> h;
     uint32_t light;
 };
-// trace_geometry_lights (lights.wgsl:183-209) without ground.
+constexpr uint32_t GROUND_HIT = 0xfffffffeu;  // HitL.light of a segment that ended on the ground
+// ground_intersection / ground_normal of a flat ground (landscape.wgsl:17-23, 71-78)
+float ground_intersection(const SceneL& S, F3 ro, F3 rd, float t_end) {
+    if (std::fabs(rd.y) < 1e-7f) return t_end;
+    const float t = (S.ground.height - ro.y) / rd.y;
+    return (t > 1e-5f && t < t_end) ? t : t_end;
+}
+inline F3 ground_normal(F3 rd) { return rd.y > 0.0f ? v3(0.0f, -1.0f, 0.0f) : v3(0.0f, 1.0f, 0.0f); }
+// trace_geometry_lights (lights.wgsl:183-209): the ground, the area lights, then the march up to
+// the nearest of them.
 HitL trace_scene(const SceneL& S, F3 ro, F3 rd, float t_end, Count* c) {
     float nearest = t_end;
     uint32_t light = NO_LIGHT;
+    if (S.ground.enabled) {
+        const float t = ground_intersection(S, ro, rd, nearest);
+        if (t > 1e-5f && t < nearest) {
+            nearest = t;
+            light = GROUND_HIT;
+        }
+    }
     for (uint32_t i = 0; i < S.light_count; ++i) {
         if (S.lights[i].kind != SYF_LIGHT_AREA) continue;
         float t = area_intersection(ro, rd, S.lights[i]);
@@ -854,7 +972,29 @@ F3 environment_emission(const SceneL& S, F3 dir, float pdf, bool use_mis, F3 n, 
     }
     return L;
 }
-// direct_light (pathtrace.wgsl:471-558), environment and area branches.
+// cone_sample / sun_pdf (pathtrace.wgsl:210-215, 432-434)
+F3 cone_sample(F3 dir, float cos_max, float r1, float r2) {
+    const float cos_t = 1.0f - r1 * (1.0f - cos_max);
+    const float sin_t = std::sqrt(std::fmax(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = 2.0f * PI_F * r2;
+    F3 t, bt;
+    tangent_frame(dir, &t, &bt);
+    return frame_mul(t, bt, dir, v3(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t));
+}
+inline float sun_pdf(float cos_max) { return 1.0f / (2.0f * PI_F * std::fmax(1.0f - cos_max, 1e-7f)); }
+// sun_radiance (pathtrace.wgsl:454-467)
+F3 sun_radiance(const SceneL& S, F3 dir, float pdf_bsdf, bool use_mis) {
+    F3 L = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < S.light_count; ++i) {
+        const LightL& l = S.lights[i];
+        if (l.kind != SYF_LIGHT_SUN || !(dot(dir, l.a) >= l.cos_max)) continue;
+        const float pl = sun_pdf(l.cos_max);
+        const float w = use_mis ? mis_power(pdf_bsdf, pl) : 1.0f;
+        L = L + v3(l.color.x * pl * w, l.color.y * pl * w, l.color.z * pl * w);
+    }
+    return L;
+}
+// direct_light (pathtrace.wgsl:471-558): environment, sun and area branches.
 F3 direct_light(const SceneL& S, F3 p, const Surf& s, F3 gn, F3 v, float p_spec, float area_total, bool last,
                 uint32_t* rng, Count* c) {
     F3 L = v3(0.0f, 0.0f, 0.0f);
@@ -895,6 +1035,19 @@ F3 direct_light(const SceneL& S, F3 p, const Surf& s, F3 gn, F3 v, float p_spec,
             F3 f = brdf_eval(s, v, dir);
             F3 e = environment_value(l, dir);
             L = L + v3(f.x * nl * e.x * 1.0f * w / pl, f.y * nl * e.y * 1.0f * w / pl, f.z * nl * e.z * 1.0f * w / pl);
+        } else if (l.kind == SYF_LIGHT_SUN) {
+            // colour = irradiance: the disk's radiance is colour * sun_pdf
+            const float r1 = rand01(rng);
+            const float r2 = rand01(rng);
+            const F3 dir = cone_sample(l.a, l.cos_max, r1, r2);
+            const float nl = dot(s.n, dir);
+            if (nl <= 0.0f || dot(gn, dir) <= 0.0f) continue;
+            if (!visible(S, p, dir, S.t_end, c)) continue;
+            const float pl = sun_pdf(l.cos_max);
+            const float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
+            const float w = last ? 1.0f : mis_power(pl, pb);
+            const F3 f = brdf_eval(s, v, dir);
+            L = L + v3(f.x * nl * l.color.x * w, f.y * nl * l.color.y * w, f.z * nl * l.color.z * w);
         } else if (l.kind == SYF_LIGHT_AREA) {
             if (i != selected) continue;
             float u1 = rand01(rng);
@@ -934,15 +1087,18 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
         HitL hl = trace_scene(S, ro, rd, S.t_end, c);
         const Hit<float>& h = hl.h;
         F3 gn = v3(0.0f, 0.0f, 0.0f);
+        const bool ground = hl.light == GROUND_HIT;
         if (h.hit) {
-            if (hl.light != NO_LIGHT) {
+            if (ground) {
+                gn = ground_normal(rd);
+            } else if (hl.light != NO_LIGHT) {
                 gn = S.lights[hl.light].n;
             } else {
                 gn = normal_at(S.F, ro + rd * h.t, std::fmax(h.t * S.pixel_cone * 0.5f, 1e-4f) * S.pixel_tol, c);
             }
             if (bounce == 0) {
                 *first_depth = h.t;
-                *first_tprev = hl.light == NO_LIGHT ? h.t_prev : -1.0f;  // refine bracket of a surface hit
+                *first_tprev = hl.light == NO_LIGHT ? h.t_prev : -1.0f;  // refine bracket of a fractal hit
                 first_aux[0] = gn.x; first_aux[1] = gn.y; first_aux[2] = gn.z; first_aux[3] = float(h.steps);
             }
         }
@@ -950,23 +1106,33 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
             if (!h.exhausted) {
                 F3 e = environment_emission(S, rd, prev_pdf, prev_from_bsdf, prev_n, prev_total);
                 L = L + mul3(throughput, e);
+                L = L + mul3(throughput, sun_radiance(S, rd, prev_pdf, prev_from_bsdf));
             }
             break;
         }
         F3 p = ro + rd * h.t;
-        if (hl.light != NO_LIGHT) {
+        if (!ground && hl.light != NO_LIGHT) {
             F3 e = area_emission(S, ro, rd, h.t, prev_pdf, prev_from_bsdf, prev_n, prev_total);
             L = L + mul3(throughput, e);
             break;
         }
         F3 n = gn;
-        if (bounce == 0) {
+        if (bounce == 0 && !ground) {
             float k = 1.0f - S.cavity * smoothstep01(float(h.steps) / 96.0f);
             throughput = throughput * k;
         }
         F3 v = neg(rd);
-        Surf s = make_surface(S, n, h.sample.traps[0], h.sample.iter);
-        float offset = std::fmax(h.t * S.pixel_cone * 2.0f, 1e-3f) * S.pixel_tol;
+        // the ground's sample: the march's initial DeSample (pathtrace.wgsl:86) with the ground
+        // finish as material (lights.wgsl:201)
+        DeOut<float> smp = h.sample;
+        if (ground) {
+            smp.traps[0] = smp.traps[1] = smp.traps[2] = smp.traps[3] = 1.0f;
+            smp.iter = 0u;
+            smp.material = S.ground.finish;
+        }
+        Surf s = make_surface(S, n, smp);
+        L = L + mul3(throughput, s.emission);
+        float offset = ground ? 1e-3f : std::fmax(h.t * S.pixel_cone * 2.0f, 1e-3f) * S.pixel_tol;
         F3 p_off = p + n * offset;
         float nv = std::fmax(dot(s.n, v), 1e-4f);
         float p_spec = spec_probability(s, nv);
@@ -1419,6 +1585,22 @@ int orc_control_trace(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t 
             status[i] = 0u;
         }
     return 0;
+}
+
+// palette_sample (common.wgsl:336-366) of a coloring's ramp at n values, rgb out; stop
+// positions resolved as lower_scene does (Ramp::position, palette.rs:106-115).
+void orc_ramp_sample(const syf_coloring* c, const float* values, size_t n, float* rgb) {
+    syf_coloring r = *c;
+    const uint32_t ns = std::min<uint32_t>(r.stop_count, SYF_MAX_STOPS);
+    r.stop_count = ns;
+    for (uint32_t i = 0; i < ns; ++i)
+        if (!r.positioned) r.stops[i][3] = float(i) / float(r.cyclic ? ns : (ns > 1u ? ns - 1u : 1u));
+    for (size_t i = 0; i < n; ++i) {
+        const F3 v = ramp_sample(r, values[i]);
+        rgb[3 * i] = v.x;
+        rgb[3 * i + 1] = v.y;
+        rgb[3 * i + 2] = v.z;
+    }
 }
 
 // Megakernel accumulation (pathtrace.wgsl:1203-1276) of a pixel rectangle: samples

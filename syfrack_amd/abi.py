@@ -30,9 +30,17 @@ FAMILY_GRAPH = 4
 CAMERA_FREE = 0
 CAMERA_ORBIT = 1
 
+LIGHT_SUN = 0
 LIGHT_AREA = 1
 LIGHT_ENV = 2
 MAX_LIGHTS = 4
+
+MAX_FINISHES = 8
+MAX_STOPS = 16
+COLOR_ITER = 4
+COLOR_NORMAL = 5
+COLOR_ORBIT = 6
+COLOR_NONE = 0xFFFF
 
 PLANE_ACCUM = 0
 PLANE_DEPTH = 1
@@ -128,6 +136,43 @@ class SyfMaterial(C.Structure):
     ]
 
 
+class SyfFinish(C.Structure):
+    _fields_ = [
+        ("albedo", F3),
+        ("roughness", C.c_float),
+        ("metallic", C.c_float),
+        ("specular", C.c_float),
+        ("emission", F3),
+        ("oren_nayar", C.c_uint32),
+    ]
+
+
+class SyfColoring(C.Structure):
+    _fields_ = [
+        ("slot", C.c_uint32),
+        ("stop_count", C.c_uint32),
+        ("cyclic", C.c_uint32),
+        ("smooth", C.c_uint32),
+        ("positioned", C.c_uint32),
+        ("frequency", C.c_float),
+        ("phase", C.c_float),
+        ("gamma", C.c_float),
+        ("stops", (C.c_float * 4) * MAX_STOPS),
+        ("weights", C.c_float * 4),
+        ("iteration_weight", C.c_float),
+        ("normal_weight", C.c_float),
+        ("logarithmic", C.c_float),
+    ]
+
+
+class SyfGround(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_uint32),
+        ("height", C.c_float),
+        ("finish", C.c_uint32),
+    ]
+
+
 class SyfJobDesc(C.Structure):
     _fields_ = [
         ("fractal", SyfFractal),
@@ -143,6 +188,10 @@ class SyfJobDesc(C.Structure):
         ("cavity_darkness", C.c_float),
         ("march_math", C.c_uint32),
         ("seed", C.c_uint64),
+        ("ground", SyfGround),
+        ("coloring", SyfColoring),
+        ("finish_count", C.c_uint32),
+        ("finishes", SyfFinish * MAX_FINISHES),
     ]
 
 

@@ -117,6 +117,85 @@ int lower_fractal(const syf_fractal& f, FractalP& F) {
     return SYF_OK;
 }
 
+// Colouring, finishes and ground (GpuColoring::from_coloring, uniforms.rs:194-214; GpuRamp,
+// :31-55; finishes, :787-817; GpuAtmosphere.ground, :227-230), and whether the shading reads
+// more of a surface hit's field sample than the one trap the march returns (ShadeP::full_sample).
+int lower_shading(const syf_job_desc& d, bool graph, JobP& P) {
+    ShadeP& S = P.sh;
+    const syf_coloring& c = d.coloring;
+    if (c.slot > SYF_COLOR_ORBIT && c.slot != SYF_COLOR_NONE)
+        return fail(SYF_E_INVALID, "coloring.slot must be a trap slot 0..3, SYF_COLOR_ITER, _NORMAL, _ORBIT or _NONE, got %u",
+                    c.slot);
+    if (c.stop_count > SYF_MAX_STOPS)
+        return fail(SYF_E_INVALID, "coloring.stop_count must be at most %d, got %u", SYF_MAX_STOPS, c.stop_count);
+    // a ramp without stops is white whatever its mapping (an all-zero coloring is no colouring)
+    if (c.stop_count > 0u &&
+        (!std::isfinite(c.frequency) || !std::isfinite(c.phase) || !(c.gamma > 0.0f) || !std::isfinite(c.gamma)))
+        return fail(SYF_E_INVALID, "coloring: frequency and phase must be finite and gamma positive");
+    for (uint32_t i = 0; i < c.stop_count; ++i) {
+        if (!finite3(c.stops[i]) || !std::isfinite(c.stops[i][3]))
+            return fail(SYF_E_INVALID, "coloring.stops[%u] must be finite", i);
+        if (c.positioned && i > 0 && !(c.stops[i][3] > c.stops[i - 1][3]))
+            return fail(SYF_E_INVALID, "coloring.stops[%u]: positions must increase strictly", i);
+    }
+    S.color_slot = c.stop_count > 0u ? c.slot : SYF_COLOR_NONE;
+    S.stop_count = c.stop_count;
+    S.cyclic = c.cyclic ? 1u : 0u;
+    S.smooth = c.smooth ? 1u : 0u;
+    S.positioned = c.positioned ? 1u : 0u;
+    S.frequency = c.frequency;
+    S.phase = c.phase;
+    S.gamma = c.gamma;
+    const uint32_t n = c.stop_count;
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) S.stops[i][k] = c.stops[i][k];
+        // Ramp::position (palette.rs:106-115): given, or evenly spaced
+        S.stops[i][3] = c.positioned ? c.stops[i][3]
+                                     : float(i) / float(c.cyclic ? n : (n > 1u ? n - 1u : 1u));
+    }
+    for (int k = 0; k < 4; ++k) S.weights[k] = c.weights[k];
+    S.iteration_weight = c.iteration_weight;
+    S.normal_weight = c.normal_weight;
+    S.logarithmic = c.logarithmic;
+    if (d.finish_count > SYF_MAX_FINISHES)
+        return fail(SYF_E_INVALID, "finish_count must be at most %d, got %u", SYF_MAX_FINISHES, d.finish_count);
+    S.finish_count = d.finish_count;
+    for (uint32_t i = 0; i < d.finish_count; ++i) {
+        const syf_finish& f = d.finishes[i];
+        if (!finite3(f.albedo) || !finite3(f.emission) || !std::isfinite(f.roughness) || !std::isfinite(f.metallic) ||
+            !std::isfinite(f.specular))
+            return fail(SYF_E_INVALID, "finishes[%u] must be finite", i);
+        if (f.oren_nayar > 1u) return fail(SYF_E_INVALID, "finishes[%u].oren_nayar must be 0 or 1", i);
+        FinishP& F = S.finishes[i];
+        for (int k = 0; k < 3; ++k) {
+            F.albedo[k] = f.albedo[k];
+            F.emission[k] = f.emission[k];
+        }
+        F.roughness = f.roughness;
+        F.metallic = f.metallic;
+        F.specular = f.specular;
+        F.sigma_scale = f.oren_nayar ? 0.45f : 0.0f;
+    }
+    S.ground_enabled = d.ground.enabled ? 1u : 0u;
+    if (d.ground.enabled && !std::isfinite(d.ground.height))
+        return fail(SYF_E_INVALID, "ground.height must be finite");
+    if (d.ground.enabled && d.ground.finish > SYF_MAX_FINISHES)  // MATERIAL_INDEX bounds
+        return fail(SYF_E_INVALID, "ground.finish must be 0..%d, got %u", SYF_MAX_FINISHES, d.ground.finish);
+    S.ground_height = d.ground.height;
+    S.ground_finish = d.ground.finish;
+    // the trap the march's field sample carries: a graph's shading slot, a classic family's slot 0
+    // (a classic family's other slots are 1, unwritten, and its material 0)
+    const uint32_t carried = graph ? P.f.gshade_slot : 0u;
+    const uint32_t cs = S.color_slot;
+    const uint32_t ss = d.material.surface_slot;
+    const bool color_full = cs == SYF_COLOR_ITER || cs == SYF_COLOR_ORBIT || (graph && cs < 4u && cs != carried);
+    const bool surface_full =
+        d.material.surface_enabled && (ss == SYF_COLOR_ITER || (graph && ss < 4u && ss != carried));
+    const bool material_full = graph && d.finish_count > 0u;
+    S.full_sample = (color_full || surface_full || material_full) ? 1u : 0u;
+    return SYF_OK;
+}
+
 int lower_job(const syf_job_desc& d, JobP& P, const FractalP* graph_fractal) {
     std::memset(&P, 0, sizeof P);
     if (d.fractal.family == SYF_FAMILY_GRAPH && !graph_fractal)
@@ -196,8 +275,16 @@ int lower_job(const syf_job_desc& d, JobP& P, const FractalP* graph_fractal) {
             st3(L.color, h3(l.zenith) * l.intensity);
             st3(L.horizon, h3(l.horizon) * l.intensity);
             st3(L.ground, h3(l.ground) * l.intensity);
+        } else if (l.kind == SYF_LIGHT_SUN) {  // GpuLight::from_light, Light::Sun (uniforms.rs:656-668)
+            // ANGULAR_SIZE bounds (scene/light.rs:126)
+            if (!(l.size[0] >= 0.01f && l.size[0] <= 90.0f))
+                return fail(SYF_E_INVALID, "lights[%u].size[0] (the sun's angular diameter) must be between 0.01 and 90 degrees, got %g",
+                            i, double(l.size[0]));
+            st3(L.a, normalize_or(h3(l.normal), H3{0.0f, 1.0f, 0.0f}));
+            L.cos_max = std::cos(rad(l.size[0] * 0.5f));
+            st3(L.color, h3(l.color) * l.intensity);
         } else {
-            return fail(SYF_E_INVALID, "lights[%u].kind: unsupported kind %u (area or environment)", i, l.kind);
+            return fail(SYF_E_INVALID, "lights[%u].kind: unsupported kind %u (sun, area or environment)", i, l.kind);
         }
     }
     const syf_material& m = d.material;
@@ -206,15 +293,16 @@ int lower_job(const syf_job_desc& d, JobP& P, const FractalP* graph_fractal) {
     P.mat.metallic = m.metallic;
     P.mat.specular = m.specular;
     P.mat.surface_enabled = m.surface_enabled;
-    if (m.surface_enabled && m.surface_slot > 3)
-        return fail(SYF_E_INVALID, "material.surface_slot must be a trap slot 0..3, got %u", m.surface_slot);
+    if (m.surface_enabled && m.surface_slot > SYF_COLOR_NORMAL)
+        return fail(SYF_E_INVALID, "material.surface_slot must be a trap slot 0..3, SYF_COLOR_ITER or SYF_COLOR_NORMAL, got %u",
+                    m.surface_slot);
     P.mat.surface_slot = m.surface_slot;
     if (graph_fractal) {
         // a graph's field evaluation returns the trap of the shading's slot as its first trap
-        P.f.gshade_slot = m.surface_enabled ? m.surface_slot : 0u;
+        P.f.gshade_slot = m.surface_enabled && m.surface_slot < 4u ? m.surface_slot : 0u;
         P.f.trap_scale = P.f.gtrap_scale[P.f.gshade_slot];
-        P.mat.surface_slot = 0u;
     }
+    if (int rc = lower_shading(d, graph_fractal != nullptr, P); rc != SYF_OK) return rc;
     P.mat.surface[0] = m.surface_roughness[0];
     P.mat.surface[1] = m.surface_roughness[1];
     P.mat.surface[2] = m.surface_metallic[0];
@@ -339,6 +427,11 @@ int syf::job_alloc(syf_ctx* ctx, const syf_job_desc& dref, const JobP& P, syf_jo
         return hip_fail(e, "job allocation");
     }
     j->ctr = reinterpret_cast<unsigned long long*>(j->counters + COUNTER_WORDS);
+    if ((e = hipMalloc(&j->dP, sizeof(JobP))) != hipSuccess ||
+        (e = hipMemcpy(j->dP, &j->P, sizeof(JobP), hipMemcpyHostToDevice)) != hipSuccess) {
+        syf_job_destroy(j);
+        return hip_fail(e, "job constants");
+    }
     rc = syf_job_clear(j);
     if (rc != SYF_OK) {
         syf_job_destroy(j);
@@ -364,6 +457,7 @@ int syf_job_destroy(syf_job* j) {
     if (j->ev1) (void)hipEventDestroy(j->ev1);
     if (j->ev2) (void)hipEventDestroy(j->ev2);
     release_wavefront(j);
+    if (j->dP) (void)hipFree(j->dP);
     if (j->gprog) (void)hipFree(j->gprog);
     if (j->stream) (void)hipStreamDestroy(j->stream);
     delete j;

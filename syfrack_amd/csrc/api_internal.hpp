@@ -37,6 +37,7 @@ struct syf_job {
     bool traced = false;  // a primary trace ran: its hit-queue error flag is checked by syf_job_stats
     double extra_ms = 0.0;
     syf::WavefrontArena* wf = nullptr;
+    syf::JobP* dP = nullptr;         // the job constants in device memory (the path-tracing kernels read them there)
     syf::GOp* gprog = nullptr;       // a graph job's compiled program (device memory)
     std::vector<uint8_t> gprog_host;  // its bytes, for the job fingerprint
 };

@@ -94,8 +94,9 @@ hipError_t launch_admit(JobP* jobs, BatchPlanes* planes, uint32_t slot, const Jo
 hipError_t launch_refine_first_hits(uint32_t fam, const JobP* jobs, const BatchPlanes* planes,
                                     const BatchItem* items, uint32_t n_items, unsigned long long* ctr,
                                     hipStream_t s);
-hipError_t launch_render_tile(const JobP& P, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0,
-                              uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
+// P: the job's host copy (its family picks the kernel); dP: the same constants in device memory.
+hipError_t launch_render_tile(const JobP& P, const JobP* dP, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                              uint32_t s0, uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
                               unsigned long long* ctr, hipStream_t s);
 
 hipError_t launch_de_probe(const FractalP& F, const float4* pts, uint32_t n, float4* out, float4* traps,

@@ -69,6 +69,33 @@ struct LightP {
     float horizon[3];
     float ground[3];
     float area;               // |cross(b, c)|
+    float cos_max;            // sun: cosine of the disk's angular radius (GpuLight a.w)
+};
+
+// A named finish as make_surface reads it (GpuFinish, uniforms.rs:787-817).
+struct FinishP {
+    float albedo[3];
+    float roughness, metallic, specular;
+    float emission[3];
+    float sigma_scale;        // Oren-Nayar: sigma = sigma_scale * roughness (0.45, or 0: Lambert)
+};
+
+// Colouring ramp, ground plane and finishes (GpuColoring, GpuAtmosphere.ground, finishes).
+struct ShadeP {
+    uint32_t color_slot;      // 0..3 trap slot, 4 iterations, 5 normal, 6 orbit; above: none
+    uint32_t stop_count, cyclic, smooth, positioned;
+    float frequency, phase, gamma;
+    float stops[SYF_MAX_STOPS][4];
+    float weights[4];
+    float iteration_weight, normal_weight, logarithmic;
+    uint32_t ground_enabled;
+    float ground_height;
+    uint32_t ground_finish;
+    uint32_t finish_count;
+    // the shading reads more of a surface hit's field sample than trap slot 0: the other traps,
+    // the iteration count or the material (then the sample is evaluated in full at the hit)
+    uint32_t full_sample;
+    FinishP finishes[SYF_MAX_FINISHES];
 };
 
 struct MaterialP {
@@ -97,6 +124,7 @@ struct JobP {
     float t_end;              // ray_extent (common.wgsl:206-208)
     float cavity_darkness;
     uint32_t march_math;      // SYF_MATH_FAST or SYF_MATH_EXACT (the primary trace's FP32 march)
+    ShadeP sh;
 };
 
 // Kernel specialisation of a family: the Mandelbulb with a non-integer power takes the

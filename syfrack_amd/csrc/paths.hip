@@ -42,13 +42,10 @@ constexpr uint32_t PATH_REFILL = 4;
 // for the rest of it (measured on single requests: 8 beats 1 by 1.4x; 4 and 16 are each best
 // on some families).  The planes do not depend on it.
 
-// The per-slot job table is read through the constant address space: its loads are then known
-// to be invariant, so the wave-uniform constants of the request being served are loaded with
-// scalar loads the compiler may hoist and keep, as it does for kernel arguments.
-typedef const __attribute__((address_space(4))) JobP ConstJob;
-__device__ __forceinline__ const JobP& job_of(const JobP* jobs, uint32_t slot) {
-    return *(const JobP*)((ConstJob*)jobs + slot);
-}
+// The per-slot job table is read through the constant address space (job_ref): its loads are
+// then known to be invariant, so the wave-uniform constants of the request being served are
+// loaded with scalar loads the compiler may hoist and keep, as it does for kernel arguments.
+__device__ __forceinline__ const JobP& job_of(const JobP* jobs, uint32_t slot) { return job_ref(jobs + slot); }
 
 __device__ __forceinline__ uint64_t below_mask() {
     const uint32_t lane = threadIdx.x & 63u;
@@ -89,7 +86,9 @@ template <uint32_t FAM> struct Lane {
     // normal taps; tap_p, g and tap_h share registers with p_off, dl and p_spec (the taps
     // end before the surface is shaded)
     uint32_t tap;
-    // surface and next-event estimation
+    // surface and next-event estimation; light_i holds the hit sample's iteration count from
+    // the evaluation after the normal taps until the surface is shaded (next-event estimation
+    // starts after that)
     Surf surf;
     F3 p_off, dl, pend;
     float p_spec, area_total, sel_w;
@@ -141,6 +140,18 @@ __device__ __forceinline__ bool continue_direct(const JobP& P, Lane<FAM>& L, uin
             const F3 f = brdf_eval(s, v, dir);
             const F3 e = environment_value(l, dir);
             L.pend = mk(f.x * nl * e.x * 1.0f * w / pl, f.y * nl * e.y * 1.0f * w / pl, f.z * nl * e.z * 1.0f * w / pl);
+            t_end = P.t_end;
+        } else if (l.kind == SYF_LIGHT_SUN) {
+            const float r1 = rand01(L.st.rng);
+            const float r2 = rand01(L.st.rng);
+            dir = cone_sample(ld3(l.a), l.cos_max, r1, r2);
+            const float nl = dot3(s.n, dir);
+            if (nl <= 0.0f || dot3(s.n, dir) <= 0.0f) continue;
+            const float pl = sun_pdf(l.cos_max);
+            const float pb = L.last ? 0.0f : bsdf_pdf(s, v, dir, L.p_spec);
+            const float w = L.last ? 1.0f : mis_power(pl, pb);
+            const F3 f = brdf_eval(s, v, dir);
+            L.pend = mk(f.x * nl * l.color[0] * w, f.y * nl * l.color[1] * w, f.z * nl * l.color[2] * w);
             t_end = P.t_end;
         } else if (l.kind == SYF_LIGHT_AREA) {
             if (L.light_i != L.sel) continue;
@@ -211,21 +222,30 @@ template <uint32_t FAM> __device__ __forceinline__ bool continue_path(const JobP
     return true;
 }
 
-// The surface part of path_bounce once the normal is known, up to next-event estimation.
-template <uint32_t FAM> __device__ __forceinline__ void shade_surface(const JobP& P, Lane<FAM>& L, F3 n) {
+// The surface part of path_bounce once the normal is known, up to next-event estimation (a
+// fractal surface, or the ground plane).  The served families are classic graphs: a hit's
+// sample is trap slot 0, unwritten slots (1) and material 0, with the iteration count the
+// extra evaluation recorded when the shading reads it.
+template <uint32_t FAM> __device__ __forceinline__ void shade_surface(const JobP& P, Lane<FAM>& L, F3 n, bool ground) {
     PathState& st = L.st;
     if (st.bounce == 0u) {
         if (L.s == L.s_first) {  // first-hit planes come from the unit's first sample
             L.fd0 = L.seg_t;
             L.fa0 = make_float4(n.x, n.y, n.z, float(L.seg_steps));
         }
-        const float t = clampT(float(L.seg_steps) / 96.0f, 0.0f, 1.0f);
-        st.throughput = scl(st.throughput, 1.0f - P.cavity_darkness * (t * t * (3.0f - 2.0f * t)));
+        if (!ground) {
+            const float t = clampT(float(L.seg_steps) / 96.0f, 0.0f, 1.0f);
+            st.throughput = scl(st.throughput, 1.0f - P.cavity_darkness * (t * t * (3.0f - 2.0f * t)));
+        }
     }
     const F3 p = add(st.o, scl(st.d, L.seg_t));
     const F3 v = neg3(st.d);
-    L.surf = make_surface(P.mat, n, L.seg_trap0);
-    const float offset = fmaxf(L.seg_t * P.pixel_cone * 2.0f, 1e-3f) * P.pixel_tolerance;
+    ShadeSample smp = ground ? ground_sample(P) : carried_sample(P, L.seg_trap0);
+    if (!ground && P.sh.full_sample) smp.iter = L.light_i;
+    F3 emission;
+    L.surf = make_surface(P, n, smp, emission);
+    st.L = add(st.L, mul3(st.throughput, emission));
+    const float offset = ground ? 1e-3f : fmaxf(L.seg_t * P.pixel_cone * 2.0f, 1e-3f) * P.pixel_tolerance;
     L.p_off = add(p, scl(n, offset));
     const float nv = fmaxf(dot3(L.surf.n, v), 1e-4f);
     L.p_spec = spec_probability(L.surf, nv);
@@ -350,10 +370,17 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
                 work.de_evals++;
                 work.de_iters += s.runs;
                 if (L.mode == L_NORMAL) {
-                    // normal_at: g += k * d over the four taps (tap_p = p_off, g = dl, h = p_spec)
-                    L.dl = add(L.dl, scl(tap_dir(L.tap), s.d));
-                    if (++L.tap < 4u) L.de_done = L.de.start(F, add(L.p_off, scl(tap_dir(L.tap), L.p_spec)));
-                    else L.mode = L_EV_NORMAL;
+                    // normal_at: g += k * d over the four taps (tap_p = p_off, g = dl, h = p_spec);
+                    // then, if the shading reads it, the hit's own sample for its iteration count
+                    if (L.tap < 4u) {
+                        L.dl = add(L.dl, scl(tap_dir(L.tap), s.d));
+                        if (++L.tap < 4u) L.de_done = L.de.start(F, add(L.p_off, scl(tap_dir(L.tap), L.p_spec)));
+                        else if (P.sh.full_sample) L.de_done = L.de.start(F, L.p_off);
+                        else L.mode = L_EV_NORMAL;
+                    } else {
+                        L.light_i = s.iter;
+                        L.mode = L_EV_NORMAL;
+                    }
                 } else {
                     work.steps++;
                     MarchEnd e;
@@ -391,7 +418,10 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
         } else if (L.mode == L_EV_SEGMENT) {
             // path_bounce up to the normal (trace_segment turns a miss in front of an
             // emitter into a hit on it)
-            if (!L.seg_hit && !L.seg_exhausted && L.ext_light != NO_LIGHT) {
+            if (!L.seg_hit && !L.seg_exhausted && L.ext_light == GROUND_HIT) {
+                shade_surface(P, L, ground_normal(st.d), true);
+                nee = true;
+            } else if (!L.seg_hit && !L.seg_exhausted && L.ext_light != NO_LIGHT) {
                 const F3 gn = light_normal(P, L.ext_light);
                 if (st.bounce == 0u && L.s == L.s_first) {
                     L.fd0 = L.seg_t;
@@ -401,9 +431,11 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
                                                                    st.prev_from_bsdf, st.prev_n, st.prev_total)));
                 path_end = true;
             } else if (!L.seg_hit) {
-                if (!L.seg_exhausted)
+                if (!L.seg_exhausted) {
                     st.L = add(st.L, mul3(st.throughput, environment_emission(P, st.d, st.prev_pdf, st.prev_from_bsdf,
                                                                               st.prev_n, st.prev_total)));
+                    st.L = add(st.L, mul3(st.throughput, sun_radiance(P, st.d, st.prev_pdf, st.prev_from_bsdf)));
+                }
                 path_end = true;
             } else {  // normal_at: four taps around the hit point
                 L.p_off = add(st.o, scl(st.d, L.seg_t));
@@ -414,7 +446,7 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
                 L.de_done = L.de.start(P.f, add(L.p_off, scl(tap_dir(0u), L.p_spec)));
             }
         } else if (L.mode == L_EV_NORMAL) {
-            shade_surface(P, L, nrm(L.dl));
+            shade_surface(P, L, nrm(L.dl), false);
             nee = true;
         } else {  // L_EV_SHADOW
             nee = true;

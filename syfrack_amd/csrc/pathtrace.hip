@@ -37,13 +37,13 @@ struct WavefrontArena {
 // ---------------------------------------------------------------- launchers
 
 // The one-launch form over a tile: four 8x8 pixel tiles per workgroup (one per wave).
-hipError_t launch_render_tile(const JobP& P, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0,
-                              uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
+hipError_t launch_render_tile(const JobP& P, const JobP* dP, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                              uint32_t s0, uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
                               unsigned long long* ctr, hipStream_t s) {
     const uint32_t tiles_x = (w + 7u) / 8u, tiles = tiles_x * ((h + 7u) / 8u);
     const dim3 grid((tiles + 3u) / 4u), block(256);
     hipError_t e = hipSuccess;
-    SYF_DISPATCH(e, P.f, k_render_tile, grid, block, 0, s, P, x0, y0, w, h, s0, ns, tiles_x, tiles, accum, depth, aux,
+    SYF_DISPATCH(e, P.f, k_render_tile, grid, block, 0, s, dP, x0, y0, w, h, s0, ns, tiles_x, tiles, accum, depth, aux,
                  depth64, ctr);
     return e;
 }
@@ -75,20 +75,21 @@ hipError_t enqueue_wavefront_sample(syf_job* j, const WfP& W, int cu_count) {
     const dim3 block(256);
     const dim3 gen_grid(std::min<uint32_t>((n + 255u) / 256u, fill));
     hipStream_t s = j->stream;
-    hipLaunchKernelGGL(k_wf_generate, gen_grid, block, 0, s, P, W, S, a->queue, a->ctl, j->ctr);
+    const JobP* dP = j->dP;
+    hipLaunchKernelGGL(k_wf_generate, gen_grid, block, 0, s, dP, W, S, a->queue, a->ctl, j->ctr);
     hipError_t e = hipGetLastError();
     for (uint32_t r = 0; r <= P.max_bounces && e == hipSuccess; ++r) {
         const uint32_t in = r & 1u;
         const dim3 grid(std::min<uint32_t>((n + 255u) / 256u, fill));
-        SYF_DISPATCH(e, P.f, k_wf_intersect, grid, block, 0, s, P, W, S, a->queue, in, a->ctl, j->ctr);
+        SYF_DISPATCH(e, P.f, k_wf_intersect, grid, block, 0, s, dP, W, S, a->queue, in, a->ctl, j->ctr);
         if (e != hipSuccess) break;
-        SYF_DISPATCH(e, P.f, k_wf_shade, grid, block, 0, s, P, W, S, a->queue, in, a->ctl, j->ctr);
+        SYF_DISPATCH(e, P.f, k_wf_shade, grid, block, 0, s, dP, W, S, a->queue, in, a->ctl, j->ctr);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_wf_next_round, dim3(1), dim3(64), 0, s, in, a->capacity, a->ctl);
         e = hipGetLastError();
     }
     if (e != hipSuccess) return e;
-    SYF_DISPATCH(e, P.f, k_wf_commit, gen_grid, block, 0, s, P, W, S, (float4*)j->plane[SYF_PLANE_ACCUM],
+    SYF_DISPATCH(e, P.f, k_wf_commit, gen_grid, block, 0, s, dP, W, S, (float4*)j->plane[SYF_PLANE_ACCUM],
                  (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
                  (double*)j->plane[SYF_PLANE_DEPTH64], a->ctl, j->ctr);
     if (e != hipSuccess) return e;
@@ -111,7 +112,7 @@ int syf_render_tile(syf_job* j, uint32_t x, uint32_t y, uint32_t w, uint32_t h, 
     hipError_t e = hipMemsetAsync(j->ctr, 0, sizeof(unsigned long long) * CTR_COUNT, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev0, j->stream);
     if (e == hipSuccess)
-        e = launch_render_tile(j->P, x, y, w, h, sample_start, sample_count, (float4*)j->plane[SYF_PLANE_ACCUM],
+        e = launch_render_tile(j->P, j->dP, x, y, w, h, sample_start, sample_count, (float4*)j->plane[SYF_PLANE_ACCUM],
                                (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
                                (double*)j->plane[SYF_PLANE_DEPTH64], j->ctr, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev1, j->stream);

@@ -28,8 +28,39 @@ template <uint32_t FAM> struct Tracer {
     const JobP& P;
     Work& w;
     uint32_t& shadow_rays;
-    __device__ F3 normal(F3 p, float h) const { return normal_at<float, FAM>(P.f, p, h, w); }
-    __device__ bool visible(F3 o, F3 d, float t_end) const {
+    __device__ __forceinline__ F3 normal(F3 p, float h) const { return normal_at<float, FAM>(P.f, p, h, w); }
+    // The field sample at a surface hit in full (the march's own evaluation at that point gives
+    // the same values): every normalised trap, the clamped iteration count, the material.
+    __device__ __forceinline__ ShadeSample sample(F3 p) const {
+        ShadeSample s;
+        w.de_evals++;
+        if constexpr (FAM == FAM_GRAPH || FAM == FAM_JIT) {
+            GRaw<float> g;
+            if constexpr (FAM == FAM_JIT) g = graph_jit<float>(P.f.gprog, p);
+            else g = graph_run<float>(P.f.gprog, p);
+            w.de_iters += g.runs;
+            auto norm = [&](float t, uint32_t k) { return isfinite(t) ? fminf(t / P.f.gtrap_scale[k], 1.0f) : 1.0f; };
+            s.t0 = norm(g.r.t0, 0u);
+            s.t1 = norm(g.r.t1, 1u);
+            s.t2 = norm(g.r.t2, 2u);
+            s.t3 = norm(g.r.t3, 3u);
+            s.iter = g.r.iter < P.f.iteration_product ? g.r.iter : P.f.iteration_product;
+            s.material = g.r.mat;
+        } else {
+            const De<float> d = de_eval<float, FAM>(P.f, p);
+            w.de_iters += d.runs;
+            s.t0 = d.trap0;
+            s.t1 = s.t2 = s.t3 = 1.0f;
+            s.iter = d.iter;
+            s.material = 0u;
+        }
+        return s;
+    }
+    // A shadow ray's march, out of line: the visibility tests of a bounce share one copy of the
+    // march, and the registers of the shading live across the call instead of beside an inlined
+    // march (the megakernel then runs three waves per SIMD instead of two).  The job constants
+    // are in device memory (job_ref), so the call passes their address without a copy.
+    __device__ __attribute__((noinline)) bool visible(F3 o, F3 d, float t_end) const {
         ++shadow_rays;
         uint32_t light;
         const float nearest = nearest_emitter(P, o, d, t_end, light);
@@ -67,8 +98,9 @@ __device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_
                 ++hits;
                 if (s == 0u) tp0 = sh.t_prev;
             }
-            path_bounce(P, st, sh, fd, fa, [&](F3 p, float hh) { return tr.normal(p, hh); },
-                        [&](F3 o, F3 d, float te) { return tr.visible(o, d, te); });
+            path_bounce(P, st, sh, fd, fa, [&](F3 p, float hh) __attribute__((always_inline)) { return tr.normal(p, hh); },
+                        [&](F3 o, F3 d, float te) __attribute__((always_inline)) { return tr.visible(o, d, te); },
+                        [&](F3 p) __attribute__((always_inline)) { return tr.sample(p); });
         }
         const F3 r = st.L;
         const bool ok = r.x == r.x && r.y == r.y && r.z == r.z && fabsf(r.x) < 1e30f && fabsf(r.y) < 1e30f &&
@@ -101,11 +133,12 @@ __device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_
 }
 
 template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_render_tile(const JobP P, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                                                     uint32_t s0, uint32_t ns, uint32_t tiles_x, uint32_t tiles,
-                                                     float4* __restrict__ accum, float* __restrict__ depth,
-                                                     float4* __restrict__ aux, double* __restrict__ depth64,
-                                                     unsigned long long* __restrict__ ctr) {
+__global__ __launch_bounds__(256) void k_render_tile(const JobP* __restrict__ job, uint32_t x0, uint32_t y0, uint32_t w,
+                                                     uint32_t h, uint32_t s0, uint32_t ns, uint32_t tiles_x,
+                                                     uint32_t tiles, float4* __restrict__ accum,
+                                                     float* __restrict__ depth, float4* __restrict__ aux,
+                                                     double* __restrict__ depth64, unsigned long long* __restrict__ ctr) {
+    const JobP& P = job_ref(job);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t tile = blockIdx.x * 4u + wave;
     if (tile >= tiles) return;
@@ -193,8 +226,9 @@ __global__ void k_wf_reset(WfCtl* ctl) {
 
 
 #ifndef __HIPCC_RTC__  // family-independent stages: the library's own build only
-__global__ __launch_bounds__(256) void k_wf_generate(const JobP P, const WfP W, Soa S, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256) void k_wf_generate(const JobP* __restrict__ job, const WfP W, Soa S, uint32_t* __restrict__ queue,
                                                      WfCtl* __restrict__ ctl, unsigned long long* __restrict__ ctr) {
+    const JobP& P = job_ref(job);
     const uint32_t n = W.w * W.h;
     const uint32_t sample = W.sample_start + ctl->sample;
     for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) {
@@ -227,9 +261,10 @@ __global__ __launch_bounds__(256) void k_wf_generate(const JobP P, const WfP W, 
 // a thread per path marching to the end 77.2, this 80.3; one Iterate body per trip 68.0, and
 // four evaluations per trip 79.6.
 template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_wf_intersect(const JobP P, const WfP W, Soa S, const uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256) void k_wf_intersect(const JobP* __restrict__ job, const WfP W, Soa S, const uint32_t* __restrict__ queue,
                                                       uint32_t in, WfCtl* __restrict__ ctl,
                                                       unsigned long long* __restrict__ ctr) {
+    const JobP& P = job_ref(job);
     const uint32_t count = ctl->count[in];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
@@ -301,9 +336,10 @@ __global__ __launch_bounds__(256) void k_wf_intersect(const JobP P, const WfP W,
 // Shade every path of queue `in` (normal, surface, next-event estimation with its shadow
 // march, continuation, roulette) and append the survivors to queue `in ^ 1`.
 template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_wf_shade(const JobP P, const WfP W, Soa S, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256) void k_wf_shade(const JobP* __restrict__ job, const WfP W, Soa S, uint32_t* __restrict__ queue,
                                                   uint32_t in, WfCtl* __restrict__ ctl,
                                                   unsigned long long* __restrict__ ctr) {
+    const JobP& P = job_ref(job);
     const uint32_t count = ctl->count[in];
     const uint32_t lane = threadIdx.x & 63u;
     Work work;
@@ -326,8 +362,9 @@ __global__ __launch_bounds__(256) void k_wf_shade(const JobP P, const WfP W, Soa
             float fd = S.F(WF_FD, id);
             float4 fa = make_float4(S.F(WF_FA0, id), S.F(WF_FA1, id), S.F(WF_FA2, id), S.F(WF_FA3, id));
             const bool first = st.bounce == 0u;
-            path_bounce(P, st, sh, fd, fa, [&](F3 p, float hh) { return tr.normal(p, hh); },
-                        [&](F3 o, F3 d, float te) { return tr.visible(o, d, te); });
+            path_bounce(P, st, sh, fd, fa, [&](F3 p, float hh) __attribute__((always_inline)) { return tr.normal(p, hh); },
+                        [&](F3 o, F3 d, float te) __attribute__((always_inline)) { return tr.visible(o, d, te); },
+                        [&](F3 p) __attribute__((always_inline)) { return tr.sample(p); });
             wf_save(S, id, st);
             if (first) {
                 S.F(WF_FD, id) = fd;
@@ -365,10 +402,11 @@ __global__ void k_wf_next_round(uint32_t in, uint32_t capacity, WfCtl* ctl) {
 
 // Accumulate each path of the tile into the film (wf_commit, wavefront.wgsl:232-260).
 template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_wf_commit(const JobP P, const WfP W, Soa S, float4* __restrict__ accum,
+__global__ __launch_bounds__(256) void k_wf_commit(const JobP* __restrict__ job, const WfP W, Soa S, float4* __restrict__ accum,
                                                    float* __restrict__ depth, float4* __restrict__ aux,
                                                    double* __restrict__ depth64, WfCtl* __restrict__ ctl,
                                                    unsigned long long* __restrict__ ctr) {
+    const JobP& P = job_ref(job);
     const uint32_t n = W.w * W.h;
     const uint32_t sample = W.sample_start + ctl->sample;
     Work work;

@@ -132,6 +132,7 @@ struct syf_server {
     uint32_t max_slots = 0;
     std::vector<uint32_t> free_slots;
     JobP* d_jobs = nullptr;
+    JobP* d_plain_job = nullptr;  // plain mode: the job constants of the request being rendered
     BatchPlanes* d_planes = nullptr;
     BatchItem* h_items[NBUF] = {};  // pinned, coherent: read by the kernels in place
     size_t items_cap[NBUF] = {};
@@ -462,10 +463,13 @@ bool run_plain(syf_server* s, std::unique_lock<std::mutex>& lk) {
     r->refs++;
     lk.unlock();
     hipError_t e = hipMemsetAsync(s->d_pctr, 0, CTR_COUNT * sizeof(unsigned long long), s->stream);
+    // the job constants, outside the timed span (the host copy stays alive: the launch is
+    // waited for below)
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_plain_job, &r->P, sizeof(JobP), hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = hipEventRecord(s->ev_pool[0], s->stream);
     if (e == hipSuccess)
-        e = launch_render_tile(r->P, 0, 0, r->P.width, r->P.height, 0, r->spp, (float4*)r->accum, (float*)r->depth,
-                               (float4*)r->aux, (double*)r->depth64, s->d_pctr, s->stream);
+        e = launch_render_tile(r->P, s->d_plain_job, 0, 0, r->P.width, r->P.height, 0, r->spp, (float4*)r->accum,
+                               (float*)r->depth, (float4*)r->aux, (double*)r->depth64, s->d_pctr, s->stream);
     if (e == hipSuccess) e = hipEventRecord(s->ev_pool[1], s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(s->ev_pool[1]);
     float ms = 0.0f;
@@ -752,6 +756,7 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     s->ns_per_unit = 0.05;
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc(&s->d_jobs, size_t(s->max_slots) * sizeof(JobP))) != hipSuccess ||
+        (e = hipMalloc(&s->d_plain_job, sizeof(JobP))) != hipSuccess ||
         (e = hipMalloc(&s->d_planes, size_t(s->max_slots) * sizeof(BatchPlanes))) != hipSuccess) {
         syf_server_destroy(s);
         return hip_fail(e, "server allocation");
@@ -816,6 +821,7 @@ int syf_server_destroy(syf_server* s) {
         delete kv.second;
     }
     hipFree(s->d_jobs);
+    hipFree(s->d_plain_job);
     hipFree(s->d_planes);
     for (int b = 0; b < NBUF; ++b) {
         hipHostFree(s->h_items[b]);

@@ -1,9 +1,11 @@
 // Device shading for the path-traced forms: the radiance estimator of pathtrace.wgsl:802-1201
-// restricted to this path's scene subset (area and environment lights, a base material with
-// an optional trap-driven SurfaceVariation giving Lambert + GGX metal; fog, sun, HDRI,
-// portals, ocean, optics, textures, guiding and phenomena are absent, so their branches are
-// inert).  The random draws happen in the reference's order so that a path is a pure function
-// of (pixel, sample, seed), whichever kernel form runs it.
+// restricted to this path's scene subset: sun, area and environment lights; a flat ground
+// plane; the base material with its colouring ramp and SurfaceVariation, and named finishes
+// (Lambert or Oren-Nayar diffuse + GGX, emission); next-event estimation with MIS, Russian
+// roulette.  Fog, HDRI, portals, ocean, dunes, optics, textures, finish detail patterns,
+// guiding and phenomena are absent, so their branches are inert.  The random draws happen in
+// the reference's order so that a path is a pure function of (pixel, sample, seed), whichever
+// kernel form runs it.
 #pragma once
 
 #include "device.hpp"
@@ -11,8 +13,17 @@
 namespace syf {
 
 typedef V3<float> F3;
+
+// The job constants of a path-tracing kernel, read from device memory through the constant
+// address space: the loads are invariant (scalar where the address is wave-uniform), and a
+// per-lane index into a table (palette stops, finishes) is an ordinary load.  (A job passed by
+// value as a kernel argument is copied to scratch memory, over a kilobyte per lane, as soon as
+// one access indexes it per lane or its address reaches a call.)
+typedef const __attribute__((address_space(4))) JobP ConstJob;
+__device__ __forceinline__ const JobP& job_ref(const JobP* p) { return *(const JobP*)((const ConstJob*)p); }
 constexpr float PI_F = 3.14159265358979f;
 constexpr uint32_t NO_LIGHT = 0xffffffffu;
+constexpr uint32_t GROUND_HIT = 0xfffffffeu;  // a segment that ended on the ground plane
 
 __device__ __forceinline__ F3 mul3(F3 a, F3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
 __device__ __forceinline__ F3 neg3(F3 a) { return {-a.x, -a.y, -a.z}; }
@@ -29,7 +40,76 @@ __device__ __forceinline__ F3 safe_normalize(F3 v) {  // common.wgsl:242-247
 struct Surf {
     F3 n, diffuse, f0;
     float alpha;
+    float sigma;  // Oren-Nayar roughness (diffuse_sigma); 0: Lambert
 };
+
+// The field sample the shading reads at a surface hit (DeSample): normalised traps, the
+// iteration count clamped to the iteration product, the material index.
+struct ShadeSample {
+    float t0, t1, t2, t3;  // traps of slots 0..3
+    uint32_t iter, material;
+};
+
+// A surface hit's sample when the shading reads only the trap the march returns (trap0: the
+// graph's shading slot, slot 0 of a classic family); the other slots are not read.
+__device__ __forceinline__ ShadeSample carried_sample(const JobP& P, float trap0) {
+    const uint32_t k = P.f.gshade_slot;
+    return ShadeSample{k == 0u ? trap0 : 1.0f, k == 1u ? trap0 : 1.0f, k == 2u ? trap0 : 1.0f, k == 3u ? trap0 : 1.0f,
+                       0u, 0u};
+}
+// The ground's sample: the march's initial DeSample (pathtrace.wgsl:86) with the ground finish
+// as material (lights.wgsl:201).
+__device__ __forceinline__ ShadeSample ground_sample(const JobP& P) {
+    return ShadeSample{1.0f, 1.0f, 1.0f, 1.0f, 0u, P.sh.ground_finish};
+}
+// The trap of slot k (0..3).  A masked sum, exact for the finite traps of a sample (normalised
+// to [0, 1]): a select chain over adjacent fields is folded into an indexed load, which puts
+// the sample in scratch memory.
+__device__ __forceinline__ float trap_at(const ShadeSample& s, uint32_t k) {
+    return (k == 0u ? 1.0f : 0.0f) * s.t0 + (k == 1u ? 1.0f : 0.0f) * s.t1 + (k == 2u ? 1.0f : 0.0f) * s.t2 +
+           (k == 3u ? 1.0f : 0.0f) * s.t3;
+}
+
+__device__ __forceinline__ float4 stop_at(const ShadeP& S, uint32_t i) {
+    return make_float4(S.stops[i][0], S.stops[i][1], S.stops[i][2], S.stops[i][3]);
+}
+// palette_mix (common.wgsl:331-335)
+__device__ __forceinline__ F3 palette_mix(const ShadeP& S, uint32_t a, uint32_t b, float amount) {
+    float f = clampT(amount, 0.0f, 1.0f);
+    if (S.smooth) f = f * f * (3.0f - 2.0f * f);
+    const float4 A = stop_at(S, a), B = stop_at(S, b);
+    return mk(A.x * (1.0f - f) + B.x * f, A.y * (1.0f - f) + B.y * f, A.z * (1.0f - f) + B.z * f);
+}
+// palette_sample / ramp_sample (common.wgsl:336-366)
+__device__ __forceinline__ F3 ramp_sample(const ShadeP& S, float value) {
+    const uint32_t n = S.stop_count;
+    if (n == 0u) return mk(1.0f, 1.0f, 1.0f);
+    if (n == 1u) return mk(S.stops[0][0], S.stops[0][1], S.stops[0][2]);
+    float t = value * S.frequency + S.phase;
+    if (S.cyclic) t = t - floorf(t);
+    else t = clampT(t, 0.0f, 1.0f);
+    if (S.gamma != 1.0f) t = powf(t, S.gamma);
+    if (!S.positioned && !S.cyclic) {
+        const float x = t * float(n - 1u);
+        const uint32_t i = min(uint32_t(floorf(x)), n - 2u);
+        return palette_mix(S, i, i + 1u, x - float(i));
+    }
+    const float first = S.stops[0][3];
+    const float last = stop_at(S, n - 1u).w;
+    if (S.cyclic && (t < first || t >= last)) {
+        const float x = t < first ? t + 1.0f : t;
+        return palette_mix(S, n - 1u, 0u, (x - last) / fmaxf(1.0f + first - last, 1e-8f));
+    }
+    if (t <= first) return mk(S.stops[0][0], S.stops[0][1], S.stops[0][2]);
+    const float4 lastc = stop_at(S, n - 1u);
+    if (t >= last) return mk(lastc.x, lastc.y, lastc.z);
+    for (uint32_t i = 0; i + 1u < n; ++i) {
+        const float4 a = stop_at(S, i), b = stop_at(S, i + 1u);
+        if (t < b.w) return palette_mix(S, i, i + 1u, (t - a.w) / (b.w - a.w));
+    }
+    return mk(lastc.x, lastc.y, lastc.z);
+}
+__device__ __forceinline__ FinishP finish_at(const ShadeP& S, uint32_t i) { return S.finishes[i]; }
 
 // Duff et al. 2017 orthonormal basis (pathtrace.wgsl:194-201); columns t, bt, n.
 __device__ __forceinline__ void tangent_frame(F3 n, F3& t, F3& bt) {
@@ -80,25 +160,65 @@ __device__ __forceinline__ F3 sample_ggx_h(F3 n, F3 v, float alpha, float r1, fl
 }
 __device__ __forceinline__ F3 reflect3(F3 e1, F3 e2) { return sub(e1, scl(e2, 2.0f * dot3(e2, e1))); }
 
-// make_surface subset (pathtrace.wgsl:271-386).
-__device__ __forceinline__ Surf make_surface(const MaterialP& m, F3 n, float trap0) {
-    float rough = m.roughness, metallic = m.metallic, spec = m.specular;
-    if (m.surface_enabled) {
-        float value = m.surface_slot == 0u ? trap0 : 1.0f;  // the classic graphs write slot 0 only
-        float t = clampT(value, 0.0f, 1.0f);
+// surface_value (pathtrace.wgsl:265-269)
+__device__ __forceinline__ float surface_value(const JobP& P, uint32_t slot, F3 n, const ShadeSample& smp) {
+    if (slot < 4u) return trap_at(smp, slot);
+    if (slot == SYF_COLOR_NORMAL) return n.y * 0.5f + 0.5f;
+    return float(smp.iter) / fmaxf(float(P.f.iteration_product), 1.0f);
+}
+// make_surface (pathtrace.wgsl:271-386) for this path's subset: the colouring ramp, a named
+// finish (albedo, roughness, metallic, specular, emission; Oren-Nayar sigma for the concrete
+// patterns), the SurfaceVariation.  The emission (Surface.emission) is returned apart: the
+// caller adds it at once and the surface record stays small for the bounce's remainder.
+__device__ __forceinline__ Surf make_surface(const JobP& P, F3 n, const ShadeSample& smp, F3& emission) {
+    const MaterialP& m = P.mat;
+    const ShadeP& S = P.sh;
+    F3 albedo = ld3(m.albedo);
+    const uint32_t slot = S.color_slot;
+    const bool assigned = smp.material > 0u && smp.material <= S.finish_count;
+    const float iters = float(smp.iter) / fmaxf(float(P.f.iteration_product), 1.0f);
+    if (!assigned && slot == SYF_COLOR_ORBIT) {
+        float value = S.weights[0] * smp.t0 + S.weights[1] * smp.t1 + S.weights[2] * smp.t2 + S.weights[3] * smp.t3 +
+                      S.iteration_weight * iters + S.normal_weight * (n.y * 0.5f + 0.5f);
+        const float gain = S.logarithmic;
+        if (gain > 0.0f) {
+            const float sg = value > 0.0f ? 1.0f : (value < 0.0f ? -1.0f : 0.0f);
+            value = sg * logf(1.0f + gain * fabsf(value)) / logf(1.0f + gain);
+        }
+        albedo = mul3(albedo, ramp_sample(S, value));
+    } else if (!assigned && slot < 4u) {
+        albedo = mul3(albedo, ramp_sample(S, trap_at(smp, slot)));
+    } else if (!assigned && slot == SYF_COLOR_NORMAL) {
+        albedo = mul3(albedo, ramp_sample(S, n.y * 0.5f + 0.5f));
+    } else if (!assigned && slot == SYF_COLOR_ITER) {
+        albedo = mul3(albedo, ramp_sample(S, iters));
+    }
+    float rough = m.roughness, metallic = m.metallic, spec = m.specular, sigma_scale = 0.0f;
+    emission = mk(0.0f, 0.0f, 0.0f);
+    if (assigned) {
+        const FinishP f = finish_at(S, smp.material - 1u);
+        albedo = ld3(f.albedo);
+        rough = f.roughness;
+        metallic = f.metallic;
+        spec = f.specular;
+        emission = ld3(f.emission);
+        sigma_scale = f.sigma_scale;
+    }
+    if (!assigned && m.surface_enabled) {
+        const float t = clampT(surface_value(P, m.surface_slot, n, smp), 0.0f, 1.0f);
         rough = m.surface[0] + (m.surface[1] - m.surface[0]) * t;
         metallic = m.surface[2] + (m.surface[3] - m.surface[2]) * t;
     }
     rough = clampT(rough, 0.02f, 1.0f);
-    F3 albedo = ld3(m.albedo);
     Surf s;
     s.n = n;
     s.diffuse = scl(albedo, 1.0f - metallic);
     s.f0 = mix3(mk(spec, spec, spec), albedo, metallic);
     s.alpha = rough * rough;
+    s.sigma = rough * sigma_scale;
     return s;
 }
-// brdf_eval (pathtrace.wgsl:394-420) without the Oren-Nayar term.
+// brdf_eval (pathtrace.wgsl:394-420): Lambert or Oren-Nayar diffuse + GGX.
 __device__ __forceinline__ F3 brdf_eval(const Surf& s, F3 v, F3 l) {
     float nl = dot3(s.n, l), nv = dot3(s.n, v);
     if (nl <= 0.0f || nv <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
@@ -109,9 +229,19 @@ __device__ __forceinline__ F3 brdf_eval(const Surf& s, F3 v, F3 l) {
     F3 f = fresnel(s.f0, vh);
     float k = ggx_d(nh, a2) * smith_g1(nv, a2) * smith_g1(nl, a2);
     float den = fmaxf(4.0f * nv * nl, 1e-4f);
-    return mk(s.diffuse.x * (1.0f - f.x) * 1.0f / PI_F + k * f.x / den,
-              s.diffuse.y * (1.0f - f.y) * 1.0f / PI_F + k * f.y / den,
-              s.diffuse.z * (1.0f - f.z) * 1.0f / PI_F + k * f.z / den);
+    float df = 1.0f;
+    if (s.sigma > 0.0f) {  // Oren-Nayar (pathtrace.wgsl:407-418)
+        const float sigma2 = s.sigma * s.sigma;
+        const float a = 1.0f - sigma2 / (2.0f * (sigma2 + 0.33f));
+        const float b = 0.45f * sigma2 / (sigma2 + 0.09f);
+        const float sin_l = sqrtf(fmaxf(0.0f, 1.0f - nl * nl));
+        const float sin_v = sqrtf(fmaxf(0.0f, 1.0f - nv * nv));
+        const float cos_phi = fmaxf(0.0f, dot3(sub(l, scl(s.n, nl)), sub(v, scl(s.n, nv))) / fmaxf(sin_l * sin_v, 1e-8f));
+        df = a + b * cos_phi * sin_l * sin_v / fmaxf(fmaxf(nl, nv), 1e-5f);
+    }
+    return mk(s.diffuse.x * (1.0f - f.x) * df / PI_F + k * f.x / den,
+              s.diffuse.y * (1.0f - f.y) * df / PI_F + k * f.y / den,
+              s.diffuse.z * (1.0f - f.z) * df / PI_F + k * f.z / den);
 }
 __device__ __forceinline__ float mis_power(float a, float b) {
     float scale = fmaxf(a, b);
@@ -180,10 +310,26 @@ __device__ __forceinline__ float area_total_weight(const JobP& P, F3 p, F3 n) {
         if (P.lights[i].kind == SYF_LIGHT_AREA) total += area_weight(p, n, P.lights[i]);
     return total;
 }
-// Nearest emitter before t_end (trace_geometry_lights, lights.wgsl:183-209, no ground).
+// ground_intersection for a flat ground (landscape.wgsl:17-23)
+__device__ __forceinline__ float ground_intersection(const ShadeP& S, F3 ro, F3 rd, float t_end) {
+    if (fabsf(rd.y) < 1e-7f) return t_end;
+    const float t = (S.ground_height - ro.y) / rd.y;
+    return (t > 1e-5f && t < t_end) ? t : t_end;
+}
+// ground_normal of a flat ground, facing the ray's origin (landscape.wgsl:71-78)
+__device__ __forceinline__ F3 ground_normal(F3 rd) { return rd.y > 0.0f ? mk(0.0f, -1.0f, 0.0f) : mk(0.0f, 1.0f, 0.0f); }
+// Nearest ground or emitter before t_end (trace_geometry_lights, lights.wgsl:183-209): light is
+// the emitter's index, GROUND_HIT, or NO_LIGHT.
 __device__ __forceinline__ float nearest_emitter(const JobP& P, F3 ro, F3 rd, float t_end, uint32_t& light) {
     float nearest = t_end;
     light = NO_LIGHT;
+    if (P.sh.ground_enabled) {
+        const float t = ground_intersection(P.sh, ro, rd, nearest);
+        if (t > 1e-5f && t < nearest) {
+            nearest = t;
+            light = GROUND_HIT;
+        }
+    }
     LIGHT_LOOP(i) {
         if (P.lights[i].kind != SYF_LIGHT_AREA) continue;
         float t = area_intersection(ro, rd, P.lights[i]);
@@ -229,16 +375,34 @@ __device__ __forceinline__ F3 environment_emission(const JobP& P, F3 dir, float 
     return L;
 }
 
+// cone_sample / sun_pdf (pathtrace.wgsl:210-215, 432-434)
+__device__ __forceinline__ F3 cone_sample(F3 dir, float cos_max, float r1, float r2) {
+    const float cos_t = 1.0f - r1 * (1.0f - cos_max);
+    const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = 2.0f * PI_F * r2;
+    F3 t, bt;
+    tangent_frame(dir, t, bt);
+    return frame_mul(t, bt, dir, mk(sin_t * cosf(phi), sin_t * sinf(phi), cos_t));
+}
+__device__ __forceinline__ float sun_pdf(float cos_max) { return 1.0f / (2.0f * PI_F * fmaxf(1.0f - cos_max, 1e-7f)); }
+// sun_radiance (pathtrace.wgsl:454-467): every sun disk containing dir, MIS-weighted against
+// the BSDF sample that found it.
+__device__ __forceinline__ F3 sun_radiance(const JobP& P, F3 dir, float pdf_bsdf, bool use_mis) {
+    F3 L = mk(0.0f, 0.0f, 0.0f);
+    LIGHT_LOOP(i) {
+        const LightP& l = P.lights[i];
+        if (l.kind != SYF_LIGHT_SUN || !(dot3(dir, ld3(l.a)) >= l.cos_max)) continue;
+        const float pl = sun_pdf(l.cos_max);
+        const float w = use_mis ? mis_power(pdf_bsdf, pl) : 1.0f;
+        L = add(L, mk(l.color[0] * pl * w, l.color[1] * pl * w, l.color[2] * pl * w));
+    }
+    return L;
+}
+
 // Next-event estimation as direct_light (pathtrace.wgsl:471-558): choose one area light by
-// weight, then the environment and the chosen area light each propose one shadow ray.
+// weight, then the environment, each sun and the chosen area light each propose one shadow ray.
 // The caller supplies the visibility test (a march to the light), so the megakernel and
 // the wavefront form share this logic.
-struct ShadowRay {
-    F3 o, d;
-    float t_end;
-    F3 contrib;  // radiance the ray carries if unoccluded (already MIS- and pdf-weighted)
-};
-
 template <class Vis>
 __device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F3 gn, F3 v, float p_spec,
                                            float area_total, bool last, uint32_t& rng, Vis&& visible) {
@@ -280,6 +444,19 @@ __device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F
             F3 f = brdf_eval(s, v, dir);
             F3 e = environment_value(l, dir);
             L = add(L, mk(f.x * nl * e.x * 1.0f * w / pl, f.y * nl * e.y * 1.0f * w / pl, f.z * nl * e.z * 1.0f * w / pl));
+        } else if (l.kind == SYF_LIGHT_SUN) {
+            // colour = irradiance: the disk's radiance is colour * sun_pdf
+            const float r1 = rand01(rng);
+            const float r2 = rand01(rng);
+            const F3 dir = cone_sample(ld3(l.a), l.cos_max, r1, r2);
+            const float nl = dot3(s.n, dir);
+            if (nl <= 0.0f || dot3(gn, dir) <= 0.0f) continue;
+            if (!visible(p, dir, P.t_end)) continue;
+            const float pl = sun_pdf(l.cos_max);
+            const float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
+            const float w = last ? 1.0f : mis_power(pl, pb);
+            const F3 f = brdf_eval(s, v, dir);
+            L = add(L, mk(f.x * nl * l.color[0] * w, f.y * nl * l.color[1] * w, f.z * nl * l.color[2] * w));
         } else if (l.kind == SYF_LIGHT_AREA) {
             if (i != selected) continue;
             float u1 = rand01(rng);
@@ -337,15 +514,19 @@ struct SegHit {
 };
 
 // The body of one bounce after its segment was traced (pathtrace.wgsl:1012-1196 subset):
-// emission on a miss or an emitter hit, otherwise the surface, next-event estimation and the
-// continuation sample.  `normal` evaluates the field gradient at a surface point; `visible`
-// marches a shadow ray.  Sets st.alive = false when the path ends.
-template <class Normal, class Vis>
+// emission on a miss or an emitter hit, otherwise the surface (fractal or ground), next-event
+// estimation and the continuation sample.  `normal` evaluates the field gradient at a surface
+// point; `visible` marches a shadow ray; `sample` evaluates the full field sample at a surface
+// hit (called only when the shading reads more than the march's trap).  Sets st.alive = false
+// when the path ends.
+template <class Normal, class Vis, class Sample>
 __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const SegHit& h, float& first_depth,
-                                            float4& first_aux, Normal&& normal, Vis&& visible) {
+                                            float4& first_aux, Normal&& normal, Vis&& visible, Sample&& sample) {
     F3 gn = mk(0.0f, 0.0f, 0.0f);
+    const bool ground = h.light == GROUND_HIT;
     if (h.hit) {
-        if (h.light != NO_LIGHT) gn = light_normal(P, h.light);
+        if (ground) gn = ground_normal(st.d);
+        else if (h.light != NO_LIGHT) gn = light_normal(P, h.light);
         else gn = normal(add(st.o, scl(st.d, h.t)), fmaxf(h.t * P.pixel_cone * 0.5f, 1e-4f) * P.pixel_tolerance);
         if (st.bounce == 0u) {
             first_depth = h.t;
@@ -353,27 +534,33 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
         }
     }
     if (!h.hit) {
-        if (!h.exhausted)
+        if (!h.exhausted) {
             st.L = add(st.L, mul3(st.throughput,
                                   environment_emission(P, st.d, st.prev_pdf, st.prev_from_bsdf, st.prev_n, st.prev_total)));
+            st.L = add(st.L, mul3(st.throughput, sun_radiance(P, st.d, st.prev_pdf, st.prev_from_bsdf)));
+        }
         st.alive = false;
         return;
     }
     const F3 p = add(st.o, scl(st.d, h.t));
-    if (h.light != NO_LIGHT) {
+    if (!ground && h.light != NO_LIGHT) {
         st.L = add(st.L, mul3(st.throughput, area_emission(P, st.o, st.d, h.t, st.prev_pdf, st.prev_from_bsdf,
                                                            st.prev_n, st.prev_total)));
         st.alive = false;
         return;
     }
     const F3 n = gn;
-    if (st.bounce == 0u) {
+    if (st.bounce == 0u && !ground) {
         float t = clampT(float(h.steps) / 96.0f, 0.0f, 1.0f);
         st.throughput = scl(st.throughput, 1.0f - P.cavity_darkness * (t * t * (3.0f - 2.0f * t)));
     }
     const F3 v = neg3(st.d);
-    const Surf s = make_surface(P.mat, n, h.trap0);
-    const float offset = fmaxf(h.t * P.pixel_cone * 2.0f, 1e-3f) * P.pixel_tolerance;
+    const ShadeSample smp =
+        ground ? ground_sample(P) : (P.sh.full_sample ? sample(p) : carried_sample(P, h.trap0));
+    F3 emission;
+    const Surf s = make_surface(P, n, smp, emission);
+    st.L = add(st.L, mul3(st.throughput, emission));
+    const float offset = ground ? 1e-3f : fmaxf(h.t * P.pixel_cone * 2.0f, 1e-3f) * P.pixel_tolerance;
     const F3 p_off = add(p, scl(n, offset));
     const float nv = fmaxf(dot3(s.n, v), 1e-4f);
     const float p_spec = spec_probability(s, nv);

@@ -4,8 +4,9 @@ Mirrors the reference's scene model on this path: the bundled graphs
 (graphs/{mandelbulb,mandelbox,menger,sphere}.ron, embedded by
 crates/syfrack-core/src/scene/graph/bundled.rs:8-49), their parameter paths
 (``fractal.nodes[1].iterations`` ... as Scene::set_param, scene/mod.rs:564), the camera modes
-(scene/camera.rs:5-35), area/environment lights (scene/light.rs:66-123), material and
-surface variation (scene/material.rs), and RenderConfig (renderer.rs:17-76).  ``Scene.desc()``
+(scene/camera.rs:5-35), sun/area/environment lights (scene/light.rs:66-123), material with
+its colouring ramp, surface variation and named finishes (scene/material.rs, palette.rs), the
+flat ground plane (scene/atmosphere.rs:19-38), and RenderConfig (renderer.rs:17-76).  ``Scene.desc()``
 produces the flat ``syf_job_desc`` the C ABI takes; the camera basis, step scale and ray
 extent are derived inside the library (syf_job_create), as Uniforms::from_scene does.
 """
@@ -176,6 +177,13 @@ class Light:
         return Light(abi.LIGHT_AREA, tuple(position), tuple(normal), tuple(size), tuple(color), intensity)
 
     @staticmethod
+    def sun(direction, color=(1.0, 1.0, 1.0), intensity=1.0, angular_size=0.53) -> "Light":
+        """Light::Sun: ``direction`` towards the sun, ``intensity`` the irradiance on a
+        perpendicular surface, ``angular_size`` the disk's full angular diameter in degrees."""
+        return Light(abi.LIGHT_SUN, normal=tuple(direction), size=(angular_size, 0.0), color=tuple(color),
+                     intensity=intensity)
+
+    @staticmethod
     def environment(zenith, horizon, ground, intensity=1.0) -> "Light":
         return Light(abi.LIGHT_ENV, intensity=intensity, zenith=tuple(zenith), horizon=tuple(horizon),
                      ground=tuple(ground))
@@ -195,12 +203,96 @@ class Light:
 
 
 @dataclass
+class Ramp:
+    """Ramp (palette.rs:74-115): linear rgb stops, optional strictly increasing positions."""
+    stops: List[tuple] = field(default_factory=lambda: [(0.0, 0.0, 0.0), (1.0, 1.0, 1.0)])
+    positions: List[float] = field(default_factory=list)
+    cyclic: bool = False
+    frequency: float = 1.0
+    phase: float = 0.0
+    gamma: float = 1.0
+    smooth: bool = False  # PaletteInterpolation::Smooth
+
+    def position(self, i: int) -> float:
+        if self.positions:
+            return float(self.positions[i])
+        n = len(self.stops)
+        return i / float(n if self.cyclic else max(n - 1, 1))
+
+
+@dataclass
+class Coloring:
+    """Coloring (material.rs:12-37): ``slot`` is a trap slot 0..3, COLOR_ITER or COLOR_NORMAL
+    for Coloring::Trap; ``orbit`` (trap weights, iterations, normal, logarithmic) makes it
+    Coloring::Orbit."""
+    ramp: Ramp = field(default_factory=Ramp)
+    slot: int = 0
+    orbit: Optional[tuple] = None  # ((w0, w1, w2, w3), iterations, normal, logarithmic)
+
+    def c_struct(self) -> abi.SyfColoring:
+        c = abi.SyfColoring()
+        r = self.ramp
+        if len(r.stops) > abi.MAX_STOPS:
+            raise ValueError(f"coloring: at most {abi.MAX_STOPS} stops, got {len(r.stops)}")
+        if r.positions and len(r.positions) != len(r.stops):
+            raise ValueError("coloring: one position per stop, or none")
+        c.slot = abi.COLOR_ORBIT if self.orbit is not None else self.slot
+        c.stop_count = len(r.stops)
+        c.cyclic = int(r.cyclic)
+        c.smooth = int(r.smooth)
+        c.positioned = int(bool(r.positions))
+        c.frequency = r.frequency
+        c.phase = r.phase
+        c.gamma = r.gamma
+        for i, st in enumerate(r.stops):
+            c.stops[i][:] = [float(st[0]), float(st[1]), float(st[2]), r.position(i)]
+        w, it, nw, lg = self.orbit if self.orbit is not None else ((1.0, 0.0, 0.0, 0.0), 0.0, 0.0, 0.0)
+        c.weights[:] = list(w)
+        c.iteration_weight = it
+        c.normal_weight = nw
+        c.logarithmic = lg
+        return c
+
+
+@dataclass
+class Finish:
+    """Finish (material.rs:161-174) without a detail pattern; ``oren_nayar`` stands for the
+    Concrete, WeatheredConcrete and WetConcrete patterns' rough diffuse lobe at zero strength."""
+    albedo: tuple = (0.8, 0.8, 0.8)
+    roughness: float = 0.5
+    metallic: float = 0.0
+    specular: float = 0.04
+    emission: tuple = (0.0, 0.0, 0.0)
+    oren_nayar: bool = False
+
+    def c_struct(self) -> abi.SyfFinish:
+        f = abi.SyfFinish()
+        f.albedo[:] = list(self.albedo)
+        f.roughness = self.roughness
+        f.metallic = self.metallic
+        f.specular = self.specular
+        f.emission[:] = list(self.emission)
+        f.oren_nayar = int(self.oren_nayar)
+        return f
+
+
+@dataclass
+class Ground:
+    """GroundPlane (atmosphere.rs:32-38) without dunes: ``finish`` 0 is the base material,
+    1..8 a named finish."""
+    height: float = 0.0
+    finish: int = 0
+
+
+@dataclass
 class Material:
     albedo: tuple = (0.8, 0.8, 0.8)
     roughness: float = 0.4
     metallic: float = 0.0
     specular: float = 0.04
     surface: Optional[tuple] = None  # (slot, (r0, r1), (m0, m1)) : SurfaceVariation
+    coloring: Optional[Coloring] = None  # None: Coloring::None
+    finishes: List[Finish] = field(default_factory=list)  # MaterialTag index i + 1
 
     def c_struct(self) -> abi.SyfMaterial:
         m = abi.SyfMaterial()
@@ -243,6 +335,7 @@ class Scene:
     material: Material = field(default_factory=Material)
     seed: int = 7
     cavity_darkness: float = 0.0
+    ground: Optional[Ground] = None
 
     def with_fractal_param(self, path: str, value: float) -> "Scene":
         s = copy.deepcopy(self)
@@ -274,6 +367,22 @@ class Scene:
             raise ValueError(f"march_math must be 'fast' or 'exact', got {cfg.march_math!r}")
         d.march_math = abi.MATH_FAST if cfg.march_math == "fast" else abi.MATH_EXACT
         d.seed = self.seed & 0xFFFFFFFFFFFFFFFF
+        m = self.material
+        if m.coloring is not None:
+            d.coloring = m.coloring.c_struct()
+        else:
+            d.coloring.slot = abi.COLOR_NONE
+            d.coloring.frequency = 1.0
+            d.coloring.gamma = 1.0
+        if len(m.finishes) > abi.MAX_FINISHES:
+            raise ValueError(f"finishes: at most {abi.MAX_FINISHES}, got {len(m.finishes)}")
+        for i, f in enumerate(m.finishes):
+            d.finishes[i] = f.c_struct()
+        d.finish_count = len(m.finishes)
+        if self.ground is not None:
+            d.ground.enabled = 1
+            d.ground.height = self.ground.height
+            d.ground.finish = self.ground.finish
         return d
 
 

@@ -23,7 +23,7 @@ from . import abi
 from .ron import from_ron, to_ron  # noqa: F401  (state.ron)
 from .graph import Graph
 from .render import Renderer, RenderOutput
-from .scene import Camera, Fractal, Light, Material, RenderConfig, Scene
+from .scene import Camera, Coloring, Finish, Fractal, Ground, Light, Material, Ramp, RenderConfig, Scene
 
 STATE_FILE = "state.ron"
 
@@ -43,13 +43,25 @@ def _build(cls, d: dict):
 def scene_from_state(d: dict) -> Scene:
     s = d
     lights = [_build(Light, l) for l in s.get("lights", [])]
-    surface = s["material"].get("surface")
-    m = _build(Material, {k: v for k, v in s["material"].items() if k != "surface"})
+    md = s["material"]
+    surface = md.get("surface")
+    m = _build(Material, {k: v for k, v in md.items() if k not in ("surface", "coloring", "finishes")})
     m.surface = None if surface is None else (int(surface[0]), tuple(surface[1]), tuple(surface[2]))
+    c = md.get("coloring")
+    if c is not None:
+        r = c["ramp"]
+        ramp = _build(Ramp, {k: v for k, v in r.items() if k not in ("stops", "positions")})
+        ramp.stops = [tuple(x) for x in r["stops"]]
+        ramp.positions = [float(x) for x in r.get("positions", [])]
+        orbit = c.get("orbit")
+        m.coloring = Coloring(ramp, int(c["slot"]),
+                              None if orbit is None else (tuple(orbit[0]), orbit[1], orbit[2], orbit[3]))
+    m.finishes = [_build(Finish, f) for f in md.get("finishes", [])]
+    g = s.get("ground")
     fr = s["fractal"]
     fractal = Graph.from_dict(fr) if fr.get("__struct__") == "Graph" else _build(Fractal, fr)
     return Scene(fractal, _build(Camera, s["camera"]), lights, m, seed=int(s["seed"]),
-                 cavity_darkness=float(s["cavity_darkness"]))
+                 cavity_darkness=float(s["cavity_darkness"]), ground=None if g is None else _build(Ground, g))
 
 
 # ---------------------------------------------------------------- the session

@@ -12,6 +12,11 @@ from syfrack_amd import abi, scene as sc, session
 def test_state_ron_round_trip_rebuilds_the_scene():
     scene, cfg = sc.mandelbox_pathtrace(64, 36, 4)
     scene.lights.append(sc.Light.environment((0.85, 0.9, 1.0), (0.55, 0.55, 0.6), (0.15, 0.13, 0.12), 1.0))
+    scene.lights.append(sc.Light.sun((0.3, 0.8, 0.5), (1.0, 0.95, 0.9), 3.0, 0.53))
+    scene.material.coloring = sc.Coloring(sc.Ramp([(1, 0, 0), (0, 1, 0), (0, 0, 1)], [0.1, 0.2, 0.8], True, 2.0, -0.1,
+                                                  1.5, True), 0, ((0.5, 0.25, 0.0, 0.0), 0.1, 0.2, 3.0))
+    scene.material.finishes = [sc.Finish((0.2, 0.3, 0.4), 0.6, 0.1, 0.05, (1.0, 0.5, 0.0), True), sc.Finish()]
+    scene.ground = sc.Ground(-1.5, 2)
     cfg.march_math = "exact"
     state = {"shader_fingerprint": 0xfedcba9876543210, "scene": scene, "cfg": cfg, "samples_done": 3, "spd": 2,
              "elapsed_secs": 0.1 + 0.2, "note": None}
