@@ -359,6 +359,7 @@ pub struct syf_server_config {
     pub grid_blocks: u32,
     pub quiet_target_us: u32,
     pub path_steps: u32,
+    pub tile_width: u32,
 }
 
 #[repr(C)]
@@ -394,6 +395,8 @@ pub struct syf_server_counters {
     pub family_grid_blocks: [u32; SYF_KERNEL_FAMILIES],
     pub family_path_steps: [u32; SYF_KERNEL_FAMILIES],
     pub autotune_ms: f64,
+    pub family_tile_width: [u32; SYF_KERNEL_FAMILIES],
+    pub family_quiet_quantum: [u32; SYF_KERNEL_FAMILIES],
 }
 
 #[repr(C)]

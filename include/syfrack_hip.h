@@ -564,7 +564,8 @@ typedef struct syf_server_config {
     uint32_t mode;               /* SYF_SERVE_BATCHED or SYF_SERVE_PLAIN */
     uint32_t max_active;         /* requests traced concurrently (0 = 4096) */
     uint32_t quantum_samples;    /* samples per pixel per work item of a long request (0 = 1 while
-                                    requests arrive, 16 once they have stopped) */
+                                    requests arrive, autotuned per family, 8 or 16, once they
+                                    have stopped) */
     uint32_t batch_target_us;    /* target device time of one launch while requests arrive
                                     (0 = 30000 us) */
     uint32_t grid_blocks;        /* workgroups of the persistent launch (0 = autotuned per family) */
@@ -573,6 +574,8 @@ typedef struct syf_server_config {
                                     all remaining work in one launch) */
     uint32_t path_steps;         /* field evaluations per lane per loop trip of the batched kernel:
                                     4, 8 or 16 (0 = autotuned per family) */
+    uint32_t tile_width;         /* width of a work item's 64-pixel tile: 8 (8 x 8), 16 (16 x 4) or
+                                    32 (32 x 2) (0 = autotuned per family) */
 } syf_server_config;
 #define SYF_QUIET_UNBOUNDED 0xffffffffu
 
@@ -607,6 +610,8 @@ typedef struct syf_server_counters {
     uint32_t family_grid_blocks[SYF_KERNEL_FAMILIES];
     uint32_t family_path_steps[SYF_KERNEL_FAMILIES];
     double autotune_ms;
+    uint32_t family_tile_width[SYF_KERNEL_FAMILIES];
+    uint32_t family_quiet_quantum[SYF_KERNEL_FAMILIES];  /* samples per work item in quiet mode */
 } syf_server_counters;
 
 int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** out);

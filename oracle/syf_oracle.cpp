@@ -43,6 +43,16 @@
 //       mistakes (a dropped cos_l, an unweighted emitter hit, a VNDF without the alpha stretch,
 //       a pdf without G1(v), a linear environment blend, a pdf without 1/pi, a missing
 //       selection probability, swapped sin/cos) each fail at least one of them.
+//   suns, flat ground, colouring ramp, finishes (cone_sample, sun_pdf, sun_radiance,
+//       ground_intersection, ramp_sample, make_surface, Oren-Nayar in brdf_eval)
+//       -> tests/test_oracle_shading.py: the ramp against palette.rs:225-260's known answers;
+//       a Lambert ground under a sun against rho/pi * E * cos(theta) * (1 + cos r) / 2, NEE alone
+//       and with MIS on disks up to 90 degrees; emission added once; every colouring slot on the
+//       ground, whose sample is known; the orbit mix against OrbitColoring::value
+//       (material.rs:39-53); the Oren-Nayar lobe against the qualitative model in its paper form
+//       (sin alpha tan beta); the ground occluding an emitter below it.  Planted mistakes (sun
+//       radiance without its pdf, max -> min in the Oren-Nayar denominator, a non-uniform cone
+//       sample, a wrong normal slot value) each fail at least one of them.
 // A function without such a pin would be marked "parity unpinned" here; none is.
 
 #include <cmath>

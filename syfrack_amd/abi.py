@@ -243,6 +243,7 @@ class SyfServerConfig(C.Structure):
         ("grid_blocks", C.c_uint32),
         ("quiet_target_us", C.c_uint32),
         ("path_steps", C.c_uint32),
+        ("tile_width", C.c_uint32),
     ]
 
 
@@ -281,6 +282,8 @@ class SyfServerCounters(C.Structure):
         ("family_grid_blocks", C.c_uint32 * 6),
         ("family_path_steps", C.c_uint32 * 6),
         ("autotune_ms", C.c_double),
+        ("family_tile_width", C.c_uint32 * 6),
+        ("family_quiet_quantum", C.c_uint32 * 6),
     ]
 
 

@@ -62,10 +62,10 @@ size_t hitq_capacity(uint32_t width, uint32_t height, int cu_count);
 // The fused march + hit-shading kernel, then the trailing pass over the hits it left.
 hipError_t launch_trace_primary(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
 hipError_t launch_hits_tail(const JobP& P, uint32_t sample, const TraceBuffers& B, int cu_count, hipStream_t s);
-// Work item of the batched serving kernel: an 8x8 tile of request `slot`, samples
-// [s0, s0 + ns).
+// Work item of the batched serving kernel: a tile of 64 pixels of request `slot`, 2^tws wide
+// (8 x 8, 16 x 4 or 32 x 2), samples [s0, s0 + ns).
 struct BatchItem {
-    uint32_t slot, x0, y0, s0, ns;
+    uint32_t slot, x0, y0, s0, ns, tws;
 };
 // Device planes of one request.
 struct BatchPlanes {

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
     // wave-uniform: the request served, the item whose pixels are being handed out, and an
     // item of another request held until the wave drains
     uint32_t slot = ~0u;
-    BatchItem pool{0, 0, 0, 0, 0}, pending{0, 0, 0, 0, 0};
+    BatchItem pool{0, 0, 0, 0, 0, 3}, pending{0, 0, 0, 0, 0, 3};
     uint32_t pool_next = 64, have_pending = 0, exhausted = 0;
     Lane<FAM> L;
     L.mode = L_IDLE;
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(256, PATHS_WAVES) void k_paths(const JobP* __restri
                 const uint32_t rank = uint32_t(__popcll(idle_mask & below_mask()));
                 if (L.mode == L_IDLE && pool_next + rank < 64u) {
                     const uint32_t u = pool_next + rank;
-                    const uint32_t px = pool.x0 + (u & 7u), py = pool.y0 + (u >> 3);
+                    const uint32_t px = pool.x0 + (u & ((1u << pool.tws) - 1u)), py = pool.y0 + (u >> pool.tws);
                     if (px < P.width && py < P.height && pool.ns > 0u) {
                         L.pix = (py << 16) | px;
                         L.s = pool.s0;
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(256) void k_refine_first_hits(const JobP* __restric
         const uint32_t slot = __builtin_amdgcn_readfirstlane(it.slot);
         const JobP& P = job_of(jobs, slot);
         const BatchPlanes pl = planes[slot];
-        const uint32_t px = it.x0 + (lane & 7u), py = it.y0 + (lane >> 3);
+        const uint32_t px = it.x0 + (lane & ((1u << it.tws) - 1u)), py = it.y0 + (lane >> it.tws);
         if (px >= P.width || py >= P.height) continue;
         const size_t idx = size_t(py) * P.width + px;
         const double tp = pl.depth64[idx];

@@ -45,6 +45,8 @@ struct Request {
     uint32_t state = SYF_REQ_QUEUED;
     uint32_t slot = ~0u;
     uint32_t tiles_x = 0, tiles = 0;
+    uint32_t tws = 3;        // its tiles are 2^tws x 2^(6 - tws) pixels (its family's choice)
+    uint32_t quantum_long = 0;  // samples per pass in quiet mode (its family's choice)
     uint64_t total = 0;      // tile-samples of the whole request
     uint64_t planned = 0;    // tile-samples launched
     uint64_t completed = 0;  // tile-samples whose launches finished
@@ -146,9 +148,11 @@ struct syf_server {
     unsigned long long* d_pctr = nullptr;  // counters of plain-mode launches
     hipEvent_t ev_pool[2 * NBUF] = {};
     uint32_t quantum = 0, quantum_long = 0, target_us = 0, quiet_us = 0, grid = 0;
-    // per kernel family: persistent workgroups and field evaluations per loop trip of k_paths
-    // (the autotuner's choices, or the configuration's)
-    uint32_t fam_grid[FAM_COUNT] = {0}, fam_steps[FAM_COUNT] = {0};
+    // per kernel family: persistent workgroups, field evaluations per loop trip of k_paths,
+    // tile width (log2) and quiet-mode samples per pass (the autotuner's choices, or the
+    // configuration's)
+    uint32_t fam_grid[FAM_COUNT] = {0}, fam_steps[FAM_COUNT] = {0}, fam_tws[FAM_COUNT] = {0},
+             fam_quantum[FAM_COUNT] = {0};
     double autotune_ms = 0.0;
     double last_submit_ms = -1e30;  // host clock of the latest submission
     double gap_ms = 0.0;            // running mean of the gaps between submissions (below 5 s)
@@ -168,12 +172,17 @@ struct syf_server {
 
 namespace {
 
-constexpr uint32_t TILE = 8;
+constexpr uint32_t TILE_PIXELS = 64;  // pixels of a work item: one per lane of a wave
 constexpr size_t BYTES_PER_PIXEL = 16 + 4 + 16 + 8;  // accum, depth, aux, depth64
 
 // Device time estimate of a work item: pixel-samples x segments per path.
 inline double item_units(const Request* r, uint32_t ns) {
-    return double(TILE * TILE) * double(ns) * double(r->P.max_bounces + 1u);
+    return double(TILE_PIXELS) * double(ns) * double(r->P.max_bounces + 1u);
+}
+// Work item t of a pass of request r
+inline BatchItem tile_item(const Request* r, uint32_t t) {
+    const uint32_t tw = 1u << r->tws, th = TILE_PIXELS >> r->tws;
+    return BatchItem{r->slot, (t % r->tiles_x) * tw, (t / r->tiles_x) * th, r->q_start, r->q_len, r->tws};
 }
 
 void set_server_error(syf_server* s, int code, const char* what, hipError_t e) {
@@ -275,7 +284,6 @@ bool plan_launch(syf_server* s) {
     const double ns = std::max(s->ns_per_unit, 1e-6);
     const double budget_units = latency ? double(s->target_us) * 1000.0 / ns
                                 : s->quiet_us == SYF_QUIET_UNBOUNDED ? 1e300 : double(s->quiet_us) * 1000.0 / ns;
-    const uint32_t quantum = latency ? s->quantum : s->quantum_long;
     std::vector<BatchItem> by_family[FAM_COUNT];
     size_t n_items = 0;
     Launch L;
@@ -294,10 +302,10 @@ bool plan_launch(syf_server* s) {
         while (r->planned < r->total && (units < budget_units || n_items < s->fill_items)) {
             if (!r->pass_open) {  // open the next pass
                 r->q_start = uint32_t(r->planned / r->tiles);
-                r->q_len = std::min(quantum, r->spp - r->q_start);
+                r->q_len = std::min(latency ? s->quantum : r->quantum_long, r->spp - r->q_start);
                 // a request whose remaining work fits in a launch goes in one piece: it then
                 // needs one launch, not one per quantum
-                if (double(r->total - r->planned) * double(TILE * TILE) * double(r->P.max_bounces + 1u) <=
+                if (double(r->total - r->planned) * double(TILE_PIXELS) * double(r->P.max_bounces + 1u) <=
                     budget_units)
                     r->q_len = r->spp - r->q_start;
                 if (r->q_start == 0u) {
@@ -314,7 +322,7 @@ bool plan_launch(syf_server* s) {
             const uint32_t t = r->q_cursor;
             // the previous pass's item of this tile is still in flight: its samples must land first
             if (t >= r->ret_prev) break;
-            items.push_back(BatchItem{r->slot, (t % r->tiles_x) * TILE, (t / r->tiles_x) * TILE, r->q_start, r->q_len});
+            items.push_back(tile_item(r, t));
             n_items++;
             units += item_units(r, r->q_len);
             r->planned += r->q_len;
@@ -609,25 +617,32 @@ syf_job_desc calibration_desc(uint32_t kfam) {
     return d;
 }
 
-// Autotuner.  For each kernel family it times its calibration scene (8x8-tile items holding all
-// 4 samples: 4096 items, the least a launch holds on a 256-CU device, in one persistent launch,
-// the better of two) at 4, 8 and 16 field evaluations per loop trip on the default grid of 3
-// workgroups per CU, then on 2 workgroups per CU with the best step count.  The grid it picks
-// serves latency-mode launches, which are of that size; quiet-mode launches, many times longer,
-// keep 3 workgroups per CU (with the tuned grid they took the uneven batch from 535 to 495
-// Mrays/s, while the 200/s serving loop gained 350 -> 363).  It leaves the
-// default (3 per CU, 8 evaluations) only for a choice at least 3 % faster: a first version on a
-// 384^2, 2-sample scene (fewer items than the grid has waves) picked 1 workgroup per CU for
-// some families and took the uneven 256-request batch from 535 to 392 Mrays/s.  A choice the
-// configuration fixes (grid_blocks, path_steps) is not tuned.  The cost model (ns per
-// pixel-sample-segment) is seeded with the power-8 Mandelbulb's best rate.  None of the
-// choices changes a served image: they only schedule the same per-pixel arithmetic.
+// Autotuner.  For each kernel family it times its calibration scene, in one persistent launch
+// over 64-pixel items holding all 4 samples (4096 items, the least a launch holds on a 256-CU
+// device; the better of two launches), choosing in turn:
+//  * the field evaluations per loop trip, 4, 8 or 16, on the default grid of 3 workgroups per
+//    CU with 8 x 8 tiles;
+//  * the grid: 2 workgroups per CU against 3.  The grid serves latency-mode launches, which are
+//    of that size; quiet-mode launches, many times longer, keep 3 workgroups per CU (with the
+//    tuned grid they took the uneven batch from 535 to 495 Mrays/s, while the 200/s serving
+//    loop gained 350 -> 363);
+//  * the tile shape, 16 x 4 or 32 x 2 pixels against 8 x 8 (rows of a tile share more of their
+//    paths' first bounces, squares more of their march lengths);
+//  * the quiet-mode samples per pass, 8 against 16, from the time per sample of launches whose
+//    items hold 8 and 16 samples (one launch each).  Longer items amortise a lane refill and a
+//    claim over more paths and lengthen the launch's tail.
+// A choice leaves the default only when it is at least 3 % faster: a first version on a 384^2,
+// 2-sample scene (fewer items than the grid has waves) picked 1 workgroup per CU for some
+// families and took the uneven 256-request batch from 535 to 392 Mrays/s.  A choice the
+// configuration fixes (grid_blocks, path_steps, tile_width, quantum_samples) is not tuned.  The
+// cost model (ns per pixel-sample-segment) is seeded with the power-8 Mandelbulb's best rate.
+// None of the choices changes a served image: they only schedule the same per-pixel arithmetic.
 void autotune(syf_server* s) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t cu = uint32_t(s->ctx->cu_count > 0 ? s->ctx->cu_count : 256);
     const uint32_t W = 512, H = 512, SPP = 4;
     const size_t pixels = size_t(W) * H;
-    const uint32_t tiles_x = W / TILE, tiles = tiles_x * (H / TILE);
+    const uint32_t tiles = uint32_t(pixels / TILE_PIXELS);
     float4 *accum = nullptr, *aux = nullptr;
     float* depth = nullptr;
     double* depth64 = nullptr;
@@ -651,14 +666,23 @@ void autotune(syf_server* s) {
               hipEventCreate(&b) == hipSuccess;
     BatchPlanes pl{accum, depth, aux, depth64};
     ok = ok && hipMemcpy(dpl, &pl, sizeof pl, hipMemcpyHostToDevice) == hipSuccess;
-    std::vector<BatchItem> items;
-    for (uint32_t t = 0; t < tiles; ++t) items.push_back(BatchItem{0, (t % tiles_x) * TILE, (t / tiles_x) * TILE, 0, SPP});
-    ok = ok && hipMemcpy(ditems, items.data(), size_t(tiles) * sizeof(BatchItem), hipMemcpyHostToDevice) == hipSuccess;
-    // device time of one launch of family f's calibration (better of two; the claim and work
-    // counters and the exit count are left zero by each launch)
-    auto time_launch = [&](uint32_t f, uint32_t grid, uint32_t steps) -> double {
+    // the calibration's items: every 64-pixel tile of the image, 2^tws pixels wide, samples [0, ns)
+    uint32_t items_tws = ~0u, items_ns = 0;
+    auto set_items = [&](uint32_t tws, uint32_t ns) {
+        if (!ok || (tws == items_tws && ns == items_ns)) return;
+        const uint32_t tw = 1u << tws, th = TILE_PIXELS >> tws, tiles_x = W / tw;
+        std::vector<BatchItem> items;
+        for (uint32_t t = 0; t < tiles; ++t) items.push_back(BatchItem{0, (t % tiles_x) * tw, (t / tiles_x) * th, 0, ns, tws});
+        ok = hipMemcpy(ditems, items.data(), size_t(tiles) * sizeof(BatchItem), hipMemcpyHostToDevice) == hipSuccess;
+        items_tws = tws;
+        items_ns = ns;
+    };
+    // device time of one launch of family f's calibration (the better of `reps`; the claim and
+    // work counters and the exit count are left zero by each launch)
+    auto time_launch = [&](uint32_t f, uint32_t grid, uint32_t steps, uint32_t tws, uint32_t ns, int reps) -> double {
+        set_items(tws, ns);
         float best = 1e30f;
-        for (int rep = 0; rep < 2 && ok; ++rep) {
+        for (int rep = 0; rep < reps && ok; ++rep) {
             float ms = 0.0f;
             ok = hipEventRecord(a, s->stream) == hipSuccess &&
                  launch_paths(f, djob, dpl, ditems, tiles, claim, ctr, exit_count, host_out, grid, steps, s->stream) ==
@@ -675,20 +699,35 @@ void autotune(syf_server* s) {
         const syf_job_desc d = calibration_desc(f);
         if (lower_job(d, P) != SYF_OK || kernel_family(P.f) != f) continue;
         ok = hipMemcpy(djob, &P, sizeof P, hipMemcpyHostToDevice) == hipSuccess;
-        uint32_t steps = s->fam_steps[f], grid = s->fam_grid[f];
-        double best = time_launch(f, grid, steps);  // the incumbent: the defaults or the fixed values
+        uint32_t steps = s->fam_steps[f], grid = s->fam_grid[f], tws = s->fam_tws[f];
+        // the incumbent: the defaults or the fixed values
+        double best = time_launch(f, grid, steps, tws, SPP, 2);
         if (s->cfg.path_steps == 0)
             for (uint32_t k : {4u, 16u}) {
-                const double ms = time_launch(f, grid, k);
+                const double ms = time_launch(f, grid, k, tws, SPP, 2);
                 if (ms < best * MARGIN) best = ms, steps = k;
             }
         if (s->cfg.grid_blocks == 0) {
-            const double ms = time_launch(f, cu * 2, steps);
+            const double ms = time_launch(f, cu * 2, steps, tws, SPP, 2);
             if (ms < best * MARGIN) best = ms, grid = cu * 2;
+        }
+        const double best_4 = best;
+        if (s->cfg.tile_width == 0)
+            for (uint32_t w : {4u, 5u}) {
+                const double ms = time_launch(f, grid, steps, w, SPP, 2);
+                if (ms < best * MARGIN) best = ms, tws = w;
+            }
+        if (s->cfg.quantum_samples == 0) {
+            // quiet-mode launches run on the default grid; ms per sample at 16 and 8 samples per item
+            const uint32_t qgrid = s->grid;
+            const double per16 = time_launch(f, qgrid, steps, tws, 16, 1) / 16.0;
+            const double per8 = time_launch(f, qgrid, steps, tws, 8, 1) / 8.0;
+            s->fam_quantum[f] = per8 < per16 * MARGIN ? 8u : 16u;
         }
         s->fam_steps[f] = steps;
         s->fam_grid[f] = grid;
-        if (f == FAM_BULB8 && best < 1e29)
+        s->fam_tws[f] = tws;
+        if (f == FAM_BULB8 && best_4 < 1e29)
             s->ns_per_unit = best * 1e6 / (double(pixels) * SPP * (d.max_bounces + 1));
     }
     hipFree(accum);
@@ -723,6 +762,8 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     if (cfg->quantum_samples > 4096) return fail(SYF_E_INVALID, "quantum_samples must be at most 4096");
     if (cfg->path_steps != 0 && cfg->path_steps != 4 && cfg->path_steps != 8 && cfg->path_steps != 16)
         return fail(SYF_E_INVALID, "path_steps must be 0 (autotuned), 4, 8 or 16, got %u", cfg->path_steps);
+    if (cfg->tile_width != 0 && cfg->tile_width != 8 && cfg->tile_width != 16 && cfg->tile_width != 32)
+        return fail(SYF_E_INVALID, "tile_width must be 0 (autotuned), 8, 16 or 32, got %u", cfg->tile_width);
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     syf_server* s = new syf_server();
@@ -746,6 +787,8 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
     for (uint32_t f = 0; f < FAM_COUNT; ++f) {
         s->fam_grid[f] = s->grid;
         s->fam_steps[f] = cfg->path_steps ? cfg->path_steps : 8u;
+        s->fam_tws[f] = cfg->tile_width == 32 ? 5u : cfg->tile_width == 16 ? 4u : 3u;
+        s->fam_quantum[f] = s->quantum_long;
     }
     // each launch ends with a tail as long as its slowest pixel's quantum: launches of a few
     // tens of ms keep that below a few per cent, and a preview waits at most two of them
@@ -792,7 +835,9 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
             syf_server_destroy(s);
             return hip_fail(e, "server streams");
         }
-    if (s->cfg.mode == SYF_SERVE_BATCHED && (cfg->grid_blocks == 0 || cfg->path_steps == 0)) autotune(s);
+    if (s->cfg.mode == SYF_SERVE_BATCHED &&
+        (cfg->grid_blocks == 0 || cfg->path_steps == 0 || cfg->tile_width == 0 || cfg->quantum_samples == 0))
+        autotune(s);
     s->worker = std::thread(worker_main, s);
     *out = s;
     return SYF_OK;
@@ -857,8 +902,12 @@ int syf_server_submit(syf_server* s, const syf_job_desc* desc, uint32_t spp, uin
     r->spp = spp;
     r->pixels = size_t(desc->width) * desc->height;
     r->footprint = uint64_t(r->pixels) * BYTES_PER_PIXEL;
-    r->tiles_x = (desc->width + TILE - 1) / TILE;
-    r->tiles = r->tiles_x * ((desc->height + TILE - 1) / TILE);
+    const uint32_t fam = kernel_family(r->P.f);
+    r->tws = s->fam_tws[fam];
+    r->quantum_long = s->fam_quantum[fam];
+    const uint32_t tw = 1u << r->tws, th = TILE_PIXELS >> r->tws;
+    r->tiles_x = (desc->width + tw - 1) / tw;
+    r->tiles = r->tiles_x * ((desc->height + th - 1) / th);
     r->total = uint64_t(r->tiles) * spp;
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->error != SYF_OK) {
@@ -1017,6 +1066,8 @@ int syf_server_stats(syf_server* s, syf_server_counters* out) {
     for (uint32_t f = 0; f < FAM_COUNT && f < SYF_KERNEL_FAMILIES; ++f) {
         out->family_grid_blocks[f] = s->fam_grid[f];
         out->family_path_steps[f] = s->fam_steps[f];
+        out->family_tile_width[f] = 1u << s->fam_tws[f];
+        out->family_quiet_quantum[f] = s->fam_quantum[f];
     }
     out->autotune_ms = s->autotune_ms;
     return SYF_OK;

@@ -57,7 +57,7 @@ class Server:
 
     def __init__(self, ctx: Context, mode: str = "batched", hbm_budget_bytes: int = 0, max_active: int = 0,
                  quantum_samples: int = 0, batch_target_us: int = 0, grid_blocks: int = 0,
-                 quiet_target_us: int = 0, path_steps: int = 0):
+                 quiet_target_us: int = 0, path_steps: int = 0, tile_width: int = 0):
         if mode not in ("batched", "plain"):
             raise ValueError(f"mode must be 'batched' or 'plain', got {mode!r}")
         self.ctx = ctx
@@ -71,6 +71,7 @@ class Server:
         cfg.grid_blocks = grid_blocks
         cfg.quiet_target_us = quiet_target_us
         cfg.path_steps = path_steps
+        cfg.tile_width = tile_width
         h = C.c_void_p()
         abi.check(self.lib.syf_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
         self.handle = h
@@ -143,6 +144,8 @@ class Server:
         out = {name: getattr(c, name) for name, _ in c._fields_ if name != "reserved"}
         out["family_grid_blocks"] = list(c.family_grid_blocks)
         out["family_path_steps"] = list(c.family_path_steps)
+        out["family_tile_width"] = list(c.family_tile_width)
+        out["family_quiet_quantum"] = list(c.family_quiet_quantum)
         return out
 
 

@@ -269,8 +269,9 @@ def test_poisson_serving_mix_scaled(ctx):
 
 
 def test_autotuner_choices_leave_images_unchanged(ctx):
-    """The autotuner picks, per kernel family, the persistent grid (1-3 workgroups per CU) and
-    the field evaluations per loop trip (4, 8, 16) of the batched kernel; every choice only
+    """The autotuner picks, per kernel family, the persistent grid (2 or 3 workgroups per CU),
+    the field evaluations per loop trip (4, 8, 16), the tile shape (8 x 8, 16 x 4, 32 x 2) and
+    the quiet-mode samples per work item (8, 16) of the batched kernel; every choice only
     schedules the same per-pixel arithmetic, so forced choices give the same planes bit for bit."""
     from syfrack_amd import abi, serve
     reqs = [q for q in serve.uneven_requests(40, seed=13, min_side=64, max_side=160)]
@@ -291,12 +292,19 @@ def test_autotuner_choices_leave_images_unchanged(ctx):
         return out, st
 
     tuned, st = serve_all()
-    print("autotuner:", st["family_grid_blocks"], st["family_path_steps"], f"{st['autotune_ms']:.0f} ms")
+    print("autotuner:", st["family_grid_blocks"], st["family_path_steps"], st["family_tile_width"],
+          st["family_quiet_quantum"], f"{st['autotune_ms']:.0f} ms")
     assert st["autotune_ms"] > 0
     assert all(k in (4, 8, 16) for k in st["family_path_steps"])
     assert all(g > 0 and g % 256 == 0 and g // 256 in (2, 3) for g in st["family_grid_blocks"])
-    for kw in (dict(path_steps=4, grid_blocks=256), dict(path_steps=16, grid_blocks=768)):
+    assert all(w in (8, 16, 32) for w in st["family_tile_width"])
+    assert all(q in (8, 16) for q in st["family_quiet_quantum"])
+    for kw in (dict(path_steps=4, grid_blocks=256, tile_width=32, quantum_samples=3),
+               dict(path_steps=16, grid_blocks=768, tile_width=16, quantum_samples=2),
+               dict(path_steps=8, grid_blocks=512, tile_width=8, quantum_samples=1)):
         forced, st2 = serve_all(**kw)
         assert st2["family_path_steps"] == [kw["path_steps"]] * 6 and st2["autotune_ms"] == 0
+        assert st2["family_tile_width"] == [kw["tile_width"]] * 6
+        assert st2["family_quiet_quantum"] == [kw["quantum_samples"]] * 6
         for a, b in zip(tuned, forced):
             assert all(np.array_equal(x, y) for x, y in zip(a, b))
