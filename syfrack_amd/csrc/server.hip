@@ -797,6 +797,16 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
                                         : uint32_t(std::min(double(s->target_us) * QUIET_FACTOR, 4.0e9));
     s->fill_items = cu * 16u;  // two workgroups of four waves per CU, twice over
     s->ns_per_unit = 0.05;
+    // The launch streams first: the runtime gives a new stream a hardware queue of its own while
+    // the process holds fewer than GPU_MAX_HW_QUEUES (4 by default), and otherwise shares one.
+    // Created before the server's other stream, the three launch streams take queues of their
+    // own whenever the process has at most one other stream (its null stream), and launches
+    // in flight then never wait for each other in a shared queue.
+    for (int b = 0; b < NBUF; ++b)
+        if ((e = hipStreamCreateWithFlags(&s->ls[b], hipStreamNonBlocking)) != hipSuccess) {
+            syf_server_destroy(s);
+            return hip_fail(e, "server streams");
+        }
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc(&s->d_jobs, size_t(s->max_slots) * sizeof(JobP))) != hipSuccess ||
         (e = hipMalloc(&s->d_plain_job, sizeof(JobP))) != hipSuccess ||
@@ -830,11 +840,6 @@ int syf_server_create(syf_ctx* ctx, const syf_server_config* cfg, syf_server** o
             return hip_fail(e, "server events");
         }
     }
-    for (int b = 0; b < NBUF; ++b)
-        if ((e = hipStreamCreateWithFlags(&s->ls[b], hipStreamNonBlocking)) != hipSuccess) {
-            syf_server_destroy(s);
-            return hip_fail(e, "server streams");
-        }
     if (s->cfg.mode == SYF_SERVE_BATCHED &&
         (cfg->grid_blocks == 0 || cfg->path_steps == 0 || cfg->tile_width == 0 || cfg->quantum_samples == 0))
         autotune(s);
