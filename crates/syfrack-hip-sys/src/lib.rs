@@ -10,7 +10,7 @@
 
 use std::os::raw::{c_char, c_int, c_void};
 
-pub const SYF_ABI_VERSION: c_int = 4;
+pub const SYF_ABI_VERSION: c_int = 5;
 
 pub const SYF_OK: c_int = 0;
 pub const SYF_E_INVALID: c_int = -1;
@@ -230,6 +230,16 @@ pub struct syf_coloring {
 
 #[repr(C)]
 #[derive(Clone, Copy, Default, Debug)]
+pub struct syf_fog {
+    pub density: f32,
+    pub color: [f32; 3],
+    pub height: f32,
+    pub falloff: f32,
+    pub anisotropy: f32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
 pub struct syf_ground {
     pub enabled: u32,
     pub height: f32,
@@ -308,6 +318,7 @@ pub struct syf_job_desc {
     pub coloring: syf_coloring,
     pub finish_count: u32,
     pub finishes: [syf_finish; SYF_MAX_FINISHES],
+    pub fog: syf_fog,
 }
 
 #[repr(C)]

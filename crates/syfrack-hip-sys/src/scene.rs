@@ -8,9 +8,9 @@
 //! itself.  [`GraphBuf::from_graph`] lowers the form graph node for node
 //! (crates/syfrack-core/src/scene/graph/mod.rs:103-257) to `syf_graph`; the library validates it with
 //! the reference's rules and messages.  What this path does not render is refused with an error naming
-//! the scene element: a 2D scene, projector, tube and HDRI lights, participating media, the seascape,
-//! portal, nebula and dunes, textures, finish detail patterns that change the shading, optics and
-//! phenomena.
+//! the scene element: a 2D scene, projector, tube and HDRI lights, the seascape, portal, nebula and
+//! dunes, textures, finish detail patterns that change the shading, optics and phenomena.  The height
+//! fog is lowered.
 
 use std::ffi::{CStr, CString};
 use std::os::raw::c_int;
@@ -210,9 +210,6 @@ pub fn job_desc_from_scene(scene: &Scene, p: &RenderParams) -> Result<syf_job_de
         return Err(refuse("phenomena: not rendered on this path"));
     }
     let a = &scene.atmosphere;
-    if a.fog_density > 0.0 {
-        return Err(refuse("atmosphere.fog_density: participating media are not rendered on this path"));
-    }
     if a.seascape.is_some() || a.portal.is_some() || a.nebula.is_some() {
         return Err(refuse("atmosphere: seascape, portal and nebula are not rendered on this path"));
     }
@@ -262,6 +259,16 @@ pub fn job_desc_from_scene(scene: &Scene, p: &RenderParams) -> Result<syf_job_de
     d.finish_count = m.finishes.len() as u32;
     if let Some(g) = &a.ground {
         d.ground = syf_ground { enabled: 1, height: g.height, finish: g.finish };
+    }
+    // GpuAtmosphere::from_atmosphere (uniforms.rs:218-226)
+    if a.fog_density > 0.0 {
+        d.fog = syf_fog {
+            density: a.fog_density,
+            color: a.fog_color,
+            height: a.fog_height,
+            falloff: a.fog_falloff,
+            anisotropy: a.anisotropy,
+        };
     }
     d.width = p.width;
     d.height = p.height;

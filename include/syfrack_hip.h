@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SYF_ABI_VERSION 4
+#define SYF_ABI_VERSION 5
 
 /* ---------------------------------------------------------------- status codes
  * Replaces anyhow::Result at the seams: config errors (renderer.rs:81-138),
@@ -152,6 +152,19 @@ typedef struct syf_coloring {
     float normal_weight;      /* orbit */
     float logarithmic;        /* orbit: signed log gain, 0 = linear */
 } syf_coloring;
+/* The atmosphere's height fog (Atmosphere, atmosphere.rs:7-16; GpuAtmosphere.a/b, uniforms.rs:
+ * 218-226): extinction density at and below `height`, decaying above it as exp(-(y - height) x
+ * falloff); single-scattering albedo `color`; Henyey-Greenstein anisotropy.  density 0: no fog.
+ * A job with fog accumulates the session's VOLUME, MOMENTS and SURFACE planes on the device
+ * (pathtrace.wgsl:1243-1266).  The batched server refuses a request with fog; the control
+ * machine's preview ignores it. */
+typedef struct syf_fog {
+    float density;            /* 0..10 */
+    float color[3];           /* linear, finite, >= 0 */
+    float height;             /* -100..100 */
+    float falloff;            /* 0..10 (0: homogeneous) */
+    float anisotropy;         /* -0.99..0.99 */
+} syf_fog;
 /* Atmosphere.ground (GpuAtmosphere.ground, uniforms.rs:227-230): a flat plane (no dunes). */
 typedef struct syf_ground {
     uint32_t enabled;
@@ -271,6 +284,7 @@ typedef struct syf_job_desc {
     syf_coloring coloring;         /* Material.coloring */
     uint32_t finish_count;         /* Material.finishes, 0..SYF_MAX_FINISHES */
     syf_finish finishes[SYF_MAX_FINISHES];
+    syf_fog fog;                   /* Atmosphere fog fields */
 } syf_job_desc;
 /* SYF_MATH_FAST: hardware reciprocal / reciprocal square root / logarithm and FMA contraction in
  * the FP32 march; held to the FP64 oracle's tolerance (>= 99.9 % hit/miss, refined hit distance
@@ -288,9 +302,11 @@ typedef struct syf_job_desc {
 #define SYF_PLANE_DEPTH64 3u /* double per pixel: FP64-refined hit distance, -1 on a miss */
 #define SYF_PLANE_STEPS 4u   /* uint32 per pixel: march steps of the primary ray, hit or miss */
 /* The session's radiance layers (session.rs:1217-1243; accumulated per sample by
- * pathtrace.wgsl:1243-1266).  This path has no participating medium, so the volume estimate of
- * every sample is zero: VOLUME and MOMENTS read (0, 0, 0, samples) and SURFACE equals ACCUM;
- * readback only (they follow from ACCUM). */
+ * pathtrace.wgsl:1243-1266).  A job with fog accumulates them on the device: the volume radiance
+ * (the camera segment's in-scattering, or the whole path when the camera ray scattered), its
+ * Welford second moments, and the surface radiance (the rest).  Without fog the volume estimate
+ * of every sample is zero: VOLUME and MOMENTS read (0, 0, 0, samples), SURFACE equals ACCUM, and
+ * they are read back only (they follow from ACCUM). */
 #define SYF_PLANE_VOLUME 5u  /* float4 per pixel: volume radiance sum, samples */
 #define SYF_PLANE_MOMENTS 6u /* float4 per pixel: the volume's Welford second moments, samples */
 #define SYF_PLANE_SURFACE 7u /* float4 per pixel: surface radiance sum, samples */
@@ -420,8 +436,8 @@ int syf_trace_primary_noalloc(syf_ctx* ctx, const syf_job_desc* desc, uint32_t s
 int syf_readback(syf_job* job, uint32_t plane, void* host, size_t bytes);
 
 /* Copy a host buffer of exactly `bytes` bytes into a device plane (ACCUM, DEPTH, AUX, DEPTH64 or
- * STEPS): the planes of a resumed session, as RenderSession::resume writes its saved buffers back
- * (session.rs:1308-1330).  Synchronous. */
+ * STEPS; VOLUME, MOMENTS and SURFACE of a job with fog): the planes of a resumed session, as
+ * RenderSession::resume writes its saved buffers back (session.rs:1308-1330).  Synchronous. */
 int syf_upload(syf_job* job, uint32_t plane, const void* host, size_t bytes);
 
 /* ---------------------------------------------------------------- checkpoints and export

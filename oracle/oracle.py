@@ -70,9 +70,19 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_escape_f128.argtypes = [P, P, P, C.c_size_t, C.c_double, P, P]
     lib.orc_ramp_sample.restype = None
     lib.orc_ramp_sample.argtypes = [P, P, C.c_size_t, P]
+    lib.orc_fog_optical_depth.restype = None
+    lib.orc_fog_optical_depth.argtypes = [P, P, P, P, C.c_size_t, P]
+    lib.orc_fog_free_flight.restype = None
+    lib.orc_fog_free_flight.argtypes = [P, P, P, C.c_float, C.c_size_t, P]
+    lib.orc_volume_coordinate.restype = C.c_float
+    lib.orc_volume_coordinate.argtypes = [C.c_uint32] * 3
+    lib.orc_hg_phase.restype = C.c_float
+    lib.orc_hg_phase.argtypes = [C.c_float, C.c_float]
+    lib.orc_hg_sample.restype = None
+    lib.orc_hg_sample.argtypes = [C.c_float, P, P, C.c_size_t, P]
     if hasattr(lib, "orc_render"):
         lib.orc_render.restype = C.c_int
-        lib.orc_render.argtypes = [P] + [C.c_uint32] * 6 + [P, P, P, P]
+        lib.orc_render.argtypes = [P] + [C.c_uint32] * 6 + [P, P, P, P, P]
     return lib
 
 
@@ -263,19 +273,56 @@ def ramp_sample(coloring, values) -> np.ndarray:
     return out
 
 
-def render(desc, rect, sample_start: int, sample_count: int):
+def render(desc, rect, sample_start: int, sample_count: int, session: bool = False):
     """Path-traced accumulation (float4 per pixel), first-hit depth, aux and FP64-refined
-    first-hit depth of a rectangle."""
+    first-hit depth of a rectangle; with session, also the VOLUME, MOMENTS and SURFACE planes
+    (3, h, w, 4) accumulated from zero."""
     lib = load()
     x0, y0, w, h = rect
     accum = np.zeros((h, w, 4), np.float32)
     depth = np.zeros((h, w), np.float32)
     aux = np.zeros((h, w, 4), np.float32)
     depth64 = np.zeros((h, w), np.float64)
+    planes = np.zeros((3, h, w, 4), np.float32) if session else None
     rc = lib.orc_render(C.byref(desc), x0, y0, w, h, sample_start, sample_count, _ptr(accum), _ptr(depth), _ptr(aux),
-                        _ptr(depth64))
+                        _ptr(depth64), _ptr(planes) if session else None)
     assert rc == 0
+    if session:
+        return accum, depth, aux, depth64, planes
     return accum, depth, aux, depth64
+
+
+def fog_optical_depth(desc, ro, rd, dist) -> np.ndarray:
+    ro = np.ascontiguousarray(np.asarray(ro, np.float32).reshape(-1, 3))
+    rd = np.ascontiguousarray(np.asarray(rd, np.float32).reshape(-1, 3))
+    dist = np.ascontiguousarray(np.broadcast_to(np.asarray(dist, np.float32), (len(ro),)))
+    out = np.zeros(len(ro), np.float32)
+    load().orc_fog_optical_depth(C.byref(desc), _ptr(ro), _ptr(rd), _ptr(dist), len(ro), _ptr(out))
+    return out
+
+
+def fog_free_flight(desc, ro, rd, t_end: float, n: int) -> np.ndarray:
+    ro = np.ascontiguousarray(np.asarray(ro, np.float32))
+    rd = np.ascontiguousarray(np.asarray(rd, np.float32))
+    out = np.zeros(n, np.float32)
+    load().orc_fog_free_flight(C.byref(desc), _ptr(ro), _ptr(rd), C.c_float(t_end), n, _ptr(out))
+    return out
+
+
+def volume_coordinate(sample_index: int, dimension: int, stream: int) -> float:
+    return load().orc_volume_coordinate(sample_index, dimension, stream & 0xFFFFFFFF)
+
+
+def hg_phase(cos_theta: float, g: float) -> float:
+    return load().orc_hg_phase(C.c_float(cos_theta), C.c_float(g))
+
+
+def hg_sample(g: float, direction, r12) -> np.ndarray:
+    d = np.ascontiguousarray(np.asarray(direction, np.float32))
+    r = np.ascontiguousarray(np.asarray(r12, np.float32).reshape(-1, 2))
+    out = np.zeros((len(r), 3), np.float32)
+    load().orc_hg_sample(C.c_float(g), _ptr(d), _ptr(r), len(r), _ptr(out))
+    return out
 
 
 def planar_f32(desc, rect=None):

@@ -53,6 +53,18 @@
 //       (sin alpha tan beta); the ground occluding an emitter below it.  Planted mistakes (sun
 //       radiance without its pdf, max -> min in the Oren-Nayar denominator, a non-uniform cone
 //       sample, a wrong normal slot value) each fail at least one of them.
+//   height fog (fog_sigma, fog_segment_tau, fog_optical_depth, fog_invert_segment, fog_free_flight,
+//       volume_coordinate, hg_phase, hg_sample, fog_segment_direct, fog_direct_light, the volume
+//       branches of radiance and the session planes) -> tests/test_oracle_fog.py: the rotated
+//       radical inverse; optical depth against quadrature of the height profile; free flight against
+//       1 - exp(-tau) (Kolmogorov-Smirnov); the phase's normalisation, mean cosine and histogram; an
+//       absorbing fog against exp(-tau) of the sky; the volume white furnace (a non-absorbing fog in a
+//       uniform sky keeps its radiance); single scattering of an emitter against a quadrature; an
+//       emitter counted once after a scattering event (radiance polynomial in the albedo); VOLUME +
+//       SURFACE = ACCUM.  Planted mistakes (the distance pdf without the equiangular average, a
+//       surface proposal for phase-sampled escapes, a sign in the phase sample, swapped Halton bases,
+//       a homogeneous inversion of a height fog, a phase hit of an emitter counted again) each fail
+//       at least one of them.
 // A function without such a pin would be marked "parity unpinned" here; none is.
 
 #include <cmath>
@@ -593,6 +605,7 @@ struct SceneL {
     uint32_t finish_count;
     syf_finish finishes[SYF_MAX_FINISHES];
     syf_ground ground;
+    syf_fog fog;  // density 0: none
 };
 
 inline F3 f3(const float* p) { return v3(p[0], p[1], p[2]); }
@@ -670,6 +683,8 @@ SceneL lower_scene(const syf_job_desc& d) {
     S.finish_count = d.finish_count;
     for (uint32_t i = 0; i < SYF_MAX_FINISHES; ++i) S.finishes[i] = d.finishes[i];
     S.ground = d.ground;
+    S.fog = d.fog;
+    if (!(S.fog.density > 0.0f)) S.fog = syf_fog{};
     return S;
 }
 
@@ -944,9 +959,12 @@ inline bool visible(const SceneL& S, F3 ro, F3 rd, float t_end, Count* c) {
     HitL h = trace_scene(S, ro, rd, t_end, c);
     return !h.h.hit && !h.h.exhausted;
 }
+inline float volume_light_probability(float weight, float total) {  // lights.wgsl:319-322
+    return total <= 0.0f ? 0.0f : std::fmin(1.0f, 3.0f * weight / total);
+}
 // area_emission (lights.wgsl:330-357)
 F3 area_emission(const SceneL& S, F3 ro, F3 rd, float distance, float prev_pdf, bool use_mis, F3 prev_n,
-                 float prev_total) {
+                 float prev_total, bool from_volume) {
     F3 L = v3(0.0f, 0.0f, 0.0f);
     for (uint32_t i = 0; i < S.light_count; ++i) {
         const LightL& l = S.lights[i];
@@ -959,7 +977,9 @@ F3 area_emission(const SceneL& S, F3 ro, F3 rd, float distance, float prev_pdf, 
         if (use_mis) {
             float pl = distance * distance / std::fmax(l.area * cos_l, 1e-20f);
             float sel = 0.0f;
-            if (prev_total > 0.0f) sel = area_weight(ro, prev_n, l) / prev_total;
+            if (prev_total > 0.0f)  // from_volume: a phase sample's light selection (lights.wgsl:346-350)
+                sel = from_volume ? volume_light_probability(area_weight(ro, v3(0.0f, 0.0f, 0.0f), l), prev_total)
+                                  : area_weight(ro, prev_n, l) / prev_total;
             pl *= sel;
             w = mis_power(prev_pdf, pl);
         }
@@ -967,15 +987,16 @@ F3 area_emission(const SceneL& S, F3 ro, F3 rd, float distance, float prev_pdf, 
     }
     return L;
 }
-// environment_emission (pathtrace.wgsl:177-189), surfaces only.
-F3 environment_emission(const SceneL& S, F3 dir, float pdf, bool use_mis, F3 n, float area_total) {
+// environment_emission (pathtrace.wgsl:177-189)
+F3 environment_emission(const SceneL& S, F3 dir, float pdf, bool use_mis, F3 n, float area_total, bool from_volume) {
     F3 L = v3(0.0f, 0.0f, 0.0f);
     for (uint32_t i = 0; i < S.light_count; ++i) {
         const LightL& l = S.lights[i];
         if (l.kind != SYF_LIGHT_ENV) continue;
         float w = 1.0f;
         if (use_mis) {
-            float proposal = environment_probability(l, area_total) * (std::fmax(dot(n, dir), 0.0f) / PI_F);
+            float proposal = environment_probability(l, area_total) *
+                             (from_volume ? 1.0f / (4.0f * PI_F) : std::fmax(dot(n, dir), 0.0f) / PI_F);
             w = mis_power(pdf, proposal);
         }
         L = L + environment_value(l, dir) * w;
@@ -1004,9 +1025,210 @@ F3 sun_radiance(const SceneL& S, F3 dir, float pdf_bsdf, bool use_mis) {
     }
     return L;
 }
+// ---------------------------------------------------------------- height fog (pathtrace.wgsl:560-779)
+float fog_sigma(const SceneL& S, F3 p) {  // :562-565
+    return S.fog.density * std::exp(std::fmin(0.0f, -(p.y - S.fog.height) * S.fog.falloff));
+}
+float fog_segment_tau(const SceneL& S, F3 ro, F3 rd, float begin, float end) {  // :569-578
+    const float distance = std::fmax(end - begin, 0.0f);
+    if (distance == 0.0f) return 0.0f;
+    const float sigma = fog_sigma(S, ro + rd * begin);
+    const float mid_height = ro.y + rd.y * ((begin + end) * 0.5f);
+    if (mid_height <= S.fog.height || S.fog.falloff == 0.0f) return S.fog.density * distance;
+    const float k = rd.y * S.fog.falloff;
+    const float x = k * distance;
+    if (std::fabs(x) < 0.001f) return sigma * distance * (1.0f - x * 0.5f + x * x / 6.0f);
+    return std::fmax(0.0f, (sigma - fog_sigma(S, ro + rd * end)) / k);
+}
+float fog_split(const SceneL& S, F3 ro, F3 rd, float distance) {  // :580-583
+    if (std::fabs(rd.y) < 1e-8f) return distance;
+    return clampv((S.fog.height - ro.y) / rd.y, 0.0f, distance);
+}
+float fog_optical_depth(const SceneL& S, F3 ro, F3 rd, float distance) {  // :585-588
+    const float split = fog_split(S, ro, rd, distance);
+    return fog_segment_tau(S, ro, rd, 0.0f, split) + fog_segment_tau(S, ro, rd, split, distance);
+}
+inline float fog_transmittance(const SceneL& S, F3 ro, F3 rd, float dist) {  // :590-592
+    return std::exp(-fog_optical_depth(S, ro, rd, dist));
+}
+float fog_invert_segment(const SceneL& S, F3 ro, F3 rd, float begin, float end, float tau) {  // :594-604
+    const float sigma = fog_sigma(S, ro + rd * begin);
+    const float k = rd.y * S.fog.falloff;
+    if (ro.y + rd.y * ((begin + end) * 0.5f) <= S.fog.height || std::fabs(k * (end - begin)) < 0.0001f)
+        return std::fmin(end, begin + tau / std::fmax(sigma, 1e-30f));
+    if (sigma > 1e-20f) return clampv(begin - std::log(std::fmax(1e-7f, 1.0f - tau * k / sigma)) / k, begin, end);
+    const float remaining = std::fmax(0.0f, fog_segment_tau(S, ro, rd, begin, end) - tau);
+    const float end_sigma = fog_sigma(S, ro + rd * end);
+    return clampv(end + std::log(std::fmax(1e-7f, 1.0f + remaining * k / std::fmax(end_sigma, 1e-30f))) / (-k), begin,
+                  end);
+}
+float fog_free_flight(const SceneL& S, F3 ro, F3 rd, float t_end, uint32_t* rng) {  // :606-615
+    if (S.fog.density <= 0.0f) return t_end;
+    const float tau = -std::log(std::fmax(1.0f - rand01(rng), 1e-7f));
+    const float split = fog_split(S, ro, rd, t_end);
+    const float first = fog_segment_tau(S, ro, rd, 0.0f, split);
+    const float total = first + fog_segment_tau(S, ro, rd, split, t_end);
+    if (tau >= total) return t_end;
+    if (tau < first) return fog_invert_segment(S, ro, rd, 0.0f, split, tau);
+    return fog_invert_segment(S, ro, rd, split, t_end, tau - first);
+}
+float volume_coordinate(uint32_t sample_index, uint32_t dimension, uint32_t stream) {  // :43-58
+    static const uint32_t bases[6] = {2u, 3u, 5u, 7u, 11u, 13u};
+    const uint32_t base = bases[dimension];
+    uint32_t index = sample_index + 1u;
+    float inverse = 1.0f / float(base);
+    float value = 0.0f;
+    while (index != 0u) {
+        value += float(index % base) * inverse;
+        index /= base;
+        inverse /= float(base);
+    }
+    const float shift = float(pcg(stream ^ (dimension * 0x9e3779b9u)) >> 8u) / 16777216.0f;
+    const float x = value + shift;
+    return x - std::floor(x);
+}
+float hg_phase(float cos_theta, float g) {  // :717-720
+    const float d = 1.0f + g * g - 2.0f * g * cos_theta;
+    return (1.0f - g * g) / (4.0f * PI_F * d * std::sqrt(d));
+}
+F3 hg_sample(F3 dir, float g, float r1, float r2) {  // :722-731
+    float cos_t = 1.0f - 2.0f * r1;
+    if (std::fabs(g) > 1e-3f) {
+        const float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * r1);
+        cos_t = (1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    const float sin_t = std::sqrt(std::fmax(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = 2.0f * PI_F * r2;
+    F3 t, bt;
+    tangent_frame(dir, &t, &bt);
+    return frame_mul(t, bt, dir, v3(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t));
+}
+float fog_segment_weight(const LightL& l, F3 ro, F3 rd, float end) {  // :621-633, area lights
+    const float along = clampv(dot(l.a - ro, rd), 0.0f, end);
+    const F3 delta = l.a - (ro + rd * along);
+    return std::fmax(std::fmax(l.color.x, l.color.y), l.color.z) * l.area / (dot(delta, delta) + l.area);
+}
+// fog_segment_direct (:635-715): the emitters' light scattered toward [0, end) of the ray.
+F3 fog_segment_direct(const SceneL& S, F3 ro, F3 rd, float end, uint32_t* rng, uint32_t point_index,
+                      uint32_t point_stream, bool primary, Count* c) {
+    if (end <= 0.0f || std::fmax(std::fmax(S.fog.color[0], S.fog.color[1]), S.fog.color[2]) <= 0.0f)
+        return v3(0.0f, 0.0f, 0.0f);
+    const float tau_end = fog_optical_depth(S, ro, rd, end);
+    if (tau_end <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    float total = 0.0f;
+    for (uint32_t i = 0; i < S.light_count; ++i)
+        if (S.lights[i].kind == SYF_LIGHT_AREA) total += fog_segment_weight(S.lights[i], ro, rd, end);
+    F3 result = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < S.light_count; ++i) {
+        const LightL& l = S.lights[i];
+        if (l.kind != SYF_LIGHT_AREA) continue;
+        const float probability = volume_light_probability(fog_segment_weight(l, ro, rd, end), total);
+        const uint32_t stream = pcg(point_stream ^ (i * 0x85ebca6bu));
+        float selection = rand01(rng);
+        if (primary) selection = volume_coordinate(point_index, 5u, stream);
+        if (probability <= 0.0f || selection >= probability) continue;
+        float u1 = rand01(rng);
+        float u2 = rand01(rng);
+        if (primary) {
+            u1 = volume_coordinate(point_index, 0u, stream);
+            u2 = volume_coordinate(point_index, 1u, stream);
+        }
+        const F3 q = l.a + l.b * (u1 - 0.5f) + l.c * (u2 - 0.5f);  // sample_emitter_point
+        const float tau_begin = fog_optical_depth(S, ro, rd, 0.0f);
+        const float tau_finish = fog_optical_depth(S, ro, rd, end);
+        const float tau_span = std::fmax(0.0f, tau_finish - tau_begin);
+        if (tau_span <= 0.0f) continue;
+        float conditional_mass = 1.0f - std::exp(-tau_span);
+        if (tau_span < 0.001f) conditional_mass = tau_span * (1.0f - tau_span * 0.5f + tau_span * tau_span / 6.0f);
+        const float interval_mass = std::exp(-tau_begin) * conditional_mass;
+        if (interval_mass < 1e-30f) continue;
+        const F3 delta = q - ro;
+        const float projection = dot(delta, rd);
+        const float perpendicular = length(delta - rd * projection);
+        const float theta_a = std::atan2(0.0f - projection, std::fmax(perpendicular, 1e-6f));
+        const float theta_b = std::atan2(end - projection, std::fmax(perpendicular, 1e-6f));
+        const float span = theta_b - theta_a;
+        const bool regular = perpendicular > 1e-4f && span > 1e-5f;
+        float xi = rand01(rng);
+        float technique = rand01(rng);
+        if (primary) {
+            xi = volume_coordinate(point_index, 3u, stream);
+            technique = volume_coordinate(point_index, 4u, stream);
+        }
+        float t = mixf(0.0f, end, xi);
+        if (technique < 0.5f) {  // equiangular
+            if (regular) t = clampv(projection + perpendicular * std::tan(mixf(theta_a, theta_b, xi)), 0.0f, end);
+        } else {  // distance, by the inverse optical depth
+            const float tau = tau_begin - std::log(std::fmax(1.0f - xi * conditional_mass, 1e-7f));
+            const float split = fog_split(S, ro, rd, end);
+            const float first = fog_segment_tau(S, ro, rd, 0.0f, split);
+            t = tau < first ? fog_invert_segment(S, ro, rd, 0.0f, split, tau)
+                            : fog_invert_segment(S, ro, rd, split, end, tau - first);
+        }
+        const F3 p = ro + rd * t;
+        const float transmittance = std::exp(-fog_optical_depth(S, ro, rd, t));
+        const float sigma = fog_sigma(S, p);
+        float angular_pdf = 1.0f / (end - 0.0f);
+        if (regular)
+            angular_pdf = perpendicular / (span * (perpendicular * perpendicular + (t - projection) * (t - projection)));
+        const float distance_pdf = 0.5f * (angular_pdf + sigma * transmittance / interval_mass);
+        const F3 to_light = q - p;
+        const float distance2 = dot(to_light, to_light);
+        if (distance2 < 1e-8f) continue;
+        const float distance = std::sqrt(distance2);
+        const F3 wi = v3(to_light.x / distance, to_light.y / distance, to_light.z / distance);
+        const float cosine = dot(l.n, neg(wi));
+        if (cosine <= 0.0f) continue;
+        if (!visible(S, p, wi, std::fmax(0.0f, distance - std::fmax(1e-5f, distance * 1e-6f)), c)) continue;
+        const float tr = transmittance * std::exp(-fog_optical_depth(S, p, wi, distance));
+        const float area_pdf = distance2 / std::fmax(l.area * cosine, 1e-20f);  // area_direction_pdf
+        const float k = sigma * tr * hg_phase(dot(rd, wi), S.fog.anisotropy) * 1.0f * 1.0f /
+                        (probability * distance_pdf * area_pdf);
+        result = result + v3(l.color.x * k, l.color.y * k, l.color.z * k);
+    }
+    return v3(result.x * S.fog.color[0], result.y * S.fog.color[1], result.z * S.fog.color[2]);
+}
+// fog_direct_light (:733-778): environment and suns at a scattering event, phase-weighted.
+F3 fog_direct_light(const SceneL& S, F3 p, F3 rd, float area_total, bool last, uint32_t* rng, Count* c) {
+    const float g = S.fog.anisotropy;
+    F3 L = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < S.light_count; ++i) {
+        const LightL& l = S.lights[i];
+        if (l.kind == SYF_LIGHT_ENV) {
+            const float mx = std::fmax(std::fmax(std::fmax(l.color.x, l.color.y), l.color.z),
+                                       std::fmax(std::fmax(std::fmax(l.horizon.x, l.horizon.y), l.horizon.z),
+                                                 std::fmax(std::fmax(l.ground.x, l.ground.y), l.ground.z)));
+            if (mx <= 0.0f) continue;
+            const float probability = environment_probability(l, area_total);
+            if (rand01(rng) >= probability) continue;
+            const float z = 1.0f - 2.0f * rand01(rng);
+            const float phi = 2.0f * PI_F * rand01(rng);
+            const float radius = std::sqrt(std::fmax(0.0f, 1.0f - z * z));
+            const F3 dir = v3(radius * std::cos(phi), z, radius * std::sin(phi));
+            if (!visible(S, p, dir, S.t_end, c)) continue;
+            const float tr = fog_transmittance(S, p, dir, S.t_end);
+            const float ph = hg_phase(dot(rd, dir), g);
+            const float pl = probability / (4.0f * PI_F);
+            const float w = last ? 1.0f : mis_power(pl, ph);
+            const F3 e = environment_value(l, dir);
+            L = L + v3(e.x * ph * tr * w / pl, e.y * ph * tr * w / pl, e.z * ph * tr * w / pl);
+        } else if (l.kind == SYF_LIGHT_SUN) {
+            const float r1 = rand01(rng);
+            const float r2 = rand01(rng);
+            const F3 dir = cone_sample(l.a, l.cos_max, r1, r2);
+            if (!visible(S, p, dir, S.t_end, c)) continue;
+            const float tr = fog_transmittance(S, p, dir, S.t_end);
+            const float ph = hg_phase(dot(rd, dir), g);
+            const float pl = sun_pdf(l.cos_max);
+            const float w = last ? 1.0f : mis_power(pl, ph);
+            L = L + v3(l.color.x * ph * tr * w, l.color.y * ph * tr * w, l.color.z * ph * tr * w);
+        }
+    }
+    return L;
+}
 // direct_light (pathtrace.wgsl:471-558): environment, sun and area branches.
 F3 direct_light(const SceneL& S, F3 p, const Surf& s, F3 gn, F3 v, float p_spec, float area_total, bool last,
-                uint32_t* rng, Count* c) {
+                bool in_fog, uint32_t* rng, Count* c) {
     F3 L = v3(0.0f, 0.0f, 0.0f);
     uint32_t selected = NO_LIGHT;
     float selected_weight = 0.0f;
@@ -1039,12 +1261,13 @@ F3 direct_light(const SceneL& S, F3 p, const Surf& s, F3 gn, F3 v, float p_spec,
             float nl = std::fmax(dot(s.n, dir), 0.0f);
             if (dot(gn, dir) <= 0.0f || nl <= 0.0f) continue;
             if (!visible(S, p, dir, S.t_end, c)) continue;
+            const float tr = in_fog ? fog_transmittance(S, p, dir, S.t_end) : 1.0f;
             float pl = prob * nl / PI_F;
             float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
             float w = last ? 1.0f : mis_power(pl, pb);
             F3 f = brdf_eval(s, v, dir);
             F3 e = environment_value(l, dir);
-            L = L + v3(f.x * nl * e.x * 1.0f * w / pl, f.y * nl * e.y * 1.0f * w / pl, f.z * nl * e.z * 1.0f * w / pl);
+            L = L + v3(f.x * nl * e.x * tr * w / pl, f.y * nl * e.y * tr * w / pl, f.z * nl * e.z * tr * w / pl);
         } else if (l.kind == SYF_LIGHT_SUN) {
             // colour = irradiance: the disk's radiance is colour * sun_pdf
             const float r1 = rand01(rng);
@@ -1053,11 +1276,12 @@ F3 direct_light(const SceneL& S, F3 p, const Surf& s, F3 gn, F3 v, float p_spec,
             const float nl = dot(s.n, dir);
             if (nl <= 0.0f || dot(gn, dir) <= 0.0f) continue;
             if (!visible(S, p, dir, S.t_end, c)) continue;
+            const float tr = in_fog ? fog_transmittance(S, p, dir, S.t_end) : 1.0f;
             const float pl = sun_pdf(l.cos_max);
             const float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
             const float w = last ? 1.0f : mis_power(pl, pb);
             const F3 f = brdf_eval(s, v, dir);
-            L = L + v3(f.x * nl * l.color.x * w, f.y * nl * l.color.y * w, f.z * nl * l.color.z * w);
+            L = L + v3(f.x * nl * l.color.x * w * tr, f.y * nl * l.color.y * w * tr, f.z * nl * l.color.z * w * tr);
         } else if (l.kind == SYF_LIGHT_AREA) {
             if (i != selected) continue;
             float u1 = rand01(rng);
@@ -1073,26 +1297,31 @@ F3 direct_light(const SceneL& S, F3 p, const Surf& s, F3 gn, F3 v, float p_spec,
             if (nl <= 0.0f || cos_l <= 0.0f || dot(gn, dir) <= 0.0f) continue;
             float end = std::fmax(0.0f, dist - std::fmax(1e-5f, dist * 1e-6f));  // shadow_endpoint
             if (!visible(S, p, dir, end, c)) continue;
+            const float tr = in_fog ? fog_transmittance(S, p, dir, dist) : 1.0f;
             float pl = (selected_weight / area_total) * (dist2 / std::fmax(l.area * cos_l, 1e-20f));
             float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
             float w = last ? 1.0f : mis_power(pl, pb);
             F3 f = brdf_eval(s, v, dir);
-            L = L + v3(f.x * nl * l.color.x * 1.0f * w / pl, f.y * nl * l.color.y * 1.0f * w / pl,
-                       f.z * nl * l.color.z * 1.0f * w / pl);
+            L = L + v3(f.x * nl * l.color.x * tr * w / pl, f.y * nl * l.color.y * tr * w / pl,
+                       f.z * nl * l.color.z * tr * w / pl);
         }
     }
     return L;
 }
 
 // radiance (pathtrace.wgsl:802-1201) for the subset; returns L and the first-hit planes.
-F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* first_depth, float* first_aux,
-            float* first_tprev, Count* c) {
+// volume_out: the sample's volume radiance (the camera segment's in-scattering, or all of L when
+// the camera ray scattered in the fog; pathtrace.wgsl:1197-1198).
+F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t sample, uint32_t* rng, float* first_depth,
+            float* first_aux, float* first_tprev, F3* volume_out, Count* c) {
     F3 ro, rd;
     camera_ray(S.cam, S.width, S.height, px, py, rng, &ro, &rd);
     F3 throughput = v3(1.0f, 1.0f, 1.0f), L = v3(0.0f, 0.0f, 0.0f);
     float prev_pdf = 0.0f, prev_total = 0.0f;
-    bool prev_from_bsdf = false;
-    F3 prev_n = v3(0.0f, 0.0f, 0.0f);
+    bool prev_from_bsdf = false, prev_from_volume = false, prev_volume_nee = false, volume_path = false;
+    F3 prev_n = v3(0.0f, 0.0f, 0.0f), primary_volume = v3(0.0f, 0.0f, 0.0f);
+    const uint32_t point_stream = pcg(pcg(px ^ S.seed) ^ pcg(py ^ 0x27d4eb2du));  // :822
+    const bool in_fog = S.fog.density > 0.0f;
     for (uint32_t bounce = 0; bounce <= S.max_bounces;) {
         HitL hl = trace_scene(S, ro, rd, S.t_end, c);
         const Hit<float>& h = hl.h;
@@ -1112,9 +1341,40 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
                 first_aux[0] = gn.x; first_aux[1] = gn.y; first_aux[2] = gn.z; first_aux[3] = float(h.steps);
             }
         }
+        if (in_fog) {  // :980-1010
+            const float t_end = h.hit ? h.t : S.t_end;
+            const F3 dv = mul3(throughput, fog_segment_direct(S, ro, rd, t_end, rng, sample, point_stream, bounce == 0, c));
+            L = L + dv;
+            if (bounce == 0) primary_volume = primary_volume + dv;
+            const float t_s = fog_free_flight(S, ro, rd, t_end, rng);
+            if (t_s < t_end) {
+                if (bounce == 0) volume_path = true;
+                const F3 p_s = ro + rd * t_s;
+                if (bounce == 0 && !h.hit) *first_depth = t_s;
+                const float vat = area_total_weight(S, p_s, v3(0.0f, 0.0f, 0.0f));
+                const bool last = bounce == S.max_bounces;
+                const F3 fc = f3(S.fog.color);
+                L = L + mul3(mul3(throughput, fc), fog_direct_light(S, p_s, rd, vat, last, rng, c));
+                throughput = mul3(throughput, fc);
+                if (last) break;
+                const float r1 = rand01(rng);
+                const float r2 = rand01(rng);
+                const F3 nd = hg_sample(rd, S.fog.anisotropy, r1, r2);
+                prev_pdf = hg_phase(dot(rd, nd), S.fog.anisotropy);
+                prev_from_bsdf = true;
+                prev_from_volume = true;
+                prev_volume_nee = true;
+                prev_n = v3(0.0f, 0.0f, 0.0f);
+                prev_total = vat;
+                ro = p_s;
+                rd = nd;
+                ++bounce;
+                continue;
+            }
+        }
         if (!h.hit) {
             if (!h.exhausted) {
-                F3 e = environment_emission(S, rd, prev_pdf, prev_from_bsdf, prev_n, prev_total);
+                F3 e = environment_emission(S, rd, prev_pdf, prev_from_bsdf, prev_n, prev_total, prev_from_volume);
                 L = L + mul3(throughput, e);
                 L = L + mul3(throughput, sun_radiance(S, rd, prev_pdf, prev_from_bsdf));
             }
@@ -1122,8 +1382,10 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
         }
         F3 p = ro + rd * h.t;
         if (!ground && hl.light != NO_LIGHT) {
-            F3 e = area_emission(S, ro, rd, h.t, prev_pdf, prev_from_bsdf, prev_n, prev_total);
-            L = L + mul3(throughput, e);
+            if (!prev_volume_nee) {  // a fog segment already counted the emitters by its line integral
+                F3 e = area_emission(S, ro, rd, h.t, prev_pdf, prev_from_bsdf, prev_n, prev_total, prev_from_volume);
+                L = L + mul3(throughput, e);
+            }
             break;
         }
         F3 n = gn;
@@ -1148,7 +1410,7 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
         float p_spec = spec_probability(s, nv);
         float area_total = area_total_weight(S, p_off, s.n);
         bool last = bounce == S.max_bounces;
-        F3 dl = direct_light(S, p_off, s, n, v, p_spec, area_total, last, rng, c);
+        F3 dl = direct_light(S, p_off, s, n, v, p_spec, area_total, last, in_fog, rng, c);
         L = L + mul3(throughput, dl);
         if (last) break;
         float r0 = rand01(rng), r1 = rand01(rng), r2 = rand01(rng);
@@ -1169,6 +1431,8 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
         prev_from_bsdf = true;
         prev_n = s.n;
         prev_total = area_total;
+        prev_from_volume = false;
+        prev_volume_nee = false;
         if (bounce >= 2) {
             float q = clampv(std::fmax(throughput.x, std::fmax(throughput.y, throughput.z)), 0.05f, 0.95f);
             if (rand01(rng) > q) break;
@@ -1178,6 +1442,7 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t* rng, float* fir
         rd = l;
         ++bounce;
     }
+    *volume_out = volume_path ? L : primary_volume;
     return L;
 }
 
@@ -1597,6 +1862,36 @@ int orc_control_trace(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t 
     return 0;
 }
 
+// The fog's pieces for tests: optical depths along rays (ro3, rd3: n x 3) to dist[i]; free-flight
+// distances along one ray with n independent PCG streams; the rotated Halton coordinate; the
+// Henyey-Greenstein phase and its samples.
+void orc_fog_optical_depth(const syf_job_desc* d, const float* ro3, const float* rd3, const float* dist, size_t n,
+                           float* out) {
+    const SceneL S = lower_scene(*d);
+    for (size_t i = 0; i < n; ++i)
+        out[i] = fog_optical_depth(S, v3(ro3[3 * i], ro3[3 * i + 1], ro3[3 * i + 2]),
+                                   v3(rd3[3 * i], rd3[3 * i + 1], rd3[3 * i + 2]), dist[i]);
+}
+void orc_fog_free_flight(const syf_job_desc* d, const float* ro, const float* rd, float t_end, size_t n, float* out) {
+    const SceneL S = lower_scene(*d);
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t rng = pcg(uint32_t(i) * 2654435761u + 12345u);
+        out[i] = fog_free_flight(S, v3(ro[0], ro[1], ro[2]), v3(rd[0], rd[1], rd[2]), t_end, &rng);
+    }
+}
+float orc_volume_coordinate(uint32_t sample_index, uint32_t dimension, uint32_t stream) {
+    return dimension < 6u ? volume_coordinate(sample_index, dimension, stream) : -1.0f;
+}
+float orc_hg_phase(float cos_theta, float g) { return hg_phase(cos_theta, g); }
+void orc_hg_sample(float g, const float* dir, const float* r12, size_t n, float* out3) {
+    for (size_t i = 0; i < n; ++i) {
+        const F3 v = hg_sample(v3(dir[0], dir[1], dir[2]), g, r12[2 * i], r12[2 * i + 1]);
+        out3[3 * i] = v.x;
+        out3[3 * i + 1] = v.y;
+        out3[3 * i + 2] = v.z;
+    }
+}
+
 // palette_sample (common.wgsl:336-366) of a coloring's ramp at n values, rgb out; stop
 // positions resolved as lower_scene does (Ramp::position, palette.rs:106-115).
 void orc_ramp_sample(const syf_coloring* c, const float* values, size_t n, float* rgb) {
@@ -1619,8 +1914,10 @@ void orc_ramp_sample(const syf_coloring* c, const float* values, size_t n, float
 // and depth64: that sample's first surface hit refined in FP64 (refine64 between the march's
 // last two positions, as the primary trace does), -1 on a miss or an emitter hit.
 // Outputs are rectangle-local, row-major: accum4, depth, aux4, depth64.
+// session4 (optional): the VOLUME, MOMENTS and SURFACE planes (3 x w x h float4, pathtrace.wgsl:
+// 1243-1266), accumulated from their current values.
 int orc_render(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sample_start,
-               uint32_t sample_count, float* accum4, float* depth, float* aux4, double* depth64) {
+               uint32_t sample_count, float* accum4, float* depth, float* aux4, double* depth64, float* session4) {
     SceneL S = lower_scene(*d);
     const Fractal<double> F64 = lower_fractal<double>(d->fractal);
     const double hit_cone = double(S.pixel_cone) * double(S.pixel_tol);
@@ -1636,13 +1933,32 @@ int orc_render(const syf_job_desc* d, uint32_t x0, uint32_t y0, uint32_t w, uint
             for (uint32_t s = 0; s < sample_count; ++s) {
                 uint32_t rng = rng_init(px, py, sample_start + s, S.seed);
                 float dd = -1.0f, aa[4] = {0.0f, 0.0f, 0.0f, -1.0f}, tp = -1.0f;
-                F3 r = radiance(S, px, py, &rng, &dd, aa, &tp, &c);
+                F3 vol;
+                F3 r = radiance(S, px, py, sample_start + s, &rng, &dd, aa, &tp, &vol, &c);
                 bool ok = r.x == r.x && r.y == r.y && r.z == r.z && std::fabs(r.x) < 1e30f && std::fabs(r.y) < 1e30f &&
                           std::fabs(r.z) < 1e30f;
                 acc[0] += ok ? r.x : 0.0f;
                 acc[1] += ok ? r.y : 0.0f;
                 acc[2] += ok ? r.z : 0.0f;
                 acc[3] += 1.0f;
+                if (session4) {  // Welford update of the volume, its moments, and the surface
+                    float* V = session4 + 4 * i;
+                    float* M = session4 + 4 * (size_t(w) * h + i);
+                    float* Sf = session4 + 4 * (2 * size_t(w) * h + i);
+                    const F3 v = ok ? vol : v3(0.0f, 0.0f, 0.0f);
+                    const F3 surface = ok ? r - vol : v3(0.0f, 0.0f, 0.0f);
+                    const float n = std::fmax(V[3], 1.0f);
+                    const F3 old_mean = v3(V[0] / n, V[1] / n, V[2] / n);
+                    const F3 delta = v - old_mean;
+                    const float n1 = V[3] + 1.0f;
+                    const F3 new_mean = old_mean + v3(delta.x / n1, delta.y / n1, delta.z / n1);
+                    M[0] += std::fmax(0.0f, delta.x * (v.x - new_mean.x));
+                    M[1] += std::fmax(0.0f, delta.y * (v.y - new_mean.y));
+                    M[2] += std::fmax(0.0f, delta.z * (v.z - new_mean.z));
+                    M[3] += 1.0f;
+                    V[0] += v.x; V[1] += v.y; V[2] += v.z; V[3] += 1.0f;
+                    Sf[0] += surface.x; Sf[1] += surface.y; Sf[2] += surface.z; Sf[3] += 1.0f;
+                }
                 if (s == 0) {
                     fd = dd;
                     for (int k = 0; k < 4; ++k) fa[k] = aa[k];

@@ -173,6 +173,16 @@ class SyfGround(C.Structure):
     ]
 
 
+class SyfFog(C.Structure):
+    _fields_ = [
+        ("density", C.c_float),
+        ("color", F3),
+        ("height", C.c_float),
+        ("falloff", C.c_float),
+        ("anisotropy", C.c_float),
+    ]
+
+
 class SyfJobDesc(C.Structure):
     _fields_ = [
         ("fractal", SyfFractal),
@@ -192,6 +202,7 @@ class SyfJobDesc(C.Structure):
         ("coloring", SyfColoring),
         ("finish_count", C.c_uint32),
         ("finishes", SyfFinish * MAX_FINISHES),
+        ("fog", SyfFog),
     ]
 
 

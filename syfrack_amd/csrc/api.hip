@@ -179,6 +179,25 @@ int lower_shading(const syf_job_desc& d, bool graph, JobP& P) {
     S.ground_enabled = d.ground.enabled ? 1u : 0u;
     if (d.ground.enabled && !std::isfinite(d.ground.height))
         return fail(SYF_E_INVALID, "ground.height must be finite");
+    // the fog's bounds (DENSITY, HEIGHT, FALLOFF, ANISOTROPY: atmosphere.rs:115-118)
+    const syf_fog& fog = d.fog;
+    if (!(fog.density >= 0.0f && fog.density <= 10.0f))
+        return fail(SYF_E_INVALID, "fog.density must be between 0 and 10, got %g", double(fog.density));
+    if (fog.density > 0.0f) {
+        if (!(fog.height >= -100.0f && fog.height <= 100.0f))
+            return fail(SYF_E_INVALID, "fog.height must be between -100 and 100, got %g", double(fog.height));
+        if (!(fog.falloff >= 0.0f && fog.falloff <= 10.0f))
+            return fail(SYF_E_INVALID, "fog.falloff must be between 0 and 10, got %g", double(fog.falloff));
+        if (!(fog.anisotropy >= -0.99f && fog.anisotropy <= 0.99f))
+            return fail(SYF_E_INVALID, "fog.anisotropy must be between -0.99 and 0.99, got %g", double(fog.anisotropy));
+        if (!finite3(fog.color) || !(fog.color[0] >= 0.0f && fog.color[1] >= 0.0f && fog.color[2] >= 0.0f))
+            return fail(SYF_E_INVALID, "fog.color must be finite and non-negative");
+        S.fog_density = fog.density;
+        S.fog_height = fog.height;
+        S.fog_falloff = fog.falloff;
+        S.fog_g = fog.anisotropy;
+        for (int k = 0; k < 3; ++k) S.fog_color[k] = fog.color[k];
+    }
     if (d.ground.enabled && d.ground.finish > SYF_MAX_FINISHES)  // MATERIAL_INDEX bounds
         return fail(SYF_E_INVALID, "ground.finish must be 0..%d, got %u", SYF_MAX_FINISHES, d.ground.finish);
     S.ground_height = d.ground.height;
@@ -418,6 +437,11 @@ int syf::job_alloc(syf_ctx* ctx, const syf_job_desc& dref, const JobP& P, syf_jo
             return hip_fail(e, "hipMalloc plane");
         }
     }
+    // a job with fog accumulates the session planes; without, they follow from ACCUM (syf_readback)
+    if (P.sh.fog_density > 0.0f && (e = hipMalloc(&j->vplanes, 3 * j->pixels * 16)) != hipSuccess) {
+        syf_job_destroy(j);
+        return hip_fail(e, "hipMalloc session planes");
+    }
     // the primary trace's counters and the work counters share one allocation (one clear per trace)
     if ((e = hipMalloc(&j->counters, COUNTER_BYTES)) != hipSuccess ||
         (e = hipMalloc(&j->hitq, (HITQ_FIELDS * j->hitq_cap + j->hitq_cap / 64) * sizeof(uint32_t))) != hipSuccess ||
@@ -451,6 +475,7 @@ int syf_job_destroy(syf_job* j) {
     if (j->stream) (void)hipStreamSynchronize(j->stream);
     for (void*& p : j->plane)
         if (p) (void)hipFree(p);
+    if (j->vplanes) (void)hipFree(j->vplanes);
     if (j->hitq) (void)hipFree(j->hitq);
     if (j->counters) (void)hipFree(j->counters);  // holds ctr too
     if (j->ev0) (void)hipEventDestroy(j->ev0);
@@ -471,6 +496,7 @@ int syf_job_clear(syf_job* j) {
     hipError_t e = hipSuccess;
     for (uint32_t p = 0; p <= SYF_PLANE_STEPS && e == hipSuccess; ++p)
         e = hipMemsetAsync(j->plane[p], 0, plane_bytes(p, j->pixels), j->stream);
+    if (e == hipSuccess && j->vplanes) e = hipMemsetAsync(j->vplanes, 0, 3 * j->pixels * 16, j->stream);
     if (e == hipSuccess) e = hipMemsetAsync(j->ctr, 0, sizeof(unsigned long long) * CTR_COUNT, j->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
     return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_job_clear");
@@ -591,11 +617,14 @@ int syf_readback(syf_job* j, uint32_t plane, void* host, size_t bytes) {
     size_t need = plane_bytes(plane, j->pixels);
     if (bytes != need) return fail(SYF_E_INVALID, "bytes must be %zu for plane %u, got %zu", need, plane, bytes);
     if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
-    const uint32_t src = plane >= SYF_PLANE_VOLUME ? SYF_PLANE_ACCUM : plane;
-    hipError_t e = hipMemcpyAsync(host, j->plane[src], bytes, hipMemcpyDeviceToHost, j->stream);
+    const void* src = plane < SYF_PLANE_VOLUME ? j->plane[plane]
+                      : j->vplanes ? static_cast<const char*>(j->vplanes) + (plane - SYF_PLANE_VOLUME) * j->pixels * 16
+                                   : j->plane[SYF_PLANE_ACCUM];
+    hipError_t e = hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, j->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
     if (e != hipSuccess) return hip_fail(e, "syf_readback");
-    // The session planes of pathtrace.wgsl:1243-1266 for this path, which has no participating
+    if (j->vplanes) return SYF_OK;
+    // The session planes of pathtrace.wgsl:1243-1266 for a job without a participating
     // medium: every sample's volume estimate is zero, so VOLUME holds (0, 0, 0, n), its Welford
     // moments max(0, delta * (v - mean)) stay (0, 0, 0, n), and SURFACE adds r - 0 = r where
     // ACCUM adds r, in the same order: the same sums bit for bit.

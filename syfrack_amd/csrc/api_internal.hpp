@@ -26,6 +26,7 @@ struct syf_job {
     size_t pixels = 0;
     hipStream_t stream = nullptr;
     void* plane[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* vplanes = nullptr;  // a job with fog: VOLUME, MOMENTS, SURFACE (float4 each), one allocation
     unsigned long long* ctr = nullptr;  // work counters: CTR_COUNT words inside the counters allocation
     uint32_t* hitq = nullptr;      // hit queue of the primary trace: HITQ_FIELDS fields x hitq_cap entries, then done flags
     size_t hitq_cap = 0;

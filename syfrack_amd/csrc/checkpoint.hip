@@ -119,12 +119,15 @@ int syf_job_fingerprint(syf_job* j, uint64_t* out) {
 int syf_upload(syf_job* j, uint32_t plane, const void* host, size_t bytes) {
     clear_error();
     if (!j || !host) return fail(SYF_E_INVALID, "job and host must not be null");
-    if (plane > SYF_PLANE_STEPS)
-        return fail(SYF_E_INVALID, "plane: %u cannot be uploaded (the session planes follow from ACCUM)", plane);
+    if (plane > SYF_PLANE_SURFACE || (plane >= SYF_PLANE_VOLUME && !j->vplanes))
+        return fail(SYF_E_INVALID, "plane: %u cannot be uploaded (without fog the session planes follow from ACCUM)",
+                    plane);
     const size_t need = plane_bytes(plane, j->pixels);
     if (bytes != need) return fail(SYF_E_INVALID, "bytes must be %zu for plane %u, got %zu", need, plane, bytes);
     if (hipError_t de = hipSetDevice(j->ctx->device); de != hipSuccess) return hip_fail(de, "hipSetDevice");
-    hipError_t e = hipMemcpyAsync(j->plane[plane], host, bytes, hipMemcpyHostToDevice, j->stream);
+    void* dst = plane < SYF_PLANE_VOLUME ? j->plane[plane]
+                                         : static_cast<char*>(j->vplanes) + (plane - SYF_PLANE_VOLUME) * j->pixels * 16;
+    hipError_t e = hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, j->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
     return e == hipSuccess ? SYF_OK : hip_fail(e, "syf_upload");
 }
@@ -207,6 +210,15 @@ int syf_checkpoint_load(syf_job* j, const char* dir_c) {
     if ((rc = syf_upload(j, SYF_PLANE_ACCUM, accum.data(), accum.size() * 4)) != SYF_OK) return rc;
     if ((rc = syf_upload(j, SYF_PLANE_DEPTH, depth.data(), depth.size() * 4)) != SYF_OK) return rc;
     if ((rc = syf_upload(j, SYF_PLANE_AUX, aux.data(), aux.size() * 4)) != SYF_OK) return rc;
+    // a job with fog resumes its session planes (volume.bin, volume-moments.bin, surface.bin)
+    if (j->vplanes) {
+        const char* names[] = {"volume.bin", "volume-moments.bin", "surface.bin"};
+        std::vector<float> v(4 * px);
+        for (uint32_t k = 0; k < 3; ++k) {
+            if ((rc = read_file(dir / names[k], v.data(), v.size() * 4)) != SYF_OK) return rc;
+            if ((rc = syf_upload(j, SYF_PLANE_VOLUME + k, v.data(), v.size() * 4)) != SYF_OK) return rc;
+        }
+    }
     // this path's own planes, when the checkpoint has them (a reference checkpoint does not)
     std::error_code ec;
     if (fs::exists(dir / "depth64.bin", ec)) {

@@ -95,9 +95,10 @@ hipError_t launch_refine_first_hits(uint32_t fam, const JobP* jobs, const BatchP
                                     const BatchItem* items, uint32_t n_items, unsigned long long* ctr,
                                     hipStream_t s);
 // P: the job's host copy (its family picks the kernel); dP: the same constants in device memory.
+// vol: the session planes (VOLUME, MOMENTS, SURFACE) of a job with fog, null otherwise.
 hipError_t launch_render_tile(const JobP& P, const JobP* dP, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                               uint32_t s0, uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
-                              unsigned long long* ctr, hipStream_t s);
+                              float4* vol, unsigned long long* ctr, hipStream_t s);
 
 hipError_t launch_de_probe(const FractalP& F, const float4* pts, uint32_t n, float4* out, float4* traps,
                            hipStream_t s);
@@ -113,6 +114,7 @@ enum : uint32_t {
     JIT_k_wf_intersect,
     JIT_k_wf_shade,
     JIT_k_wf_commit,
+    JIT_k_render_tile_fog,
     JIT_KERNELS
 };
 // Launches kernel `which` of the compiled topology `jit` (FractalP::gjit) with its arguments'

@@ -95,6 +95,9 @@ struct ShadeP {
     // the shading reads more of a surface hit's field sample than trap slot 0: the other traps,
     // the iteration count or the material (then the sample is evaluated in full at the hit)
     uint32_t full_sample;
+    // height fog (GpuAtmosphere.a, b): density 0 = none
+    float fog_density, fog_height, fog_falloff, fog_g;
+    float fog_color[3];
     FinishP finishes[SYF_MAX_FINISHES];
 };
 

@@ -39,12 +39,17 @@ struct WavefrontArena {
 // The one-launch form over a tile: four 8x8 pixel tiles per workgroup (one per wave).
 hipError_t launch_render_tile(const JobP& P, const JobP* dP, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                               uint32_t s0, uint32_t ns, float4* accum, float* depth, float4* aux, double* depth64,
-                              unsigned long long* ctr, hipStream_t s) {
+                              float4* vol, unsigned long long* ctr, hipStream_t s) {
     const uint32_t tiles_x = (w + 7u) / 8u, tiles = tiles_x * ((h + 7u) / 8u);
     const dim3 grid((tiles + 3u) / 4u), block(256);
     hipError_t e = hipSuccess;
-    SYF_DISPATCH(e, P.f, k_render_tile, grid, block, 0, s, dP, x0, y0, w, h, s0, ns, tiles_x, tiles, accum, depth, aux,
-                 depth64, ctr);
+    if (P.sh.fog_density > 0.0f) {
+        SYF_DISPATCH(e, P.f, k_render_tile_fog, grid, block, 0, s, dP, x0, y0, w, h, s0, ns, tiles_x, tiles, accum,
+                     depth, aux, depth64, vol, ctr);
+    } else {
+        SYF_DISPATCH(e, P.f, k_render_tile, grid, block, 0, s, dP, x0, y0, w, h, s0, ns, tiles_x, tiles, accum, depth,
+                     aux, depth64, vol, ctr);
+    }
     return e;
 }
 
@@ -91,7 +96,7 @@ hipError_t enqueue_wavefront_sample(syf_job* j, const WfP& W, int cu_count) {
     if (e != hipSuccess) return e;
     SYF_DISPATCH(e, P.f, k_wf_commit, gen_grid, block, 0, s, dP, W, S, (float4*)j->plane[SYF_PLANE_ACCUM],
                  (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
-                 (double*)j->plane[SYF_PLANE_DEPTH64], a->ctl, j->ctr);
+                 (double*)j->plane[SYF_PLANE_DEPTH64], (float4*)j->vplanes, a->ctl, j->ctr);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_wf_end_sample, dim3(1), dim3(64), 0, s, a->ctl);
     return hipGetLastError();
@@ -114,7 +119,7 @@ int syf_render_tile(syf_job* j, uint32_t x, uint32_t y, uint32_t w, uint32_t h, 
     if (e == hipSuccess)
         e = launch_render_tile(j->P, j->dP, x, y, w, h, sample_start, sample_count, (float4*)j->plane[SYF_PLANE_ACCUM],
                                (float*)j->plane[SYF_PLANE_DEPTH], (float4*)j->plane[SYF_PLANE_AUX],
-                               (double*)j->plane[SYF_PLANE_DEPTH64], j->ctr, j->stream);
+                               (double*)j->plane[SYF_PLANE_DEPTH64], (float4*)j->vplanes, j->ctr, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev1, j->stream);
     if (e == hipSuccess) e = hipEventRecord(j->ev2, j->stream);
     j->timed = true;

@@ -75,11 +75,13 @@ template <uint32_t FAM> struct Tracer {
 // accum += (r if finite, 1) (pathtrace.wgsl:1203-1276).  A call that starts at sample 0 also
 // writes the first-hit planes: depth, normal + steps, and the first surface hit refined in FP64
 // between the march's last two positions (refine64, as the primary trace), -1 otherwise.
-template <uint32_t FAM>
+// FOG: the job has fog (the session planes vol are accumulated too).
+template <uint32_t FAM, bool FOG>
 __device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_t py, uint32_t s0, uint32_t ns,
                                              float4* __restrict__ accum, float* __restrict__ depth,
-                                             float4* __restrict__ aux, double* __restrict__ depth64, Work& work,
-                                             uint32_t& segs, uint32_t& shadow, uint32_t& prim, uint32_t& hits) {
+                                             float4* __restrict__ aux, double* __restrict__ depth64,
+                                             float4* __restrict__ vol, Work& work, uint32_t& segs, uint32_t& shadow,
+                                             uint32_t& prim, uint32_t& hits) {
     const size_t idx = size_t(py) * P.width + px;
     Tracer<FAM> tr{P, work, shadow};
     float4 acc = accum[idx];
@@ -98,9 +100,10 @@ __device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_
                 ++hits;
                 if (s == 0u) tp0 = sh.t_prev;
             }
-            path_bounce(P, st, sh, fd, fa, [&](F3 p, float hh) __attribute__((always_inline)) { return tr.normal(p, hh); },
-                        [&](F3 o, F3 d, float te) __attribute__((always_inline)) { return tr.visible(o, d, te); },
-                        [&](F3 p) __attribute__((always_inline)) { return tr.sample(p); });
+            path_bounce<FOG>(P, st, sh, fd, fa,
+                             [&](F3 p, float hh) __attribute__((always_inline)) { return tr.normal(p, hh); },
+                             [&](F3 o, F3 d, float te) __attribute__((always_inline)) { return tr.visible(o, d, te); },
+                             [&](F3 p) __attribute__((always_inline)) { return tr.sample(p); });
         }
         const F3 r = st.L;
         const bool ok = r.x == r.x && r.y == r.y && r.z == r.z && fabsf(r.x) < 1e30f && fabsf(r.y) < 1e30f &&
@@ -109,6 +112,7 @@ __device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_
         acc.y += ok ? r.y : 0.0f;
         acc.z += ok ? r.z : 0.0f;
         acc.w += 1.0f;
+        if (FOG) accumulate_session(vol, size_t(P.width) * P.height, idx, r, st, ok);
         if (s == 0u) {
             fd0 = fd;
             fa0 = fa;
@@ -132,12 +136,12 @@ __device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_
     }
 }
 
-template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_render_tile(const JobP* __restrict__ job, uint32_t x0, uint32_t y0, uint32_t w,
-                                                     uint32_t h, uint32_t s0, uint32_t ns, uint32_t tiles_x,
-                                                     uint32_t tiles, float4* __restrict__ accum,
-                                                     float* __restrict__ depth, float4* __restrict__ aux,
-                                                     double* __restrict__ depth64, unsigned long long* __restrict__ ctr) {
+template <uint32_t FAM, bool FOG>
+__device__ __forceinline__ void render_tile(const JobP* __restrict__ job, uint32_t x0, uint32_t y0, uint32_t w,
+                                            uint32_t h, uint32_t s0, uint32_t ns, uint32_t tiles_x, uint32_t tiles,
+                                            float4* __restrict__ accum, float* __restrict__ depth,
+                                            float4* __restrict__ aux, double* __restrict__ depth64,
+                                            float4* __restrict__ vol, unsigned long long* __restrict__ ctr) {
     const JobP& P = job_ref(job);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t tile = blockIdx.x * 4u + wave;
@@ -146,10 +150,31 @@ __global__ __launch_bounds__(256) void k_render_tile(const JobP* __restrict__ jo
     Work work;
     uint32_t segs = 0, shadow = 0, prim = 0, hits = 0;
     if (lx < w && ly < h)
-        render_pixel<FAM>(P, x0 + lx, y0 + ly, s0, ns, accum, depth, aux, depth64, work, segs, shadow, prim, hits);
+        render_pixel<FAM, FOG>(P, x0 + lx, y0 + ly, s0, ns, accum, depth, aux, depth64, vol, work, segs, shadow, prim,
+                               hits);
     flush_work(work, hits, segs + shadow, ctr);
     uint32_t p = wave_sum(prim);
     if (lane == 0u) atomicAdd(&ctr[CTR_PRIMARY], (unsigned long long)p);
+}
+// The megakernel over a tile of a job without fog (vol unused).
+template <uint32_t FAM>
+__global__ __launch_bounds__(256) void k_render_tile(const JobP* __restrict__ job, uint32_t x0, uint32_t y0, uint32_t w,
+                                                     uint32_t h, uint32_t s0, uint32_t ns, uint32_t tiles_x,
+                                                     uint32_t tiles, float4* __restrict__ accum,
+                                                     float* __restrict__ depth, float4* __restrict__ aux,
+                                                     double* __restrict__ depth64, float4* __restrict__ vol,
+                                                     unsigned long long* __restrict__ ctr) {
+    render_tile<FAM, false>(job, x0, y0, w, h, s0, ns, tiles_x, tiles, accum, depth, aux, depth64, vol, ctr);
+}
+// The megakernel of a job with fog: 30-40 more registers (two waves per SIMD instead of three).
+template <uint32_t FAM>
+__global__ __launch_bounds__(256) void k_render_tile_fog(const JobP* __restrict__ job, uint32_t x0, uint32_t y0,
+                                                         uint32_t w, uint32_t h, uint32_t s0, uint32_t ns,
+                                                         uint32_t tiles_x, uint32_t tiles, float4* __restrict__ accum,
+                                                         float* __restrict__ depth, float4* __restrict__ aux,
+                                                         double* __restrict__ depth64, float4* __restrict__ vol,
+                                                         unsigned long long* __restrict__ ctr) {
+    render_tile<FAM, true>(job, x0, y0, w, h, s0, ns, tiles_x, tiles, accum, depth, aux, depth64, vol, ctr);
 }
 
 // ---------------------------------------------------------------- wavefront
@@ -159,11 +184,13 @@ enum : uint32_t {
     WF_OX, WF_OY, WF_OZ, WF_DX, WF_DY, WF_DZ, WF_TX, WF_TY, WF_TZ, WF_LX, WF_LY, WF_LZ, WF_NX, WF_NY, WF_NZ,
     WF_PDF, WF_TOTAL, WF_FD, WF_FA0, WF_FA1, WF_FA2, WF_FA3, WF_HT, WF_HTRAP,
     WF_TPREV,  // first segment: march position before a surface hit (FP64 refine bracket), else -1
+    WF_VX, WF_VY, WF_VZ,  // fog: the camera segment's in-scattered radiance
     // u32 planes
     WF_RNG, WF_BOUNCE, WF_HSTEPS, WF_HLIGHT,
     WF_PLANES
 };
-// WF_BOUNCE packs bounce (bits 0-15), alive (16), prev_from_bsdf (17), hit (18), exhausted (19).
+// WF_BOUNCE packs bounce (bits 0-15), alive (16), prev_from_bsdf (17), hit (18), exhausted (19),
+// prev_from_volume (20), prev_volume_nee (21), volume_path (22).
 
 struct WfCtl {
     uint32_t count[2];  // queue lengths (ping-pong)
@@ -192,9 +219,12 @@ __device__ __forceinline__ void wf_save(const Soa& S, uint32_t id, const PathSta
     S.F(WF_LX, id) = st.L.x; S.F(WF_LY, id) = st.L.y; S.F(WF_LZ, id) = st.L.z;
     S.F(WF_NX, id) = st.prev_n.x; S.F(WF_NY, id) = st.prev_n.y; S.F(WF_NZ, id) = st.prev_n.z;
     S.F(WF_PDF, id) = st.prev_pdf; S.F(WF_TOTAL, id) = st.prev_total;
+    S.F(WF_VX, id) = st.primary_volume.x; S.F(WF_VY, id) = st.primary_volume.y; S.F(WF_VZ, id) = st.primary_volume.z;
     S.U(WF_RNG, id) = st.rng;
-    uint32_t b = S.U(WF_BOUNCE, id) & ~0x3ffffu;
-    S.U(WF_BOUNCE, id) = b | (st.bounce & 0xffffu) | (st.alive ? 0x10000u : 0u) | (st.prev_from_bsdf ? 0x20000u : 0u);
+    uint32_t b = S.U(WF_BOUNCE, id) & 0xc0000u;  // hit and exhausted belong to the intersect stage
+    S.U(WF_BOUNCE, id) = b | (st.bounce & 0xffffu) | (st.alive ? 0x10000u : 0u) | (st.prev_from_bsdf ? 0x20000u : 0u) |
+                         (st.prev_from_volume ? 0x100000u : 0u) | (st.prev_volume_nee ? 0x200000u : 0u) |
+                         (st.volume_path ? 0x400000u : 0u);
 }
 __device__ __forceinline__ PathState wf_load(const Soa& S, uint32_t id) {
     PathState st;
@@ -210,6 +240,12 @@ __device__ __forceinline__ PathState wf_load(const Soa& S, uint32_t id) {
     st.bounce = b & 0xffffu;
     st.alive = (b & 0x10000u) != 0u;
     st.prev_from_bsdf = (b & 0x20000u) != 0u;
+    st.prev_from_volume = (b & 0x100000u) != 0u;
+    st.prev_volume_nee = (b & 0x200000u) != 0u;
+    st.volume_path = (b & 0x400000u) != 0u;
+    st.primary_volume = mk(S.F(WF_VX, id), S.F(WF_VY, id), S.F(WF_VZ, id));
+    st.sample = 0u;  // set by the stage that shades (it knows the pixel and the sample)
+    st.point_stream = 0u;
     return st;
 }
 
@@ -355,6 +391,8 @@ __global__ __launch_bounds__(256) void k_wf_shade(const JobP* __restrict__ job, 
         if (i < count) {
             id = queue[size_t(in) * W.capacity + i];
             PathState st = wf_load(S, id);
+            st.sample = W.sample_start + ctl->sample;
+            st.point_stream = pcg(pcg((W.x0 + id % W.w) ^ P.seed) ^ pcg((W.y0 + id / W.w) ^ 0x27d4eb2du));
             const uint32_t bw = S.U(WF_BOUNCE, id);
             SegHit sh{(bw & 0x40000u) != 0u, (bw & 0x80000u) != 0u, S.F(WF_HT, id), S.U(WF_HSTEPS, id),
                       S.U(WF_HLIGHT, id), S.F(WF_HTRAP, id)};
@@ -362,7 +400,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(const JobP* __restrict__ job, 
             float fd = S.F(WF_FD, id);
             float4 fa = make_float4(S.F(WF_FA0, id), S.F(WF_FA1, id), S.F(WF_FA2, id), S.F(WF_FA3, id));
             const bool first = st.bounce == 0u;
-            path_bounce(P, st, sh, fd, fa, [&](F3 p, float hh) __attribute__((always_inline)) { return tr.normal(p, hh); },
+            path_bounce<true>(P, st, sh, fd, fa, [&](F3 p, float hh) __attribute__((always_inline)) { return tr.normal(p, hh); },
                         [&](F3 o, F3 d, float te) __attribute__((always_inline)) { return tr.visible(o, d, te); },
                         [&](F3 p) __attribute__((always_inline)) { return tr.sample(p); });
             wf_save(S, id, st);
@@ -404,8 +442,8 @@ __global__ void k_wf_next_round(uint32_t in, uint32_t capacity, WfCtl* ctl) {
 template <uint32_t FAM>
 __global__ __launch_bounds__(256) void k_wf_commit(const JobP* __restrict__ job, const WfP W, Soa S, float4* __restrict__ accum,
                                                    float* __restrict__ depth, float4* __restrict__ aux,
-                                                   double* __restrict__ depth64, WfCtl* __restrict__ ctl,
-                                                   unsigned long long* __restrict__ ctr) {
+                                                   double* __restrict__ depth64, float4* __restrict__ vol,
+                                                   WfCtl* __restrict__ ctl, unsigned long long* __restrict__ ctr) {
     const JobP& P = job_ref(job);
     const uint32_t n = W.w * W.h;
     const uint32_t sample = W.sample_start + ctl->sample;
@@ -426,6 +464,7 @@ __global__ __launch_bounds__(256) void k_wf_commit(const JobP* __restrict__ job,
         a.z += ok ? r.z : 0.0f;
         a.w += 1.0f;
         accum[idx] = a;
+        if (vol) accumulate_session(vol, size_t(P.width) * P.height, idx, r, wf_load(S, id), ok);
         if (sample == 0u) {
             const float fd = S.F(WF_FD, id), tp = S.F(WF_TPREV, id);
             depth[idx] = fd;

@@ -477,7 +477,7 @@ bool run_plain(syf_server* s, std::unique_lock<std::mutex>& lk) {
     if (e == hipSuccess) e = hipEventRecord(s->ev_pool[0], s->stream);
     if (e == hipSuccess)
         e = launch_render_tile(r->P, s->d_plain_job, 0, 0, r->P.width, r->P.height, 0, r->spp, (float4*)r->accum,
-                               (float*)r->depth, (float4*)r->aux, (double*)r->depth64, s->d_pctr, s->stream);
+                               (float*)r->depth, (float4*)r->aux, (double*)r->depth64, nullptr, s->d_pctr, s->stream);
     if (e == hipSuccess) e = hipEventRecord(s->ev_pool[1], s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(s->ev_pool[1]);
     float ms = 0.0f;
@@ -903,6 +903,11 @@ int syf_server_submit(syf_server* s, const syf_job_desc* desc, uint32_t spp, uin
     if (rc != SYF_OK) {
         delete r;
         return rc;
+    }
+    if (r->P.sh.fog_density > 0.0f) {
+        delete r;
+        return fail(SYF_E_INVALID, "fog.density: the server traces no participating medium; render a job with fog "
+                                   "with syf_render_tile or syf_render_wavefront");
     }
     r->spp = spp;
     r->pixels = size_t(desc->width) * desc->height;

@@ -340,8 +340,14 @@ __device__ __forceinline__ float nearest_emitter(const JobP& P, F3 ro, F3 rd, fl
     }
     return nearest;
 }
+// volume_light_probability (lights.wgsl:319-322)
+__device__ __forceinline__ float volume_light_probability(float weight, float total) {
+    return total <= 0.0f ? 0.0f : fminf(1.0f, 3.0f * weight / total);
+}
+// area_emission (lights.wgsl:330-357); from_volume: the segment started at a scattering event in
+// the fog, whose light selection is volume_light_probability.
 __device__ __forceinline__ F3 area_emission(const JobP& P, F3 ro, F3 rd, float distance, float prev_pdf, bool use_mis,
-                                            F3 prev_n, float prev_total) {  // lights.wgsl:330-357
+                                            F3 prev_n, float prev_total, bool from_volume = false) {
     F3 L = mk(0.0f, 0.0f, 0.0f);
     LIGHT_LOOP(i) {
         const LightP& l = P.lights[i];
@@ -354,7 +360,9 @@ __device__ __forceinline__ F3 area_emission(const JobP& P, F3 ro, F3 rd, float d
         if (use_mis) {
             float pl = distance * distance / fmaxf(l.area * cos_l, 1e-20f);
             float sel = 0.0f;
-            if (prev_total > 0.0f) sel = area_weight(ro, prev_n, l) / prev_total;
+            if (prev_total > 0.0f)
+                sel = from_volume ? volume_light_probability(area_weight(ro, mk(0.0f, 0.0f, 0.0f), l), prev_total)
+                                  : area_weight(ro, prev_n, l) / prev_total;
             pl *= sel;
             w = mis_power(prev_pdf, pl);
         }
@@ -362,14 +370,18 @@ __device__ __forceinline__ F3 area_emission(const JobP& P, F3 ro, F3 rd, float d
     }
     return L;
 }
+// environment_emission (pathtrace.wgsl:177-189): from_volume, the direction came from a phase
+// sample in the fog, whose NEE proposal is uniform over the sphere.
 __device__ __forceinline__ F3 environment_emission(const JobP& P, F3 dir, float pdf, bool use_mis, F3 n,
-                                                   float area_total) {  // pathtrace.wgsl:177-189
+                                                   float area_total, bool from_volume = false) {
     F3 L = mk(0.0f, 0.0f, 0.0f);
     LIGHT_LOOP(i) {
         const LightP& l = P.lights[i];
         if (l.kind != SYF_LIGHT_ENV) continue;
         float w = 1.0f;
-        if (use_mis) w = mis_power(pdf, environment_probability(l, area_total) * (fmaxf(dot3(n, dir), 0.0f) / PI_F));
+        if (use_mis)
+            w = mis_power(pdf, environment_probability(l, area_total) *
+                                   (from_volume ? 1.0f / (4.0f * PI_F) : fmaxf(dot3(n, dir), 0.0f) / PI_F));
         L = add(L, scl(environment_value(l, dir), w));
     }
     return L;
@@ -399,13 +411,236 @@ __device__ __forceinline__ F3 sun_radiance(const JobP& P, F3 dir, float pdf_bsdf
     return L;
 }
 
+// ---------------------------------------------------------------- height fog
+// The fog of pathtrace.wgsl:560-779: extinction sigma(p) = density exp(min(0, -(y - height)
+// falloff)), exact Beer-Lambert optical depths split at the fog height, free-flight sampling by
+// inverting the optical depth, the emitters' line integral along a segment (equiangular and
+// distance sampling combined), Henyey-Greenstein scattering with next-event estimation of the
+// environment and suns.
+__device__ __forceinline__ float fog_sigma(const ShadeP& S, F3 p) {
+    return S.fog_density * expf(fminf(0.0f, -(p.y - S.fog_height) * S.fog_falloff));
+}
+__device__ __forceinline__ float fog_segment_tau(const ShadeP& S, F3 ro, F3 rd, float begin, float end) {
+    const float distance = fmaxf(end - begin, 0.0f);
+    if (distance == 0.0f) return 0.0f;
+    const float sigma = fog_sigma(S, add(ro, scl(rd, begin)));
+    const float mid_height = ro.y + rd.y * ((begin + end) * 0.5f);
+    if (mid_height <= S.fog_height || S.fog_falloff == 0.0f) return S.fog_density * distance;
+    const float k = rd.y * S.fog_falloff;
+    const float x = k * distance;
+    if (fabsf(x) < 0.001f) return sigma * distance * (1.0f - x * 0.5f + x * x / 6.0f);
+    return fmaxf(0.0f, (sigma - fog_sigma(S, add(ro, scl(rd, end)))) / k);
+}
+__device__ __forceinline__ float fog_split(const ShadeP& S, F3 ro, F3 rd, float distance) {
+    if (fabsf(rd.y) < 1e-8f) return distance;
+    return clampT((S.fog_height - ro.y) / rd.y, 0.0f, distance);
+}
+__device__ __forceinline__ float fog_optical_depth(const ShadeP& S, F3 ro, F3 rd, float distance) {
+    const float split = fog_split(S, ro, rd, distance);
+    return fog_segment_tau(S, ro, rd, 0.0f, split) + fog_segment_tau(S, ro, rd, split, distance);
+}
+__device__ __forceinline__ float fog_transmittance(const ShadeP& S, F3 ro, F3 rd, float dist) {
+    return expf(-fog_optical_depth(S, ro, rd, dist));
+}
+__device__ __forceinline__ float fog_invert_segment(const ShadeP& S, F3 ro, F3 rd, float begin, float end, float tau) {
+    const float sigma = fog_sigma(S, add(ro, scl(rd, begin)));
+    const float k = rd.y * S.fog_falloff;
+    if (ro.y + rd.y * ((begin + end) * 0.5f) <= S.fog_height || fabsf(k * (end - begin)) < 0.0001f)
+        return fminf(end, begin + tau / fmaxf(sigma, 1e-30f));
+    if (sigma > 1e-20f) return clampT(begin - logf(fmaxf(1e-7f, 1.0f - tau * k / sigma)) / k, begin, end);
+    const float remaining = fmaxf(0.0f, fog_segment_tau(S, ro, rd, begin, end) - tau);
+    const float end_sigma = fog_sigma(S, add(ro, scl(rd, end)));
+    return clampT(end + logf(fmaxf(1e-7f, 1.0f + remaining * k / fmaxf(end_sigma, 1e-30f))) / (-k), begin, end);
+}
+// The first collision along [0, t_end), t_end without one (fog_free_flight).
+__device__ __forceinline__ float fog_free_flight(const ShadeP& S, F3 ro, F3 rd, float t_end, uint32_t& rng) {
+    if (S.fog_density <= 0.0f) return t_end;
+    const float tau = -logf(fmaxf(1.0f - rand01(rng), 1e-7f));
+    const float split = fog_split(S, ro, rd, t_end);
+    const float first = fog_segment_tau(S, ro, rd, 0.0f, split);
+    const float total = first + fog_segment_tau(S, ro, rd, split, t_end);
+    if (tau >= total) return t_end;
+    if (tau < first) return fog_invert_segment(S, ro, rd, 0.0f, split, tau);
+    return fog_invert_segment(S, ro, rd, split, t_end, tau - first);
+}
+// volume_coordinate (pathtrace.wgsl:43-58): the radical inverse of sample_index + 1 in the
+// dimension's prime base, rotated by a per-stream shift.
+__device__ __forceinline__ float volume_coordinate(uint32_t sample_index, uint32_t dimension, uint32_t stream) {
+    const uint32_t base = dimension == 0u ? 2u : dimension == 1u ? 3u : dimension == 2u ? 5u
+                        : dimension == 3u ? 7u : dimension == 4u ? 11u : 13u;
+    uint32_t index = sample_index + 1u;
+    float inverse = 1.0f / float(base);
+    float value = 0.0f;
+    while (index != 0u) {
+        value += float(index % base) * inverse;
+        index /= base;
+        inverse /= float(base);
+    }
+    const float shift = float(pcg(stream ^ (dimension * 0x9e3779b9u)) >> 8u) / 16777216.0f;
+    const float x = value + shift;
+    return x - floorf(x);
+}
+__device__ __forceinline__ float hg_phase(float cos_theta, float g) {  // pathtrace.wgsl:717-720
+    const float d = 1.0f + g * g - 2.0f * g * cos_theta;
+    return (1.0f - g * g) / (4.0f * PI_F * d * sqrtf(d));
+}
+__device__ __forceinline__ F3 hg_sample(F3 dir, float g, float r1, float r2) {  // pathtrace.wgsl:722-731
+    float cos_t = 1.0f - 2.0f * r1;
+    if (fabsf(g) > 1e-3f) {
+        const float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * r1);
+        cos_t = (1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = 2.0f * PI_F * r2;
+    F3 t, bt;
+    tangent_frame(dir, t, bt);
+    return frame_mul(t, bt, dir, mk(sin_t * cosf(phi), sin_t * sinf(phi), cos_t));
+}
+// fog_segment_weight (pathtrace.wgsl:621-633) of an area light
+__device__ __forceinline__ float fog_segment_weight(const LightP& l, F3 ro, F3 rd, float end) {
+    const F3 center = ld3(l.a);
+    const float along = clampT(dot3(sub(center, ro), rd), 0.0f, end);
+    const F3 delta = sub(center, add(ro, scl(rd, along)));
+    return fmaxf(fmaxf(l.color[0], l.color[1]), l.color[2]) * l.area / (dot3(delta, delta) + l.area);
+}
+// fog_segment_direct (pathtrace.wgsl:635-715): the emitters' light scattered toward the ray
+// along [0, end), each emitter proposing one point chosen by equiangular or distance sampling
+// (one-sample MIS of the two), its distance pdf the mean of theirs.  The camera segment
+// (primary) takes its selection, emitter point and distance coordinates from the per-pixel
+// rotated Halton sequence of the sample index; every random draw happens either way.
+template <class Vis>
+__device__ __forceinline__ F3 fog_segment_direct(const JobP& P, F3 ro, F3 rd, float end, uint32_t& rng,
+                                                 uint32_t point_index, uint32_t point_stream, bool primary,
+                                                 Vis&& visible) {
+    const ShadeP& S = P.sh;
+    if (end <= 0.0f || fmaxf(fmaxf(S.fog_color[0], S.fog_color[1]), S.fog_color[2]) <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    const float tau_end = fog_optical_depth(S, ro, rd, end);
+    if (tau_end <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    float total = 0.0f;
+    LIGHT_LOOP(i)
+        if (P.lights[i].kind == SYF_LIGHT_AREA) total += fog_segment_weight(P.lights[i], ro, rd, end);
+    F3 result = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+    for (uint32_t i = 0; i < P.light_count; ++i) {
+        const LightP& l = P.lights[i];
+        if (l.kind != SYF_LIGHT_AREA) continue;
+        const float probability = volume_light_probability(fog_segment_weight(l, ro, rd, end), total);
+        const uint32_t stream = pcg(point_stream ^ (i * 0x85ebca6bu));
+        float selection = rand01(rng);
+        if (primary) selection = volume_coordinate(point_index, 5u, stream);
+        if (probability <= 0.0f || selection >= probability) continue;
+        float u1 = rand01(rng);
+        float u2 = rand01(rng);
+        if (primary) {
+            u1 = volume_coordinate(point_index, 0u, stream);
+            u2 = volume_coordinate(point_index, 1u, stream);
+        }
+        const F3 q = add(add(ld3(l.a), scl(ld3(l.b), u1 - 0.5f)), scl(ld3(l.c), u2 - 0.5f));
+        const float tau_begin = 0.0f;  // fog_optical_depth over [0, 0)
+        const float tau_finish = fog_optical_depth(S, ro, rd, end);
+        const float tau_span = fmaxf(0.0f, tau_finish - tau_begin);
+        if (tau_span <= 0.0f) continue;
+        float conditional_mass = 1.0f - expf(-tau_span);
+        if (tau_span < 0.001f) conditional_mass = tau_span * (1.0f - tau_span * 0.5f + tau_span * tau_span / 6.0f);
+        const float interval_mass = expf(-tau_begin) * conditional_mass;
+        if (interval_mass < 1e-30f) continue;
+        const F3 delta = sub(q, ro);
+        const float projection = dot3(delta, rd);
+        const float perpendicular = len3(sub(delta, scl(rd, projection)));
+        const float theta_a = atan2f(0.0f - projection, fmaxf(perpendicular, 1e-6f));
+        const float theta_b = atan2f(end - projection, fmaxf(perpendicular, 1e-6f));
+        const float span = theta_b - theta_a;
+        const bool regular = perpendicular > 1e-4f && span > 1e-5f;
+        float xi = rand01(rng);
+        float technique = rand01(rng);
+        if (primary) {
+            xi = volume_coordinate(point_index, 3u, stream);
+            technique = volume_coordinate(point_index, 4u, stream);
+        }
+        float t = mixf(0.0f, end, xi);
+        if (technique < 0.5f) {
+            if (regular) t = clampT(projection + perpendicular * tanf(mixf(theta_a, theta_b, xi)), 0.0f, end);
+        } else {
+            const float tau = tau_begin - logf(fmaxf(1.0f - xi * conditional_mass, 1e-7f));
+            const float split = fog_split(S, ro, rd, end);
+            const float first = fog_segment_tau(S, ro, rd, 0.0f, split);
+            t = tau < first ? fog_invert_segment(S, ro, rd, 0.0f, split, tau)
+                            : fog_invert_segment(S, ro, rd, split, end, tau - first);
+        }
+        const F3 p = add(ro, scl(rd, t));
+        const float transmittance = expf(-fog_optical_depth(S, ro, rd, t));
+        const float sigma = fog_sigma(S, p);
+        float angular_pdf = 1.0f / (end - 0.0f);
+        if (regular)
+            angular_pdf = perpendicular / (span * (perpendicular * perpendicular + (t - projection) * (t - projection)));
+        const float distance_pdf = 0.5f * (angular_pdf + sigma * transmittance / interval_mass);
+        const F3 to_light = sub(q, p);
+        const float distance2 = dot3(to_light, to_light);
+        if (distance2 < 1e-8f) continue;
+        const float distance = sqrtf(distance2);
+        const F3 wi = mk(to_light.x / distance, to_light.y / distance, to_light.z / distance);
+        const float cosine = dot3(ld3(l.n), neg3(wi));
+        if (cosine <= 0.0f) continue;
+        if (!visible(p, wi, fmaxf(0.0f, distance - fmaxf(1e-5f, distance * 1e-6f)))) continue;
+        const float tr = transmittance * expf(-fog_optical_depth(S, p, wi, distance));
+        const float area_pdf = distance2 / fmaxf(l.area * cosine, 1e-20f);  // area_direction_pdf
+        const float k = sigma * tr * hg_phase(dot3(rd, wi), S.fog_g) * 1.0f * 1.0f /
+                        (probability * distance_pdf * area_pdf);
+        result = add(result, mk(l.color[0] * k, l.color[1] * k, l.color[2] * k));
+    }
+    return mk(result.x * S.fog_color[0], result.y * S.fog_color[1], result.z * S.fog_color[2]);
+}
+// fog_direct_light (pathtrace.wgsl:733-778): the environment and suns at a scattering event,
+// weighted by the phase function instead of a BRDF.
+template <class Vis>
+__device__ __forceinline__ F3 fog_direct_light(const JobP& P, F3 p, F3 rd, float area_total, bool last, uint32_t& rng,
+                                               Vis&& visible) {
+    const float g = P.sh.fog_g;
+    F3 L = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+    for (uint32_t i = 0; i < P.light_count; ++i) {
+        const LightP& l = P.lights[i];
+        if (l.kind == SYF_LIGHT_ENV) {
+            const float mx = fmaxf(fmaxf(fmaxf(l.color[0], l.color[1]), l.color[2]),
+                                   fmaxf(fmaxf(fmaxf(l.horizon[0], l.horizon[1]), l.horizon[2]),
+                                         fmaxf(fmaxf(l.ground[0], l.ground[1]), l.ground[2])));
+            if (mx <= 0.0f) continue;
+            const float probability = environment_probability(l, area_total);
+            if (rand01(rng) >= probability) continue;
+            const float z = 1.0f - 2.0f * rand01(rng);
+            const float phi = 2.0f * PI_F * rand01(rng);
+            const float radius = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            const F3 dir = mk(radius * cosf(phi), z, radius * sinf(phi));
+            if (!visible(p, dir, P.t_end)) continue;
+            const float tr = fog_transmittance(P.sh, p, dir, P.t_end);
+            const float ph = hg_phase(dot3(rd, dir), g);
+            const float pl = probability / (4.0f * PI_F);
+            const float w = last ? 1.0f : mis_power(pl, ph);
+            const F3 e = environment_value(l, dir);
+            L = add(L, mk(e.x * ph * tr * w / pl, e.y * ph * tr * w / pl, e.z * ph * tr * w / pl));
+        } else if (l.kind == SYF_LIGHT_SUN) {
+            const float r1 = rand01(rng);
+            const float r2 = rand01(rng);
+            const F3 dir = cone_sample(ld3(l.a), l.cos_max, r1, r2);
+            if (!visible(p, dir, P.t_end)) continue;
+            const float tr = fog_transmittance(P.sh, p, dir, P.t_end);
+            const float ph = hg_phase(dot3(rd, dir), g);
+            const float pl = sun_pdf(l.cos_max);
+            const float w = last ? 1.0f : mis_power(pl, ph);
+            L = add(L, mk(l.color[0] * ph * tr * w, l.color[1] * ph * tr * w, l.color[2] * ph * tr * w));
+        }
+    }
+    return L;
+}
+
 // Next-event estimation as direct_light (pathtrace.wgsl:471-558): choose one area light by
 // weight, then the environment, each sun and the chosen area light each propose one shadow ray.
 // The caller supplies the visibility test (a march to the light), so the megakernel and
 // the wavefront form share this logic.
+// in_fog: the shadow rays' light is attenuated by the fog's transmittance.
 template <class Vis>
 __device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F3 gn, F3 v, float p_spec,
-                                           float area_total, bool last, uint32_t& rng, Vis&& visible) {
+                                           float area_total, bool last, bool in_fog, uint32_t& rng, Vis&& visible) {
     F3 L = mk(0.0f, 0.0f, 0.0f);
     uint32_t selected = NO_LIGHT;
     float selected_weight = 0.0f;
@@ -438,12 +673,13 @@ __device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F
             float nl = fmaxf(dot3(s.n, dir), 0.0f);
             if (dot3(gn, dir) <= 0.0f || nl <= 0.0f) continue;
             if (!visible(p, dir, P.t_end)) continue;
+            const float tr = in_fog ? fog_transmittance(P.sh, p, dir, P.t_end) : 1.0f;
             float pl = prob * nl / PI_F;
             float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
             float w = last ? 1.0f : mis_power(pl, pb);
             F3 f = brdf_eval(s, v, dir);
             F3 e = environment_value(l, dir);
-            L = add(L, mk(f.x * nl * e.x * 1.0f * w / pl, f.y * nl * e.y * 1.0f * w / pl, f.z * nl * e.z * 1.0f * w / pl));
+            L = add(L, mk(f.x * nl * e.x * tr * w / pl, f.y * nl * e.y * tr * w / pl, f.z * nl * e.z * tr * w / pl));
         } else if (l.kind == SYF_LIGHT_SUN) {
             // colour = irradiance: the disk's radiance is colour * sun_pdf
             const float r1 = rand01(rng);
@@ -452,11 +688,12 @@ __device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F
             const float nl = dot3(s.n, dir);
             if (nl <= 0.0f || dot3(gn, dir) <= 0.0f) continue;
             if (!visible(p, dir, P.t_end)) continue;
+            const float tr = in_fog ? fog_transmittance(P.sh, p, dir, P.t_end) : 1.0f;
             const float pl = sun_pdf(l.cos_max);
             const float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
             const float w = last ? 1.0f : mis_power(pl, pb);
             const F3 f = brdf_eval(s, v, dir);
-            L = add(L, mk(f.x * nl * l.color[0] * w, f.y * nl * l.color[1] * w, f.z * nl * l.color[2] * w));
+            L = add(L, mk(f.x * nl * l.color[0] * w * tr, f.y * nl * l.color[1] * w * tr, f.z * nl * l.color[2] * w * tr));
         } else if (l.kind == SYF_LIGHT_AREA) {
             if (i != selected) continue;
             float u1 = rand01(rng);
@@ -472,12 +709,13 @@ __device__ __forceinline__ F3 direct_light(const JobP& P, F3 p, const Surf& s, F
             if (nl <= 0.0f || cos_l <= 0.0f || dot3(gn, dir) <= 0.0f) continue;
             float end = fmaxf(0.0f, dist - fmaxf(1e-5f, dist * 1e-6f));  // shadow_endpoint (lights.wgsl:253-255)
             if (!visible(p, dir, end)) continue;
+            const float tr = in_fog ? fog_transmittance(P.sh, p, dir, dist) : 1.0f;
             float pl = (selected_weight / area_total) * (dist2 / fmaxf(l.area * cos_l, 1e-20f));
             float pb = last ? 0.0f : bsdf_pdf(s, v, dir, p_spec);
             float w = last ? 1.0f : mis_power(pl, pb);
             F3 f = brdf_eval(s, v, dir);
-            L = add(L, mk(f.x * nl * l.color[0] * 1.0f * w / pl, f.y * nl * l.color[1] * 1.0f * w / pl,
-                          f.z * nl * l.color[2] * 1.0f * w / pl));
+            L = add(L, mk(f.x * nl * l.color[0] * tr * w / pl, f.y * nl * l.color[1] * tr * w / pl,
+                          f.z * nl * l.color[2] * tr * w / pl));
         }
     }
     return L;
@@ -489,7 +727,40 @@ struct PathState {
     float prev_pdf, prev_total;
     uint32_t rng, bounce;
     bool prev_from_bsdf, alive;
+    // fog: the segment starts at a scattering event (its NEE proposals are the phase's), whose
+    // emitters the segment's line integral already counted; the camera segment scattered; the
+    // light scattered toward the camera segment; the sample index and pixel stream of its
+    // Halton coordinates
+    bool prev_from_volume, prev_volume_nee, volume_path;
+    F3 primary_volume;
+    uint32_t sample, point_stream;
 };
+
+// The session planes of one sample (pathtrace.wgsl:1243-1266) of a job with fog: the volume
+// radiance (the camera segment's in-scattering, or the whole path when the camera ray
+// scattered), its Welford second moments, and the surface radiance r - volume.  vol holds the
+// VOLUME, MOMENTS and SURFACE planes, `pixels` entries each.
+__device__ __forceinline__ void accumulate_session(float4* __restrict__ vol, size_t pixels, size_t idx, F3 r,
+                                                   const PathState& st, bool ok) {
+    const F3 volume = st.volume_path ? st.L : st.primary_volume;
+    const F3 v = ok ? volume : mk(0.0f, 0.0f, 0.0f);
+    const F3 surface = ok ? sub(r, volume) : mk(0.0f, 0.0f, 0.0f);
+    const float4 old = vol[idx];
+    const float n = fmaxf(old.w, 1.0f);
+    const F3 old_mean = mk(old.x / n, old.y / n, old.z / n);
+    const F3 delta = sub(v, old_mean);
+    const float n1 = old.w + 1.0f;
+    const F3 new_mean = add(old_mean, mk(delta.x / n1, delta.y / n1, delta.z / n1));
+    float4 m = vol[pixels + idx];
+    m.x += fmaxf(0.0f, delta.x * (v.x - new_mean.x));
+    m.y += fmaxf(0.0f, delta.y * (v.y - new_mean.y));
+    m.z += fmaxf(0.0f, delta.z * (v.z - new_mean.z));
+    m.w += 1.0f;
+    vol[pixels + idx] = m;
+    vol[idx] = make_float4(old.x + v.x, old.y + v.y, old.z + v.z, old.w + 1.0f);
+    float4 sf = vol[2 * pixels + idx];
+    vol[2 * pixels + idx] = make_float4(sf.x + surface.x, sf.y + surface.y, sf.z + surface.z, sf.w + 1.0f);
+}
 
 __device__ __forceinline__ void path_start(const JobP& P, uint32_t px, uint32_t py, uint32_t sample, PathState& st) {
     st.rng = rng_init(px, py, sample, P.seed);
@@ -502,6 +773,12 @@ __device__ __forceinline__ void path_start(const JobP& P, uint32_t px, uint32_t 
     st.bounce = 0;
     st.prev_from_bsdf = false;
     st.alive = true;
+    st.prev_from_volume = false;
+    st.prev_volume_nee = false;
+    st.volume_path = false;
+    st.primary_volume = mk(0.0f, 0.0f, 0.0f);
+    st.sample = sample;
+    st.point_stream = pcg(pcg(px ^ P.seed) ^ pcg(py ^ 0x27d4eb2du));  // pathtrace.wgsl:822
 }
 
 // One segment's outcome as the march and emitter test leave it.
@@ -518,8 +795,9 @@ struct SegHit {
 // estimation and the continuation sample.  `normal` evaluates the field gradient at a surface
 // point; `visible` marches a shadow ray; `sample` evaluates the full field sample at a surface
 // hit (called only when the shading reads more than the march's trap).  Sets st.alive = false
-// when the path ends.
-template <class Normal, class Vis, class Sample>
+// when the path ends.  FOG = false: a kernel for jobs without fog (the fog's code is not
+// compiled in, so that kernel keeps its registers for the rest).
+template <bool FOG, class Normal, class Vis, class Sample>
 __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const SegHit& h, float& first_depth,
                                             float4& first_aux, Normal&& normal, Vis&& visible, Sample&& sample) {
     F3 gn = mk(0.0f, 0.0f, 0.0f);
@@ -533,10 +811,46 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
             first_aux = make_float4(gn.x, gn.y, gn.z, float(h.steps));
         }
     }
+    const bool in_fog = FOG && P.sh.fog_density > 0.0f;
+    if (in_fog) {  // pathtrace.wgsl:980-1010
+        const float t_end = h.hit ? h.t : P.t_end;
+        const F3 dv = mul3(st.throughput, fog_segment_direct(P, st.o, st.d, t_end, st.rng, st.sample, st.point_stream,
+                                                             st.bounce == 0u, visible));
+        st.L = add(st.L, dv);
+        if (st.bounce == 0u) st.primary_volume = add(st.primary_volume, dv);
+        const float t_s = fog_free_flight(P.sh, st.o, st.d, t_end, st.rng);
+        if (t_s < t_end) {  // a scattering event before the segment's end
+            if (st.bounce == 0u) st.volume_path = true;
+            const F3 p_s = add(st.o, scl(st.d, t_s));
+            if (st.bounce == 0u && !h.hit) first_depth = t_s;
+            const float vat = area_total_weight(P, p_s, mk(0.0f, 0.0f, 0.0f));
+            const bool last = st.bounce == P.max_bounces;
+            const F3 fc = ld3(P.sh.fog_color);
+            st.L = add(st.L, mul3(mul3(st.throughput, fc), fog_direct_light(P, p_s, st.d, vat, last, st.rng, visible)));
+            st.throughput = mul3(st.throughput, fc);
+            if (last) {
+                st.alive = false;
+                return;
+            }
+            const float r1 = rand01(st.rng);
+            const float r2 = rand01(st.rng);
+            const F3 nd = hg_sample(st.d, P.sh.fog_g, r1, r2);
+            st.prev_pdf = hg_phase(dot3(st.d, nd), P.sh.fog_g);
+            st.prev_from_bsdf = true;
+            st.prev_from_volume = true;
+            st.prev_volume_nee = true;
+            st.prev_n = mk(0.0f, 0.0f, 0.0f);
+            st.prev_total = vat;
+            st.o = p_s;
+            st.d = nd;
+            st.bounce++;
+            return;
+        }
+    }
     if (!h.hit) {
         if (!h.exhausted) {
-            st.L = add(st.L, mul3(st.throughput,
-                                  environment_emission(P, st.d, st.prev_pdf, st.prev_from_bsdf, st.prev_n, st.prev_total)));
+            st.L = add(st.L, mul3(st.throughput, environment_emission(P, st.d, st.prev_pdf, st.prev_from_bsdf, st.prev_n,
+                                                                      st.prev_total, st.prev_from_volume)));
             st.L = add(st.L, mul3(st.throughput, sun_radiance(P, st.d, st.prev_pdf, st.prev_from_bsdf)));
         }
         st.alive = false;
@@ -544,8 +858,10 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
     }
     const F3 p = add(st.o, scl(st.d, h.t));
     if (!ground && h.light != NO_LIGHT) {
-        st.L = add(st.L, mul3(st.throughput, area_emission(P, st.o, st.d, h.t, st.prev_pdf, st.prev_from_bsdf,
-                                                           st.prev_n, st.prev_total)));
+        // a fog segment already counted every emitter path by its line integral
+        if (!st.prev_volume_nee)
+            st.L = add(st.L, mul3(st.throughput, area_emission(P, st.o, st.d, h.t, st.prev_pdf, st.prev_from_bsdf,
+                                                               st.prev_n, st.prev_total, st.prev_from_volume)));
         st.alive = false;
         return;
     }
@@ -566,7 +882,7 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
     const float p_spec = spec_probability(s, nv);
     const float area_total = area_total_weight(P, p_off, s.n);
     const bool last = st.bounce == P.max_bounces;
-    F3 dl = direct_light(P, p_off, s, n, v, p_spec, area_total, last, st.rng, visible);
+    F3 dl = direct_light(P, p_off, s, n, v, p_spec, area_total, last, in_fog, st.rng, visible);
     st.L = add(st.L, mul3(st.throughput, dl));
     if (last) {
         st.alive = false;
@@ -593,6 +909,8 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
     st.prev_from_bsdf = true;
     st.prev_n = s.n;
     st.prev_total = area_total;
+    st.prev_from_volume = false;
+    st.prev_volume_nee = false;
     if (st.bounce >= 2u) {
         float q = clampT(fmaxf(st.throughput.x, fmaxf(st.throughput.y, st.throughput.z)), 0.05f, 0.95f);
         if (rand01(st.rng) > q) {

@@ -6,7 +6,7 @@ crates/syfrack-core/src/scene/graph/bundled.rs:8-49), their parameter paths
 (``fractal.nodes[1].iterations`` ... as Scene::set_param, scene/mod.rs:564), the camera modes
 (scene/camera.rs:5-35), sun/area/environment lights (scene/light.rs:66-123), material with
 its colouring ramp, surface variation and named finishes (scene/material.rs, palette.rs), the
-flat ground plane (scene/atmosphere.rs:19-38), and RenderConfig (renderer.rs:17-76).  ``Scene.desc()``
+flat ground plane and height fog (scene/atmosphere.rs:7-38), and RenderConfig (renderer.rs:17-76).  ``Scene.desc()``
 produces the flat ``syf_job_desc`` the C ABI takes; the camera basis, step scale and ray
 extent are derived inside the library (syf_job_create), as Uniforms::from_scene does.
 """
@@ -285,6 +285,18 @@ class Ground:
 
 
 @dataclass
+class Fog:
+    """The atmosphere's height fog (Atmosphere fog fields, atmosphere.rs:7-16): extinction
+    ``density`` at and below ``height``, decaying above it by exp(-(y - height) falloff);
+    single-scattering albedo ``color``; Henyey-Greenstein ``anisotropy``."""
+    density: float = 0.02
+    color: tuple = (1.0, 1.0, 1.0)
+    height: float = 0.0
+    falloff: float = 1.0
+    anisotropy: float = 0.0
+
+
+@dataclass
 class Material:
     albedo: tuple = (0.8, 0.8, 0.8)
     roughness: float = 0.4
@@ -336,6 +348,7 @@ class Scene:
     seed: int = 7
     cavity_darkness: float = 0.0
     ground: Optional[Ground] = None
+    fog: Optional[Fog] = None
 
     def with_fractal_param(self, path: str, value: float) -> "Scene":
         s = copy.deepcopy(self)
@@ -383,6 +396,12 @@ class Scene:
             d.ground.enabled = 1
             d.ground.height = self.ground.height
             d.ground.finish = self.ground.finish
+        if self.fog is not None:
+            d.fog.density = self.fog.density
+            d.fog.color[:] = list(self.fog.color)
+            d.fog.height = self.fog.height
+            d.fog.falloff = self.fog.falloff
+            d.fog.anisotropy = self.fog.anisotropy
         return d
 
 

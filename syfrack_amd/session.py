@@ -23,7 +23,7 @@ from . import abi
 from .ron import from_ron, to_ron  # noqa: F401  (state.ron)
 from .graph import Graph
 from .render import Renderer, RenderOutput
-from .scene import Camera, Coloring, Finish, Fractal, Ground, Light, Material, Ramp, RenderConfig, Scene
+from .scene import Camera, Coloring, Finish, Fog, Fractal, Ground, Light, Material, Ramp, RenderConfig, Scene
 
 STATE_FILE = "state.ron"
 
@@ -58,10 +58,12 @@ def scene_from_state(d: dict) -> Scene:
                               None if orbit is None else (tuple(orbit[0]), orbit[1], orbit[2], orbit[3]))
     m.finishes = [_build(Finish, f) for f in md.get("finishes", [])]
     g = s.get("ground")
+    fog = s.get("fog")
     fr = s["fractal"]
     fractal = Graph.from_dict(fr) if fr.get("__struct__") == "Graph" else _build(Fractal, fr)
     return Scene(fractal, _build(Camera, s["camera"]), lights, m, seed=int(s["seed"]),
-                 cavity_darkness=float(s["cavity_darkness"]), ground=None if g is None else _build(Ground, g))
+                 cavity_darkness=float(s["cavity_darkness"]), ground=None if g is None else _build(Ground, g),
+                 fog=None if fog is None else _build(Fog, fog))
 
 
 # ---------------------------------------------------------------- the session

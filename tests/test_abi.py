@@ -19,7 +19,7 @@ def test_library_exports_every_header_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) == set(abi.SIGNATURES), set(names) ^ set(abi.SIGNATURES)
-    assert lib.syf_abi_version() == 4
+    assert lib.syf_abi_version() == 5
 
 
 STRUCTS = [
@@ -28,6 +28,7 @@ STRUCTS = [
     ("syf_planar_desc", abi.SyfPlanarDesc), ("syf_server_config", abi.SyfServerConfig),
     ("syf_request_info", abi.SyfRequestInfo), ("syf_server_counters", abi.SyfServerCounters),
     ("syf_finish", abi.SyfFinish), ("syf_coloring", abi.SyfColoring), ("syf_ground", abi.SyfGround),
+    ("syf_fog", abi.SyfFog),
 ]
 
 

@@ -124,8 +124,10 @@ def _against_oracle(name, out, acc, dep, aux, min_frac=0.98):
     assert np.mean((out.depth >= 0) == hits) >= 0.999
     both = hits & (out.depth >= 0)
     assert np.allclose(out.depth[both], dep[both], rtol=1e-5)
-    # first-hit normals of the path tracer (AUX), where the first segments agree
-    same = both & (out.normals[..., 3] == aux[..., 3])
+    # first-hit normals of the path tracer (AUX), where the first segments agree (in fog, a camera
+    # ray that scattered before escaping has a depth, the scattering distance, and no normal)
+    same = both & (out.normals[..., 3] == aux[..., 3]) & (aux[..., 3] >= 0)
+    both = both & (aux[..., 3] >= 0)
     n_g, n_o = out.normals[same][:, :3].astype(np.float64), aux[same][:, :3].astype(np.float64)
     cosang = np.clip(np.sum(n_g * n_o, axis=1), -1.0, 1.0)
     assert same.sum() >= 0.99 * both.sum()

@@ -188,3 +188,111 @@ def test_shading_validation(renderer):
     d = scene.desc(cfg)
     d.coloring = abi.SyfColoring()
     Job(renderer.ctx, d).close()
+
+
+# ---------------------------------------------------------------- height fog
+
+def fog_scenes():
+    """The reference's exhibition-like atmosphere: a height fog under sun, sky and an emitter,
+    over a Mandelbulb and over a Menger sponge on the ground."""
+    from syfrack_amd import scene as sc
+    sky = sc.Light.environment((0.35, 0.4, 0.5), (0.25, 0.25, 0.28), (0.05, 0.05, 0.05), 1.0)
+    sun = sc.Light.sun((0.4, 0.8, 0.3), (1.0, 0.95, 0.85), 3.0, 0.53)
+    lamp = sc.Light.area((2.0, 1.5, -2.0), (-2.0, -1.5, 2.0), (1.0, 1.0), (1.0, 0.9, 0.8), 15.0)
+    out = {}
+    f = sc.Fractal.bundled("mandelbulb")
+    f.set_param("fractal.nodes[1].iterations", 8)
+    out["bulb_fog_sun_lamp"] = sc.Scene(f, sc.Camera.orbit((0, 0, 0), 3.4, 30.0, 20.0, 35.0), [sky, sun, lamp],
+                                        sc.Material(albedo=(0.8, 0.75, 0.7), roughness=0.4), seed=51,
+                                        fog=sc.Fog(0.12, (0.85, 0.8, 0.78), -0.5, 0.4, 0.55))
+    f = sc.Fractal.bundled("menger")
+    f.set_param("fractal.nodes[1].iterations", 4)
+    out["menger_fog_ground"] = sc.Scene(f, sc.Camera.orbit((0, 0, 0), 4.5, 40.0, 20.0, 35.0), [sky, sun, lamp],
+                                        sc.Material(albedo=(0.8, 0.8, 0.8), roughness=0.5), seed=52,
+                                        ground=sc.Ground(-1.2, 0), fog=sc.Fog(0.2, (0.82, 0.76, 0.68), -1.0, 0.35, 0.75))
+    return out
+
+
+@pytest.mark.parametrize("name", ["bulb_fog_sun_lamp", "menger_fog_ground"])
+@pytest.mark.parametrize("form", ["megakernel", "wavefront"])
+def test_fog_matches_oracle(renderer, name, form):
+    """Radiance and the session planes (VOLUME, MOMENTS, SURFACE) of a fog scene against the
+    oracle's, with the family tolerance (the fog's equiangular sampling calls atan2 and tan, whose
+    device results differ from glibc's by an ulp now and then)."""
+    from syfrack_amd import abi
+    from syfrack_amd import scene as sc
+    from oracle import oracle
+    from test_gpu_pathtrace import _against_oracle
+    scene = fog_scenes()[name]
+    cfg = sc.RenderConfig(96, 54, 8, 4, 256)
+    cfg.tile_size = 32
+    job = renderer.job(scene, cfg)
+    try:
+        for s0 in range(0, cfg.samples_per_pixel, 4):
+            for y in range(0, cfg.height, 32):
+                for x in range(0, cfg.width, 32):
+                    w, h = min(32, cfg.width - x), min(32, cfg.height - y)
+                    if form == "wavefront":
+                        job.render_wavefront(x, y, w, h, s0, 4)
+                    else:
+                        job.render_tile(x, y, w, h, s0, 4)
+        job.sync()
+        from syfrack_amd.render import RenderOutput
+        out = RenderOutput(job.readback(abi.PLANE_ACCUM), job.readback(abi.PLANE_DEPTH), job.readback(abi.PLANE_AUX),
+                           depth64=job.readback(abi.PLANE_DEPTH64))
+        planes = [job.readback(p) for p in (abi.PLANE_VOLUME, abi.PLANE_MOMENTS, abi.PLANE_SURFACE)]
+    finally:
+        job.close()
+    acc, dep, aux, _, sess = oracle.render(scene.desc(cfg), (0, 0, cfg.width, cfg.height), 0, cfg.samples_per_pixel,
+                                           session=True)
+    assert acc[..., :3].mean() > 1e-3
+    _against_oracle(f"{name} {form}", out, acc, dep, aux)
+    vol, mom, surf = planes
+    assert np.array_equal(vol[..., 3], sess[0][..., 3]) and np.array_equal(surf[..., 3], sess[2][..., 3])
+    assert np.allclose(vol[..., :3] + surf[..., :3], out.accum[..., :3], rtol=1e-4, atol=1e-5)
+    for k, (g, o) in enumerate(((vol, sess[0]), (surf, sess[2]))):
+        close = np.abs(g[..., :3] - o[..., :3]) <= 1e-3 * np.abs(o[..., :3]) + 1e-4
+        frac = np.mean(np.all(close, axis=-1))
+        print(f"{name} {form} session plane {k}: {frac:.4f} within 1e-3")
+        assert frac >= 0.98
+    assert np.all(mom[..., :3] >= 0)
+
+
+def test_fog_wavefront_equals_megakernel(renderer):
+    from syfrack_amd import abi
+    from syfrack_amd import scene as sc
+    scene = fog_scenes()["bulb_fog_sun_lamp"]
+    cfg = sc.RenderConfig(64, 40, 4, 4, 256)
+    mk = renderer.render(scene, cfg)
+    wf = renderer.render(scene, cfg, wavefront=True)
+    assert np.allclose(wf.accum, mk.accum, rtol=1e-6, atol=1e-7)
+    assert np.array_equal(wf.depth, mk.depth)
+
+
+def test_fog_refused_by_the_server_and_resumed_by_checkpoints(renderer, tmp_path):
+    from syfrack_amd import abi, serve
+    from syfrack_amd import scene as sc
+    from syfrack_amd.render import Job
+    scene = fog_scenes()["bulb_fog_sun_lamp"]
+    cfg = sc.RenderConfig(32, 24, 2, 2, 128)
+    with serve.Server(renderer.ctx, "batched", path_steps=8, grid_blocks=512, tile_width=8, quantum_samples=1) as s:
+        with pytest.raises(abi.SyfError) as e:
+            s.submit(scene, cfg)
+        assert e.value.code == abi.SYF_E_INVALID and "fog" in str(e.value)
+    # a checkpoint of a fog job resumes its session planes
+    a = renderer.job(scene, cfg)
+    try:
+        a.render_tile(0, 0, cfg.width, cfg.height, 0, 2)
+        a.sync()
+        a.checkpoint_save(str(tmp_path / "ck"), "(samples_done: 2)")
+        ref = [a.readback(p) for p in (abi.PLANE_ACCUM, abi.PLANE_VOLUME, abi.PLANE_MOMENTS, abi.PLANE_SURFACE)]
+    finally:
+        a.close()
+    b = renderer.job(scene, cfg)
+    try:
+        b.checkpoint_load(str(tmp_path / "ck"))
+        got = [b.readback(p) for p in (abi.PLANE_ACCUM, abi.PLANE_VOLUME, abi.PLANE_MOMENTS, abi.PLANE_SURFACE)]
+    finally:
+        b.close()
+    for x, y in zip(ref, got):
+        assert np.array_equal(x, y)

@@ -17,6 +17,7 @@ def test_state_ron_round_trip_rebuilds_the_scene():
                                                   1.5, True), 0, ((0.5, 0.25, 0.0, 0.0), 0.1, 0.2, 3.0))
     scene.material.finishes = [sc.Finish((0.2, 0.3, 0.4), 0.6, 0.1, 0.05, (1.0, 0.5, 0.0), True), sc.Finish()]
     scene.ground = sc.Ground(-1.5, 2)
+    scene.fog = sc.Fog(0.025, (0.85, 0.8, 0.78), -2.0, 0.4, 0.55)
     cfg.march_math = "exact"
     state = {"shader_fingerprint": 0xfedcba9876543210, "scene": scene, "cfg": cfg, "samples_done": 3, "spd": 2,
              "elapsed_secs": 0.1 + 0.2, "note": None}
