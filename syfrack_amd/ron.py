@@ -4,8 +4,10 @@
 Option (None / Some), strings, booleans and numbers (floats by repr, so they read back exactly):
 the form of ``state.ron``.  ``from_ron`` reads that subset and the reference's graph files
 (graphs/*.ron: enum variants with fields, unit variants, tuples for glam vectors, line
-comments): a named struct or variant with fields becomes a dict with its name under
-"__struct__", a unit variant its name as a string, a tuple a tuple, a list a list, Some(x) x.
+comments) and scene files: a named struct or variant with fields becomes a dict with its name
+under "__struct__", a variant with positional values (``Kelvin(4300.0)``, ``Graph((...))``) a
+``Variant`` (a tuple that carries the name), a unit variant its name as a string, a tuple a tuple,
+a list a list, Some(x) x.
 """
 from __future__ import annotations
 
@@ -59,8 +61,17 @@ def to_ron(value: Any, indent: int = 0) -> str:
     raise TypeError(f"cannot write {type(value).__name__} as RON")
 
 
-_TOKEN = re.compile(r'\s*(?:(//[^\n]*)|("(?:[^"\\]|\\.)*")|([-+]?(?:\d+\.?\d*(?:[eE][-+]?\d+)?|inf(?![A-Za-z_0-9])|NaN(?![A-Za-z_0-9])))'
-                    r'|([A-Za-z_][A-Za-z_0-9]*)|(.))', re.S)
+class Variant(tuple):
+    """An enum variant with positional values: the values as a tuple, the name in ``variant``."""
+
+    def __new__(cls, name: str, values):
+        obj = super().__new__(cls, values)
+        obj.variant = name
+        return obj
+
+
+_TOKEN = re.compile(r'\s*(?:(//[^\n]*)|("(?:[^"\\]|\\.)*")|([-+]?(?:(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?|inf(?![A-Za-z_0-9])|NaN(?![A-Za-z_0-9])))'
+                    r'|(?:r#)?([A-Za-z_][A-Za-z_0-9]*)|(.))', re.S)
 
 
 def from_ron(text: str) -> Any:
@@ -118,7 +129,8 @@ def from_ron(text: str) -> Any:
                 d = group()
                 if isinstance(d, dict):
                     d["__struct__"] = name
-                return d
+                    return d
+                return Variant(name, d if isinstance(d, tuple) else (d,))
             return name
         if t == ("p", "["):
             take()

@@ -350,6 +350,13 @@ class Scene:
     ground: Optional[Ground] = None
     fog: Optional[Fog] = None
 
+    @staticmethod
+    def from_ron(text: str) -> "Scene":
+        """A scene file of the reference (scenes/*.ron) for this path; what it does not render is
+        refused with a ValueError naming the scene element (syfrack_amd.scene_file)."""
+        from .scene_file import scene_from_ron
+        return scene_from_ron(text)
+
     def with_fractal_param(self, path: str, value: float) -> "Scene":
         s = copy.deepcopy(self)
         s.fractal.set_param(path, value)
