@@ -10,7 +10,7 @@
 
 use std::os::raw::{c_char, c_int, c_void};
 
-pub const SYF_ABI_VERSION: c_int = 5;
+pub const SYF_ABI_VERSION: c_int = 6;
 
 pub const SYF_OK: c_int = 0;
 pub const SYF_E_INVALID: c_int = -1;
@@ -36,6 +36,20 @@ pub const SYF_LIGHT_ENV: u32 = 2;
 pub const SYF_MAX_LIGHTS: usize = 4;
 
 pub const SYF_MAX_FINISHES: usize = 8;
+/// FinishPattern in the order of GpuFinish.detail_kind (uniforms.rs:797-812); 0: no detail.
+pub const SYF_PATTERN_NONE: u32 = 0;
+pub const SYF_PATTERN_VEINED_STONE: u32 = 1;
+pub const SYF_PATTERN_PATINA: u32 = 2;
+pub const SYF_PATTERN_BRUSHED_METAL: u32 = 3;
+pub const SYF_PATTERN_BASALT: u32 = 4;
+pub const SYF_PATTERN_RIPPLES: u32 = 5;
+pub const SYF_PATTERN_ROSETTES: u32 = 6;
+pub const SYF_PATTERN_COASTAL_ROCK: u32 = 7;
+pub const SYF_PATTERN_SAND: u32 = 8;
+pub const SYF_PATTERN_DUST: u32 = 9;
+pub const SYF_PATTERN_CONCRETE: u32 = 10;
+pub const SYF_PATTERN_WEATHERED_CONCRETE: u32 = 11;
+pub const SYF_PATTERN_WET_CONCRETE: u32 = 12;
 pub const SYF_MAX_STOPS: usize = 16;
 pub const SYF_COLOR_ITER: u32 = 4;
 pub const SYF_COLOR_NORMAL: u32 = 5;
@@ -208,6 +222,13 @@ pub struct syf_finish {
     pub specular: f32,
     pub emission: [f32; 3],
     pub oren_nayar: u32,
+    pub pattern: u32,
+    pub seed: u32,
+    pub scale: f32,
+    pub strength: f32,
+    pub bump: f32,
+    pub roughness_delta: f32,
+    pub tint: [f32; 3],
 }
 
 #[repr(C)]

@@ -9,7 +9,7 @@
 //! (crates/syfrack-core/src/scene/graph/mod.rs:103-257) to `syf_graph`; the library validates it with
 //! the reference's rules and messages.  What this path does not render is refused with an error naming
 //! the scene element: a 2D scene, projector, tube and HDRI lights, the seascape, portal, nebula and
-//! dunes, textures, finish detail patterns that change the shading, optics and phenomena.  The height
+//! dunes, textures, optics and phenomena.  The height
 //! fog is lowered.
 
 use std::ffi::{CStr, CString};
@@ -172,32 +172,44 @@ fn coloring(c: &Coloring) -> Result<syf_coloring, Error> {
     Ok(out)
 }
 
-/// Finish as GpuFinish (uniforms.rs:787-817).  A detail pattern is lowered only where it leaves the
-/// shading unchanged (strength, bump and roughness_delta zero: make_surface, pathtrace.wgsl:307-345);
-/// the concrete patterns then keep their Oren-Nayar diffuse lobe (pathtrace.wgsl:381-384).
+/// Finish with its FinishDetail as GpuFinish (uniforms.rs:787-817): the pattern numbered as
+/// detail_kind, its seed, scale, strength, bump, roughness delta and tint.
 fn finish(i: usize, f: &Finish) -> Result<syf_finish, Error> {
     if f.optics.is_some() {
         return Err(refuse(format!("material.finishes[{i}].optics: not rendered on this path")));
     }
-    let mut oren_nayar = 0;
-    if let Some(d) = &f.detail {
-        if d.strength != 0.0 || d.bump != 0.0 || d.roughness_delta != 0.0 {
-            return Err(refuse(format!(
-                "material.finishes[{i}].detail: finish patterns are not rendered on this path (a detail with zero \
-                 strength, bump and roughness_delta is)"
-            )));
-        }
-        use FinishPattern::*;
-        oren_nayar = matches!(d.pattern, Concrete | WeatheredConcrete | WetConcrete) as u32;
-    }
-    Ok(syf_finish {
+    let mut out = syf_finish {
         albedo: f.albedo,
         roughness: f.roughness,
         metallic: f.metallic,
         specular: f.specular,
         emission: f.emission,
-        oren_nayar,
-    })
+        ..Default::default()
+    };
+    if let Some(d) = &f.detail {
+        use FinishPattern::*;
+        out.pattern = match d.pattern {
+            VeinedStone => 1,
+            Patina => 2,
+            BrushedMetal => 3,
+            Basalt => 4,
+            Ripples => 5,
+            Rosettes => 6,
+            CoastalRock => 7,
+            Sand => 8,
+            Dust => 9,
+            Concrete => 10,
+            WeatheredConcrete => 11,
+            WetConcrete => 12,
+        };
+        out.seed = d.seed;
+        out.scale = d.scale;
+        out.strength = d.strength;
+        out.bump = d.bump;
+        out.roughness_delta = d.roughness_delta;
+        out.tint = d.tint;
+    }
+    Ok(out)
 }
 
 /// The job description of a 3D scene (everything but the form graph, which goes to

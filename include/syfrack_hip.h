@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SYF_ABI_VERSION 5
+#define SYF_ABI_VERSION 6
 
 /* ---------------------------------------------------------------- status codes
  * Replaces anyhow::Result at the seams: config errors (renderer.rs:81-138),
@@ -119,10 +119,29 @@ typedef struct syf_material {
  * flat ground plane.  All zero: no finishes, no colouring, no ground. */
 #define SYF_MAX_FINISHES 8   /* GpuFinish slots (uniforms.rs:787) */
 #define SYF_MAX_STOPS 16     /* MAX_PALETTE_STOPS (palette.rs:6) */
-/* Finish (material.rs:161-174) as GpuFinish (uniforms.rs:787-817).  A finish's detail pattern
- * is accepted only where it leaves the shading unchanged (strength, bump and roughness_delta
- * zero); then the Concrete, WeatheredConcrete and WetConcrete patterns keep their Oren-Nayar
- * diffuse lobe (pathtrace.wgsl:381-384, 407-418), selected by oren_nayar = 1. */
+/* FinishPattern (material.rs:177-191) in the order of GpuFinish.detail_kind (uniforms.rs:797-812). */
+#define SYF_PATTERN_NONE 0u
+#define SYF_PATTERN_VEINED_STONE 1u
+#define SYF_PATTERN_PATINA 2u
+#define SYF_PATTERN_BRUSHED_METAL 3u
+#define SYF_PATTERN_BASALT 4u
+#define SYF_PATTERN_RIPPLES 5u
+#define SYF_PATTERN_ROSETTES 6u
+#define SYF_PATTERN_COASTAL_ROCK 7u
+#define SYF_PATTERN_SAND 8u
+#define SYF_PATTERN_DUST 9u
+#define SYF_PATTERN_CONCRETE 10u
+#define SYF_PATTERN_WEATHERED_CONCRETE 11u
+#define SYF_PATTERN_WET_CONCRETE 12u
+/* Finish (material.rs:161-174) with its FinishDetail (material.rs:193-221) as GpuFinish
+ * (uniforms.rs:787-817).  A detail pattern shapes the finish at the hit (finish_detail.wgsl,
+ * make_surface pathtrace.wgsl:306-329): its height blends albedo toward the tint by strength
+ * and moves the roughness by roughness_delta, its slope tilts the shading normal by bump, each
+ * band faded by the path's ray footprint.  The Concrete, WeatheredConcrete and WetConcrete
+ * patterns also give the diffuse lobe Oren-Nayar's sigma = 0.45 x roughness
+ * (pathtrace.wgsl:381-384, 407-418); oren_nayar = 1 asks for that lobe without a pattern.
+ * The coastal rock's tide line and the sand's wind direction belong to the seascape and the
+ * dunes, absent here: the tide is dry and the wind blows along +x. */
 typedef struct syf_finish {
     float albedo[3];
     float roughness;
@@ -130,6 +149,13 @@ typedef struct syf_finish {
     float specular;
     float emission[3];        /* linear emitted radiance */
     uint32_t oren_nayar;      /* 1: rough diffuse with sigma = 0.45 x roughness */
+    uint32_t pattern;         /* SYF_PATTERN_*; 0: no detail */
+    uint32_t seed;            /* 0..65535 */
+    float scale;              /* 0.01..200: pattern cells per scene unit */
+    float strength;           /* 0..1 */
+    float bump;               /* -1..1 */
+    float roughness_delta;    /* -1..1 */
+    float tint[3];            /* 0..1 */
 } syf_finish;
 #define SYF_COLOR_ITER 4u     /* COLOR_SLOT_ITER (common.wgsl:13): iterations / iteration product */
 #define SYF_COLOR_NORMAL 5u   /* COLOR_SLOT_NORMAL: n.y / 2 + 1/2 */

@@ -76,6 +76,8 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_fog_free_flight.argtypes = [P, P, P, C.c_float, C.c_size_t, P]
     lib.orc_volume_coordinate.restype = C.c_float
     lib.orc_volume_coordinate.argtypes = [C.c_uint32] * 3
+    lib.orc_finish_relief.restype = None
+    lib.orc_finish_relief.argtypes = [C.c_uint32, C.c_uint32, C.c_float, P, C.c_int, C.c_float, P]
     lib.orc_hg_phase.restype = C.c_float
     lib.orc_hg_phase.argtypes = [C.c_float, C.c_float]
     lib.orc_hg_sample.restype = None
@@ -311,6 +313,15 @@ def fog_free_flight(desc, ro, rd, t_end: float, n: int) -> np.ndarray:
 
 def volume_coordinate(sample_index: int, dimension: int, stream: int) -> float:
     return load().orc_volume_coordinate(sample_index, dimension, stream & 0xFFFFFFFF)
+
+
+def finish_relief(pattern: int, seed: int, scale: float, points, footprint: float) -> np.ndarray:
+    """finish_eval (finish_detail.wgsl:38-151) at each point: rows (slope x, y, z, height)."""
+    p = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
+    out = np.zeros((len(p), 4), np.float32)
+    load().orc_finish_relief(pattern, seed, C.c_float(scale), p.ctypes.data, len(p), C.c_float(footprint),
+                             out.ctypes.data)
+    return out
 
 
 def hg_phase(cos_theta: float, g: float) -> float:

@@ -65,7 +65,16 @@
 //       surface proposal for phase-sampled escapes, a sign in the phase sample, swapped Halton bases,
 //       a homogeneous inversion of a height fog, a phase hit of an emitter counted again) each fail
 //       at least one of them.
-// A function without such a pin would be marked "parity unpinned" here; none is.
+//   finish detail patterns (lattice, lattice_noise, lattice_fbm, finish_relief, the detail block
+//       of make_surface, the ray footprint of radiance) -> tests/test_oracle_finish.py: the slope
+//       against a finite difference of the height for patterns 1-7, 9 and 10; every band's fade to
+//       its closed-form mean at a huge footprint; sand's ripple slope, weathered concrete's rain
+//       slope and wet concrete's pool ripples in footprint ranges where their parts separate; the
+//       lattice's range, mean and seeding.  Planted mistakes (the warp's Jacobian factor of dust, the
+//       rosette's angular term, the pore slope's frequency, the rain slope's weight, an octave slope
+//       without its frequency) each fail at least one of them.  Parity unpinned: the grain weights of
+//       sand (2.5) and wet concrete (16), which add a shading slope no height follows.
+// Every other function has such a pin.
 
 #include <cmath>
 #include <cstdint>
@@ -779,9 +788,191 @@ float surface_value(const SceneL& S, uint32_t slot, F3 n, const DeOut<float>& sm
     if (slot == SYF_COLOR_NORMAL) return n.y * 0.5f + 0.5f;
     return float(smp.iter) / std::fmax(float(S.F.iteration_product), 1.0f);
 }
+// ---------------------------------------------------------------- finish detail patterns
+// finish_detail.wgsl: per pattern, a height h in [0, 1] and its slope (the analytic gradient of
+// h in world space for patterns 1-7, 9 and 10; a shading slope for 8, 11 and 12).
+struct Relief {
+    F3 g;
+    float h;
+};
+inline float sstep(float a, float b, float x) {
+    const float t = clampv((x - a) / (b - a), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+inline float sstep_d(float a, float b, float x) {  // finish_smooth_derivative
+    const float t = clampv((x - a) / (b - a), 0.0f, 1.0f);
+    return 6.0f * t * (1.0f - t) / (b - a);
+}
+inline float sgn(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+inline F3 hadamard(F3 a, float x, float y, float z) { return v3(a.x * x, a.y * y, a.z * z); }
+
+// finish_hash (finish_detail.wgsl:3-7)
+float lattice(int32_t x, int32_t y, int32_t z, uint32_t seed) {
+    const uint32_t h = pcg(uint32_t(x) ^ pcg(uint32_t(y) + 0x9e3779b9u) ^ pcg(uint32_t(z) + seed));
+    return float(h >> 8) / 16777216.0f;
+}
+// finish_noise (finish_detail.wgsl:8-20): smoothstep-weighted trilinear lattice noise
+Relief lattice_noise(F3 p, uint32_t seed) {
+    const float px = std::floor(p.x), py = std::floor(p.y), pz = std::floor(p.z);
+    const int32_t i = int32_t(px), j = int32_t(py), k = int32_t(pz);
+    const float fx = p.x - px, fy = p.y - py, fz = p.z - pz;
+    const float wx = fx * fx * (3.0f - 2.0f * fx), wy = fy * fy * (3.0f - 2.0f * fy), wz = fz * fz * (3.0f - 2.0f * fz);
+    float c[2][2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int d = 0; d < 2; ++d) c[a][b][d] = lattice(i + a, j + b, k + d, seed);
+    const float y00 = mixf(c[0][0][0], c[1][0][0], wx), y10 = mixf(c[0][1][0], c[1][1][0], wx);
+    const float y01 = mixf(c[0][0][1], c[1][0][1], wx), y11 = mixf(c[0][1][1], c[1][1][1], wx);
+    const float z0 = mixf(y00, y10, wy), z1 = mixf(y01, y11, wy);
+    const float ddx = mixf(mixf(c[1][0][0] - c[0][0][0], c[1][1][0] - c[0][1][0], wy),
+                           mixf(c[1][0][1] - c[0][0][1], c[1][1][1] - c[0][1][1], wy), wz);
+    const float ddy = mixf(y10 - y00, y11 - y01, wz);
+    const float ddz = z1 - z0;
+    Relief r;
+    r.g = v3(ddx * 6.0f * fx * (1.0f - fx), ddy * 6.0f * fy * (1.0f - fy), ddz * 6.0f * fz * (1.0f - fz));
+    r.h = mixf(z0, z1, wz);
+    return r;
+}
+// finish_fbm (finish_detail.wgsl:21-33)
+Relief lattice_fbm(F3 p, float width, uint32_t seed) {
+    Relief out{v3(0.0f, 0.0f, 0.0f), 0.0f};
+    float freq = 1.0f, amp = 0.5f;
+    for (uint32_t o = 0; o < 3; ++o) {
+        const float vis = 1.0f - sstep(0.35f, 0.85f, width * freq);
+        if (vis > 0.0f) {
+            const Relief n = lattice_noise(p * freq, seed + o * 97u);
+            out.g = out.g + n.g * (amp * freq * vis);
+            out.h += amp * mixf(0.5f, n.h, vis);
+        } else {
+            out.h += amp * 0.5f;
+        }
+        freq *= 2.13f;
+        amp *= 0.5f;
+    }
+    out.g = v3(out.g.x / 0.875f, out.g.y / 0.875f, out.g.z / 0.875f);
+    out.h /= 0.875f;
+    return out;
+}
+// finish_eval (finish_detail.wgsl:38-151) without dunes (wind 0) or ocean
+Relief finish_relief(uint32_t kind, uint32_t seed, float scale, F3 p, float footprint) {
+    const F3 q = p * scale;
+    const float width = footprint * scale;
+    switch (kind) {
+    case 12u: {
+        const Relief basin = lattice_fbm(hadamard(q, 0.31f, 0.31f, 0.22f), width * 0.31f, seed + 911u);
+        const float wet = sstep(0.47f, 0.59f, basin.h);
+        const Relief grit = lattice_fbm(q * 16.0f, width * 16.0f, seed + 113u);
+        const float vis = 1.0f - sstep(0.15f, 0.6f, width * 3.0f);
+        const float ph = q.x * 2.8f + q.z * 4.1f;
+        const F3 ripple = v3(2.8f, 0.0f, 4.1f) * (std::cos(ph) * 0.018f * vis);
+        return Relief{mix3(grit.g * 16.0f, ripple, wet) * scale, wet};
+    }
+    case 10u:
+    case 11u: {
+        const Relief stains = lattice_fbm(q * 0.14f, width * 0.14f, seed + 17u);
+        const Relief aggregate = lattice_fbm(q * 7.0f, width * 7.0f, seed + 113u);
+        const Relief pore = lattice_noise(q * 23.0f, seed + 307u);
+        const float vis = 1.0f - sstep(0.25f, 0.75f, width * 23.0f);
+        const float pit = 1.0f - sstep(0.24f, 0.43f, pore.h);
+        const F3 pit_g = pore.g * (-sstep_d(0.24f, 0.43f, pore.h) * 23.0f * vis);
+        const float ph = (q.y / 1.25f + 0.5f) - std::floor(q.y / 1.25f + 0.5f) - 0.5f;
+        const float board = 1.0f - sstep(0.03f, 0.15f, width);
+        const float joint = (1.0f - sstep(0.006f, 0.027f, std::fabs(ph))) * board;
+        const float joint_g = -sstep_d(0.006f, 0.027f, std::fabs(ph)) * sgn(ph) / 1.25f * board;
+        const float height = 0.18f * stains.h + 0.65f * aggregate.h - 0.14f * mixf(0.2f, pit, vis) - 0.09f * joint;
+        const F3 g = (stains.g * (0.18f * 0.14f) + aggregate.g * (0.65f * 7.0f) - pit_g * 0.14f -
+                      v3(0.0f, 0.09f * joint_g, 0.0f)) * scale;
+        if (kind == 11u) {
+            const Relief rain = lattice_fbm(hadamard(q, 1.9f, 0.055f, 1.9f), width * 1.9f, seed + 601u);
+            const float deposit = sstep(0.33f, 0.68f, rain.h);
+            const F3 rain_g = hadamard(rain.g, 1.9f, 0.055f, 1.9f) * (scale * sstep_d(0.33f, 0.68f, rain.h));
+            return Relief{g + rain_g * 0.12f, clampv(0.72f * deposit + 0.28f * stains.h, 0.0f, 1.0f)};
+        }
+        return Relief{g, clampv(height + 0.12f, 0.0f, 1.0f)};
+    }
+    case 9u: {
+        const Relief wa = lattice_fbm(q * 0.17f, width * 0.17f, seed + 17u);
+        const Relief wb = lattice_fbm(q * 0.17f, width * 0.17f, seed + 53u);
+        const F3 warped = q + v3(2.8f * (wa.h - 0.5f), 0.0f, 2.8f * (wb.h - 0.5f));
+        const Relief cloud = lattice_fbm(warped * 0.16f, width * 0.48f, seed + 211u);
+        const Relief streak = lattice_fbm(hadamard(warped, 0.65f, 0.18f, 0.09f), width * 1.95f, seed + 397u);
+        const float density = 0.78f * cloud.h + 0.22f * streak.h;
+        const F3 g = cloud.g * (0.78f * 0.16f) + hadamard(streak.g, 0.65f, 0.18f, 0.09f) * 0.22f;
+        const F3 dg = (g + (wa.g * g.x + wb.g * g.z) * (2.8f * 0.17f)) * scale;
+        const float deposit = sstep(0.36f, 0.64f, density);
+        const Relief grain = lattice_fbm(p * 48.0f, footprint * 48.0f, seed + 739u);
+        const float powder = 0.85f + 0.15f * grain.h;
+        return Relief{dg * (sstep_d(0.36f, 0.64f, density) * powder) + grain.g * (deposit * 0.15f * 48.0f),
+                      deposit * powder};
+    }
+    case 8u: {
+        const float x = p.x * scale, z = p.z * scale;
+        const Relief warp = lattice_noise(v3(x * 0.045f, 0.0f, z * 0.045f), seed);
+        const float ph = 2.0f * PI_F * (x + 1.7f * warp.h + 0.12f * std::sin(z * 0.22f));
+        const float vis = std::exp(-((width * 3.0f) * (width * 3.0f)));
+        const float ripple = 0.5f + 0.36f * std::sin(ph) + 0.10f * std::sin(2.0f * ph);
+        const float a = 2.0f * PI_F * (0.36f * std::cos(ph) + 0.20f * std::cos(2.0f * ph)) * scale * vis;
+        const Relief grain = lattice_fbm(p * 90.0f, footprint * 90.0f, seed + 139u);
+        return Relief{v3(a * (1.0f + 0.0765f * warp.g.x), 0.0f, a * (0.0264f * std::cos(z * 0.22f) + 0.0765f * warp.g.z)) +
+                          grain.g * 2.5f,
+                      0.70f * mixf(0.5f, ripple, vis) + 0.30f * grain.h};
+    }
+    case 6u: {
+        const float u = (q.x + 0.5f) - std::floor(q.x + 0.5f) - 0.5f, w = (q.z + 0.5f) - std::floor(q.z + 0.5f) - 0.5f;
+        const float r = std::fmax(std::sqrt(u * u + w * w), 1e-5f);
+        const float ang = std::atan2(w, u);
+        const float flower = r - 0.25f - 0.07f * std::cos(ang * 8.0f);
+        const float ring = r - 0.45f;
+        float gu = sgn(flower) * (u / r + 0.56f * std::sin(ang * 8.0f) * (-w / (r * r)));
+        float gw = sgn(flower) * (w / r + 0.56f * std::sin(ang * 8.0f) * (u / (r * r)));
+        float dist = std::fabs(flower);
+        if (std::fabs(ring) < dist) {
+            dist = std::fabs(ring);
+            gu = sgn(ring) * u / r;
+            gw = sgn(ring) * w / r;
+        }
+        const float vis = 1.0f - sstep(0.10f, 0.40f, width);
+        const float carving = 1.0f - sstep(0.012f, 0.045f, dist);
+        const float k = -sstep_d(0.012f, 0.045f, dist) * vis * scale;
+        return Relief{v3(gu * k, 0.0f, gw * k), mixf(0.2f, carving, vis)};
+    }
+    case 3u: {
+        const Relief n = lattice_fbm(hadamard(q, 18.0f, 0.45f, 3.0f), width * 18.0f, seed);
+        return Relief{hadamard(n.g, 18.0f, 0.45f, 3.0f) * scale, n.h};
+    }
+    default:
+        break;
+    }
+    const Relief n = lattice_fbm(q, width, seed);
+    Relief r = n;
+    if (kind == 1u) {
+        const F3 axis = v3(0.16f, 0.11f, 0.08f);
+        const float ph = (dot(q, axis) + n.h * 2.8f) * PI_F * 2.0f;
+        const float wave = std::sin(ph);
+        const float veins = 1.0f - sstep(0.035f, 0.19f, std::fabs(wave));
+        const F3 vg = (axis + n.g * 2.8f) * (-sstep_d(0.035f, 0.19f, std::fabs(wave)) * sgn(wave) * std::cos(ph) * PI_F * 2.0f);
+        const float vis = 1.0f - sstep(0.18f, 0.45f, width);
+        r.g = mix3(n.g, n.g * 0.65f + vg * 0.35f, vis);
+        r.h = mixf(n.h, 0.65f * n.h + 0.35f * veins, vis);
+    } else if (kind == 2u) {
+        r.g = n.g * sstep_d(0.35f, 0.67f, n.h);
+        r.h = sstep(0.35f, 0.67f, n.h);
+    } else if (kind == 4u || kind == 7u) {
+        const Relief grain = lattice_fbm(q * 3.7f, width * 3.7f, seed + 53u);
+        r.g = n.g * 0.8f + grain.g * (0.2f * 3.7f);
+        r.h = 0.8f * n.h + 0.2f * grain.h;
+    } else if (kind == 5u) {
+        const float a = q.x + 0.75f * n.h, b = q.z * 1.37f + q.x * 0.4f;
+        const float vis = 1.0f - sstep(0.35f, 1.0f, width);
+        r.g = ((v3(1.0f, 0.0f, 0.0f) + n.g * 0.75f) * (0.32f * std::cos(a)) + v3(0.4f, 0.0f, 1.37f) * (0.18f * std::cos(b))) * vis;
+        r.h = 0.5f + (0.32f * std::sin(a) + 0.18f * std::sin(b)) * vis;
+    }
+    return Relief{r.g * scale, r.h};
+}
+
 // make_surface (pathtrace.wgsl:271-386) for the subset: colouring ramp, named finishes
 // (emission; Oren-Nayar for the concrete patterns, sigma = 0.45 roughness), SurfaceVariation.
-Surf make_surface(const SceneL& S, F3 n, const DeOut<float>& smp) {
+Surf make_surface(const SceneL& S, F3 p, F3 n, const DeOut<float>& smp, float footprint) {
     const syf_material& m = S.mat;
     F3 albedo = f3(m.albedo);
     const uint32_t slot = S.color_slot;
@@ -813,16 +1004,46 @@ Surf make_surface(const SceneL& S, F3 n, const DeOut<float>& smp) {
         metallic = f.metallic;
         spec = f.specular;
         emission = f3(f.emission);
-        sigma_scale = f.oren_nayar ? 0.45f : 0.0f;
+        sigma_scale = (f.oren_nayar || f.pattern >= 10u) ? 0.45f : 0.0f;
     }
     if (!assigned && m.surface_enabled) {
         const float t = clampv(surface_value(S, m.surface_slot, n, smp), 0.0f, 1.0f);
         rough = m.surface_roughness[0] + (m.surface_roughness[1] - m.surface_roughness[0]) * t;
         metallic = m.surface_metallic[0] + (m.surface_metallic[1] - m.surface_metallic[0]) * t;
     }
+    F3 shading_n = n;
+    if (assigned && S.finishes[smp.material - 1u].pattern != 0u) {  // pathtrace.wgsl:306-329
+        const syf_finish& f = S.finishes[smp.material - 1u];
+        Relief d = finish_relief(f.pattern, f.seed, f.scale, p, footprint);
+        if (f.pattern == 9u) {
+            const float up = sstep(0.10f, 0.80f, n.y);
+            d.g = d.g * up;
+            d.h *= up;
+        }
+        if (f.pattern == 12u) d.h *= sstep(0.45f, 0.95f, n.y);
+        const float amount = f.strength * d.h;
+        albedo = mix3(albedo, f3(f.tint), amount);
+        if (f.pattern == 9u) {
+            rough += std::fmax(f.roughness_delta, 0.0f) * amount;
+            metallic *= 1.0f - amount;
+            spec = mixf(spec, 0.025f, amount);
+        } else if (f.pattern == 12u) {
+            rough = mixf(rough, clampv(rough + f.roughness_delta, 0.025f, 1.0f), amount);
+            spec = mixf(spec, 0.0204f, amount);
+            metallic *= 1.0f - amount;
+        } else {
+            rough += f.roughness_delta * (2.0f * d.h - 1.0f);
+        }
+        if (f.pattern == 2u) metallic *= 1.0f - 0.75f * amount;
+        if (std::fabs(f.bump) > 0.0f) {  // finish_normal
+            const F3 slope = d.g - n * dot(d.g, n);
+            const float lim = 0.75f / std::fmax(std::sqrt(dot(slope, slope)), 1e-5f);
+            shading_n = safe_normalize(n - slope * clampv(f.bump, -lim, lim));
+        }
+    }
     rough = clampv(rough, 0.02f, 1.0f);
     Surf s;
-    s.n = n;
+    s.n = shading_n;
     s.diffuse = albedo * (1.0f - metallic);
     s.f0 = mix3(v3(spec, spec, spec), albedo, metallic);
     s.alpha = rough * rough;
@@ -1318,6 +1539,7 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t sample, uint32_t
     camera_ray(S.cam, S.width, S.height, px, py, rng, &ro, &rd);
     F3 throughput = v3(1.0f, 1.0f, 1.0f), L = v3(0.0f, 0.0f, 0.0f);
     float prev_pdf = 0.0f, prev_total = 0.0f;
+    float tex_width = 0.0f, tex_spread = S.pixel_cone * 0.5f;  // pathtrace.wgsl:811-814
     bool prev_from_bsdf = false, prev_from_volume = false, prev_volume_nee = false, volume_path = false;
     F3 prev_n = v3(0.0f, 0.0f, 0.0f), primary_volume = v3(0.0f, 0.0f, 0.0f);
     const uint32_t point_stream = pcg(pcg(px ^ S.seed) ^ pcg(py ^ 0x27d4eb2du));  // :822
@@ -1350,6 +1572,8 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t sample, uint32_t
             if (t_s < t_end) {
                 if (bounce == 0) volume_path = true;
                 const F3 p_s = ro + rd * t_s;
+                tex_width += t_s * tex_spread;
+                tex_spread = std::fmax(tex_spread, 0.25f);
                 if (bounce == 0 && !h.hit) *first_depth = t_s;
                 const float vat = area_total_weight(S, p_s, v3(0.0f, 0.0f, 0.0f));
                 const bool last = bounce == S.max_bounces;
@@ -1402,7 +1626,8 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t sample, uint32_t
             smp.iter = 0u;
             smp.material = S.ground.finish;
         }
-        Surf s = make_surface(S, n, smp);
+        tex_width += h.t * tex_spread;
+        Surf s = make_surface(S, p, n, smp, std::fmax(tex_width, 1e-4f));
         L = L + mul3(throughput, s.emission);
         float offset = ground ? 1e-3f : std::fmax(h.t * S.pixel_cone * 2.0f, 1e-3f) * S.pixel_tol;
         F3 p_off = p + n * offset;
@@ -1440,6 +1665,7 @@ F3 radiance(const SceneL& S, uint32_t px, uint32_t py, uint32_t sample, uint32_t
         }
         ro = p_off;
         rd = l;
+        tex_spread = std::fmax(tex_spread, s.alpha * 0.25f);
         ++bounce;
     }
     *volume_out = volume_path ? L : primary_volume;
@@ -1877,6 +2103,17 @@ void orc_fog_free_flight(const syf_job_desc* d, const float* ro, const float* rd
     for (size_t i = 0; i < n; ++i) {
         uint32_t rng = pcg(uint32_t(i) * 2654435761u + 12345u);
         out[i] = fog_free_flight(S, v3(ro[0], ro[1], ro[2]), v3(rd[0], rd[1], rd[2]), t_end, &rng);
+    }
+}
+// finish_relief at n points (xyz rows): out4 = slope xyz, height
+void orc_finish_relief(uint32_t kind, uint32_t seed, float scale, const float* p, int n, float footprint,
+                       float* out4) {
+    for (int i = 0; i < n; ++i) {
+        const Relief r = finish_relief(kind, seed, scale, v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), footprint);
+        out4[4 * i] = r.g.x;
+        out4[4 * i + 1] = r.g.y;
+        out4[4 * i + 2] = r.g.z;
+        out4[4 * i + 3] = r.h;
     }
 }
 float orc_volume_coordinate(uint32_t sample_index, uint32_t dimension, uint32_t stream) {

@@ -36,6 +36,9 @@ LIGHT_ENV = 2
 MAX_LIGHTS = 4
 
 MAX_FINISHES = 8
+# FinishPattern (material.rs:177-191) numbered as GpuFinish.detail_kind (uniforms.rs:797-812)
+PATTERNS = ("VeinedStone", "Patina", "BrushedMetal", "Basalt", "Ripples", "Rosettes", "CoastalRock", "Sand", "Dust",
+            "Concrete", "WeatheredConcrete", "WetConcrete")
 MAX_STOPS = 16
 COLOR_ITER = 4
 COLOR_NORMAL = 5
@@ -144,6 +147,13 @@ class SyfFinish(C.Structure):
         ("specular", C.c_float),
         ("emission", F3),
         ("oren_nayar", C.c_uint32),
+        ("pattern", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("scale", C.c_float),
+        ("strength", C.c_float),
+        ("bump", C.c_float),
+        ("roughness_delta", C.c_float),
+        ("tint", F3),
     ]
 
 

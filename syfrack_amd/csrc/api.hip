@@ -166,6 +166,27 @@ int lower_shading(const syf_job_desc& d, bool graph, JobP& P) {
             !std::isfinite(f.specular))
             return fail(SYF_E_INVALID, "finishes[%u] must be finite", i);
         if (f.oren_nayar > 1u) return fail(SYF_E_INVALID, "finishes[%u].oren_nayar must be 0 or 1", i);
+        if (f.pattern > SYF_PATTERN_WET_CONCRETE)
+            return fail(SYF_E_INVALID, "finishes[%u].pattern must be 0..%u, got %u", i, SYF_PATTERN_WET_CONCRETE,
+                        f.pattern);
+        if (f.pattern != SYF_PATTERN_NONE) {
+            // FinishDetail bounds (material.rs:223-250)
+            if (f.seed > 65535u) return fail(SYF_E_INVALID, "finishes[%u].detail.seed must be 0..65535, got %u", i, f.seed);
+            if (!(f.scale >= 0.01f && f.scale <= 200.0f))
+                return fail(SYF_E_INVALID, "finishes[%u].detail.scale must be between 0.01 and 200, got %g", i,
+                            double(f.scale));
+            if (!(f.strength >= 0.0f && f.strength <= 1.0f))
+                return fail(SYF_E_INVALID, "finishes[%u].detail.strength must be between 0 and 1, got %g", i,
+                            double(f.strength));
+            if (!(f.bump >= -1.0f && f.bump <= 1.0f))
+                return fail(SYF_E_INVALID, "finishes[%u].detail.bump must be between -1 and 1, got %g", i, double(f.bump));
+            if (!(f.roughness_delta >= -1.0f && f.roughness_delta <= 1.0f))
+                return fail(SYF_E_INVALID, "finishes[%u].detail.roughness_delta must be between -1 and 1, got %g", i,
+                            double(f.roughness_delta));
+            for (int k = 0; k < 3; ++k)
+                if (!(f.tint[k] >= 0.0f && f.tint[k] <= 1.0f))
+                    return fail(SYF_E_INVALID, "finishes[%u].detail.tint must be within [0, 1]", i);
+        }
         FinishP& F = S.finishes[i];
         for (int k = 0; k < 3; ++k) {
             F.albedo[k] = f.albedo[k];
@@ -174,7 +195,15 @@ int lower_shading(const syf_job_desc& d, bool graph, JobP& P) {
         F.roughness = f.roughness;
         F.metallic = f.metallic;
         F.specular = f.specular;
-        F.sigma_scale = f.oren_nayar ? 0.45f : 0.0f;
+        F.sigma_scale = (f.oren_nayar || f.pattern >= SYF_PATTERN_CONCRETE) ? 0.45f : 0.0f;
+        F.pattern = f.pattern;
+        F.seed = f.pattern ? f.seed : 0u;
+        F.scale = f.pattern ? f.scale : 1.0f;
+        F.strength = f.pattern ? f.strength : 0.0f;
+        F.bump = f.pattern ? f.bump : 0.0f;
+        F.roughness_delta = f.pattern ? f.roughness_delta : 0.0f;
+        for (int k = 0; k < 3; ++k) F.tint[k] = f.pattern ? f.tint[k] : 0.0f;
+        if (f.pattern) S.detail = 1u;
     }
     S.ground_enabled = d.ground.enabled ? 1u : 0u;
     if (d.ground.enabled && !std::isfinite(d.ground.height))

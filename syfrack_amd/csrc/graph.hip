@@ -1076,7 +1076,7 @@ std::string jit_source(const std::vector<GOp>& prog) {
 const char* const JIT_NAMES[JIT_KERNELS] = {
     "syf::k_primary<7u>",   "syf::k_primary_exact<7u>", "syf::k_hits_tail<7u>", "syf::k_de_probe<7u>",
     "syf::k_render_tile<7u>", "syf::k_wf_intersect<7u>", "syf::k_wf_shade<7u>",  "syf::k_wf_commit<7u>",
-    "syf::k_render_tile_fog<7u>"};
+    "syf::k_render_tile_full<7u>"};
 static_assert(FAM_JIT == 7u, "the run-time kernel names carry the family");
 
 struct JitModule {

@@ -114,7 +114,7 @@ enum : uint32_t {
     JIT_k_wf_intersect,
     JIT_k_wf_shade,
     JIT_k_wf_commit,
-    JIT_k_render_tile_fog,
+    JIT_k_render_tile_full,
     JIT_KERNELS
 };
 // Launches kernel `which` of the compiled topology `jit` (FractalP::gjit) with its arguments'

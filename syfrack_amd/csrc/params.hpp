@@ -78,6 +78,9 @@ struct FinishP {
     float roughness, metallic, specular;
     float emission[3];
     float sigma_scale;        // Oren-Nayar: sigma = sigma_scale * roughness (0.45, or 0: Lambert)
+    uint32_t pattern, seed;   // detail pattern 1..12 (0: none) and its seed (detail_kind)
+    float scale, strength, bump, roughness_delta;  // detail_mapping
+    float tint[3];            // detail_tint
 };
 
 // Colouring ramp, ground plane and finishes (GpuColoring, GpuAtmosphere.ground, finishes).
@@ -95,6 +98,8 @@ struct ShadeP {
     // the shading reads more of a surface hit's field sample than trap slot 0: the other traps,
     // the iteration count or the material (then the sample is evaluated in full at the hit)
     uint32_t full_sample;
+    // a finish has a detail pattern (the path carries its ray footprint to the hits)
+    uint32_t detail;
     // height fog (GpuAtmosphere.a, b): density 0 = none
     float fog_density, fog_height, fog_falloff, fog_g;
     float fog_color[3];

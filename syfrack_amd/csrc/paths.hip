@@ -243,7 +243,7 @@ template <uint32_t FAM> __device__ __forceinline__ void shade_surface(const JobP
     ShadeSample smp = ground ? ground_sample(P) : carried_sample(P, L.seg_trap0);
     if (!ground && P.sh.full_sample) smp.iter = L.light_i;
     F3 emission;
-    L.surf = make_surface(P, n, smp, emission);
+    L.surf = make_surface<false>(P, p, n, smp, 1e-4f, emission);  // the server takes no finish detail patterns
     st.L = add(st.L, mul3(st.throughput, emission));
     const float offset = ground ? 1e-3f : fmaxf(L.seg_t * P.pixel_cone * 2.0f, 1e-3f) * P.pixel_tolerance;
     L.p_off = add(p, scl(n, offset));

@@ -43,8 +43,8 @@ hipError_t launch_render_tile(const JobP& P, const JobP* dP, uint32_t x0, uint32
     const uint32_t tiles_x = (w + 7u) / 8u, tiles = tiles_x * ((h + 7u) / 8u);
     const dim3 grid((tiles + 3u) / 4u), block(256);
     hipError_t e = hipSuccess;
-    if (P.sh.fog_density > 0.0f) {
-        SYF_DISPATCH(e, P.f, k_render_tile_fog, grid, block, 0, s, dP, x0, y0, w, h, s0, ns, tiles_x, tiles, accum,
+    if (P.sh.fog_density > 0.0f || P.sh.detail) {
+        SYF_DISPATCH(e, P.f, k_render_tile_full, grid, block, 0, s, dP, x0, y0, w, h, s0, ns, tiles_x, tiles, accum,
                      depth, aux, depth64, vol, ctr);
     } else {
         SYF_DISPATCH(e, P.f, k_render_tile, grid, block, 0, s, dP, x0, y0, w, h, s0, ns, tiles_x, tiles, accum, depth,

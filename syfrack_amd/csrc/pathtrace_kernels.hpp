@@ -112,7 +112,7 @@ __device__ __forceinline__ void render_pixel(const JobP& P, uint32_t px, uint32_
         acc.y += ok ? r.y : 0.0f;
         acc.z += ok ? r.z : 0.0f;
         acc.w += 1.0f;
-        if (FOG) accumulate_session(vol, size_t(P.width) * P.height, idx, r, st, ok);
+        if (FOG && vol) accumulate_session(vol, size_t(P.width) * P.height, idx, r, st, ok);
         if (s == 0u) {
             fd0 = fd;
             fa0 = fa;
@@ -166,9 +166,11 @@ __global__ __launch_bounds__(256) void k_render_tile(const JobP* __restrict__ jo
                                                      unsigned long long* __restrict__ ctr) {
     render_tile<FAM, false>(job, x0, y0, w, h, s0, ns, tiles_x, tiles, accum, depth, aux, depth64, vol, ctr);
 }
-// The megakernel of a job with fog: 30-40 more registers (two waves per SIMD instead of three).
+// The megakernel of a job with fog or finish detail patterns (both compiled in): 30-40 more
+// registers (two waves per SIMD instead of three).  vol: the session planes of a job with fog,
+// else null.
 template <uint32_t FAM>
-__global__ __launch_bounds__(256) void k_render_tile_fog(const JobP* __restrict__ job, uint32_t x0, uint32_t y0,
+__global__ __launch_bounds__(256) void k_render_tile_full(const JobP* __restrict__ job, uint32_t x0, uint32_t y0,
                                                          uint32_t w, uint32_t h, uint32_t s0, uint32_t ns,
                                                          uint32_t tiles_x, uint32_t tiles, float4* __restrict__ accum,
                                                          float* __restrict__ depth, float4* __restrict__ aux,
@@ -185,6 +187,7 @@ enum : uint32_t {
     WF_PDF, WF_TOTAL, WF_FD, WF_FA0, WF_FA1, WF_FA2, WF_FA3, WF_HT, WF_HTRAP,
     WF_TPREV,  // first segment: march position before a surface hit (FP64 refine bracket), else -1
     WF_VX, WF_VY, WF_VZ,  // fog: the camera segment's in-scattered radiance
+    WF_TW, WF_TS,         // the ray footprint for finish detail: width, spread
     // u32 planes
     WF_RNG, WF_BOUNCE, WF_HSTEPS, WF_HLIGHT,
     WF_PLANES
@@ -220,6 +223,7 @@ __device__ __forceinline__ void wf_save(const Soa& S, uint32_t id, const PathSta
     S.F(WF_NX, id) = st.prev_n.x; S.F(WF_NY, id) = st.prev_n.y; S.F(WF_NZ, id) = st.prev_n.z;
     S.F(WF_PDF, id) = st.prev_pdf; S.F(WF_TOTAL, id) = st.prev_total;
     S.F(WF_VX, id) = st.primary_volume.x; S.F(WF_VY, id) = st.primary_volume.y; S.F(WF_VZ, id) = st.primary_volume.z;
+    S.F(WF_TW, id) = st.tex_width; S.F(WF_TS, id) = st.tex_spread;
     S.U(WF_RNG, id) = st.rng;
     uint32_t b = S.U(WF_BOUNCE, id) & 0xc0000u;  // hit and exhausted belong to the intersect stage
     S.U(WF_BOUNCE, id) = b | (st.bounce & 0xffffu) | (st.alive ? 0x10000u : 0u) | (st.prev_from_bsdf ? 0x20000u : 0u) |
@@ -244,6 +248,8 @@ __device__ __forceinline__ PathState wf_load(const Soa& S, uint32_t id) {
     st.prev_volume_nee = (b & 0x200000u) != 0u;
     st.volume_path = (b & 0x400000u) != 0u;
     st.primary_volume = mk(S.F(WF_VX, id), S.F(WF_VY, id), S.F(WF_VZ, id));
+    st.tex_width = S.F(WF_TW, id);
+    st.tex_spread = S.F(WF_TS, id);
     st.sample = 0u;  // set by the stage that shades (it knows the pixel and the sample)
     st.point_stream = 0u;
     return st;

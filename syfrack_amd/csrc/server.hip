@@ -909,6 +909,11 @@ int syf_server_submit(syf_server* s, const syf_job_desc* desc, uint32_t spp, uin
         return fail(SYF_E_INVALID, "fog.density: the server traces no participating medium; render a job with fog "
                                    "with syf_render_tile or syf_render_wavefront");
     }
+    if (r->P.sh.detail) {
+        delete r;
+        return fail(SYF_E_INVALID, "finishes: the server renders no finish detail patterns; render a job with them "
+                                   "with syf_render_tile or syf_render_wavefront");
+    }
     r->spp = spp;
     r->pixels = size_t(desc->width) * desc->height;
     r->footprint = uint64_t(r->pixels) * BYTES_PER_PIXEL;

@@ -2,8 +2,8 @@
 // restricted to this path's scene subset: sun, area and environment lights; a flat ground
 // plane; the base material with its colouring ramp and SurfaceVariation, and named finishes
 // (Lambert or Oren-Nayar diffuse + GGX, emission); next-event estimation with MIS, Russian
-// roulette.  Fog, HDRI, portals, ocean, dunes, optics, textures, finish detail patterns,
-// guiding and phenomena are absent, so their branches are inert.  The random draws happen in
+// roulette; height fog and finish detail patterns.  HDRI, portals, ocean, dunes, optics,
+// textures, guiding and phenomena are absent, so their branches are inert.  The random draws happen in
 // the reference's order so that a path is a pure function of (pixel, sample, seed), whichever
 // kernel form runs it.
 #pragma once
@@ -166,11 +166,17 @@ __device__ __forceinline__ float surface_value(const JobP& P, uint32_t slot, F3 
     if (slot == SYF_COLOR_NORMAL) return n.y * 0.5f + 0.5f;
     return float(smp.iter) / fmaxf(float(P.f.iteration_product), 1.0f);
 }
+}  // namespace syf
+#include "finish_detail.hpp"
+namespace syf {
+
 // make_surface (pathtrace.wgsl:271-386) for this path's subset: the colouring ramp, a named
 // finish (albedo, roughness, metallic, specular, emission; Oren-Nayar sigma for the concrete
 // patterns), the SurfaceVariation.  The emission (Surface.emission) is returned apart: the
 // caller adds it at once and the surface record stays small for the bounce's remainder.
-__device__ __forceinline__ Surf make_surface(const JobP& P, F3 n, const ShadeSample& smp, F3& emission) {
+template <bool DETAIL>
+__device__ __forceinline__ Surf make_surface(const JobP& P, F3 p, F3 n, const ShadeSample& smp, float footprint,
+                                             F3& emission) {
     const MaterialP& m = P.mat;
     const ShadeP& S = P.sh;
     F3 albedo = ld3(m.albedo);
@@ -209,9 +215,14 @@ __device__ __forceinline__ Surf make_surface(const JobP& P, F3 n, const ShadeSam
         rough = m.surface[0] + (m.surface[1] - m.surface[0]) * t;
         metallic = m.surface[2] + (m.surface[3] - m.surface[2]) * t;
     }
+    F3 shading_n = n;
+    if (DETAIL && assigned && S.detail) {
+        const FinishP& f = S.finishes[smp.material - 1u];
+        if (f.pattern != 0u) apply_finish_detail(f, p, n, footprint, albedo, rough, metallic, spec, shading_n);
+    }
     rough = clampT(rough, 0.02f, 1.0f);
     Surf s;
-    s.n = n;
+    s.n = shading_n;
     s.diffuse = scl(albedo, 1.0f - metallic);
     s.f0 = mix3(mk(spec, spec, spec), albedo, metallic);
     s.alpha = rough * rough;
@@ -734,6 +745,9 @@ struct PathState {
     bool prev_from_volume, prev_volume_nee, volume_path;
     F3 primary_volume;
     uint32_t sample, point_stream;
+    // the ray footprint carried to the hits for finish detail (texture_width, texture_spread,
+    // pathtrace.wgsl:811-814): width at the ray origin and its growth per unit length
+    float tex_width, tex_spread;
 };
 
 // The session planes of one sample (pathtrace.wgsl:1243-1266) of a job with fog: the volume
@@ -779,6 +793,8 @@ __device__ __forceinline__ void path_start(const JobP& P, uint32_t px, uint32_t 
     st.primary_volume = mk(0.0f, 0.0f, 0.0f);
     st.sample = sample;
     st.point_stream = pcg(pcg(px ^ P.seed) ^ pcg(py ^ 0x27d4eb2du));  // pathtrace.wgsl:822
+    st.tex_width = 0.0f;
+    st.tex_spread = P.pixel_cone * 0.5f;
 }
 
 // One segment's outcome as the march and emitter test leave it.
@@ -795,8 +811,9 @@ struct SegHit {
 // estimation and the continuation sample.  `normal` evaluates the field gradient at a surface
 // point; `visible` marches a shadow ray; `sample` evaluates the full field sample at a surface
 // hit (called only when the shading reads more than the march's trap).  Sets st.alive = false
-// when the path ends.  FOG = false: a kernel for jobs without fog (the fog's code is not
-// compiled in, so that kernel keeps its registers for the rest).
+// when the path ends.  FOG = false: a kernel for jobs without fog or finish detail patterns
+// (neither's code is compiled in, so that kernel keeps its registers for the rest); FOG = true
+// also carries the ray footprint to the hits for the detail patterns.
 template <bool FOG, class Normal, class Vis, class Sample>
 __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const SegHit& h, float& first_depth,
                                             float4& first_aux, Normal&& normal, Vis&& visible, Sample&& sample) {
@@ -822,6 +839,8 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
         if (t_s < t_end) {  // a scattering event before the segment's end
             if (st.bounce == 0u) st.volume_path = true;
             const F3 p_s = add(st.o, scl(st.d, t_s));
+            st.tex_width += t_s * st.tex_spread;  // FOG: finish detail is compiled in as well
+            st.tex_spread = fmaxf(st.tex_spread, 0.25f);
             if (st.bounce == 0u && !h.hit) first_depth = t_s;
             const float vat = area_total_weight(P, p_s, mk(0.0f, 0.0f, 0.0f));
             const bool last = st.bounce == P.max_bounces;
@@ -874,7 +893,8 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
     const ShadeSample smp =
         ground ? ground_sample(P) : (P.sh.full_sample ? sample(p) : carried_sample(P, h.trap0));
     F3 emission;
-    const Surf s = make_surface(P, n, smp, emission);
+    if (FOG) st.tex_width += h.t * st.tex_spread;
+    const Surf s = make_surface<FOG>(P, p, n, smp, fmaxf(st.tex_width, 1e-4f), emission);
     st.L = add(st.L, mul3(st.throughput, emission));
     const float offset = ground ? 1e-3f : fmaxf(h.t * P.pixel_cone * 2.0f, 1e-3f) * P.pixel_tolerance;
     const F3 p_off = add(p, scl(n, offset));
@@ -921,6 +941,7 @@ __device__ __forceinline__ void path_bounce(const JobP& P, PathState& st, const 
     }
     st.o = p_off;
     st.d = l;
+    if (FOG) st.tex_spread = fmaxf(st.tex_spread, s.alpha * 0.25f);
     st.bounce++;
 }
 

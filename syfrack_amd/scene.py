@@ -256,14 +256,22 @@ class Coloring:
 
 @dataclass
 class Finish:
-    """Finish (material.rs:161-174) without a detail pattern; ``oren_nayar`` stands for the
-    Concrete, WeatheredConcrete and WetConcrete patterns' rough diffuse lobe at zero strength."""
+    """Finish (material.rs:161-174) with its FinishDetail (material.rs:193-221): ``pattern`` 1..12
+    numbers FinishPattern as abi.PATTERNS (0: no detail); the Concrete patterns (10..12) bring
+    Oren-Nayar's rough diffuse lobe, which ``oren_nayar`` also asks for without a pattern."""
     albedo: tuple = (0.8, 0.8, 0.8)
     roughness: float = 0.5
     metallic: float = 0.0
     specular: float = 0.04
     emission: tuple = (0.0, 0.0, 0.0)
     oren_nayar: bool = False
+    pattern: int = 0
+    seed: int = 7
+    scale: float = 2.0
+    strength: float = 0.25
+    bump: float = 0.004
+    roughness_delta: float = 0.15
+    tint: tuple = (0.3, 0.27, 0.2)
 
     def c_struct(self) -> abi.SyfFinish:
         f = abi.SyfFinish()
@@ -273,6 +281,14 @@ class Finish:
         f.specular = self.specular
         f.emission[:] = list(self.emission)
         f.oren_nayar = int(self.oren_nayar)
+        f.pattern = self.pattern
+        if self.pattern:
+            f.seed = self.seed
+            f.scale = self.scale
+            f.strength = self.strength
+            f.bump = self.bump
+            f.roughness_delta = self.roughness_delta
+            f.tint[:] = list(self.tint)
         return f
 
 

@@ -4,11 +4,9 @@ A scene file is the RON form of ``Scene`` (crates/syfrack-core/src/scene/mod.rs:
 form graph, camera, lights, material, atmosphere and look.  ``scene_from_ron`` reads the fields
 this path renders and refuses, with an error naming the scene element, what it does not:
 
-- a 2D scene, a creation graph, phenomena (mod.rs:92-110);
+- a 2D scene, a creation graph, phenomena (mod.rs:92-110), a form graph with an Instances node;
 - projector, tube and HDRI lights (light.rs:66-123);
-- a texture, optics, more than 8 finishes, and finish detail patterns that change the shading
-  (material.rs:140-255; a detail with zero strength, bump and roughness_delta leaves the surface
-  as it is, and the concrete patterns then keep their Oren-Nayar diffuse lobe);
+- a texture, optics and more than 8 finishes (material.rs:140-255);
 - the seascape, portal, nebula and dunes (atmosphere.rs:7-48).
 
 The look (tonemap, grading, grain, vignette) is the display pipeline applied after readback and is
@@ -31,7 +29,6 @@ from . import abi
 
 LEGACY_MESSAGE = "legacy scene format: use the previous version to convert this scene"
 MAX_LIGHTS_REFERENCE = 96  # light.rs:7
-_CONCRETE = ("Concrete", "WeatheredConcrete", "WetConcrete")
 
 
 class SceneFileError(ValueError):
@@ -141,20 +138,25 @@ def _coloring(c) -> Coloring:
 
 
 def _finish(i: int, d: dict) -> Finish:
-    """Finish (material.rs:161-174); see the module docstring for the detail patterns."""
+    """Finish with its FinishDetail (material.rs:161-221), the pattern numbered as abi.PATTERNS."""
     where = f"material.finishes[{i}]"
     if d.get("optics") is not None:
         raise SceneFileError(f"{where}.optics: not rendered on this path")
-    oren_nayar = False
+    f = Finish(albedo=_v3(d["albedo"]), roughness=float(d["roughness"]), metallic=float(d["metallic"]),
+               specular=float(d["specular"]), emission=_v3(d.get("emission", (0.0, 0.0, 0.0))))
     det = d.get("detail")
     if det is not None:
-        if any(float(det.get(k, 0.0)) != 0.0 for k in ("strength", "bump", "roughness_delta")):
-            raise SceneFileError(f"{where}.detail: finish patterns are not rendered on this path (a detail with "
-                                 "zero strength, bump and roughness_delta is)")
-        oren_nayar = det.get("pattern") in _CONCRETE
-    return Finish(albedo=_v3(d["albedo"]), roughness=float(d["roughness"]), metallic=float(d["metallic"]),
-                  specular=float(d["specular"]), emission=_v3(d.get("emission", (0.0, 0.0, 0.0))),
-                  oren_nayar=oren_nayar)
+        name = det.get("pattern")
+        if name not in abi.PATTERNS:
+            raise SceneFileError(f"{where}.detail.pattern: unknown pattern {name!r}")
+        f.pattern = abi.PATTERNS.index(name) + 1
+        f.seed = int(det["seed"])
+        f.scale = float(det["scale"])
+        f.strength = float(det["strength"])
+        f.bump = float(det["bump"])
+        f.roughness_delta = float(det["roughness_delta"])
+        f.tint = _v3(det["tint"])
+    return f
 
 
 def _material(d: dict) -> Material:
@@ -180,10 +182,15 @@ def _form(d: dict) -> Graph:
     if fr is None:
         return Graph.bundled("sphere")
     if isinstance(fr, Variant) and fr.variant == "Graph" and len(fr) == 1 and isinstance(fr[0], dict):
-        return Graph.from_dict(fr[0])
-    if isinstance(fr, dict) and fr.get("__struct__") == "Graph":
-        return Graph.from_dict(fr)
-    raise SceneFileError(f"fractal: a Graph, got {type(fr).__name__}")
+        g = Graph.from_dict(fr[0])
+    elif isinstance(fr, dict) and fr.get("__struct__") == "Graph":
+        g = Graph.from_dict(fr)
+    else:
+        raise SceneFileError(f"fractal: a Graph, got {type(fr).__name__}")
+    for n in g.nodes:
+        if n.kind == "Instances" and n.enabled:
+            raise SceneFileError(f"fractal: node {n.id}: Instances (independent copies) is not evaluated on this path")
+    return g
 
 
 def scene_from_dict(d: dict) -> Scene:

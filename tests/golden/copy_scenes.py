@@ -1,8 +1,9 @@
 """Copy three of the reference's bundled scene files into tests/golden/scenes/ as fixtures.
 
-Source: scenes/{exhibition,forme-exposition,orfevrerie}.ron of the SyFrack tree: the bundled
+Source: scenes/{exhibition,forme-exposition,orfevrerie}.ron of the SyFrack tree, the bundled
 scenes that combine a sun, an area emitter, a sky gradient and the height fog over a form graph
-(Kelvin and linear light colours, a trap colouring ramp).  They are data (RON scene
+(Kelvin and linear light colours, a trap colouring ramp); scenes/studies/epic-20261014/3d/3d-01.ron (brushed metal and two concrete finishes on a ground)
+and scenes/studies/epic-20261014/3d-volumes/3d-volumes-01.ron (brushed metal).  They are data (RON scene
 descriptions), copied byte for byte.  Run from the repo root:
     python tests/golden/copy_scenes.py /root/reference
 """
@@ -10,13 +11,14 @@ import os
 import shutil
 import sys
 
-NAMES = ("exhibition", "forme-exposition", "orfevrerie")
+NAMES = ("exhibition", "forme-exposition", "orfevrerie", "studies/epic-20261014/3d-volumes/3d-volumes-01", "studies/epic-20261014/3d/3d-01")
 
 
 def copy(ref_root: str, out_dir: str) -> None:
     os.makedirs(out_dir, exist_ok=True)
     for name in NAMES:
-        shutil.copyfile(os.path.join(ref_root, "scenes", name + ".ron"), os.path.join(out_dir, name + ".ron"))
+        shutil.copyfile(os.path.join(ref_root, "scenes", name + ".ron"),
+                        os.path.join(out_dir, os.path.basename(name) + ".ron"))
 
 
 if __name__ == "__main__":

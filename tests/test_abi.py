@@ -19,7 +19,7 @@ def test_library_exports_every_header_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) == set(abi.SIGNATURES), set(names) ^ set(abi.SIGNATURES)
-    assert lib.syf_abi_version() == 5
+    assert lib.syf_abi_version() == 6
 
 
 STRUCTS = [
