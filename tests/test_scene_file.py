@@ -191,3 +191,12 @@ def test_every_reference_scene_loads_or_is_refused_by_name():
         loaded += 1
     assert loaded >= 71 and refused > 0
     print(f"{loaded} scene files lowered, {refused} refused by name")
+
+
+def test_patterned_scene_survives_the_session_state():
+    from syfrack_amd.ron import to_ron
+    from syfrack_amd.session import scene_from_state
+    s = load_scene(os.path.join(SCENES, "3d-01.ron"))
+    back = scene_from_state(from_ron(to_ron({"scene": s}))["scene"])
+    assert back.material.finishes == s.material.finishes and back.ground == s.ground
+    assert [f.pattern for f in back.material.finishes] == [11, 3, 0, 10]
