@@ -626,6 +626,9 @@ def main() -> int:
     counters = read_counters("k_primary", args.width, args.height)
     if counters:
         out["roofline"]["valu_counters"] = counters
+    # the frame job's stream holds one of the process's hardware queues: release it before the
+    # servers create their launch streams, so that each of those gets a queue of its own
+    job.close()
     if not args.no_serve:
         sv = serving_bench(r.ctx, args.serve_requests)
         sv["serving_loop"] = serving_loop_bench(r.ctx, [float(x) for x in args.loop_rates.split(",") if x.strip()],
@@ -651,7 +654,6 @@ def main() -> int:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.width, args.height)
         out["errors"] = error_report(r, args.march_math)
-    job.close()
     print(json.dumps(out), flush=True)
     return 0
 
